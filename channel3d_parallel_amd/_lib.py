@@ -1,0 +1,78 @@
+"""ctypes binding of libc3d.so (the C ABI declared in include/c3d.h).
+
+The library is the only compute path: if it is missing or cannot be loaded
+this module raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libc3d.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+C3D_OK = 0
+ERRORS = {-1: "C3D_EINVAL", -2: "C3D_EHIP", -3: "C3D_ENOMEM", -4: "C3D_ENOCONV", -5: "C3D_ENAN"}
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class c3d_desc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("grid_type", C.c_int),
+        ("Lxp", C.c_double), ("Lzp", C.c_double), ("Ly", C.c_double), ("alpha", C.c_double),
+        ("nu", C.c_double), ("precision", C.c_int),
+    ]
+
+
+# (name, restype, argtypes) of every entry point in include/c3d.h
+SIGNATURES = [
+    ("c3d_create", C.c_int, [C.POINTER(c3d_desc), C.POINTER(C.c_void_p)]),
+    ("c3d_destroy", C.c_int, [C.c_void_p]),
+    ("c3d_last_error", C.c_char_p, [C.c_void_p]),
+    ("c3d_grid_get", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
+    ("c3d_set_fields", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    ("c3d_get_fields", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    ("c3d_apply_boundary_conditions", C.c_int, [C.c_void_p]),
+    ("c3d_divergence", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_gradient", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    ("c3d_solve_poisson_equation", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double]),
+    ("c3d_compute_non_ib_projection", C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    ("c3d_check_divergence", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_time_projection", C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp, _dp, C.c_int]),
+    ("c3d_stage_count", C.c_int, []),
+    ("c3d_stage_name", C.c_char_p, [C.c_int]),
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libc3d.so; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m channel3d_parallel_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    """Pointer to a float64 numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    if a.dtype.kind != "f" or a.dtype.itemsize != 8:
+        raise TypeError("float64 array required")
+    if not (a.flags["F_CONTIGUOUS"] or a.ndim == 1):
+        raise ValueError("arrays must be Fortran-contiguous")
+    return a.ctypes.data_as(_dp)

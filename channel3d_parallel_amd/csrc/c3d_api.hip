@@ -1,0 +1,472 @@
+// C ABI of the channel3D projection library: context, grid set-up and the
+// entry points declared in include/c3d.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/c3d.h"
+#include "fft_lds.hpp"
+#include "kernels.hpp"
+
+using c3d::cpx;
+using c3d::DevGrid;
+
+struct c3d_ctx {
+  c3d_desc desc{};
+  int nx = 0, ny = 0, nz = 0, nxg = 0, nyg = 0, nzg = 0, nxp = 0, nzp = 0, nym = 0, nxh = 0;
+  // host grid (Fortran index i stored at [i-1])
+  std::vector<double> x, y, z, xg, yg, zg, kxx, kzz, dyy_d, dyy_l, dyy_u;
+  double dx = 0, dz = 0, dxmin = 0, dymin = 0, dzmin = 0, Lx = 0, Ly = 0, Lz = 0, Lxp = 0, Lzp = 0;
+  // device state
+  hipStream_t stream = nullptr;
+  DevGrid g{};
+  std::vector<void*> allocs;
+  double *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr, *Pint = nullptr, *rhs = nullptr;
+  cpx<double>* S = nullptr;
+  double* CP = nullptr;
+  unsigned long long* red = nullptr;
+  size_t nu = 0, nv = 0, nw = 0, np = 0, nr = 0, nspec = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(c3d_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(ctx, C3D_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),        \
+                  __FILE__, __LINE__);                                                        \
+  } while (0)
+
+template <typename T>
+int dalloc(c3d_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
+  if (e != hipSuccess)
+    return fail(c, C3D_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T),
+                hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return 0;
+}
+
+template <typename T>
+int upload(c3d_ctx* c, const T** dst, const std::vector<T>& v) {
+  T* p = nullptr;
+  int rc = dalloc(c, &p, v.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *dst = p;
+  return 0;
+}
+
+// Centres with mirror ghosts, src/initialization.f90:173-199.
+void centres(const std::vector<double>& f, std::vector<double>& g) {
+  const size_t n = f.size();
+  g.assign(n + 1, 0.0);
+  for (size_t i = 0; i + 1 < n; i++) g[i + 1] = 0.5 * (f[i] + f[i + 1]);
+  g[0] = g[1] - 2.0 * (g[1] - f[0]);
+  g[n] = g[n - 1] + 2.0 * (f[n - 1] - g[n - 1]);
+}
+
+double min_diff(const std::vector<double>& a) {
+  double m = a[1] - a[0];
+  for (size_t i = 1; i + 1 < a.size(); i++) m = std::fmin(m, a[i + 1] - a[i]);
+  return m;
+}
+
+// Modified wavenumbers 2(cos(2 pi i/n) - 1)/d^2, src/initialization.f90:290-305
+// (dx**2d0 evaluated as dx*dx).
+std::vector<double> modified_wavenumbers(int n, double d) {
+  const double pi = 4.0 * std::atan(1.0);  // src/global.f90:34
+  std::vector<double> k(n, 0.0);
+  const int half = (n + 1) / 2;
+  for (int i = 0; i <= half && i < n; i++)
+    k[i] = 2.0 * (std::cos(2.0 * pi * (double)i / (double)n) - 1.0) / (d * d);
+  for (int i = half + 1; i < n; i++)
+    k[i] = 2.0 * (std::cos(2.0 * pi * (double)(-n + i) / (double)n) - 1.0) / (d * d);
+  return k;
+}
+
+template <typename real>
+std::vector<cpx<real>> twiddles(int n) {
+  std::vector<cpx<real>> t(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < n; m++) {
+    const long double a = -2.0L * pi * (long double)m / (long double)n;
+    t[m].x = (real)cosl(a);
+    t[m].y = (real)sinl(a);
+  }
+  return t;
+}
+
+// Grid generation, src/input_output.f90:140-206 (types 0, 1) and the
+// derived quantities of src/initialization.f90:166-390.
+void build_grid(c3d_ctx* c) {
+  const c3d_desc& d = c->desc;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  c->x.resize(nx);
+  c->z.resize(nz);
+  c->y.resize(ny);
+  for (int i = 0; i < nx; i++) c->x[i] = d.Lxp * (double)i / (double)(nx - 2);
+  for (int i = 0; i < nz; i++) c->z[i] = d.Lzp * (double)i / (double)(nz - 2);
+  if (d.grid_type == 0) {
+    for (int i = 0; i < ny; i++) c->y[i] = d.Ly * (double)i / (double)(ny - 1);
+  } else {
+    for (int i = 0; i < ny; i++) c->y[i] = d.Ly * (double)i / (double)(ny - 1) - 1.0;
+    if (d.alpha > 0.0)
+      for (int i = 0; i < ny; i++) c->y[i] = std::tanh(d.alpha * c->y[i]) / std::tanh(d.alpha);
+    double mn = c->y[0];
+    for (int i = 1; i < ny; i++) mn = std::fmin(mn, c->y[i]);
+    for (int i = 0; i < ny; i++) c->y[i] = c->y[i] - mn;
+    double mx = c->y[0];
+    for (int i = 1; i < ny; i++) mx = std::fmax(mx, c->y[i]);
+    for (int i = 0; i < ny; i++) c->y[i] = c->y[i] * d.Ly / mx;
+  }
+  centres(c->x, c->xg);
+  centres(c->y, c->yg);
+  centres(c->z, c->zg);
+  c->dxmin = min_diff(c->xg);
+  c->dymin = min_diff(c->yg);
+  c->dzmin = min_diff(c->zg);
+  c->Lx = c->x[nx - 1] - c->x[0];
+  c->Ly = c->y[ny - 1] - c->y[0];
+  c->Lz = c->z[nz - 1] - c->z[0];
+  c->dx = c->dxmin;
+  c->dz = c->dzmin;
+  c->Lxp = c->Lx - c->dx;
+  c->Lzp = c->Lz - c->dz;
+  c->kxx = modified_wavenumbers(nx - 2, c->dx);
+  c->kzz = modified_wavenumbers(nz - 2, c->dz);
+  // Dyy, src/initialization.f90:353-383 (Fortran j = 2..nyg-1 -> jj = j-2)
+  const int nyg = ny + 1, nym = ny - 1;
+  c->dyy_d.assign(nym, 0.0);
+  c->dyy_l.assign(nym > 1 ? nym - 1 : 1, 0.0);
+  c->dyy_u.assign(nym > 1 ? nym - 1 : 1, 0.0);
+  const std::vector<double>&y = c->y, &yg = c->yg;
+  for (int j = 2; j <= nyg - 1; j++) {
+    const double dyf = y[j - 1] - y[j - 2];
+    const double a = 1.0 / dyf / (yg[j] - yg[j - 1]);
+    const double b = 1.0 / dyf * (-1.0 / (yg[j] - yg[j - 1]) - 1.0 / (yg[j - 1] - yg[j - 2]));
+    const double cc = 1.0 / dyf / (yg[j - 1] - yg[j - 2]);
+    const int jj = j - 2;
+    if (j == 2) {
+      c->dyy_d[jj] = b + cc;
+      c->dyy_u[jj] = a;
+    } else if (j == nyg - 1) {
+      c->dyy_d[jj] = a + b;
+      c->dyy_l[jj - 1] = cc;
+    } else {
+      c->dyy_d[jj] = b;
+      c->dyy_u[jj] = a;
+      c->dyy_l[jj - 1] = cc;
+    }
+  }
+}
+
+// Fortran-indexed spacing table: t[i] = a(i) - a(i-1) for i = 2..n (t[0], t[1] = 0).
+std::vector<double> face_spacing(const std::vector<double>& a) {
+  std::vector<double> t(a.size() + 2, 0.0);
+  for (size_t i = 2; i <= a.size(); i++) t[i] = a[i - 1] - a[i - 2];
+  return t;
+}
+// t[i] = a(i+1) - a(i) for i = 1..n-1.
+std::vector<double> centre_spacing(const std::vector<double>& a) {
+  std::vector<double> t(a.size() + 2, 0.0);
+  for (size_t i = 1; i + 1 <= a.size(); i++) t[i] = a[i] - a[i - 1];
+  return t;
+}
+
+int sync(c3d_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// The Poisson pipeline between the forward and the inverse x transforms.
+int spectral_core(c3d_ctx* c) {
+  HIPCHK(c, c3d::launch_fft_z(c->g, -1, c->S, c->stream));
+  HIPCHK(c, c3d::launch_tri(c->g, c->S, c->CP, c->stream));
+  HIPCHK(c, c3d::launch_fft_z(c->g, +1, c->S, c->stream));
+  return 0;
+}
+
+const char* kStageNames[] = {"fwd_x_div", "fft_z_fwd", "tri_y", "fft_z_inv", "inv_x",
+                             "grad_correct"};
+constexpr int kNumStages = 6;
+
+}  // namespace
+
+extern "C" {
+
+int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
+  if (!desc || !out) return C3D_EINVAL;
+  *out = nullptr;
+  const c3d_desc& d = *desc;
+  // src/initialization.f90:46-47: even nx, nz; FFT lengths must be supported.
+  if (d.nx < 6 || d.nz < 6 || d.ny < 4 || (d.nx % 2) || (d.nz % 2)) return C3D_EINVAL;
+  if (!c3d::fft_length_supported(d.nx - 2) || !c3d::fft_length_supported(d.nz - 2))
+    return C3D_EINVAL;
+  if (d.grid_type != 0 && d.grid_type != 1) return C3D_EINVAL;
+  if (d.precision != C3D_PREC_FP64) return C3D_EINVAL;
+  if (!(d.Lxp > 0) || !(d.Lzp > 0) || !(d.Ly > 0)) return C3D_EINVAL;
+
+  c3d_ctx* c = new c3d_ctx();
+  c->desc = d;
+  c->nx = d.nx; c->ny = d.ny; c->nz = d.nz;
+  c->nxg = d.nx + 1; c->nyg = d.ny + 1; c->nzg = d.nz + 1;
+  c->nxp = d.nx - 2; c->nzp = d.nz - 2; c->nym = d.ny - 1; c->nxh = c->nxp / 2 + 1;
+  build_grid(c);
+
+  auto cleanup = [&](int rc) {
+    c3d_destroy(c);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(C3D_EHIP);
+
+  DevGrid& g = c->g;
+  g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
+  g.nxg = c->nxg; g.nyg = c->nyg; g.nzg = c->nzg;
+  g.nxp = c->nxp; g.nzp = c->nzp; g.nym = c->nym; g.nxh = c->nxh;
+  int rc = 0;
+  if ((rc = upload(c, &g.dxf, face_spacing(c->x))) || (rc = upload(c, &g.dyf, face_spacing(c->y))) ||
+      (rc = upload(c, &g.dzf, face_spacing(c->z))) ||
+      (rc = upload(c, &g.dxg, centre_spacing(c->xg))) ||
+      (rc = upload(c, &g.dyg, centre_spacing(c->yg))) ||
+      (rc = upload(c, &g.dzg, centre_spacing(c->zg))) || (rc = upload(c, &g.kxx, c->kxx)) ||
+      (rc = upload(c, &g.kzz, c->kzz)) || (rc = upload(c, &g.dyy_d, c->dyy_d)) ||
+      (rc = upload(c, &g.dyy_l, c->dyy_l)) || (rc = upload(c, &g.dyy_u, c->dyy_u)) ||
+      (rc = upload(c, &g.twx64, twiddles<double>(c->nxp))) ||
+      (rc = upload(c, &g.twz64, twiddles<double>(c->nzp))) ||
+      (rc = upload(c, &g.twx32, twiddles<float>(c->nxp))) ||
+      (rc = upload(c, &g.twz32, twiddles<float>(c->nzp))))
+    return cleanup(rc);
+
+  c->nu = (size_t)c->nx * c->nyg * c->nzg;
+  c->nv = (size_t)c->nxg * c->ny * c->nzg;
+  c->nw = (size_t)c->nxg * c->nyg * c->nz;
+  c->np = (size_t)c->nxg * c->nyg * c->nzg;
+  c->nr = (size_t)(c->nxg - 2) * (c->nyg - 2) * (c->nzg - 1);
+  c->nspec = (size_t)c->nym * c->nxh * c->nzp;
+  if ((rc = dalloc(c, &c->U, c->nu)) || (rc = dalloc(c, &c->V, c->nv)) ||
+      (rc = dalloc(c, &c->W, c->nw)) || (rc = dalloc(c, &c->P, c->np)) ||
+      (rc = dalloc(c, &c->Pint, c->nr)) || (rc = dalloc(c, &c->rhs, c->nr)) ||
+      (rc = dalloc(c, &c->S, c->nspec)) || (rc = dalloc(c, &c->CP, c->nspec)) ||
+      (rc = dalloc(c, &c->red, 1)))
+    return cleanup(rc);
+  if (hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->V, 0, c->nv * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->W, 0, c->nw * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->P, 0, c->np * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->Pint, 0, c->nr * 8, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return cleanup(C3D_EHIP);
+  *out = c;
+  return C3D_OK;
+}
+
+int c3d_destroy(c3d_ctx* c) {
+  if (!c) return C3D_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return C3D_OK;
+}
+
+const char* c3d_last_error(const c3d_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int c3d_grid_get(const c3d_ctx* c, int what, double* out, int n) {
+  if (!c || !out) return C3D_EINVAL;
+  const std::vector<double>* v = nullptr;
+  std::vector<double> s;
+  switch (what) {
+    case 0: v = &c->x; break;
+    case 1: v = &c->y; break;
+    case 2: v = &c->z; break;
+    case 3: v = &c->xg; break;
+    case 4: v = &c->yg; break;
+    case 5: v = &c->zg; break;
+    case 6: v = &c->kxx; break;
+    case 7: v = &c->kzz; break;
+    case 8: v = &c->dyy_d; break;
+    case 9: v = &c->dyy_l; break;
+    case 10: v = &c->dyy_u; break;
+    case 11:
+      s = {c->dx, c->dz, c->dxmin, c->dymin, c->dzmin, c->Lx, c->Ly, c->Lz, c->Lxp, c->Lzp};
+      v = &s;
+      break;
+    default: return C3D_EINVAL;
+  }
+  if ((int)v->size() > n) return C3D_EINVAL;
+  std::memcpy(out, v->data(), v->size() * sizeof(double));
+  return (int)v->size();
+}
+
+int c3d_set_fields(c3d_ctx* c, const double* U, const double* V, const double* W, const double* P) {
+  if (!c) return C3D_EINVAL;
+  if (U) HIPCHK(c, hipMemcpyAsync(c->U, U, c->nu * 8, hipMemcpyHostToDevice, c->stream));
+  if (V) HIPCHK(c, hipMemcpyAsync(c->V, V, c->nv * 8, hipMemcpyHostToDevice, c->stream));
+  if (W) HIPCHK(c, hipMemcpyAsync(c->W, W, c->nw * 8, hipMemcpyHostToDevice, c->stream));
+  if (P) HIPCHK(c, hipMemcpyAsync(c->P, P, c->np * 8, hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+int c3d_get_fields(c3d_ctx* c, double* U, double* V, double* W, double* P, double* Pint) {
+  if (!c) return C3D_EINVAL;
+  if (U) HIPCHK(c, hipMemcpyAsync(U, c->U, c->nu * 8, hipMemcpyDeviceToHost, c->stream));
+  if (V) HIPCHK(c, hipMemcpyAsync(V, c->V, c->nv * 8, hipMemcpyDeviceToHost, c->stream));
+  if (W) HIPCHK(c, hipMemcpyAsync(W, c->W, c->nw * 8, hipMemcpyDeviceToHost, c->stream));
+  if (P) HIPCHK(c, hipMemcpyAsync(P, c->P, c->np * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Pint) HIPCHK(c, hipMemcpyAsync(Pint, c->Pint, c->nr * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_apply_boundary_conditions(c3d_ctx* c) {
+  if (!c) return C3D_EINVAL;
+  HIPCHK(c, c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream));
+  return sync(c);
+}
+
+int c3d_divergence(c3d_ctx* c, double* div) {
+  if (!c || !div) return C3D_EINVAL;
+  HIPCHK(c, c3d::launch_divergence(c->g, c->U, c->V, c->W, c->rhs, c->stream));
+  HIPCHK(c, hipMemcpyAsync(div, c->rhs, c->nr * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_gradient(c3d_ctx* c, const double* p, double* Gx, double* Gy, double* Gz) {
+  if (!c || !p || !Gx || !Gy || !Gz) return C3D_EINVAL;
+  double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+  int rc = 0;
+  // scratch for the host-array drop-in (not on the resident hot path)
+  if (hipMalloc(&gx, c->nu * 8) != hipSuccess || hipMalloc(&gy, c->nv * 8) != hipSuccess ||
+      hipMalloc(&gz, c->nw * 8) != hipSuccess) {
+    rc = fail(c, C3D_ENOMEM, "gradient scratch allocation failed");
+  } else if (hipMemcpyAsync(c->rhs, p, c->nr * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+             hipMemsetAsync(gx, 0, c->nu * 8, c->stream) != hipSuccess ||
+             hipMemsetAsync(gy, 0, c->nv * 8, c->stream) != hipSuccess ||
+             hipMemsetAsync(gz, 0, c->nw * 8, c->stream) != hipSuccess ||
+             c3d::launch_gradient(c->g, c->rhs, gx, gy, gz, c->stream) != hipSuccess ||
+             hipMemcpyAsync(Gx, gx, c->nu * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+             hipMemcpyAsync(Gy, gy, c->nv * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+             hipMemcpyAsync(Gz, gz, c->nw * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+             hipStreamSynchronize(c->stream) != hipSuccess) {
+    rc = fail(c, C3D_EHIP, "gradient failed");
+  }
+  (void)hipStreamSynchronize(c->stream);
+  if (gx) (void)hipFree(gx);
+  if (gy) (void)hipFree(gy);
+  if (gz) (void)hipFree(gz);
+  return rc;
+}
+
+int c3d_solve_poisson_equation(c3d_ctx* c, double* rhs, int rk_step, double dt) {
+  if (!c || !rhs) return C3D_EINVAL;
+  if (rk_step == 1 && !(dt != 0.0)) return fail(c, C3D_EINVAL, "dt must be nonzero at rk_step 1");
+  HIPCHK(c, hipMemcpyAsync(c->rhs, rhs, c->nr * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, c3d::launch_fwd_x(c->g, false, nullptr, nullptr, nullptr, c->rhs, c->S, c->stream));
+  int rc = spectral_core(c);
+  if (rc) return rc;
+  HIPCHK(c, c3d::launch_inv_x(c->g, 0, c->S, c->rhs, c->P, dt, rk_step == 1, c->stream));
+  if (rk_step == 1) HIPCHK(c, c3d::launch_p_ghosts(c->g, c->P, c->stream));
+  HIPCHK(c, hipMemcpyAsync(rhs, c->rhs, c->nr * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev) {
+  const bool rk1 = rk_step == 1;
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  HIPCHK(c, c3d::launch_fwd_x(c->g, true, c->U, c->V, c->W, nullptr, c->S, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  HIPCHK(c, c3d::launch_fft_z(c->g, -1, c->S, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  HIPCHK(c, c3d::launch_tri(c->g, c->S, c->CP, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+  HIPCHK(c, c3d::launch_fft_z(c->g, +1, c->S, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
+  HIPCHK(c, c3d::launch_inv_x(c->g, 1, c->S, c->Pint, c->P, dt, rk1, c->stream));
+  if (rk1) HIPCHK(c, c3d::launch_p_ghosts(c->g, c->P, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[5], c->stream));
+  HIPCHK(c, c3d::launch_grad_correct(c->g, c->U, c->V, c->W, c->Pint, dt, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[6], c->stream));
+  return 0;
+}
+
+int c3d_compute_non_ib_projection(c3d_ctx* c, double dt, int rk_step) {
+  if (!c) return C3D_EINVAL;
+  if (!(dt != 0.0)) return fail(c, C3D_EINVAL, "dt must be nonzero");
+  int rc = projection_launch(c, dt, rk_step, nullptr);
+  if (rc) return rc;
+  return sync(c);
+}
+
+int c3d_check_divergence(c3d_ctx* c, double* max_div) {
+  if (!c || !max_div) return C3D_EINVAL;
+  HIPCHK(c, hipMemsetAsync(c->red, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, c3d::launch_max_div(c->g, c->U, c->V, c->W, c->red, c->stream));
+  unsigned long long bits = 0;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->red, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  std::memcpy(max_div, &bits, sizeof(double));
+  return C3D_OK;
+}
+
+int c3d_stage_count(void) { return kNumStages; }
+const char* c3d_stage_name(int i) { return (i >= 0 && i < kNumStages) ? kStageNames[i] : ""; }
+
+int c3d_time_projection(c3d_ctx* c, int iters, double dt, double* total_ms, double* kernel_ms,
+                        int nk) {
+  if (!c || iters <= 0 || !(dt != 0.0)) return C3D_EINVAL;
+  std::vector<hipEvent_t> ev((size_t)iters * (kNumStages + 1));
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  int rc = 0;
+  for (int it = 0; it < iters && !rc; it++)
+    rc = projection_launch(c, dt, 2, &ev[(size_t)it * (kNumStages + 1)]);
+  if (!rc) rc = sync(c);
+  if (!rc) {
+    std::vector<double> acc(kNumStages, 0.0);
+    double tot = 0.0;
+    for (int it = 0; it < iters && !rc; it++) {
+      hipEvent_t* e = &ev[(size_t)it * (kNumStages + 1)];
+      for (int s = 0; s < kNumStages; s++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e[s], e[s + 1]) != hipSuccess) {
+          rc = fail(c, C3D_EHIP, "hipEventElapsedTime failed");
+          break;
+        }
+        acc[s] += ms;
+      }
+      float ms = 0.f;
+      if (!rc && hipEventElapsedTime(&ms, e[0], e[kNumStages]) == hipSuccess) tot += ms;
+    }
+    if (!rc) {
+      if (total_ms) *total_ms = tot / iters;
+      if (kernel_ms)
+        for (int s = 0; s < nk && s < kNumStages; s++) kernel_ms[s] = acc[s] / iters;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,577 @@
+// Projection-path kernels for gfx950 (FP64 plain path).
+//
+// Spectral layout: the half spectrum S is stored [j][kx][kz] with kz
+// fastest (nym x nxh x nzp complex), so that
+//   * the x transform writes 16-wide kz runs (LDS transpose),
+//   * the z transform reads one contiguous column per (j, kx),
+//   * the y tridiagonal runs one lane per (kx, kz) with lanes along kz,
+//     coalesced at every j.
+// Physical fields keep the reference's Fortran layout (x fastest,
+// ghost-inclusive), so host copies need no reshaping.
+#include "fft_lds.hpp"
+#include "kernels.hpp"
+
+namespace c3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Rows of the z tile handled by one x-transform workgroup (row pairs are
+// packed into one complex transform).
+__host__ __device__ constexpr int tile_rows(int nzp) {
+  return (nzp % 16 == 0) ? 16 : (nzp % 8 == 0) ? 8 : (nzp % 4 == 0) ? 4 : 2;
+}
+
+// ---------------------------------------------------------------------------
+// Forward x transform: div (or rhs) rows -> R2C half spectrum.
+// Block (kBlock) handles y-plane j = blockIdx.y + 2 and z rows
+// k = blockIdx.x*TK + 2 .. +TK-1.  Rows 2p and 2p+1 are transformed together
+// as the real and imaginary parts of one complex row and separated after.
+// ---------------------------------------------------------------------------
+template <typename real, int N, int TK, bool FROM_UVW>
+__global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __restrict__ U,
+                                                  const double* __restrict__ V,
+                                                  const double* __restrict__ W,
+                                                  const double* __restrict__ rhs,
+                                                  const cpx<real>* __restrict__ tw,
+                                                  cpx<real>* __restrict__ S) {
+  constexpr int NP = TK / 2;
+  constexpr int NT = kBlock / NP;
+  constexpr int NH = N / 2 + 1;
+  constexpr int BUF = (NP * N > TK * NH) ? NP * N : TK * NH;
+  __shared__ cpx<real> buf[BUF];
+
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TK;
+  const int jj = blockIdx.y;
+  const int j = jj + 2;
+  const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
+
+  // 1) rows -> LDS (x contiguous, coalesced)
+  for (int e = tid; e < TK * N; e += kBlock) {
+    const int n = e % N, t = e / N;
+    const int i = n + 2, k = k0 + t + 2;
+    double d;
+    if constexpr (FROM_UVW) {
+      // divergence, src/differential_operators.f90:64-72
+      d = (U[fidx(i, j, k, nx, nyg)] - U[fidx(i - 1, j, k, nx, nyg)]) / g.dxf[i] +
+          (V[fidx(i, j, k, nxg, ny)] - V[fidx(i, j - 1, k, nxg, ny)]) / g.dyf[j] +
+          (W[fidx(i, j, k, nxg, nyg)] - W[fidx(i, j, k - 1, nxg, nyg)]) / g.dzf[k];
+    } else {
+      d = rhs[ridx(i, j, k, nxg, nyg)];
+    }
+    real* bp = reinterpret_cast<real*>(&buf[(t >> 1) * N + n]);
+    bp[t & 1] = (real)d;
+  }
+  __syncthreads();
+
+  // 2) complex FFT of each packed pair
+  fft_lds<real, N, NT, -1>(buf + (tid / NT) * N, tw, tid % NT);
+
+  // 3) separate the two real transforms: A_k = (Z_k + conj Z_{N-k})/2,
+  //    B_k = (Z_k - conj Z_{N-k})/(2i); store transposed [kx][row].
+  constexpr int PER = (NP * NH + kBlock - 1) / kBlock;
+  cpx<real> A[PER], B[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + q * kBlock;
+    if (e < NP * NH) {
+      const int p = e / NH, kx = e % NH;
+      const cpx<real> z = buf[p * N + kx];
+      const cpx<real> zc = buf[p * N + (kx == 0 ? 0 : N - kx)];
+      A[q] = cmk((real)0.5 * (z.x + zc.x), (real)0.5 * (z.y - zc.y));
+      B[q] = cmk((real)0.5 * (z.y + zc.y), (real)0.5 * (zc.x - z.x));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + q * kBlock;
+    if (e < NP * NH) {
+      const int p = e / NH, kx = e % NH;
+      buf[kx * TK + 2 * p] = A[q];
+      buf[kx * TK + 2 * p + 1] = B[q];
+    }
+  }
+  __syncthreads();
+
+  // 4) LDS -> S[j][kx][k0 .. k0+TK)
+  const int nzp = g.nzp;
+  for (int e = tid; e < NH * TK; e += kBlock) {
+    const int kx = e / TK, t = e % TK;
+    S[((size_t)jj * NH + kx) * nzp + k0 + t] = buf[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z transform (C2C) of every column S[j][kx][:], in place.
+// ---------------------------------------------------------------------------
+template <typename real, int N, int SIGN>
+__global__ __launch_bounds__(kBlock) void k_fft_z(cpx<real>* __restrict__ S,
+                                                  const cpx<real>* __restrict__ tw, int ncol) {
+  constexpr int NT = 64;
+  constexpr int NC = kBlock / NT;
+  __shared__ cpx<real> buf[NC * N];
+  const int tid = threadIdx.x;
+  const size_t col0 = (size_t)blockIdx.x * NC;
+  for (int e = tid; e < NC * N; e += kBlock) {
+    const int c = e / N;
+    if (col0 + c < (size_t)ncol) buf[e] = S[col0 * N + e];
+  }
+  __syncthreads();
+  fft_lds<real, N, NT, SIGN>(buf + (tid / NT) * N, tw, tid % NT);
+  for (int e = tid; e < NC * N; e += kBlock) {
+    const int c = e / N;
+    if (col0 + c < (size_t)ncol) S[col0 * N + e] = buf[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonal solve along y for every mode (kx, kz): src/poisson.f90:50-75.
+// Matrix: D(j) = (Dyy(j,j) + kxx) + kzz, DL(j) = Dyy(j+1,j), DU(j) = Dyy(j,j+1),
+// and D(2) *= 3/2 for the (0,0) mode.  Thomas elimination without
+// pivoting (rows are diagonally dominant; agrees with partial pivoting to
+// round-off).  One lane per mode, lanes along kz; CP holds the forward
+// multipliers for the back substitution.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_tri64(DevGrid g, cpx<double>* __restrict__ S,
+                                                  double* __restrict__ CP) {
+  const int nzp = g.nzp, nym = g.nym;
+  const size_t stride = (size_t)g.nxh * nzp;
+  const size_t m = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= stride) return;
+  const int kx = (int)(m / nzp), kz = (int)(m % nzp);
+  const double lamx = g.kxx[kx], lamz = g.kzz[kz];
+  cpx<double>* s = S + m;
+  double* cp = CP + m;
+
+  double d0 = (g.dyy_d[0] + lamx) + lamz;
+  if (kx == 0 && kz == 0) d0 = 1.5 * d0;
+  cpx<double> bprev = s[0];
+  double den = d0;
+  double cprev = (nym > 1) ? g.dyy_u[0] / den : 0.0;
+  bprev = cmk(bprev.x / den, bprev.y / den);
+  s[0] = bprev;
+  cp[0] = cprev;
+  for (int jj = 1; jj < nym; jj++) {
+    const double l = g.dyy_l[jj - 1];
+    den = ((g.dyy_d[jj] + lamx) + lamz) - l * cprev;
+    cpx<double> b = s[(size_t)jj * stride];
+    b = cmk((b.x - l * bprev.x) / den, (b.y - l * bprev.y) / den);
+    s[(size_t)jj * stride] = b;
+    if (jj < nym - 1) {
+      cprev = g.dyy_u[jj] / den;
+      cp[(size_t)jj * stride] = cprev;
+    }
+    bprev = b;
+  }
+  cpx<double> xnext = bprev;
+  for (int jj = nym - 2; jj >= 0; jj--) {
+    const double c = cp[(size_t)jj * stride];
+    cpx<double> b = s[(size_t)jj * stride];
+    b = cmk(b.x - c * xnext.x, b.y - c * xnext.y);
+    s[(size_t)jj * stride] = b;
+    xnext = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse x transform (C2R) of S[j][kx][k0..k0+TK) and the output stage of
+// solve_poisson_equation (src/poisson.f90:78-99):
+//   phi = Re(IDFT)/(nxp*nzp); periodic copies i = nxg-1 <- 2, k = nzp+2 <- 2.
+// MODE 0: out = phi (rhs_p), MODE 1: out = phi/dt (P_interim, with the zero
+// plane nzg that the divergence leaves), rk1: P = phi/(dt*8/15).
+// ---------------------------------------------------------------------------
+template <typename real, int N, int TK, int MODE>
+__global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __restrict__ S,
+                                                  const cpx<real>* __restrict__ tw,
+                                                  double* __restrict__ out, double* __restrict__ P,
+                                                  double dt, int rk1) {
+  constexpr int NP = TK / 2;
+  constexpr int NT = kBlock / NP;
+  constexpr int NH = N / 2 + 1;
+  constexpr int BUF = (NP * N > TK * NH) ? NP * N : TK * NH;
+  __shared__ cpx<real> buf[BUF];
+
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TK;
+  const int jj = blockIdx.y;
+  const int j = jj + 2;
+  const int nzp = g.nzp, nxg = g.nxg, nyg = g.nyg;
+
+  for (int e = tid; e < NH * TK; e += kBlock) {
+    const int kx = e / TK, t = e % TK;
+    buf[e] = S[((size_t)jj * NH + kx) * nzp + k0 + t];
+  }
+  __syncthreads();
+
+  // Hermitian extension and packing Z = A + iB (imaginary parts of the
+  // DC and Nyquist bins dropped: the real part of the full inverse).
+  constexpr int PER = (NP * N + kBlock - 1) / kBlock;
+  cpx<real> Z[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + q * kBlock;
+    if (e < NP * N) {
+      const int p = e / N, n = e % N;
+      cpx<real> a, b;
+      if (n <= N / 2) {
+        a = buf[n * TK + 2 * p];
+        b = buf[n * TK + 2 * p + 1];
+        if (n == 0 || n == N / 2) {
+          a.y = (real)0;
+          b.y = (real)0;
+        }
+      } else {
+        a = buf[(N - n) * TK + 2 * p];
+        b = buf[(N - n) * TK + 2 * p + 1];
+        a.y = -a.y;
+        b.y = -b.y;
+      }
+      Z[q] = cmk(a.x - b.y, a.y + b.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + q * kBlock;
+    if (e < NP * N) buf[e] = Z[q];
+  }
+  __syncthreads();
+
+  fft_lds<real, N, NT, +1>(buf + (tid / NT) * N, tw, tid % NT);
+
+  const double norm = (double)((long long)g.nxp * (long long)nzp);
+  const double cP = dt * (8.0 / 15.0);
+  const int idup = nxg - 1;     // x duplicate of i = 2
+  const int kdup = nzp + 2;     // z duplicate of k = 2
+  for (int e = tid; e < TK * N; e += kBlock) {
+    const int t = e / N, n = e % N;
+    const cpx<real> z = buf[(t >> 1) * N + n];
+    const double phi = (double)((t & 1) ? z.y : z.x) / norm;
+    const double o = (MODE == 1) ? phi / dt : phi;
+    const int i = n + 2, k = k0 + t + 2;
+    out[ridx(i, j, k, nxg, nyg)] = o;
+    if (rk1) P[fidx(i, j, k, nxg, nyg)] = phi / cP;
+    if (n == 0) {
+      out[ridx(idup, j, k, nxg, nyg)] = o;
+      if (rk1) P[fidx(idup, j, k, nxg, nyg)] = phi / cP;
+    }
+    if (k0 == 0 && t == 0) {
+      out[ridx(i, j, kdup, nxg, nyg)] = o;
+      if (rk1) P[fidx(i, j, kdup, nxg, nyg)] = phi / cP;
+      if (n == 0) {
+        out[ridx(idup, j, kdup, nxg, nyg)] = o;
+        if (rk1) P[fidx(idup, j, kdup, nxg, nyg)] = phi / cP;
+      }
+    }
+  }
+  if (MODE == 1 && k0 == 0) {
+    // rhs_p plane nzg is zero after the divergence; P_interim = 0/dt there.
+    for (int i = 2 + tid; i <= nxg - 1; i += kBlock) out[ridx(i, j, g.nzg, nxg, nyg)] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P ghost copies (src/poisson.f90:93-98): x, then y, then z.
+// ---------------------------------------------------------------------------
+__global__ void k_pghost_x(DevGrid g, double* __restrict__ P) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
+  if (j > g.nyg) return;
+  P[fidx(1, j, k, g.nxg, g.nyg)] = P[fidx(2, j, k, g.nxg, g.nyg)];
+  P[fidx(g.nxg, j, k, g.nxg, g.nyg)] = P[fidx(g.nxg - 1, j, k, g.nxg, g.nyg)];
+}
+__global__ void k_pghost_y(DevGrid g, double* __restrict__ P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
+  if (i > g.nxg) return;
+  P[fidx(i, 1, k, g.nxg, g.nyg)] = P[fidx(i, 2, k, g.nxg, g.nyg)];
+  P[fidx(i, g.nyg, k, g.nxg, g.nyg)] = P[fidx(i, g.nyg - 1, k, g.nxg, g.nyg)];
+}
+__global__ void k_pghost_z(DevGrid g, double* __restrict__ P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  if (i > g.nxg) return;
+  P[fidx(i, j, 1, g.nxg, g.nyg)] = P[fidx(i, j, 2, g.nxg, g.nyg)];
+  P[fidx(i, j, g.nzg, g.nxg, g.nyg)] = P[fidx(i, j, g.nzg - 1, g.nxg, g.nyg)];
+}
+
+// ---------------------------------------------------------------------------
+// Gradient correction: U = U - G(dt*Pint) on the index ranges of
+// src/differential_operators.f90:28-50 (faces outside them keep U).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __restrict__ U,
+                                                         double* __restrict__ V,
+                                                         double* __restrict__ W,
+                                                         const double* __restrict__ Pint,
+                                                         double dt) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  if (i > nx - 1 || j > nyg - 1) return;
+  const double p = dt * Pint[ridx(i, j, k, nxg, nyg)];
+  const double px = dt * Pint[ridx(i + 1, j, k, nxg, nyg)];
+  const size_t iu = fidx(i, j, k, nx, nyg);
+  U[iu] = U[iu] - (px - p) / g.dxg[i];
+  if (j <= ny - 1) {
+    const double py = dt * Pint[ridx(i, j + 1, k, nxg, nyg)];
+    const size_t iv = fidx(i, j, k, nxg, ny);
+    V[iv] = V[iv] - (py - p) / g.dyg[j];
+  }
+  if (k <= nz - 1) {
+    const double pz = dt * Pint[ridx(i, j, k + 1, nxg, nyg)];
+    const size_t iw = fidx(i, j, k, nxg, nyg);
+    W[iw] = W[iw] - (pz - p) / g.dzg[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions, src/boundary_conditions.f90:20-252, one domain.
+// ---------------------------------------------------------------------------
+__global__ void k_bc_x(DevGrid g, double* __restrict__ U, double* __restrict__ V,
+                       double* __restrict__ W) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  if (j > nyg) return;
+  {  // U: x faces
+    const double a = U[fidx(nx - 1, j, k, nx, nyg)], b = U[fidx(2, j, k, nx, nyg)];
+    U[fidx(1, j, k, nx, nyg)] = a;
+    U[fidx(nx, j, k, nx, nyg)] = b;
+  }
+  if (j <= ny) {  // V: x centres
+    const double a = V[fidx(nxg - 2, j, k, nxg, ny)], b = V[fidx(2, j, k, nxg, ny)],
+                 c = V[fidx(3, j, k, nxg, ny)];
+    V[fidx(1, j, k, nxg, ny)] = a;
+    V[fidx(nxg - 1, j, k, nxg, ny)] = b;
+    V[fidx(nxg, j, k, nxg, ny)] = c;
+  }
+  if (k <= nz) {  // W: x centres
+    const double a = W[fidx(nxg - 2, j, k, nxg, nyg)], b = W[fidx(2, j, k, nxg, nyg)],
+                 c = W[fidx(3, j, k, nxg, nyg)];
+    W[fidx(1, j, k, nxg, nyg)] = a;
+    W[fidx(nxg - 1, j, k, nxg, nyg)] = b;
+    W[fidx(nxg, j, k, nxg, nyg)] = c;
+  }
+}
+__global__ void k_bc_z(DevGrid g, double* __restrict__ U, double* __restrict__ V,
+                       double* __restrict__ W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
+  if (i > nxg) return;
+  if (i <= nx) {  // U: z centres
+    const double a = U[fidx(i, j, nzg - 2, nx, nyg)], b = U[fidx(i, j, 2, nx, nyg)],
+                 c = U[fidx(i, j, 3, nx, nyg)];
+    U[fidx(i, j, 1, nx, nyg)] = a;
+    U[fidx(i, j, nzg - 1, nx, nyg)] = b;
+    U[fidx(i, j, nzg, nx, nyg)] = c;
+  }
+  if (j <= ny) {  // V: z centres
+    const double a = V[fidx(i, j, nzg - 2, nxg, ny)], b = V[fidx(i, j, 2, nxg, ny)],
+                 c = V[fidx(i, j, 3, nxg, ny)];
+    V[fidx(i, j, 1, nxg, ny)] = a;
+    V[fidx(i, j, nzg - 1, nxg, ny)] = b;
+    V[fidx(i, j, nzg, nxg, ny)] = c;
+  }
+  {  // W: z faces
+    const double a = W[fidx(i, j, nz - 1, nxg, nyg)], b = W[fidx(i, j, 2, nxg, nyg)];
+    W[fidx(i, j, 1, nxg, nyg)] = a;
+    W[fidx(i, j, nz, nxg, nyg)] = b;
+  }
+}
+__global__ void k_bc_y(DevGrid g, double* __restrict__ U, double* __restrict__ V,
+                       double* __restrict__ W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  if (i > nxg) return;
+  if (i <= nx) {  // U: y centres
+    U[fidx(i, 1, k, nx, nyg)] = -U[fidx(i, 2, k, nx, nyg)];
+    U[fidx(i, nyg, k, nx, nyg)] = -U[fidx(i, nyg - 1, k, nx, nyg)];
+  }
+  V[fidx(i, 1, k, nxg, ny)] = 0.0;  // V: y faces
+  V[fidx(i, ny, k, nxg, ny)] = 0.0;
+  if (k <= nz) {  // W: y centres
+    W[fidx(i, 1, k, nxg, nyg)] = -W[fidx(i, 2, k, nxg, nyg)];
+    W[fidx(i, nyg, k, nxg, nyg)] = -W[fidx(i, nyg - 1, k, nxg, nyg)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Divergence into rhs_p shape and max|div| reduction (src/monitor.f90:210-221).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double div_at(const DevGrid& g, const double* __restrict__ U,
+                                         const double* __restrict__ V,
+                                         const double* __restrict__ W, int i, int j, int k) {
+  return (U[fidx(i, j, k, g.nx, g.nyg)] - U[fidx(i - 1, j, k, g.nx, g.nyg)]) / g.dxf[i] +
+         (V[fidx(i, j, k, g.nxg, g.ny)] - V[fidx(i, j - 1, k, g.nxg, g.ny)]) / g.dyf[j] +
+         (W[fidx(i, j, k, g.nxg, g.nyg)] - W[fidx(i, j, k - 1, g.nxg, g.nyg)]) / g.dzf[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_divergence(DevGrid g, const double* __restrict__ U,
+                                                       const double* __restrict__ V,
+                                                       const double* __restrict__ W,
+                                                       double* __restrict__ div) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;  // 2..nzg
+  if (i > g.nxg - 1 || j > g.nyg - 1) return;
+  div[ridx(i, j, k, g.nxg, g.nyg)] = (k <= g.nzg - 1) ? div_at(g, U, V, W, i, j, k) : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_max_div(DevGrid g, const double* __restrict__ U,
+                                                    const double* __restrict__ V,
+                                                    const double* __restrict__ W,
+                                                    unsigned long long* __restrict__ red) {
+  __shared__ double sm[kBlock / 64];
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  double v = 0.0;
+  if (i <= g.nxg - 1 && j <= g.nyg - 1) v = fabs(div_at(g, U, V, W, i, j, k));
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sm[0];
+    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, sm[w]);
+    atomicMax(red, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gradient(DevGrid g, const double* __restrict__ p,
+                                                     double* __restrict__ Gx,
+                                                     double* __restrict__ Gy,
+                                                     double* __restrict__ Gz) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  if (i > nx - 1 || j > nyg - 1) return;
+  const double pc = p[ridx(i, j, k, nxg, nyg)];
+  Gx[fidx(i, j, k, nx, nyg)] = (p[ridx(i + 1, j, k, nxg, nyg)] - pc) / g.dxg[i];
+  if (j <= ny - 1) Gy[fidx(i, j, k, nxg, ny)] = (p[ridx(i, j + 1, k, nxg, nyg)] - pc) / g.dyg[j];
+  if (k <= nz - 1) Gz[fidx(i, j, k, nxg, nyg)] = (p[ridx(i, j, k + 1, nxg, nyg)] - pc) / g.dzg[k];
+}
+
+// Dispatch a compile-time FFT length.
+#define C3D_DISPATCH_N(n, BODY)                     \
+  switch (n) {                                      \
+    case 16: { constexpr int NN = 16; BODY; } break;   \
+    case 32: { constexpr int NN = 32; BODY; } break;   \
+    case 48: { constexpr int NN = 48; BODY; } break;   \
+    case 64: { constexpr int NN = 64; BODY; } break;   \
+    case 96: { constexpr int NN = 96; BODY; } break;   \
+    case 128: { constexpr int NN = 128; BODY; } break; \
+    case 192: { constexpr int NN = 192; BODY; } break; \
+    case 256: { constexpr int NN = 256; BODY; } break; \
+    case 384: { constexpr int NN = 384; BODY; } break; \
+    case 512: { constexpr int NN = 512; BODY; } break; \
+    default: return hipErrorInvalidValue;           \
+  }
+
+#define C3D_DISPATCH_TK(tk, BODY)                  \
+  switch (tk) {                                    \
+    case 16: { constexpr int TK = 16; BODY; } break; \
+    case 8: { constexpr int TK = 8; BODY; } break;   \
+    case 4: { constexpr int TK = 4; BODY; } break;   \
+    case 2: { constexpr int TK = 2; BODY; } break;   \
+    default: return hipErrorInvalidValue;          \
+  }
+
+inline dim3 grid_pts(const DevGrid& g, int ni, int nj, int nk) {
+  return dim3((ni + 63) / 64, (nj + 3) / 4, nk);
+}
+
+}  // namespace
+
+hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const double* V,
+                        const double* W, const double* rhs, cpx<double>* S, hipStream_t s) {
+  const int tk = tile_rows(g.nzp);
+  dim3 grid(g.nzp / tk, g.nym);
+  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
+    if (from_uvw)
+      hipLaunchKernelGGL((k_fwd_x<double, NN, TK, true>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         rhs, g.twx64, S);
+    else
+      hipLaunchKernelGGL((k_fwd_x<double, NN, TK, false>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         rhs, g.twx64, S);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_z(const DevGrid& g, int sign, cpx<double>* S, hipStream_t s) {
+  const int ncol = g.nym * g.nxh;
+  dim3 grid((ncol + 3) / 4);
+  C3D_DISPATCH_N(g.nzp, {
+    if (sign < 0)
+      hipLaunchKernelGGL((k_fft_z<double, NN, -1>), grid, dim3(kBlock), 0, s, S, g.twz64, ncol);
+    else
+      hipLaunchKernelGGL((k_fft_z<double, NN, +1>), grid, dim3(kBlock), 0, s, S, g.twz64, ncol);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s) {
+  const size_t nm = (size_t)g.nxh * g.nzp;
+  hipLaunchKernelGGL(k_tri64, dim3((unsigned)((nm + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, g,
+                     S, CP);
+  return hipGetLastError();
+}
+
+hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
+                        double dt, bool rk1, hipStream_t s) {
+  const int tk = tile_rows(g.nzp);
+  dim3 grid(g.nzp / tk, g.nym);
+  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_inv_x<double, NN, TK, 0>), grid, dim3(kBlock), 0, s, g, S, g.twx64,
+                         out, P, dt, rk1 ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_inv_x<double, NN, TK, 1>), grid, dim3(kBlock), 0, s, g, S, g.twx64,
+                         out, P, dt, rk1 ? 1 : 0);
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s) {
+  hipLaunchKernelGGL(k_pghost_x, dim3((g.nyg + 63) / 64, g.nzg), dim3(64), 0, s, g, P);
+  hipLaunchKernelGGL(k_pghost_y, dim3((g.nxg + 63) / 64, g.nzg), dim3(64), 0, s, g, P);
+  hipLaunchKernelGGL(k_pghost_z, dim3((g.nxg + 63) / 64, g.nyg), dim3(64), 0, s, g, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
+                               double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_correct, grid_pts(g, g.nx - 2, g.nyg - 2, g.nzg - 2), dim3(kBlock), 0,
+                     s, g, U, V, W, Pint, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s) {
+  hipLaunchKernelGGL(k_bc_x, dim3((g.nyg + 63) / 64, g.nzg), dim3(64), 0, s, g, U, V, W);
+  hipLaunchKernelGGL(k_bc_z, dim3((g.nxg + 63) / 64, g.nyg), dim3(64), 0, s, g, U, V, W);
+  hipLaunchKernelGGL(k_bc_y, dim3((g.nxg + 63) / 64, g.nzg), dim3(64), 0, s, g, U, V, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_divergence(const DevGrid& g, const double* U, const double* V, const double* W,
+                             double* div, hipStream_t s) {
+  hipLaunchKernelGGL(k_divergence, grid_pts(g, g.nxg - 2, g.nyg - 2, g.nzg - 1), dim3(kBlock), 0,
+                     s, g, U, V, W, div);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient(const DevGrid& g, const double* p, double* Gx, double* Gy, double* Gz,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_gradient, grid_pts(g, g.nx - 2, g.nyg - 2, g.nzg - 2), dim3(kBlock), 0, s,
+                     g, p, Gx, Gy, Gz);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_div(const DevGrid& g, const double* U, const double* V, const double* W,
+                          unsigned long long* red, hipStream_t s) {
+  hipLaunchKernelGGL(k_max_div, grid_pts(g, g.nxg - 2, g.nyg - 2, g.nzg - 2), dim3(kBlock), 0, s,
+                     g, U, V, W, red);
+  return hipGetLastError();
+}
+
+}  // namespace c3d

@@ -1,0 +1,65 @@
+// Host-side launchers of the projection kernels (defined in kernels.hip).
+#pragma once
+
+#include "c3d_common.hpp"
+
+namespace c3d {
+
+// Grid geometry and solver tables resident on the device.
+struct DevGrid {
+  int nx, ny, nz;     // face counts
+  int nxg, nyg, nzg;  // centre counts incl. ghosts (= n + 1)
+  int nxp, nzp, nym;  // FFT lengths and tridiagonal length
+  int nxh;            // nxp/2 + 1 (R2C spectrum length)
+  // Fortran-indexed spacing tables (entry [i] for 1-based index i):
+  const double* dxf;  // x(i) - x(i-1), i = 2..nx
+  const double* dyf;  // y(j) - y(j-1), j = 2..ny
+  const double* dzf;  // z(k) - z(k-1), k = 2..nz
+  const double* dxg;  // xg(i+1) - xg(i), i = 1..nxg-1
+  const double* dyg;  // yg(j+1) - yg(j)
+  const double* dzg;  // zg(k+1) - zg(k)
+  const double* kxx;  // [nxp] modified wavenumbers
+  const double* kzz;  // [nzp]
+  const double* dyy_d;  // [nym]   Dyy(j,j)
+  const double* dyy_l;  // [nym-1] Dyy(j+1,j)
+  const double* dyy_u;  // [nym-1] Dyy(j,j+1)
+  const cpx<double>* twx64;
+  const cpx<double>* twz64;
+  const cpx<float>* twx32;
+  const cpx<float>* twz32;
+};
+
+// rhs_p offset of Fortran (i,j,k) with bounds (2:nxg-1, 2:nyg-1, 2:nzg).
+__host__ __device__ __forceinline__ size_t ridx(int i, int j, int k, int nxg, int nyg) {
+  return (size_t)(i - 2) + (size_t)(nxg - 2) * ((size_t)(j - 2) + (size_t)(nyg - 2) * (size_t)(k - 2));
+}
+
+// Forward x transform.  from_uvw: input is div(U,V,W) computed on the fly,
+// else the rhs_p array.  Writes the half spectrum S[j][kx][kz].
+hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const double* V,
+                        const double* W, const double* rhs, cpx<double>* S, hipStream_t s);
+hipError_t launch_fft_z(const DevGrid& g, int sign, cpx<double>* S, hipStream_t s);
+// Tridiagonal solve along y for every (kx,kz); CP is nym*nxh*nzp scratch.
+hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s);
+// Inverse x transform.  mode 0: out = solution rhs_p (Poisson drop-in).
+// mode 1: out = P_interim = solution/dt (projection).  rk1: also write P.
+hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
+                        double dt, bool rk1, hipStream_t s);
+// Ghost copies of P (src/poisson.f90:93-98), in the reference order.
+hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s);
+// U -= G(dt * Pint) on the gradient's index ranges.
+hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
+                               double dt, hipStream_t s);
+// apply_boundary_conditions on device-resident staggered fields.
+hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s);
+// Divergence into an rhs_p-shaped array (plane nzg zero).
+hipError_t launch_divergence(const DevGrid& g, const double* U, const double* V, const double* W,
+                             double* div, hipStream_t s);
+// Gradient of an rhs_p-shaped array into zeroed staggered fields.
+hipError_t launch_gradient(const DevGrid& g, const double* p, double* Gx, double* Gy, double* Gz,
+                           hipStream_t s);
+// max |div(U,V,W)| over the rhs_p shape; result as double bits in *red.
+hipError_t launch_max_div(const DevGrid& g, const double* U, const double* V, const double* W,
+                          unsigned long long* red, hipStream_t s);
+
+}  // namespace c3d

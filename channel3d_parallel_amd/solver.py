@@ -1,0 +1,144 @@
+"""Python host mirror of the reference's projection-path module procedures.
+
+Each method calls one C-ABI entry point of libc3d.so and keeps the name and
+argument meaning of the reference routine it replaces:
+
+========================================  ======================================
+reference (file:line)                     here
+========================================  ======================================
+``solve_poisson_equation(rhs)``           :meth:`Channel.solve_poisson_equation`
+src/poisson.f90:33-101
+``compute_non_IB_projection``             :meth:`Channel.compute_non_IB_projection`
+src/projection.f90:22-46
+``apply_boundary_conditions(U,V,W)``      :meth:`Channel.apply_boundary_conditions`
+src/boundary_conditions.f90:20-55
+``divergence(div,U,V,W)``                 :meth:`Channel.divergence`
+src/differential_operators.f90:54-74
+``gradient(Gpx,Gpy,Gpz,p)``               :meth:`Channel.gradient`
+src/differential_operators.f90:16-52
+``check_divergence(max_div)``             :meth:`Channel.check_divergence`
+src/monitor.f90:210-221
+========================================  ======================================
+
+Arrays are numpy float64 in the reference's Fortran layout.  The reference
+stops the program on fatal conditions and ignores solver status; here every
+failure raises :class:`C3DError` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ptr
+
+
+class C3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_lib.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Channel:
+    """One GPU-resident channel state (the reference's module-global state)."""
+
+    def __init__(self, nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1,
+                 nu=1.0, precision=_lib.PREC_FP64):
+        self.L = _lib.load()
+        d = _lib.c3d_desc(nx, ny, nz, grid_type, Lxp, Lzp, Ly, alpha, nu, precision)
+        h = C.c_void_p()
+        rc = self.L.c3d_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise C3DError(rc, f"c3d_create({nx},{ny},{nz}) failed")
+        self.h = h
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nxg, self.nyg, self.nzg = nx + 1, ny + 1, nz + 1
+        self.nxp, self.nzp, self.nym = nx - 2, nz - 2, ny - 1
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.c3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise C3DError(rc, self.L.c3d_last_error(self.h).decode())
+
+    # shapes ----------------------------------------------------------------
+    shape_u = property(lambda s: (s.nx, s.nyg, s.nzg))
+    shape_v = property(lambda s: (s.nxg, s.ny, s.nzg))
+    shape_w = property(lambda s: (s.nxg, s.nyg, s.nz))
+    shape_p = property(lambda s: (s.nxg, s.nyg, s.nzg))
+    shape_rhs = property(lambda s: (s.nxg - 2, s.nyg - 2, s.nzg - 1))
+
+    def zeros(self, which):
+        return np.zeros(getattr(self, "shape_" + which), order="F")
+
+    def grid(self, what: int) -> np.ndarray:
+        out = np.zeros(max(self.nxg, self.nyg, self.nzg) + 16)
+        n = self.L.c3d_grid_get(self.h, what, ptr(out), out.size)
+        if n < 0:
+            self._chk(n)
+        return out[:n].copy()
+
+    # state -------------------------------------------------------------------
+    def set_fields(self, U=None, V=None, W=None, P=None):
+        for a, s in ((U, self.shape_u), (V, self.shape_v), (W, self.shape_w), (P, self.shape_p)):
+            if a is not None and a.shape != s:
+                raise ValueError(f"shape {a.shape} != {s}")
+        self._chk(self.L.c3d_set_fields(self.h, ptr(U), ptr(V), ptr(W), ptr(P)))
+
+    def get_fields(self):
+        U, V, W, P, Pint = (self.zeros("u"), self.zeros("v"), self.zeros("w"), self.zeros("p"),
+                            self.zeros("rhs"))
+        self._chk(self.L.c3d_get_fields(self.h, ptr(U), ptr(V), ptr(W), ptr(P), ptr(Pint)))
+        return dict(U=U, V=V, W=W, P=P, P_interim=Pint)
+
+    # reference procedures ----------------------------------------------------
+    def apply_boundary_conditions(self):
+        self._chk(self.L.c3d_apply_boundary_conditions(self.h))
+
+    def divergence(self):
+        d = self.zeros("rhs")
+        self._chk(self.L.c3d_divergence(self.h, ptr(d)))
+        return d
+
+    def gradient(self, p):
+        gx, gy, gz = self.zeros("u"), self.zeros("v"), self.zeros("w")
+        p = np.asfortranarray(p, dtype=np.float64)
+        self._chk(self.L.c3d_gradient(self.h, ptr(p), ptr(gx), ptr(gy), ptr(gz)))
+        return gx, gy, gz
+
+    def solve_poisson_equation(self, rhs, rk_step=2, dt=1.0):
+        """In place on a Fortran-ordered rhs_p-shaped array."""
+        if rhs.shape != self.shape_rhs or not rhs.flags["F_CONTIGUOUS"]:
+            raise ValueError("rhs must be Fortran-ordered with shape %s" % (self.shape_rhs,))
+        self._chk(self.L.c3d_solve_poisson_equation(self.h, ptr(rhs), rk_step, dt))
+        return rhs
+
+    def compute_non_IB_projection(self, dt, rk_step=2):
+        self._chk(self.L.c3d_compute_non_ib_projection(self.h, dt, rk_step))
+
+    def check_divergence(self) -> float:
+        v = C.c_double()
+        self._chk(self.L.c3d_check_divergence(self.h, C.byref(v)))
+        return v.value
+
+    # measurement ---------------------------------------------------------------
+    def time_projection(self, iters, dt=1.0):
+        ns = self.L.c3d_stage_count()
+        tot = C.c_double()
+        ks = np.zeros(ns)
+        self._chk(self.L.c3d_time_projection(self.h, iters, dt, C.byref(tot), ptr(ks), ns))
+        names = [self.L.c3d_stage_name(i).decode() for i in range(ns)]
+        return tot.value, dict(zip(names, ks))

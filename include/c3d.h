@@ -1,0 +1,117 @@
+/*
+ * c3d.h -- C ABI of the MI355X (gfx950) channel3D projection library.
+ *
+ * Drop-in boundary for the fractional-step projection path of the
+ * reference's Fortran modules (poisson, projection, differential_operators,
+ * boundary_conditions, monitor).  The reference exposes module procedures
+ * that act on module-global state; here one opaque context (c3d_ctx) holds
+ * that state on one GPU, and a Fortran host binds these entry points with
+ * iso_c_binding (see the bind(C) interfaces in
+ * channel3d_parallel_amd/fortran/c3d_bindings.f90).
+ *
+ * Conventions
+ *  - Plain pointers and sizes, no framework types.  Host arrays use the
+ *    reference's Fortran layout: column-major (x fastest), ghost-inclusive,
+ *    FP64:  U(nx,nyg,nzg)  V(nxg,ny,nzg)  W(nxg,nyg,nz)  P(nxg,nyg,nzg)
+ *    rhs_p / P_interim with bounds (2:nxg-1, 2:nyg-1, 2:nzg)
+ *    where nxg = nx+1, nyg = ny+1, nzg = nz+1 and nx,ny,nz are the face
+ *    counts (nx_global, ny_global, nz_global of the reference namelist).
+ *  - Every call returns 0 on success or a negative C3D_E* code; the message
+ *    is available from c3d_last_error().  Calls are synchronous on return
+ *    (one HIP stream per context).  One host thread per context.
+ *  - Fields stay resident in device memory between calls.
+ */
+#ifndef C3D_H
+#define C3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define C3D_OK 0
+#define C3D_EINVAL (-1)   /* invalid argument or size */
+#define C3D_EHIP (-2)     /* HIP runtime error */
+#define C3D_ENOMEM (-3)   /* device allocation failed */
+#define C3D_ENOCONV (-4)  /* iterative solve hit its iteration limit */
+#define C3D_ENAN (-5)     /* non-finite value detected */
+
+#define C3D_PREC_FP64 0  /* plain FP64 path */
+#define C3D_PREC_MIXED 1 /* FP32 spectral solve + FP64 iterative refinement */
+
+typedef struct c3d_ctx c3d_ctx;
+
+/* Grid and physics description.  Mirrors the &params namelist entries of
+ * src/input_output.f90:27-40 that the projection path reads. */
+typedef struct c3d_desc {
+  int nx, ny, nz;   /* nx_global, ny_global, nz_global (face counts, even nx,nz) */
+  int grid_type;    /* 0 uniform y, 1 tanh-stretched y (src/input_output.f90:156-178) */
+  double Lxp;       /* periodic length in x (default 2*pi) */
+  double Lzp;       /* periodic length in z (default pi) */
+  double Ly;        /* Ly_channel (default 2) */
+  double alpha;     /* alpha_stretch */
+  double nu;        /* kinematic viscosity */
+  int precision;    /* C3D_PREC_FP64 or C3D_PREC_MIXED */
+} c3d_desc;
+
+/* Creates a context on the current HIP device: builds the grid, spacing
+ * tables, modified wavenumbers and Dyy (src/initialization.f90:166-390),
+ * allocates the resident fields.  Replaces `initialize`
+ * (src/initialization.f90:24-460) for the projection state. */
+int c3d_create(const c3d_desc *desc, c3d_ctx **out);
+int c3d_destroy(c3d_ctx *ctx);
+const char *c3d_last_error(const c3d_ctx *ctx);
+
+/* Copies a grid vector to host memory (n = capacity in doubles).
+ * what: 0 x 1 y 2 z 3 xg 4 yg 5 zg 6 kxx 7 kzz 8 Dyy diag 9 Dyy lower
+ *       10 Dyy upper 11 scalars{dx,dz,dxmin,dymin,dzmin,Lx,Ly,Lz,Lxp,Lzp}.
+ * Returns the number of values written (>= 0) or an error code. */
+int c3d_grid_get(const c3d_ctx *ctx, int what, double *out, int n);
+
+/* Host <-> device copies of the resident state (reference shapes above).
+ * NULL pointers are skipped.  P_interim uses the rhs_p shape. */
+int c3d_set_fields(c3d_ctx *ctx, const double *U, const double *V, const double *W,
+                   const double *P);
+int c3d_get_fields(c3d_ctx *ctx, double *U, double *V, double *W, double *P, double *P_interim);
+
+/* apply_boundary_conditions(U,V,W) on the resident velocity
+ * (src/boundary_conditions.f90:20-55, single domain). */
+int c3d_apply_boundary_conditions(c3d_ctx *ctx);
+
+/* divergence(div, U, V, W) of the resident velocity into a host array of
+ * the rhs_p shape (src/differential_operators.f90:54-74). */
+int c3d_divergence(c3d_ctx *ctx, double *div);
+
+/* gradient(Gpx, Gpy, Gpz, p) of a host rhs_p-shaped array into host
+ * staggered arrays (src/differential_operators.f90:16-52). */
+int c3d_gradient(c3d_ctx *ctx, const double *p, double *Gpx, double *Gpy, double *Gpz);
+
+/* solve_poisson_equation(rhs) on a host rhs_p-shaped array, in place
+ * (src/poisson.f90:33-101).  If rk_step == 1 the resident P receives
+ * rhs/(dt*8/15) with neighbour-copied ghosts, as at src/poisson.f90:91-99. */
+int c3d_solve_poisson_equation(c3d_ctx *ctx, double *rhs, int rk_step, double dt);
+
+/* compute_non_IB_projection on the resident fields
+ * (src/projection.f90:22-46): U <- U - G(phi), P_interim <- phi/dt, and
+ * the P side effect at rk_step == 1.  Boundary faces are left as the
+ * reference leaves them; callers apply c3d_apply_boundary_conditions after,
+ * as src/time_integration.f90:117-123 does. */
+int c3d_compute_non_ib_projection(c3d_ctx *ctx, double dt, int rk_step);
+
+/* check_divergence: max|div(U,V,W)| over the rhs_p shape
+ * (src/monitor.f90:210-221). */
+int c3d_check_divergence(c3d_ctx *ctx, double *max_div);
+
+/* Timing of `iters` back-to-back resident projections with HIP events on
+ * the context stream.  total_ms: average ms per projection;
+ * kernel_ms[0..nk): average ms per kernel stage (order from
+ * c3d_stage_name), nk <= c3d_stage_count().  No reference counterpart:
+ * measurement hook for the hot path. */
+int c3d_time_projection(c3d_ctx *ctx, int iters, double dt, double *total_ms, double *kernel_ms,
+                        int nk);
+int c3d_stage_count(void);
+const char *c3d_stage_name(int i);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* C3D_H */

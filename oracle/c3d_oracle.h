@@ -1,0 +1,82 @@
+/*
+ * c3d_oracle.h -- CPU FP64 restatement of the channel3D projection path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * never by the product library (channel3d_parallel_amd/), which must fail
+ * loudly when its HIP build is missing instead of falling back to this code.
+ *
+ * Arrays follow the reference's Fortran layout: column-major (x fastest),
+ * 1-based Fortran index (i,j,k) stored at 0-based memory offset
+ * (i-lo1) + n1*((j-lo2) + n2*(k-lo3)), ghost-inclusive shapes:
+ *   U(nx ,nyg,nzg)  V(nxg,ny ,nzg)  W(nxg,nyg,nz )  P(nxg,nyg,nzg)
+ *   rhs_p / P_interim (2:nxg-1, 2:nyg-1, 2:nzg)   -- src/initialization.f90:116-155
+ * with nxg = nx+1, nyg = ny+1, nzg = nz+1 on a single domain.
+ * Complex data is interleaved (re, im) doubles.
+ */
+#ifndef C3D_ORACLE_H
+#define C3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid orc_grid;
+
+/* Grid and solver constants (src/input_output.f90:140-206,
+ * src/initialization.f90:166-305,346-390). grid_type 0 = uniform y,
+ * 1 = tanh-stretched y. Returns NULL on invalid sizes. */
+orc_grid *orc_grid_new(int nx, int ny, int nz, double Lxp, double Lzp,
+                       double Ly, double alpha, int grid_type);
+void orc_grid_free(orc_grid *g);
+/* what: 0 x[nx] 1 y[ny] 2 z[nz] 3 xg[nxg] 4 yg[nyg] 5 zg[nzg]
+ *       6 kxx[nxp] 7 kzz[nzp] 8 Dyy diag[nym] 9 Dyy lower[nym-1]
+ *       10 Dyy upper[nym-1]
+ *       11 scalars[10] = dx dz dxmin dymin dzmin Lx Ly Lz Lxp Lzp
+ * Returns the number of doubles written, or -1. */
+int orc_grid_get(const orc_grid *g, int what, double *out);
+
+/* Boundary conditions (src/boundary_conditions.f90:20-252). */
+void orc_apply_bc(const orc_grid *g, double *U, double *V, double *W);
+/* Staggered divergence faces -> centres (src/differential_operators.f90:54-74). */
+void orc_divergence(const orc_grid *g, double *div, const double *U,
+                    const double *V, const double *W);
+/* Staggered gradient centres -> faces (src/differential_operators.f90:16-52). */
+void orc_gradient(const orc_grid *g, double *Gx, double *Gy, double *Gz,
+                  const double *p);
+
+/* 1-D complex DFT, unnormalised, sign -1 forward / +1 backward
+ * (FFTW_FORWARD / FFTW_BACKWARD convention, src/poisson.f90:45,80). */
+int orc_fft(int n, int sign, double *data);
+
+/* LAPACK zgtsv restated: complex tridiagonal solve with partial pivoting,
+ * one right-hand side (src/poisson.f90:72).  dl[n-1], d[n], du[n-1], b[n]
+ * complex interleaved, all overwritten.  Returns INFO. */
+int orc_zgtsv(int n, double *dl, double *d, double *du, double *b);
+
+/* solve_poisson_equation (src/poisson.f90:33-101).  rhs has the rhs_p shape
+ * and is overwritten with the solution.  If rk_step == 1 and P != NULL the
+ * pressure side effect of src/poisson.f90:91-99 is applied.
+ * tri_mode 0 = zgtsv restatement, 1 = Thomas without pivoting (diagnostic).
+ * nthreads <= 0 uses the OpenMP default. */
+void orc_poisson(const orc_grid *g, double *rhs, int rk_step, double dt,
+                 double *P, int tri_mode, int nthreads);
+
+/* compute_non_IB_projection (src/projection.f90:22-46) on caller arrays.
+ * U,V,W in/out; Uint,Vint,Wint receive the interim copies (may be NULL);
+ * Pint receives P_interim (rhs_p shape); rhs is rhs_p scratch/out;
+ * P (full shape) receives the rk_step==1 side effect (may be NULL). */
+void orc_projection(const orc_grid *g, double *U, double *V, double *W,
+                    double *Uint, double *Vint, double *Wint, double *Pint,
+                    double *rhs, double *P, int rk_step, double dt,
+                    int tri_mode, int nthreads);
+
+/* check_divergence (src/monitor.f90:210-221): max|div| over the rhs_p shape
+ * with the extra z plane zero. */
+double orc_max_abs_div(const orc_grid *g, const double *U, const double *V,
+                       const double *W);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,170 @@
+"""ctypes wrapper of the C oracle (test infrastructure, see package docstring).
+
+Arrays are numpy float64 arrays in Fortran order with the reference's shapes:
+``U(nx,nyg,nzg)``, ``V(nxg,ny,nzg)``, ``W(nxg,nyg,nz)``, ``P(nxg,nyg,nzg)``
+and ``rhs_p(nxg-2, nyg-2, nzg-1)`` (reference bounds ``2:nxg-1, 2:nyg-1,
+2:nzg``), see src/initialization.f90:116-155.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "liborc.so")
+_dp = C.POINTER(C.c_double)
+
+
+def build(force: bool = False) -> str:
+    """Compile liborc.so with the committed Makefile."""
+    if force or not os.path.exists(_SO):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _SO
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        L.orc_grid_new.restype = C.c_void_p
+        L.orc_grid_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_int]
+        L.orc_grid_free.argtypes = [C.c_void_p]
+        L.orc_grid_get.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_grid_get.restype = C.c_int
+        L.orc_apply_bc.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_divergence.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_gradient.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_fft.argtypes = [C.c_int, C.c_int, _dp]
+        L.orc_fft.restype = C.c_int
+        L.orc_zgtsv.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_zgtsv.restype = C.c_int
+        L.orc_poisson.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_int]
+        L.orc_projection.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     C.c_int, C.c_double, C.c_int, C.c_int]
+        L.orc_max_abs_div.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_max_abs_div.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 or a.dtype == np.complex128
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1, "oracle arrays must be Fortran-contiguous"
+    return a.ctypes.data_as(_dp)
+
+
+def fft(a: np.ndarray, sign: int = -1) -> np.ndarray:
+    """Unnormalised 1-D complex DFT (sign -1 forward, +1 backward)."""
+    b = np.array(a, dtype=np.complex128, copy=True)
+    if lib().orc_fft(b.size, sign, _p(b)) != 0:
+        raise ValueError("unsupported FFT length %d" % b.size)
+    return b
+
+
+def zgtsv(dl, d, du, b):
+    """Restated LAPACK zgtsv on copies; returns (x, info)."""
+    dl = np.array(dl, dtype=np.complex128)
+    d = np.array(d, dtype=np.complex128)
+    du = np.array(du, dtype=np.complex128)
+    x = np.array(b, dtype=np.complex128)
+    info = lib().orc_zgtsv(d.size, _p(dl), _p(d), _p(du), _p(x))
+    return x, info
+
+
+class Grid:
+    """Single-domain grid (src/input_output.f90:140-206, src/initialization.f90:166-390).
+
+    ``nx, ny, nz`` are face counts as in the reference namelist
+    (``nx_global`` ...).  ``grid_type`` 0 uniform y, 1 tanh-stretched y.
+    """
+
+    def __init__(self, nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nxg, self.nyg, self.nzg = nx + 1, ny + 1, nz + 1
+        self.nxp, self.nzp, self.nym = nx - 2, nz - 2, ny - 1
+        self.h = lib().orc_grid_new(nx, ny, nz, Lxp, Lzp, Ly, alpha, grid_type)
+        if not self.h:
+            raise ValueError("invalid grid")
+        self.Lxp_in, self.Lzp_in, self.Ly_in, self.alpha, self.grid_type = Lxp, Lzp, Ly, alpha, grid_type
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_grid_free(self.h)
+            self.h = None
+
+    def get(self, what: int) -> np.ndarray:
+        sizes = {0: self.nx, 1: self.ny, 2: self.nz, 3: self.nxg, 4: self.nyg, 5: self.nzg,
+                 6: self.nxp, 7: self.nzp, 8: self.nym, 9: self.nym - 1, 10: self.nym - 1, 11: 10}
+        out = np.zeros(sizes[what])
+        lib().orc_grid_get(self.h, what, _p(out))
+        return out
+
+    @property
+    def scalars(self):
+        s = self.get(11)
+        return dict(zip(["dx", "dz", "dxmin", "dymin", "dzmin", "Lx", "Ly", "Lz", "Lxp", "Lzp"], s))
+
+    # shapes -------------------------------------------------------------
+    @property
+    def shape_u(self):
+        return (self.nx, self.nyg, self.nzg)
+
+    @property
+    def shape_v(self):
+        return (self.nxg, self.ny, self.nzg)
+
+    @property
+    def shape_w(self):
+        return (self.nxg, self.nyg, self.nz)
+
+    @property
+    def shape_p(self):
+        return (self.nxg, self.nyg, self.nzg)
+
+    @property
+    def shape_rhs(self):
+        return (self.nxg - 2, self.nyg - 2, self.nzg - 1)
+
+    def zeros(self, which):
+        return np.zeros(getattr(self, "shape_" + which), order="F")
+
+    # operators ----------------------------------------------------------
+    def apply_bc(self, U, V, W):
+        lib().orc_apply_bc(self.h, _p(U), _p(V), _p(W))
+
+    def divergence(self, U, V, W):
+        d = self.zeros("rhs")
+        lib().orc_divergence(self.h, _p(d), _p(U), _p(V), _p(W))
+        return d
+
+    def gradient(self, p):
+        gx, gy, gz = self.zeros("u"), self.zeros("v"), self.zeros("w")
+        lib().orc_gradient(self.h, _p(gx), _p(gy), _p(gz), _p(p))
+        return gx, gy, gz
+
+    def poisson(self, rhs, rk_step=2, dt=1.0, P=None, tri_mode=0, nthreads=0):
+        """In-place solve_poisson_equation (src/poisson.f90:33-101)."""
+        lib().orc_poisson(self.h, _p(rhs), rk_step, dt, _p(P), tri_mode, nthreads)
+        return rhs
+
+    def projection(self, U, V, W, rk_step=2, dt=1.0, P=None, tri_mode=0, nthreads=0):
+        """compute_non_IB_projection in place on U,V,W; returns dict of outputs."""
+        Ui, Vi, Wi = self.zeros("u"), self.zeros("v"), self.zeros("w")
+        Pint, rhs = self.zeros("rhs"), self.zeros("rhs")
+        lib().orc_projection(self.h, _p(U), _p(V), _p(W), _p(Ui), _p(Vi), _p(Wi), _p(Pint),
+                             _p(rhs), _p(P), rk_step, dt, tri_mode, nthreads)
+        return dict(U_interim=Ui, V_interim=Vi, W_interim=Wi, P_interim=Pint, rhs_p=rhs)
+
+    def max_abs_div(self, U, V, W) -> float:
+        return lib().orc_max_abs_div(self.h, _p(U), _p(V), _p(W))
