@@ -247,6 +247,14 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   g.nxg = c->nxg; g.nyg = c->nyg; g.nzg = c->nzg;
   g.nxp = c->nxp; g.nzp = c->nzp; g.nym = c->nym; g.nxh = c->nxh;
   int rc = 0;
+  auto recip = [](std::vector<double> v) {
+    for (double& e : v) e = (e != 0.0) ? 1.0 / e : 0.0;
+    return v;
+  };
+  if ((rc = upload(c, &g.rdxf, recip(face_spacing(c->x)))) ||
+      (rc = upload(c, &g.rdyf, recip(face_spacing(c->y)))) ||
+      (rc = upload(c, &g.rdzf, recip(face_spacing(c->z)))))
+    return cleanup(rc);
   if ((rc = upload(c, &g.dxf, face_spacing(c->x))) || (rc = upload(c, &g.dyf, face_spacing(c->y))) ||
       (rc = upload(c, &g.dzf, face_spacing(c->z))) ||
       (rc = upload(c, &g.dxg, centre_spacing(c->xg))) ||

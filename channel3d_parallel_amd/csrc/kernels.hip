@@ -48,21 +48,49 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   const int j = jj + 2;
   const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
 
-  // 1) rows -> LDS (x contiguous, coalesced)
-  for (int e = tid; e < TK * N; e += kBlock) {
-    const int n = e % N, t = e / N;
-    const int i = n + 2, k = k0 + t + 2;
-    double d;
-    if constexpr (FROM_UVW) {
-      // divergence, src/differential_operators.f90:64-72
-      d = (U[fidx(i, j, k, nx, nyg)] - U[fidx(i - 1, j, k, nx, nyg)]) / g.dxf[i] +
-          (V[fidx(i, j, k, nxg, ny)] - V[fidx(i, j - 1, k, nxg, ny)]) / g.dyf[j] +
-          (W[fidx(i, j, k, nxg, nyg)] - W[fidx(i, j, k - 1, nxg, nyg)]) / g.dzf[k];
-    } else {
-      d = rhs[ridx(i, j, k, nxg, nyg)];
+  // 1) rows -> LDS (x contiguous, coalesced).  All loads of a batch are
+  //    issued before any is used so that many are in flight per lane.
+  constexpr int ITEMS = (TK * N + kBlock - 1) / kBlock;
+  constexpr int BATCH = ITEMS < 4 ? ITEMS : 4;
+  for (int q0 = 0; q0 < ITEMS; q0 += BATCH) {
+    double v[BATCH][6];
+#pragma unroll
+    for (int q = 0; q < BATCH; q++) {
+      const int e = tid + (q0 + q) * kBlock;
+      if ((TK * N) % kBlock == 0 || e < TK * N) {
+        const int n = e % N, t = e / N;
+        const int i = n + 2, k = k0 + t + 2;
+        if constexpr (FROM_UVW) {
+          v[q][0] = U[fidx(i, j, k, nx, nyg)];
+          v[q][1] = U[fidx(i - 1, j, k, nx, nyg)];
+          v[q][2] = V[fidx(i, j, k, nxg, ny)];
+          v[q][3] = V[fidx(i, j - 1, k, nxg, ny)];
+          v[q][4] = W[fidx(i, j, k, nxg, nyg)];
+          v[q][5] = W[fidx(i, j, k - 1, nxg, nyg)];
+        } else {
+          v[q][0] = rhs[ridx(i, j, k, nxg, nyg)];
+        }
+      }
     }
-    real* bp = reinterpret_cast<real*>(&buf[(t >> 1) * N + n]);
-    bp[t & 1] = (real)d;
+#pragma unroll
+    for (int q = 0; q < BATCH; q++) {
+      const int e = tid + (q0 + q) * kBlock;
+      if ((TK * N) % kBlock == 0 || e < TK * N) {
+        const int n = e % N, t = e / N;
+        const int i = n + 2, k = k0 + t + 2;
+        double d;
+        if constexpr (FROM_UVW) {
+          // divergence, src/differential_operators.f90:64-72, with the
+          // face-spacing reciprocals tabulated
+          d = (v[q][0] - v[q][1]) * g.rdxf[i] + (v[q][2] - v[q][3]) * g.rdyf[j] +
+              (v[q][4] - v[q][5]) * g.rdzf[k];
+        } else {
+          d = v[q][0];
+        }
+        real* bp = reinterpret_cast<real*>(&buf[(t >> 1) * N + n]);
+        bp[t & 1] = (real)d;
+      }
+    }
   }
   __syncthreads();
 
@@ -115,9 +143,25 @@ __global__ __launch_bounds__(kBlock) void k_fft_z(cpx<real>* __restrict__ S,
   __shared__ cpx<real> buf[NC * N];
   const int tid = threadIdx.x;
   const size_t col0 = (size_t)blockIdx.x * NC;
-  for (int e = tid; e < NC * N; e += kBlock) {
-    const int c = e / N;
-    if (col0 + c < (size_t)ncol) buf[e] = S[col0 * N + e];
+  {
+    constexpr int ITEMS = (NC * N + kBlock - 1) / kBlock;
+    real vx[ITEMS], vy[ITEMS];
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int e = tid + q * kBlock;
+      vx[q] = (real)0;
+      vy[q] = (real)0;
+      if (e < NC * N && col0 + e / N < (size_t)ncol) {
+        const cpx<real> t = S[col0 * N + e];
+        vx[q] = t.x;
+        vy[q] = t.y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int e = tid + q * kBlock;
+      if (e < NC * N) buf[e] = cmk(vx[q], vy[q]);
+    }
   }
   __syncthreads();
   fft_lds<real, N, NT, SIGN>(buf + (tid / NT) * N, tw, tid % NT);
@@ -177,6 +221,83 @@ __global__ __launch_bounds__(kBlock) void k_tri64(DevGrid g, cpx<double>* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Same Thomas solve, chunked for a compile-time length NYM = NCH*C: rows are
+// loaded C at a time (C independent loads in flight per lane) and the
+// forward sweep is re-run chunk by chunk during the back substitution from
+// checkpoints (cp and the eliminated rhs entering each chunk, 3 doubles per
+// chunk in CK), so the memory traffic is the rhs read twice, the solution
+// written once and the small checkpoint array.
+// ---------------------------------------------------------------------------
+template <int NCH, int C>
+__global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>* __restrict__ S,
+                                                          double* __restrict__ CK) {
+  const int nzp = g.nzp;
+  const size_t stride = (size_t)g.nxh * nzp;
+  const size_t m = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (m >= stride) return;
+  const int kx = (int)(m / nzp), kz = (int)(m % nzp);
+  const double lamx = g.kxx[kx];
+  const double lamz = g.kzz[kz];
+  const bool zero_mode = (kx == 0 && kz == 0);
+  cpx<double>* s = S + m;
+  double* ck = CK + m;
+  constexpr int N = NCH * C;
+
+  // forward sweep: y_j = (b_j - l_{j-1} y_{j-1}) / den_j, cp_j = u_j / den_j
+  double cprev = 0.0;
+  cpx<double> yprev = cmk(0.0, 0.0);
+  for (int c = 0; c < NCH; c++) {
+    ck[(size_t)(3 * c) * stride] = cprev;
+    ck[(size_t)(3 * c + 1) * stride] = yprev.x;
+    ck[(size_t)(3 * c + 2) * stride] = yprev.y;
+    cpx<double> b[C];
+#pragma unroll
+    for (int r = 0; r < C; r++) b[r] = s[(size_t)(c * C + r) * stride];
+#pragma unroll
+    for (int r = 0; r < C; r++) {
+      const int jj = c * C + r;
+      double d = (g.dyy_d[jj] + lamx) + lamz;
+      if (jj == 0 && zero_mode) d = 1.5 * d;
+      const double l = (jj == 0) ? 0.0 : g.dyy_l[jj - 1];
+      const double inv = 1.0 / (d - l * cprev);
+      yprev = cmk((b[r].x - l * yprev.x) * inv, (b[r].y - l * yprev.y) * inv);
+      cprev = (jj < N - 1) ? g.dyy_u[jj] * inv : 0.0;
+    }
+  }
+  // back substitution x_j = y_j - cp_j x_{j+1}, chunk by chunk from the top,
+  // re-running the forward sweep inside each chunk from its checkpoint
+  cpx<double> xnext = cmk(0.0, 0.0);
+  for (int c = NCH - 1; c >= 0; c--) {
+    cpx<double> y[C];
+    double cpv[C];
+#pragma unroll
+    for (int r = 0; r < C; r++) y[r] = s[(size_t)(c * C + r) * stride];
+    double cp = ck[(size_t)(3 * c) * stride];
+    cpx<double> yp = cmk(ck[(size_t)(3 * c + 1) * stride], ck[(size_t)(3 * c + 2) * stride]);
+#pragma unroll
+    for (int r = 0; r < C; r++) {
+      const int jj = c * C + r;
+      double d = (g.dyy_d[jj] + lamx) + lamz;
+      if (jj == 0 && zero_mode) d = 1.5 * d;
+      const double l = (jj == 0) ? 0.0 : g.dyy_l[jj - 1];
+      const double inv = 1.0 / (d - l * cp);
+      yp = cmk((y[r].x - l * yp.x) * inv, (y[r].y - l * yp.y) * inv);
+      cp = (jj < N - 1) ? g.dyy_u[jj] * inv : 0.0;
+      cpv[r] = cp;
+      y[r] = yp;
+    }
+#pragma unroll
+    for (int r = C - 1; r >= 0; r--) {
+      const int jj = c * C + r;
+      cpx<double> x = y[r];
+      if (jj < N - 1) x = cmk(x.x - cpv[r] * xnext.x, x.y - cpv[r] * xnext.y);
+      xnext = x;
+      s[(size_t)jj * stride] = x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Inverse x transform (C2R) of S[j][kx][k0..k0+TK) and the output stage of
 // solve_poisson_equation (src/poisson.f90:78-99):
 //   phi = Re(IDFT)/(nxp*nzp); periodic copies i = nxg-1 <- 2, k = nzp+2 <- 2.
@@ -200,9 +321,26 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
   const int j = jj + 2;
   const int nzp = g.nzp, nxg = g.nxg, nyg = g.nyg;
 
-  for (int e = tid; e < NH * TK; e += kBlock) {
-    const int kx = e / TK, t = e % TK;
-    buf[e] = S[((size_t)jj * NH + kx) * nzp + k0 + t];
+  {
+    constexpr int ITEMS = (NH * TK + kBlock - 1) / kBlock;
+    real vx[ITEMS], vy[ITEMS];
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int e = tid + q * kBlock;
+      vx[q] = (real)0;
+      vy[q] = (real)0;
+      if (e < NH * TK) {
+        const int kx = e / TK, t = e % TK;
+        const cpx<real> v = S[((size_t)jj * NH + kx) * nzp + k0 + t];
+        vx[q] = v.x;
+        vy[q] = v.y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int e = tid + q * kBlock;
+      if (e < NH * TK) buf[e] = cmk(vx[q], vy[q]);
+    }
   }
   __syncthreads();
 
@@ -270,6 +408,11 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
   if (MODE == 1 && k0 == 0) {
     // rhs_p plane nzg is zero after the divergence; P_interim = 0/dt there.
     for (int i = 2 + tid; i <= nxg - 1; i += kBlock) out[ridx(i, j, g.nzg, nxg, nyg)] = 0.0;
+  }
+  if (MODE == 0 && k0 == 0 && tid == 0) {
+    // plane nzg keeps its input, but the x-periodic copy (src/poisson.f90:110)
+    // also covers it: rhs(nxg-1, j, nzg) = rhs(2, j, nzg).
+    out[ridx(idup, j, g.nzg, nxg, nyg)] = out[ridx(2, j, g.nzg, nxg, nyg)];
   }
 }
 
@@ -424,9 +567,10 @@ __global__ __launch_bounds__(kBlock) void k_max_div(DevGrid g, const double* __r
   __shared__ double sm[kBlock / 64];
   const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
   double v = 0.0;
-  if (i <= g.nxg - 1 && j <= g.nyg - 1) v = fabs(div_at(g, U, V, W, i, j, k));
+  if (i <= g.nxg - 1 && j <= g.nyg - 1)
+    for (int k = blockIdx.z + 2; k <= g.nzg - 1; k += gridDim.z)
+      v = fmax(v, fabs(div_at(g, U, V, W, i, j, k)));
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -512,8 +656,13 @@ hipError_t launch_fft_z(const DevGrid& g, int sign, cpx<double>* S, hipStream_t 
 
 hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
-  hipLaunchKernelGGL(k_tri64, dim3((unsigned)((nm + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, g,
-                     S, CP);
+  const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
+  switch (g.nym) {
+    case 64: hipLaunchKernelGGL((k_tri64_chunked<4, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 128: hipLaunchKernelGGL((k_tri64_chunked<8, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 192: hipLaunchKernelGGL((k_tri64_chunked<12, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    default: hipLaunchKernelGGL(k_tri64, grid, dim3(kBlock), 0, s, g, S, CP); break;
+  }
   return hipGetLastError();
 }
 
@@ -569,8 +718,8 @@ hipError_t launch_gradient(const DevGrid& g, const double* p, double* Gx, double
 
 hipError_t launch_max_div(const DevGrid& g, const double* U, const double* V, const double* W,
                           unsigned long long* red, hipStream_t s) {
-  hipLaunchKernelGGL(k_max_div, grid_pts(g, g.nxg - 2, g.nyg - 2, g.nzg - 2), dim3(kBlock), 0, s,
-                     g, U, V, W, red);
+  hipLaunchKernelGGL(k_max_div, grid_pts(g, g.nxg - 2, g.nyg - 2, g.nzg - 2 < 16 ? g.nzg - 2 : 16),
+                     dim3(kBlock), 0, s, g, U, V, W, red);
   return hipGetLastError();
 }
 

@@ -18,6 +18,9 @@ struct DevGrid {
   const double* dxg;  // xg(i+1) - xg(i), i = 1..nxg-1
   const double* dyg;  // yg(j+1) - yg(j)
   const double* dzg;  // zg(k+1) - zg(k)
+  const double* rdxf;  // 1/dxf (same indexing)
+  const double* rdyf;  // 1/dyf
+  const double* rdzf;  // 1/dzf
   const double* kxx;  // [nxp] modified wavenumbers
   const double* kzz;  // [nzp]
   const double* dyy_d;  // [nym]   Dyy(j,j)

@@ -1,0 +1,117 @@
+!------------------------------------------------------------------------------!
+! iso_c_binding interfaces to the gfx950 projection library (include/c3d.h). !
+! Fortran hosts use these through the modules in c3d_modules.f90, which keep !
+! the reference's module and procedure names.                                !
+!------------------------------------------------------------------------------!
+Module c3d_bindings
+
+  Use, Intrinsic :: iso_c_binding
+  Implicit None
+
+  Integer(c_int), Parameter :: C3D_OK = 0
+  Integer(c_int), Parameter :: C3D_PREC_FP64 = 0
+  Integer(c_int), Parameter :: C3D_PREC_MIXED = 1
+
+  ! struct c3d_desc
+  Type, Bind(C) :: c3d_desc
+     Integer(c_int) :: nx, ny, nz, grid_type
+     Real(c_double) :: Lxp, Lzp, Ly, alpha, nu
+     Integer(c_int) :: precision
+  End Type c3d_desc
+
+  Interface
+
+     Integer(c_int) Function c3d_create(desc, ctx) Bind(C, name="c3d_create")
+       Import :: c_int, c_ptr, c3d_desc
+       Type(c3d_desc), Intent(In) :: desc
+       Type(c_ptr), Intent(Out) :: ctx
+     End Function c3d_create
+
+     Integer(c_int) Function c3d_destroy(ctx) Bind(C, name="c3d_destroy")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+     End Function c3d_destroy
+
+     Type(c_ptr) Function c3d_last_error(ctx) Bind(C, name="c3d_last_error")
+       Import :: c_ptr
+       Type(c_ptr), Value :: ctx
+     End Function c3d_last_error
+
+     Integer(c_int) Function c3d_grid_get(ctx, what, out, n) Bind(C, name="c3d_grid_get")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: what, n
+       Real(c_double) :: out(*)
+     End Function c3d_grid_get
+
+     Integer(c_int) Function c3d_set_fields(ctx, U, V, W, P) Bind(C, name="c3d_set_fields")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx, U, V, W, P
+     End Function c3d_set_fields
+
+     Integer(c_int) Function c3d_get_fields(ctx, U, V, W, P, Pint) Bind(C, name="c3d_get_fields")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx, U, V, W, P, Pint
+     End Function c3d_get_fields
+
+     Integer(c_int) Function c3d_apply_boundary_conditions(ctx) &
+          Bind(C, name="c3d_apply_boundary_conditions")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+     End Function c3d_apply_boundary_conditions
+
+     Integer(c_int) Function c3d_divergence(ctx, div) Bind(C, name="c3d_divergence")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: div(*)
+     End Function c3d_divergence
+
+     Integer(c_int) Function c3d_gradient(ctx, p, Gx, Gy, Gz) Bind(C, name="c3d_gradient")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: p(*)
+       Real(c_double) :: Gx(*), Gy(*), Gz(*)
+     End Function c3d_gradient
+
+     Integer(c_int) Function c3d_solve_poisson_equation(ctx, rhs, rk_step, dt) &
+          Bind(C, name="c3d_solve_poisson_equation")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: rhs(*)
+       Integer(c_int), Value :: rk_step
+       Real(c_double), Value :: dt
+     End Function c3d_solve_poisson_equation
+
+     Integer(c_int) Function c3d_compute_non_ib_projection(ctx, dt, rk_step) &
+          Bind(C, name="c3d_compute_non_ib_projection")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: dt
+       Integer(c_int), Value :: rk_step
+     End Function c3d_compute_non_ib_projection
+
+     Integer(c_int) Function c3d_check_divergence(ctx, max_div) Bind(C, name="c3d_check_divergence")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: max_div
+     End Function c3d_check_divergence
+
+  End Interface
+
+Contains
+
+  ! Library error message as a Fortran string.
+  Function c3d_error_message(ctx) Result(msg)
+    Type(c_ptr), Intent(In) :: ctx
+    Character(len=512) :: msg
+    Character(kind=c_char), Pointer :: cs(:)
+    Integer :: i
+    msg = ' '
+    Call c_f_pointer(c3d_last_error(ctx), cs, [512])
+    Do i = 1, 512
+       If (cs(i) == c_null_char) Exit
+       msg(i:i) = cs(i)
+    End Do
+  End Function c3d_error_message
+
+End Module c3d_bindings
