@@ -31,6 +31,14 @@ struct c3d_ctx {
   double* CP = nullptr;
   unsigned long long* red = nullptr;
   size_t nu = 0, nv = 0, nw = 0, np = 0, nr = 0, nspec = 0;
+  // RK3 state (allocated on first use): step-start copies, stage
+  // right-hand sides F[stage][component], reduction scratch
+  double *Uo = nullptr, *Vo = nullptr, *Wo = nullptr;
+  double* F[3][3] = {{nullptr}};
+  double* part = nullptr;  // mass-flow row partials, then 2 result slots
+  double dPdx = 0.0, dPdy = 0.0, dPdz = 0.0;
+  // interior Heaviside masks (null: no body, H = 1)
+  const double *Hu = nullptr, *Hv = nullptr, *Hw = nullptr;
   std::string err;
 };
 
@@ -278,7 +286,7 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
       (rc = dalloc(c, &c->W, c->nw)) || (rc = dalloc(c, &c->P, c->np)) ||
       (rc = dalloc(c, &c->Pint, c->nr)) || (rc = dalloc(c, &c->rhs, c->nr)) ||
       (rc = dalloc(c, &c->S, c->nspec)) || (rc = dalloc(c, &c->CP, c->nspec)) ||
-      (rc = dalloc(c, &c->red, 1)))
+      (rc = dalloc(c, &c->red, 4)))
     return cleanup(rc);
   if (hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->V, 0, c->nv * 8, c->stream) != hipSuccess ||
@@ -437,6 +445,215 @@ int c3d_check_divergence(c3d_ctx* c, double* max_div) {
   if (rc) return rc;
   std::memcpy(max_div, &bits, sizeof(double));
   return C3D_OK;
+}
+
+int c3d_copy_bandwidth(c3d_ctx* c, size_t nbytes, int iters, double* ms) {
+  if (!c || !ms || iters <= 0 || nbytes < 16 || (nbytes % 16)) return C3D_EINVAL;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  if (hipMalloc(&a, nbytes) != hipSuccess || hipMalloc(&b, nbytes) != hipSuccess) {
+    rc = fail(c, C3D_ENOMEM, "copy calibration buffers");
+  } else if (hipMemsetAsync(a, 0, nbytes, c->stream) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+             c3d::launch_copy(a, b, nbytes, c->stream) != hipSuccess ||
+             hipEventRecord(e0, c->stream) != hipSuccess) {
+    rc = fail(c, C3D_EHIP, "copy calibration set-up failed");
+  } else {
+    for (int it = 0; it < iters && !rc; it++)
+      if (c3d::launch_copy(a, b, nbytes, c->stream) != hipSuccess) rc = fail(c, C3D_EHIP, "copy");
+    float t = 0.f;
+    if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = fail(c, C3D_EHIP, "copy timing failed");
+    if (!rc) *ms = t / iters;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
+// ----------------------------------------------------------------------------
+// RK3 time integration (src/time_integration.f90, src/equations.f90,
+// src/mass_flow.f90)
+// ----------------------------------------------------------------------------
+static int ensure_rk(c3d_ctx* c) {
+  if (c->Uo) return 0;
+  const size_t fu = (size_t)(c->nx - 2) * (c->nyg - 2) * (c->nzg - 2);
+  const size_t fv = (size_t)(c->nxg - 2) * (c->ny - 2) * (c->nzg - 2);
+  const size_t fw = (size_t)(c->nxg - 2) * (c->nyg - 2) * (c->nz - 2);
+  int rc = 0;
+  if ((rc = dalloc(c, &c->Uo, c->nu)) || (rc = dalloc(c, &c->Vo, c->nv)) ||
+      (rc = dalloc(c, &c->Wo, c->nw)))
+    return rc;
+  for (int s = 0; s < 3; s++)
+    if ((rc = dalloc(c, &c->F[s][0], fu)) || (rc = dalloc(c, &c->F[s][1], fv)) ||
+        (rc = dalloc(c, &c->F[s][2], fw)))
+      return rc;
+  for (int s = 0; s < 3; s++) {
+    HIPCHK(c, hipMemsetAsync(c->F[s][0], 0, fu * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->F[s][1], 0, fv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->F[s][2], 0, fw * 8, c->stream));
+  }
+  if ((rc = dalloc(c, &c->part, 2 * (size_t)c3d::massflow_rows(c->g) + 2))) return rc;
+  return 0;
+}
+
+static int save_state(c3d_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->Uo, c->U, c->nu * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->Vo, c->V, c->nv * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->Wo, c->W, c->nw * 8, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+static int rhs_launch(c3d_ctx* c, int slot) {
+  double** F = c->F[slot - 1];
+  HIPCHK(c, c3d::launch_rhs(c->g, c->U, c->V, c->W, c->Hu, c->Hv, c->Hw, c->desc.nu, c->dPdx,
+                            c->dPdy, c->dPdz, F[0], F[1], F[2], c->stream));
+  return 0;
+}
+
+static int rk_update_launch(c3d_ctx* c, int rk_step, double dt) {
+  double* X[3] = {c->U, c->V, c->W};
+  const double* Xo[3] = {c->Uo, c->Vo, c->Wo};
+  for (int q = 0; q < 3; q++)
+    HIPCHK(c, c3d::launch_rk_update(c->g, q, rk_step, dt, c->F[0][q], c->F[1][q], c->F[2][q],
+                                    Xo[q], X[q], c->stream));
+  return 0;
+}
+
+static int mass_flow(c3d_ctx* c, double* q) {
+  double* out = c->part + 2 * (size_t)c3d::massflow_rows(c->g);
+  HIPCHK(c, c3d::launch_massflow(c->g, c->U, c->Hu, c->part, out, c->stream));
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  *q = h[0] / h[1];
+  return 0;
+}
+
+int c3d_set_pressure_gradient(c3d_ctx* c, double dPdx, double dPdy, double dPdz) {
+  if (!c) return C3D_EINVAL;
+  c->dPdx = dPdx;
+  c->dPdy = dPdy;
+  c->dPdz = dPdz;
+  return C3D_OK;
+}
+
+int c3d_get_pressure_gradient(const c3d_ctx* c, double* dPdx, double* dPdy, double* dPdz) {
+  if (!c) return C3D_EINVAL;
+  if (dPdx) *dPdx = c->dPdx;
+  if (dPdy) *dPdy = c->dPdy;
+  if (dPdz) *dPdz = c->dPdz;
+  return C3D_OK;
+}
+
+int c3d_save_state(c3d_ctx* c) {
+  if (!c) return C3D_EINVAL;
+  int rc = ensure_rk(c);
+  if (!rc) rc = save_state(c);
+  if (!rc) rc = sync(c);
+  return rc;
+}
+
+int c3d_compute_rhs(c3d_ctx* c, int slot) {
+  if (!c || slot < 1 || slot > 3) return C3D_EINVAL;
+  int rc = ensure_rk(c);
+  if (!rc) rc = rhs_launch(c, slot);
+  if (!rc) rc = sync(c);
+  return rc;
+}
+
+int c3d_get_rhs(c3d_ctx* c, int slot, double* Fu, double* Fv, double* Fw) {
+  if (!c || slot < 1 || slot > 3) return C3D_EINVAL;
+  int rc = ensure_rk(c);
+  if (rc) return rc;
+  const size_t fu = (size_t)(c->nx - 2) * (c->nyg - 2) * (c->nzg - 2);
+  const size_t fv = (size_t)(c->nxg - 2) * (c->ny - 2) * (c->nzg - 2);
+  const size_t fw = (size_t)(c->nxg - 2) * (c->nyg - 2) * (c->nz - 2);
+  double** F = c->F[slot - 1];
+  if (Fu) HIPCHK(c, hipMemcpyAsync(Fu, F[0], fu * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Fv) HIPCHK(c, hipMemcpyAsync(Fv, F[1], fv * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Fw) HIPCHK(c, hipMemcpyAsync(Fw, F[2], fw * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_rk3_update(c3d_ctx* c, int rk_step, double dt) {
+  if (!c || rk_step < 1 || rk_step > 3) return C3D_EINVAL;
+  int rc = ensure_rk(c);
+  if (!rc) rc = rk_update_launch(c, rk_step, dt);
+  if (!rc) rc = sync(c);
+  return rc;
+}
+
+int c3d_compute_dt(c3d_ctx* c, double cfl, double* dt_out) {
+  if (!c || !dt_out) return C3D_EINVAL;
+  unsigned long long h[3] = {0, 0, 0};
+  unsigned long long* red3 = c->red + 1;  // slots 1..3 of the reduction buffer
+  HIPCHK(c, hipMemsetAsync(red3, 0, sizeof(h), c->stream));
+  HIPCHK(c, c3d::launch_dt_max(c->g, c->U, c->V, c->W, red3, c->stream));
+  HIPCHK(c, hipMemcpyAsync(h, red3, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  double lmax[3];
+  std::memcpy(lmax, h, sizeof(lmax));
+  // src/time_integration.f90:241-265
+  const double dtc = std::fmin(std::fmin(cfl * lmax[0], cfl * lmax[1]), cfl * lmax[2]);
+  const double nu = c->desc.nu;
+  const double dtv = std::fmin(std::fmin(cfl * (c->dxmin * c->dxmin) / nu,
+                                         cfl * (c->dymin * c->dymin) / nu),
+                               cfl * (c->dzmin * c->dzmin) / nu);
+  double dt = std::fmin(dtc, dtv);
+  dt = std::fmin(dt, 0.1);
+  if (cfl < 0) dt = -cfl;
+  *dt_out = dt;
+  return C3D_OK;
+}
+
+int c3d_mean_mass_flow_u(c3d_ctx* c, double* q) {
+  if (!c || !q) return C3D_EINVAL;
+  int rc = ensure_rk(c);
+  if (!rc) rc = mass_flow(c, q);
+  return rc;
+}
+
+int c3d_add_to_u(c3d_ctx* c, double d) {
+  if (!c) return C3D_EINVAL;
+  HIPCHK(c, c3d::launch_add_u(c->g, c->U, d, c->stream));
+  return sync(c);
+}
+
+int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
+  if (!c || !(dt > 0.0)) return C3D_EINVAL;
+  int rc = ensure_rk(c);
+  if (!rc) rc = save_state(c);  // src/time_integration.f90:91-94
+  for (int st = 1; st <= 3 && !rc; st++) {
+    rc = rhs_launch(c, st);                                     // :101-103
+    if (!rc) rc = rk_update_launch(c, st, dt);                  // :105-107
+    if (!rc && c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
+      rc = fail(c, C3D_EHIP, "boundary conditions");            // :117
+    if (!rc) rc = projection_launch(c, dt, st, nullptr);        // :118
+    if (!rc && c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
+      rc = fail(c, C3D_EHIP, "boundary conditions");            // :123
+  }
+  if (!rc && x_mass_cte == 1) {  // :187-193
+    const double prev = c->dPdx;
+    double ub = 0.0;
+    rc = mass_flow(c, &ub);
+    if (!rc) {
+      const double d = q0 - ub;
+      if (c3d::launch_add_u(c->g, c->U, d, c->stream) != hipSuccess ||
+          c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
+        rc = fail(c, C3D_EHIP, "mass-flow correction");
+      c->dPdx = 0.5 * (prev + d / dt);
+    }
+  }
+  if (!rc) rc = sync(c);
+  return rc;
 }
 
 int c3d_stage_count(void) { return kNumStages; }

@@ -1,6 +1,7 @@
 // Stockham autosort FFT on LDS-resident rows, for wave64 workgroups on gfx950.
 //
-// A length-N complex row lives in LDS; NT threads of the workgroup own it.
+// A length-N complex row lives in LDS at padded positions lp(i) (see below);
+// NT threads of the workgroup own it.
 // Each radix-R pass loads its R inputs per butterfly into registers, syncs,
 // applies the twiddles and the radix-R DFT and stores the autosorted
 // outputs back into the same LDS row (so a row needs N complex of LDS, not
@@ -18,6 +19,13 @@
 #include "c3d_common.hpp"
 
 namespace c3d {
+
+// Padded LDS position of element i of a row: one spare complex after every
+// 8, so the stride-R stores of a first radix-8 pass (and stride-8 patterns
+// in general) spread over all LDS banks.  A padded row occupies
+// lds_row_len(N) complex.
+__host__ __device__ __forceinline__ constexpr int lp(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int lds_row_len(int n) { return n + n / 8; }
 
 template <typename real, int R, int SIGN>
 __device__ __forceinline__ void dft_small(cpx<real>* v) {
@@ -74,7 +82,7 @@ __device__ __forceinline__ void stockham_pass(cpx<real>* buf, const cpx<real>* _
     const int j = t + b * NT;
     if (PER * NT == NB || j < NB) {
 #pragma unroll
-      for (int r = 0; r < R; r++) v[b][r] = buf[j + r * NB];
+      for (int r = 0; r < R; r++) v[b][r] = buf[lp(j + r * NB)];
       if constexpr (NS > 1) {
         const int k = j % NS;
 #pragma unroll
@@ -94,7 +102,7 @@ __device__ __forceinline__ void stockham_pass(cpx<real>* buf, const cpx<real>* _
       const int k = j % NS;
       const int idx = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[idx + r * NS] = v[b][r];
+      for (int r = 0; r < R; r++) buf[lp(idx + r * NS)] = v[b][r];
     }
   }
   __syncthreads();

@@ -8,6 +8,8 @@
 //     coalesced at every j.
 // Physical fields keep the reference's Fortran layout (x fastest,
 // ghost-inclusive), so host copies need no reshaping.
+#include <cstdlib>
+
 #include "fft_lds.hpp"
 #include "kernels.hpp"
 
@@ -16,11 +18,23 @@ namespace c3d {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kGradKC = 8;  // z planes per lane in the gradient correction
 
 // Rows of the z tile handled by one x-transform workgroup (row pairs are
-// packed into one complex transform).
-__host__ __device__ constexpr int tile_rows(int nzp) {
-  return (nzp % 16 == 0) ? 16 : (nzp % 8 == 0) ? 8 : (nzp % 4 == 0) ? 4 : 2;
+// packed into one complex transform).  Default: 16 rows (256-byte kz runs in
+// the spectrum) up to x length 192, 8 rows above (LDS per workgroup stays
+// near 28 KB so that several workgroups share a CU).  C3D_XTILE overrides it
+// (tuning knob; must divide nzp).
+int tile_rows(int nxp, int nzp) {
+  static const int forced = [] {
+    const char* e = getenv("C3D_XTILE");
+    return e ? atoi(e) : 0;
+  }();
+  if ((forced == 2 || forced == 4 || forced == 8 || forced == 16) && nzp % forced == 0) return forced;
+  const int want = nxp > 192 ? 8 : 16;
+  for (int t = want; t >= 2; t /= 2)
+    if (nzp % t == 0) return t;
+  return 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -39,7 +53,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   constexpr int NP = TK / 2;
   constexpr int NT = kBlock / NP;
   constexpr int NH = N / 2 + 1;
-  constexpr int BUF = (NP * N > TK * NH) ? NP * N : TK * NH;
+  constexpr int LP = lds_row_len(N);  // padded FFT row
+  constexpr int TKP = TK + 1;         // padded pitch of the [kx][row] layout
+  constexpr int BUF = (NP * LP > NH * TKP) ? NP * LP : NH * TKP;
   __shared__ cpx<real> buf[BUF];
 
   const int tid = threadIdx.x;
@@ -87,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
         } else {
           d = v[q][0];
         }
-        real* bp = reinterpret_cast<real*>(&buf[(t >> 1) * N + n]);
+        real* bp = reinterpret_cast<real*>(&buf[(t >> 1) * LP + lp(n)]);
         bp[t & 1] = (real)d;
       }
     }
@@ -95,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   __syncthreads();
 
   // 2) complex FFT of each packed pair
-  fft_lds<real, N, NT, -1>(buf + (tid / NT) * N, tw, tid % NT);
+  fft_lds<real, N, NT, -1>(buf + (tid / NT) * LP, tw, tid % NT);
 
   // 3) separate the two real transforms: A_k = (Z_k + conj Z_{N-k})/2,
   //    B_k = (Z_k - conj Z_{N-k})/(2i); store transposed [kx][row].
@@ -106,8 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
     const int e = tid + q * kBlock;
     if (e < NP * NH) {
       const int p = e / NH, kx = e % NH;
-      const cpx<real> z = buf[p * N + kx];
-      const cpx<real> zc = buf[p * N + (kx == 0 ? 0 : N - kx)];
+      const cpx<real> z = buf[p * LP + lp(kx)];
+      const cpx<real> zc = buf[p * LP + lp(kx == 0 ? 0 : N - kx)];
       A[q] = cmk((real)0.5 * (z.x + zc.x), (real)0.5 * (z.y - zc.y));
       B[q] = cmk((real)0.5 * (z.y + zc.y), (real)0.5 * (zc.x - z.x));
     }
@@ -118,8 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
     const int e = tid + q * kBlock;
     if (e < NP * NH) {
       const int p = e / NH, kx = e % NH;
-      buf[kx * TK + 2 * p] = A[q];
-      buf[kx * TK + 2 * p + 1] = B[q];
+      buf[kx * TKP + 2 * p] = A[q];
+      buf[kx * TKP + 2 * p + 1] = B[q];
     }
   }
   __syncthreads();
@@ -128,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   const int nzp = g.nzp;
   for (int e = tid; e < NH * TK; e += kBlock) {
     const int kx = e / TK, t = e % TK;
-    S[((size_t)jj * NH + kx) * nzp + k0 + t] = buf[e];
+    S[((size_t)jj * NH + kx) * nzp + k0 + t] = buf[kx * TKP + t];
   }
 }
 
@@ -140,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_z(cpx<real>* __restrict__ S,
                                                   const cpx<real>* __restrict__ tw, int ncol) {
   constexpr int NT = 64;
   constexpr int NC = kBlock / NT;
-  __shared__ cpx<real> buf[NC * N];
+  constexpr int LP = lds_row_len(N);
+  __shared__ cpx<real> buf[NC * LP];
   const int tid = threadIdx.x;
   const size_t col0 = (size_t)blockIdx.x * NC;
   {
@@ -160,14 +177,14 @@ __global__ __launch_bounds__(kBlock) void k_fft_z(cpx<real>* __restrict__ S,
 #pragma unroll
     for (int q = 0; q < ITEMS; q++) {
       const int e = tid + q * kBlock;
-      if (e < NC * N) buf[e] = cmk(vx[q], vy[q]);
+      if (e < NC * N) buf[(e / N) * LP + lp(e % N)] = cmk(vx[q], vy[q]);
     }
   }
   __syncthreads();
-  fft_lds<real, N, NT, SIGN>(buf + (tid / NT) * N, tw, tid % NT);
+  fft_lds<real, N, NT, SIGN>(buf + (tid / NT) * LP, tw, tid % NT);
   for (int e = tid; e < NC * N; e += kBlock) {
     const int c = e / N;
-    if (col0 + c < (size_t)ncol) S[col0 * N + e] = buf[e];
+    if (col0 + c < (size_t)ncol) S[col0 * N + e] = buf[c * LP + lp(e % N)];
   }
 }
 
@@ -312,7 +329,9 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
   constexpr int NP = TK / 2;
   constexpr int NT = kBlock / NP;
   constexpr int NH = N / 2 + 1;
-  constexpr int BUF = (NP * N > TK * NH) ? NP * N : TK * NH;
+  constexpr int LP = lds_row_len(N);
+  constexpr int TKP = TK + 1;
+  constexpr int BUF = (NP * LP > NH * TKP) ? NP * LP : NH * TKP;
   __shared__ cpx<real> buf[BUF];
 
   const int tid = threadIdx.x;
@@ -339,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
 #pragma unroll
     for (int q = 0; q < ITEMS; q++) {
       const int e = tid + q * kBlock;
-      if (e < NH * TK) buf[e] = cmk(vx[q], vy[q]);
+      if (e < NH * TK) buf[(e / TK) * TKP + e % TK] = cmk(vx[q], vy[q]);
     }
   }
   __syncthreads();
@@ -355,15 +374,15 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
       const int p = e / N, n = e % N;
       cpx<real> a, b;
       if (n <= N / 2) {
-        a = buf[n * TK + 2 * p];
-        b = buf[n * TK + 2 * p + 1];
+        a = buf[n * TKP + 2 * p];
+        b = buf[n * TKP + 2 * p + 1];
         if (n == 0 || n == N / 2) {
           a.y = (real)0;
           b.y = (real)0;
         }
       } else {
-        a = buf[(N - n) * TK + 2 * p];
-        b = buf[(N - n) * TK + 2 * p + 1];
+        a = buf[(N - n) * TKP + 2 * p];
+        b = buf[(N - n) * TKP + 2 * p + 1];
         a.y = -a.y;
         b.y = -b.y;
       }
@@ -374,11 +393,11 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int e = tid + q * kBlock;
-    if (e < NP * N) buf[e] = Z[q];
+    if (e < NP * N) buf[(e / N) * LP + lp(e % N)] = Z[q];
   }
   __syncthreads();
 
-  fft_lds<real, N, NT, +1>(buf + (tid / NT) * N, tw, tid % NT);
+  fft_lds<real, N, NT, +1>(buf + (tid / NT) * LP, tw, tid % NT);
 
   const double norm = (double)((long long)g.nxp * (long long)nzp);
   const double cP = dt * (8.0 / 15.0);
@@ -386,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
   const int kdup = nzp + 2;     // z duplicate of k = 2
   for (int e = tid; e < TK * N; e += kBlock) {
     const int t = e / N, n = e % N;
-    const cpx<real> z = buf[(t >> 1) * N + n];
+    const cpx<real> z = buf[(t >> 1) * LP + lp(n)];
     const double phi = (double)((t & 1) ? z.y : z.x) / norm;
     const double o = (MODE == 1) ? phi / dt : phi;
     const int i = n + 2, k = k0 + t + 2;
@@ -447,24 +466,34 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
                                                          double* __restrict__ W,
                                                          const double* __restrict__ Pint,
                                                          double dt) {
+  // one lane per (i, j), walking kGradKC consecutive z planes so that
+  // p(k+1) of one plane is p(k) of the next
   const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  const int kb = blockIdx.z * kGradKC + 2;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   if (i > nx - 1 || j > nyg - 1) return;
-  const double p = dt * Pint[ridx(i, j, k, nxg, nyg)];
-  const double px = dt * Pint[ridx(i + 1, j, k, nxg, nyg)];
-  const size_t iu = fidx(i, j, k, nx, nyg);
-  U[iu] = U[iu] - (px - p) / g.dxg[i];
-  if (j <= ny - 1) {
-    const double py = dt * Pint[ridx(i, j + 1, k, nxg, nyg)];
-    const size_t iv = fidx(i, j, k, nxg, ny);
-    V[iv] = V[iv] - (py - p) / g.dyg[j];
-  }
-  if (k <= nz - 1) {
-    const double pz = dt * Pint[ridx(i, j, k + 1, nxg, nyg)];
-    const size_t iw = fidx(i, j, k, nxg, nyg);
-    W[iw] = W[iw] - (pz - p) / g.dzg[k];
+  const double hx = g.dxg[i];
+  const double hy = (j <= ny - 1) ? g.dyg[j] : 1.0;
+  double p = dt * Pint[ridx(i, j, kb, nxg, nyg)];
+#pragma unroll
+  for (int kk = 0; kk < kGradKC; kk++) {
+    const int k = kb + kk;
+    if (k > nzg - 1) break;
+    const double px = dt * Pint[ridx(i + 1, j, k, nxg, nyg)];
+    const size_t iu = fidx(i, j, k, nx, nyg);
+    U[iu] = U[iu] - (px - p) / hx;
+    if (j <= ny - 1) {
+      const double py = dt * Pint[ridx(i, j + 1, k, nxg, nyg)];
+      const size_t iv = fidx(i, j, k, nxg, ny);
+      V[iv] = V[iv] - (py - p) / hy;
+    }
+    if (k <= nz - 1) {
+      const double pz = dt * Pint[ridx(i, j, k + 1, nxg, nyg)];
+      const size_t iw = fidx(i, j, k, nxg, nyg);
+      W[iw] = W[iw] - (pz - p) / g.dzg[k];
+      p = pz;
+    }
   }
 }
 
@@ -596,6 +625,13 @@ __global__ __launch_bounds__(kBlock) void k_gradient(DevGrid g, const double* __
   if (k <= nz - 1) Gz[fidx(i, j, k, nxg, nyg)] = (p[ridx(i, j, k + 1, nxg, nyg)] - pc) / g.dzg[k];
 }
 
+// Plain streaming copy (calibration of bandwidth and traffic counters).
+__global__ __launch_bounds__(kBlock) void k_copy(const double2* __restrict__ src,
+                                                 double2* __restrict__ dst, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (size_t)gridDim.x * kBlock)
+    dst[q] = src[q];
+}
+
 // Dispatch a compile-time FFT length.
 #define C3D_DISPATCH_N(n, BODY)                     \
   switch (n) {                                      \
@@ -629,7 +665,7 @@ inline dim3 grid_pts(const DevGrid& g, int ni, int nj, int nk) {
 
 hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const double* V,
                         const double* W, const double* rhs, cpx<double>* S, hipStream_t s) {
-  const int tk = tile_rows(g.nzp);
+  const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
     if (from_uvw)
@@ -668,7 +704,7 @@ hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t 
 
 hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
                         double dt, bool rk1, hipStream_t s) {
-  const int tk = tile_rows(g.nzp);
+  const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
     if (mode == 0)
@@ -690,8 +726,9 @@ hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s) {
 
 hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
                                double dt, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad_correct, grid_pts(g, g.nx - 2, g.nyg - 2, g.nzg - 2), dim3(kBlock), 0,
-                     s, g, U, V, W, Pint, dt);
+  hipLaunchKernelGGL(k_grad_correct,
+                     grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kGradKC - 1) / kGradKC),
+                     dim3(kBlock), 0, s, g, U, V, W, Pint, dt);
   return hipGetLastError();
 }
 
@@ -720,6 +757,12 @@ hipError_t launch_max_div(const DevGrid& g, const double* U, const double* V, co
                           unsigned long long* red, hipStream_t s) {
   hipLaunchKernelGGL(k_max_div, grid_pts(g, g.nxg - 2, g.nyg - 2, g.nzg - 2 < 16 ? g.nzg - 2 : 16),
                      dim3(kBlock), 0, s, g, U, V, W, red);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t nbytes, hipStream_t s) {
+  const size_t n = nbytes / sizeof(double2);
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(kBlock), 0, s, (const double2*)src, (double2*)dst, n);
   return hipGetLastError();
 }
 

@@ -65,4 +65,27 @@ hipError_t launch_gradient(const DevGrid& g, const double* p, double* Gx, double
 hipError_t launch_max_div(const DevGrid& g, const double* U, const double* V, const double* W,
                           unsigned long long* red, hipStream_t s);
 
+// RK3 (rk3.hip): right-hand sides into interior-shaped F arrays; H masks may
+// be null (H = 1).
+hipError_t launch_rhs(const DevGrid& g, const double* U, const double* V, const double* W,
+                      const double* Hu, const double* Hv, const double* Hw, double nu, double dPdx,
+                      double dPdy, double dPdz, double* Fu, double* Fv, double* Fw, hipStream_t s);
+// Stage update X(int) = Xo(int) + dt*sum a_kl F_l; which 0 U, 1 V, 2 W.
+hipError_t launch_rk_update(const DevGrid& g, int which, int stage, double dt, const double* F1,
+                            const double* F2, const double* F3, const double* Xo, double* X,
+                            hipStream_t s);
+hipError_t launch_add_u(const DevGrid& g, double* U, double d, hipStream_t s);
+// compute_dt maxima of dx/|u|, dy/|v|, dz/|w| as double bits into red3[0..2]
+// (red3 zeroed by the caller).
+hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, const double* W,
+                         unsigned long long* red3, hipStream_t s);
+// Bulk-flow integral and norm of U into out2[0..1]; part holds
+// 2*massflow_rows(g) doubles.
+int massflow_rows(const DevGrid& g);
+hipError_t launch_massflow(const DevGrid& g, const double* U, const double* H, double* part,
+                           double* out2, hipStream_t s);
+
+// Streaming copy of nbytes (multiple of 16) for bandwidth calibration.
+hipError_t launch_copy(const void* src, void* dst, size_t nbytes, hipStream_t s);
+
 }  // namespace c3d

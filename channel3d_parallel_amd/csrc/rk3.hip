@@ -1,0 +1,304 @@
+// RK3 advection-diffusion kernels (src/equations.f90, src/time_integration.f90,
+// src/mass_flow.f90) for gfx950, FP64.
+//
+// Each right-hand side is evaluated pointwise: the reference's whole-array
+// passes through term_1/term_2 reduce, at every interior point, to the
+// expressions below (same operands, same operation order, same order of
+// accumulation: convective x, y, z, viscous, pressure gradient).  FMA
+// contraction is disabled so the results round exactly as written.
+#include "kernels.hpp"
+
+namespace c3d {
+
+namespace {
+
+constexpr int kB = 256;
+
+#define U_(i, j, k) U[fidx(i, j, k, g.nx, g.nyg)]
+#define V_(i, j, k) V[fidx(i, j, k, g.nxg, g.ny)]
+#define W_(i, j, k) W[fidx(i, j, k, g.nxg, g.nyg)]
+
+// compute_rhs_u, src/equations.f90:24-153; F bounds (2:nx-1, 2:nyg-1, 2:nzg-1)
+__global__ __launch_bounds__(kB) void k_rhs_u(DevGrid g, const double* __restrict__ U,
+                                              const double* __restrict__ V,
+                                              const double* __restrict__ W,
+                                              const double* __restrict__ H, double nu,
+                                              double dPdx, double* __restrict__ F) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  if (i > g.nx - 1 || j > g.nyg - 1) return;
+  const double uc1 = 0.5 * (U_(i - 1, j, k) + U_(i, j, k));
+  const double uc2 = 0.5 * (U_(i, j, k) + U_(i + 1, j, k));
+  const double a = -(uc2 * uc2 - uc1 * uc1) / g.dxg[i];
+  const double uv1 = (0.5 * (U_(i, j, k) + U_(i, j + 1, k))) * (0.5 * (V_(i, j, k) + V_(i + 1, j, k)));
+  const double uv0 =
+      (0.5 * (U_(i, j - 1, k) + U_(i, j, k))) * (0.5 * (V_(i, j - 1, k) + V_(i + 1, j - 1, k)));
+  const double b = -(uv1 - uv0) / g.dyf[j];
+  const double uw1 = (0.5 * (U_(i, j, k) + U_(i, j, k + 1))) * (0.5 * (W_(i, j, k) + W_(i + 1, j, k)));
+  const double uw0 =
+      (0.5 * (U_(i, j, k - 1) + U_(i, j, k))) * (0.5 * (W_(i, j, k - 1) + W_(i + 1, j, k - 1)));
+  const double c = -(uw1 - uw0) / g.dzf[k];
+  const double dx_1 = g.dxf[i], dx_2 = g.dxf[i + 1], dx_3 = g.dxg[i];
+  const double dy_3 = g.dyf[j];
+  const double dz_1 = g.dzg[k - 1], dz_2 = g.dzg[k], dz_3 = g.dzf[k];
+  const double t1 = (U_(i, j + 1, k) - U_(i, j, k)) / g.dyg[j];
+  const double t1m = (U_(i, j, k) - U_(i, j - 1, k)) / g.dyg[j - 1];
+  const double visc = 1.0 / dx_3 * (nu * 1.0 / dx_2 * (U_(i + 1, j, k) - U_(i, j, k)) -
+                                    nu * 1.0 / dx_1 * (U_(i, j, k) - U_(i - 1, j, k))) +
+                      1.0 / dy_3 * (nu * t1 - nu * t1m) +
+                      1.0 / dz_3 * (nu * 1.0 / dz_2 * (U_(i, j, k + 1) - U_(i, j, k)) -
+                                    nu * 1.0 / dz_1 * (U_(i, j, k) - U_(i, j, k - 1)));
+  const double h = H ? H[fidx(i, j, k, g.nx, g.nyg)] : 1.0;
+  double r = a;
+  r = r + b;
+  r = r + c;
+  r = r + visc;
+  r = r + dPdx * h;
+  F[(size_t)(i - 2) + (size_t)(g.nx - 2) * ((size_t)(j - 2) + (size_t)(g.nyg - 2) * (size_t)(k - 2))] = r;
+}
+
+// compute_rhs_v, src/equations.f90:160-288; F bounds (2:nxg-1, 2:ny-1, 2:nzg-1)
+__global__ __launch_bounds__(kB) void k_rhs_v(DevGrid g, const double* __restrict__ U,
+                                              const double* __restrict__ V,
+                                              const double* __restrict__ W,
+                                              const double* __restrict__ H, double nu,
+                                              double dPdy, double* __restrict__ F) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  if (i > g.nxg - 1 || j > g.ny - 1) return;
+  const double vc1 = 0.5 * (V_(i, j - 1, k) + V_(i, j, k));
+  const double vc2 = 0.5 * (V_(i, j, k) + V_(i, j + 1, k));
+  const double a = -(vc2 * vc2 - vc1 * vc1) / g.dyg[j];
+  const double uv1 = (0.5 * (U_(i, j, k) + U_(i, j + 1, k))) * (0.5 * (V_(i, j, k) + V_(i + 1, j, k)));
+  const double uv0 =
+      (0.5 * (U_(i - 1, j, k) + U_(i - 1, j + 1, k))) * (0.5 * (V_(i - 1, j, k) + V_(i, j, k)));
+  const double b = -(uv1 - uv0) / g.dxf[i];
+  const double vw1 = (0.5 * (V_(i, j, k) + V_(i, j, k + 1))) * (0.5 * (W_(i, j, k) + W_(i, j + 1, k)));
+  const double vw0 =
+      (0.5 * (V_(i, j, k - 1) + V_(i, j, k))) * (0.5 * (W_(i, j, k - 1) + W_(i, j + 1, k - 1)));
+  const double c = -(vw1 - vw0) / g.dzf[k];
+  const double dx_1 = g.dxg[i - 1], dx_2 = g.dxg[i], dx_3 = g.dxf[i];
+  const double dy_1 = g.dyf[j], dy_2 = g.dyf[j + 1], dy_3 = g.dyg[j];
+  const double dz_1 = g.dzg[k - 1], dz_2 = g.dzg[k], dz_3 = g.dzf[k];
+  const double visc = 1.0 / dx_3 * (nu * 1.0 / dx_2 * (V_(i + 1, j, k) - V_(i, j, k)) -
+                                    nu * 1.0 / dx_1 * (V_(i, j, k) - V_(i - 1, j, k))) +
+                      1.0 / dy_3 * (nu * 1.0 / dy_2 * (V_(i, j + 1, k) - V_(i, j, k)) -
+                                    nu * 1.0 / dy_1 * (V_(i, j, k) - V_(i, j - 1, k))) +
+                      1.0 / dz_3 * (nu * 1.0 / dz_2 * (V_(i, j, k + 1) - V_(i, j, k)) -
+                                    nu * 1.0 / dz_1 * (V_(i, j, k) - V_(i, j, k - 1)));
+  const double h = H ? H[fidx(i, j, k, g.nxg, g.ny)] : 1.0;
+  double r = a;
+  r = r + b;
+  r = r + c;
+  r = r + visc;
+  r = r + dPdy * h;
+  F[(size_t)(i - 2) + (size_t)(g.nxg - 2) * ((size_t)(j - 2) + (size_t)(g.ny - 2) * (size_t)(k - 2))] = r;
+}
+
+// compute_rhs_w, src/equations.f90:295-423; F bounds (2:nxg-1, 2:nyg-1, 2:nz-1)
+__global__ __launch_bounds__(kB) void k_rhs_w(DevGrid g, const double* __restrict__ U,
+                                              const double* __restrict__ V,
+                                              const double* __restrict__ W,
+                                              const double* __restrict__ H, double nu,
+                                              double dPdz, double* __restrict__ F) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  if (i > g.nxg - 1 || j > g.nyg - 1) return;
+  const double wc1 = 0.5 * (W_(i, j, k - 1) + W_(i, j, k));
+  const double wc2 = 0.5 * (W_(i, j, k) + W_(i, j, k + 1));
+  const double a = -(wc2 * wc2 - wc1 * wc1) / g.dzg[k];
+  const double uw1 = (0.5 * (U_(i, j, k) + U_(i, j, k + 1))) * (0.5 * (W_(i, j, k) + W_(i + 1, j, k)));
+  const double uw0 =
+      (0.5 * (U_(i - 1, j, k) + U_(i - 1, j, k + 1))) * (0.5 * (W_(i - 1, j, k) + W_(i, j, k)));
+  const double b = -(uw1 - uw0) / g.dxf[i];
+  const double vw1 = (0.5 * (V_(i, j, k) + V_(i, j, k + 1))) * (0.5 * (W_(i, j, k) + W_(i, j + 1, k)));
+  const double vw0 =
+      (0.5 * (V_(i, j - 1, k) + V_(i, j - 1, k + 1))) * (0.5 * (W_(i, j - 1, k) + W_(i, j, k)));
+  const double c = -(vw1 - vw0) / g.dyf[j];
+  const double dx_1 = g.dxg[i - 1], dx_2 = g.dxg[i], dx_3 = g.dxf[i];
+  const double dy_3 = g.dyf[j];
+  const double dz_1 = g.dzf[k], dz_2 = g.dzf[k + 1], dz_3 = g.dzg[k];
+  const double t1 = (W_(i, j + 1, k) - W_(i, j, k)) / g.dyg[j];
+  const double t1m = (W_(i, j, k) - W_(i, j - 1, k)) / g.dyg[j - 1];
+  const double visc = 1.0 / dx_3 * (nu * 1.0 / dx_2 * (W_(i + 1, j, k) - W_(i, j, k)) -
+                                    nu * 1.0 / dx_1 * (W_(i, j, k) - W_(i - 1, j, k))) +
+                      1.0 / dy_3 * (nu * t1 - nu * t1m) +
+                      1.0 / dz_3 * (nu * 1.0 / dz_2 * (W_(i, j, k + 1) - W_(i, j, k)) -
+                                    nu * 1.0 / dz_1 * (W_(i, j, k) - W_(i, j, k - 1)));
+  const double h = H ? H[fidx(i, j, k, g.nxg, g.nyg)] : 1.0;
+  double r = a;
+  r = r + b;
+  r = r + c;
+  r = r + visc;
+  r = r + dPdz * h;
+  F[(size_t)(i - 2) + (size_t)(g.nxg - 2) * ((size_t)(j - 2) + (size_t)(g.nyg - 2) * (size_t)(k - 2))] = r;
+}
+
+// RK3 stage update on the interior of one component
+// (src/time_integration.f90:105-107, 134-136, 163-168).
+__global__ __launch_bounds__(kB) void k_rk_update(int n1, int n2, int i1, int j1, int k1, int stage,
+                                                  double dt, const double* __restrict__ F1,
+                                                  const double* __restrict__ F2,
+                                                  const double* __restrict__ F3,
+                                                  const double* __restrict__ Xo,
+                                                  double* __restrict__ X) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  if (i > i1 || j > j1) return;
+  const double c11 = 8.0 / 15.0, c21 = 1.0 / 4.0, c22 = 5.0 / 12.0, c31 = 1.0 / 4.0, c32 = 0.0,
+               c33 = 3.0 / 4.0;
+  const size_t f = (size_t)(i - 2) + (size_t)(i1 - 1) * ((size_t)(j - 2) + (size_t)(j1 - 1) * (size_t)(k - 2));
+  const size_t o = fidx(i, j, k, n1, n2);
+  double v;
+  if (stage == 1) v = Xo[o] + dt * c11 * F1[f];
+  else if (stage == 2) v = Xo[o] + dt * (c21 * F1[f] + c22 * F2[f]);
+  else v = Xo[o] + dt * (c31 * F1[f] + c32 * F2[f] + c33 * F3[f]);
+  X[o] = v;
+  (void)k1;
+}
+
+// U(2:nx-1, 2:nyg-1, 2:nzg-1) += d  (src/time_integration.f90:190)
+__global__ __launch_bounds__(kB) void k_add_u(DevGrid g, double* __restrict__ U, double d) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  if (i > g.nx - 1 || j > g.nyg - 1) return;
+  const size_t o = fidx(i, j, k, g.nx, g.nyg);
+  U[o] = U[o] + d;
+}
+
+// compute_dt maxima (src/time_integration.f90:228-239): max over the
+// centre range of dx/|u| per component, as bit patterns of non-negative
+// doubles (+Inf where u = 0).
+__global__ __launch_bounds__(kB) void k_dt_max(DevGrid g, const double* __restrict__ U,
+                                               const double* __restrict__ V,
+                                               const double* __restrict__ W,
+                                               unsigned long long* __restrict__ red) {
+  __shared__ double sm[3][kB / 64];
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  double m[3] = {0.0, 0.0, 0.0};
+  if (i <= g.nxg - 1 && j <= g.nyg - 1)
+    for (int k = blockIdx.z + 2; k <= g.nzg - 1; k += gridDim.z) {
+      m[0] = fmax(m[0], g.dxg[i] / fabs(U_(i, j, k)));
+      m[1] = fmax(m[1], g.dyg[j] / fabs(V_(i, j, k)));
+      m[2] = fmax(m[2], g.dzg[k] / fabs(W_(i, j, k)));
+    }
+  for (int c = 0; c < 3; c++) {
+    double v = m[c];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sm[c][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = sm[threadIdx.x][0];
+    for (int w = 1; w < kB / 64; w++) v = fmax(v, sm[threadIdx.x][w]);
+    atomicMax(red + threadIdx.x, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// Bulk-flow partial sums (src/mass_flow.f90:36-44) per (j,k) row, summed
+// over i in order: part[2*r] = sum of (H U + H U)*dy/2, part[2*r+1] = norm.
+__global__ __launch_bounds__(kB) void k_massflow_rows(DevGrid g, const double* __restrict__ U,
+                                                      const double* __restrict__ H,
+                                                      double* __restrict__ part) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * kB + threadIdx.x;  // row index over (j = 3..nyg-1, k = 2..nzg-1)
+  const int nj = g.nyg - 3, nk = g.nzg - 2;
+  if (r >= nj * nk) return;
+  const int j = r % nj + 3, k = r / nj + 2;
+  double s = 0.0, n = 0.0;
+  for (int i = 2; i <= g.nx - 1; i++) {
+    const double h1 = H ? H[fidx(i, j, k, g.nx, g.nyg)] : 1.0;
+    const double h0 = H ? H[fidx(i, j - 1, k, g.nx, g.nyg)] : 1.0;
+    s = s + (h1 * U_(i, j, k) + h0 * U_(i, j - 1, k)) * 0.5 * g.dyg[j - 1];
+    n = n + (h1 + h0) * 0.5 * g.dyg[j - 1];
+  }
+  part[2 * (size_t)r] = s;
+  part[2 * (size_t)r + 1] = n;
+}
+
+// Ordered sum of the row partials (single workgroup, fixed order).
+__global__ __launch_bounds__(kB) void k_sum_pairs(const double* __restrict__ part, int nrow,
+                                                  double* __restrict__ out) {
+  __shared__ double ss[kB], sn[kB];
+  double s = 0.0, n = 0.0;
+  for (int r = threadIdx.x; r < nrow; r += kB) {
+    s += part[2 * (size_t)r];
+    n += part[2 * (size_t)r + 1];
+  }
+  ss[threadIdx.x] = s;
+  sn[threadIdx.x] = n;
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+      sn[threadIdx.x] += sn[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = ss[0];
+    out[1] = sn[0];
+  }
+}
+
+inline dim3 gpts(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
+
+}  // namespace
+
+hipError_t launch_rhs(const DevGrid& g, const double* U, const double* V, const double* W,
+                      const double* Hu, const double* Hv, const double* Hw, double nu, double dPdx,
+                      double dPdy, double dPdz, double* Fu, double* Fv, double* Fw, hipStream_t s) {
+  hipLaunchKernelGGL(k_rhs_u, gpts(g.nx - 2, g.nyg - 2, g.nzg - 2), dim3(kB), 0, s, g, U, V, W, Hu,
+                     nu, dPdx, Fu);
+  hipLaunchKernelGGL(k_rhs_v, gpts(g.nxg - 2, g.ny - 2, g.nzg - 2), dim3(kB), 0, s, g, U, V, W, Hv,
+                     nu, dPdy, Fv);
+  hipLaunchKernelGGL(k_rhs_w, gpts(g.nxg - 2, g.nyg - 2, g.nz - 2), dim3(kB), 0, s, g, U, V, W, Hw,
+                     nu, dPdz, Fw);
+  return hipGetLastError();
+}
+
+hipError_t launch_rk_update(const DevGrid& g, int which, int stage, double dt, const double* F1,
+                            const double* F2, const double* F3, const double* Xo, double* X,
+                            hipStream_t s) {
+  int n1, n2, i1, j1, k1;
+  if (which == 0) { n1 = g.nx; n2 = g.nyg; i1 = g.nx - 1; j1 = g.nyg - 1; k1 = g.nzg - 1; }
+  else if (which == 1) { n1 = g.nxg; n2 = g.ny; i1 = g.nxg - 1; j1 = g.ny - 1; k1 = g.nzg - 1; }
+  else { n1 = g.nxg; n2 = g.nyg; i1 = g.nxg - 1; j1 = g.nyg - 1; k1 = g.nz - 1; }
+  hipLaunchKernelGGL(k_rk_update, gpts(i1 - 1, j1 - 1, k1 - 1), dim3(kB), 0, s, n1, n2, i1, j1, k1,
+                     stage, dt, F1, F2, F3, Xo, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_u(const DevGrid& g, double* U, double d, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_u, gpts(g.nx - 2, g.nyg - 2, g.nzg - 2), dim3(kB), 0, s, g, U, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, const double* W,
+                         unsigned long long* red3, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt_max, gpts(g.nxg - 2, g.nyg - 2, g.nzg - 2 < 16 ? g.nzg - 2 : 16), dim3(kB),
+                     0, s, g, U, V, W, red3);
+  return hipGetLastError();
+}
+
+int massflow_rows(const DevGrid& g) { return (g.nyg - 3) * (g.nzg - 2); }
+
+hipError_t launch_massflow(const DevGrid& g, const double* U, const double* H, double* part,
+                           double* out2, hipStream_t s) {
+  const int nrow = massflow_rows(g);
+  hipLaunchKernelGGL(k_massflow_rows, dim3((nrow + kB - 1) / kB), dim3(kB), 0, s, g, U, H, part);
+  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kB), 0, s, part, nrow, out2);
+  return hipGetLastError();
+}
+
+}  // namespace c3d

@@ -62,6 +62,80 @@ def white_faces(nx, ny, nz, seed=2):
     return U, V, W
 
 
+def grid_faces(nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1):
+    """Face coordinates of create_grid (src/input_output.f90:140-178)."""
+    x = Lxp * np.arange(nx, dtype=np.float64) / (nx - 2)
+    z = Lzp * np.arange(nz, dtype=np.float64) / (nz - 2)
+    y = Ly * np.arange(ny, dtype=np.float64) / (ny - 1)
+    if grid_type == 1:
+        y = y - 1.0
+        if alpha > 0:
+            y = np.tanh(alpha * y) / np.tanh(alpha)
+        y = y - y.min()
+        y = y * Ly / y.max()
+    return x, y, z
+
+
+def channel_ic(nx, ny, nz, Ub=1.0, pert=0.1, seed=1, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0,
+               grid_type=1):
+    """Laminar parabola with bulk velocity Ub plus a divergence-free
+    perturbation of RMS pert*Ub.
+
+    The perturbation is the discrete curl of a vector potential A whose
+    components live on cell edges (Ax on (x-centre, y-face, z-face), Ay on
+    (x-face, y-centre, z-face), Az on (x-face, y-face, z-centre)), i.i.d.
+    N(0,1) from the SplitMix64 stream `seed`, periodic in x and z and zero on
+    the walls.  The curl uses the face spacings that the divergence uses, so
+    the discrete divergence of the perturbation telescopes to round-off.
+    Boundary conditions are not applied."""
+    x, y, z = grid_faces(nx, ny, nz, Lxp, Lzp, Ly, alpha, grid_type)
+    nxg, nyg, nzg = nx + 1, ny + 1, nz + 1
+    nxp, nzp = nx - 2, nz - 2
+    hx = np.zeros(nxg + 1)
+    hx[2:nx + 1] = x[1:] - x[:-1]  # hx[i] = x(i) - x(i-1), Fortran index
+    hy = np.zeros(nyg + 1)
+    hy[2:ny + 1] = y[1:] - y[:-1]
+    hz = np.zeros(nzg + 1)
+    hz[2:nz + 1] = z[1:] - z[:-1]
+    # periodic potentials on the unique index box; A[i,j,k] for i in 0..nxp-1 (x index 2..nxp+1)
+    n = nxp * ny * nzp
+    a = normal(seed, 3 * n)
+    Ax = a[:n].reshape((nxp, ny, nzp), order="F").copy()
+    Ay = a[n:2 * n].reshape((nxp, ny, nzp), order="F").copy()
+    Az = a[2 * n:].reshape((nxp, ny, nzp), order="F").copy()
+    Ax[:, 0, :] = Ax[:, -1, :] = 0.0  # y-face components vanish on the walls
+    Az[:, 0, :] = Az[:, -1, :] = 0.0
+
+    def px(A, s):  # periodic shift in x: value at index i-s
+        return np.roll(A, s, axis=0)
+
+    def pz(A, s):
+        return np.roll(A, s, axis=2)
+
+    # unique faces/centres: x index i = 2..nxp+1 -> 0..nxp-1; z likewise;
+    # y: faces j = 1..ny -> 0..ny-1, centres j = 2..ny -> use face pairs
+    hxu = hx[2:nxp + 2][:, None, None]
+    hzu = hz[2:nzp + 2][None, None, :]
+    hyc = hy[2:ny + 1][None, :, None]  # spacing of y centre j = 2..ny
+    # U at (x-face i, y-centre j, z-centre k): (Az(j) - Az(j-1))/hy_j - (Ay(k) - Ay(k-1))/hz_k
+    Up = (Az[:, 1:, :] - Az[:, :-1, :]) / hyc - (Ay[:, 1:, :] - pz(Ay, 1)[:, 1:, :]) / hzu
+    # V at (x-centre i, y-face j, z-centre k): (Ax(k) - Ax(k-1))/hz_k - (Az(i) - Az(i-1))/hx_i
+    Vp = (Ax - pz(Ax, 1)) / hzu - (Az - px(Az, 1)) / hxu
+    # W at (x-centre i, y-centre j, z-face k): (Ay(i) - Ay(i-1))/hx_i - (Ax(j) - Ax(j-1))/hy_j
+    Wp = (Ay[:, 1:, :] - px(Ay, 1)[:, 1:, :]) / hxu - (Ax[:, 1:, :] - Ax[:, :-1, :]) / hyc
+    rms = np.sqrt((np.mean(Up ** 2) + np.mean(Vp[:, 1:-1, :] ** 2) + np.mean(Wp ** 2)) / 3.0)
+    s = pert * Ub / rms
+    U = np.zeros((nx, nyg, nzg), order="F")
+    V = np.zeros((nxg, ny, nzg), order="F")
+    W = np.zeros((nxg, nyg, nz), order="F")
+    ym = 0.5 * (y[:-1] + y[1:])  # centres yg(2..ny)
+    lam = 1.5 * Ub * ym * (Ly - ym) / (0.25 * Ly * Ly)  # parabola with bulk Ub
+    U[1:nxp + 1, 1:ny, 1:nzp + 1] = lam[None, :, None] + s * Up
+    V[1:nxp + 1, :, 1:nzp + 1] = s * Vp
+    W[1:nxp + 1, 1:ny, 1:nzp + 1] = s * Wp
+    return U, V, W
+
+
 def spectral_faces(nx, ny, nz, seed=3, slope=-11.0 / 6.0):
     """Random-phase fields with |u_hat(k)| ~ |k|^slope on the index box
     (nxp, nym, nzp) (slope -11/6 gives an E(k) ~ k^-5/3 shell spectrum),

@@ -96,6 +96,50 @@ Module c3d_bindings
        Real(c_double), Intent(Out) :: max_div
      End Function c3d_check_divergence
 
+     Integer(c_int) Function c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz) &
+          Bind(C, name="c3d_set_pressure_gradient")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: dPdx, dPdy, dPdz
+     End Function c3d_set_pressure_gradient
+
+     Integer(c_int) Function c3d_save_state(ctx) Bind(C, name="c3d_save_state")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+     End Function c3d_save_state
+
+     Integer(c_int) Function c3d_compute_rhs(ctx, slot) Bind(C, name="c3d_compute_rhs")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: slot
+     End Function c3d_compute_rhs
+
+     Integer(c_int) Function c3d_rk3_update(ctx, rk_step, dt) Bind(C, name="c3d_rk3_update")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: rk_step
+       Real(c_double), Value :: dt
+     End Function c3d_rk3_update
+
+     Integer(c_int) Function c3d_compute_dt(ctx, cfl, dt) Bind(C, name="c3d_compute_dt")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: cfl
+       Real(c_double), Intent(Out) :: dt
+     End Function c3d_compute_dt
+
+     Integer(c_int) Function c3d_mean_mass_flow_u(ctx, q) Bind(C, name="c3d_mean_mass_flow_u")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: q
+     End Function c3d_mean_mass_flow_u
+
+     Integer(c_int) Function c3d_add_to_u(ctx, d) Bind(C, name="c3d_add_to_u")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: d
+     End Function c3d_add_to_u
+
   End Interface
 
 Contains

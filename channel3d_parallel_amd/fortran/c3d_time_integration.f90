@@ -1,0 +1,98 @@
+!------------------------------------------------------------------------------!
+! RK3 host loop over the gfx950 library, keeping the reference's module and   !
+! procedure names and call order:                                            !
+!   equations        :: compute_rhs  (src/equations.f90, u, v and w at once) !
+!   mass_flow        :: compute_mean_mass_flow_U,                            !
+!                       compute_dPx_for_constant_mass_flow (src/mass_flow.f90)!
+!   time_integration :: compute_time_step_RK3, compute_dt                    !
+!                       (src/time_integration.f90:86-267)                    !
+! The velocity stays resident on the GPU; dPdx lives in c3d_flow and is      !
+! passed to the library before every right-hand side.                       !
+!------------------------------------------------------------------------------!
+Module c3d_flow
+  Use iso_fortran_env, Only : Int32, Int64
+  Implicit None
+  Real(Int64)    :: t = 0d0
+  Real(Int64)    :: CFL = 0.5d0
+  Real(Int64)    :: dPdx = 0d0, dPdy = 0d0, dPdz = 0d0, dPdx_prev = 0d0
+  Real(Int64)    :: Qflow_x_0 = 0d0
+  Integer(Int32) :: x_mass_cte = 0
+  ! Wray RK3 stage times (src/initialization.f90:408-410)
+  Real(Int64), Parameter :: rk_t(3) = [8d0/15d0, 2d0/3d0, 1d0]
+End Module c3d_flow
+
+!------------------------------------------------------------------------------!
+Module equations
+  Use c3d_global
+  Use c3d_flow
+  Implicit None
+Contains
+  ! compute_rhs_u/v/w(U,V,W,F<slot>) of the resident velocity
+  ! (src/equations.f90:24-423).
+  Subroutine compute_rhs(slot)
+    Integer(Int32), Intent(In) :: slot
+    Call c3d_check(c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz), 'set_pressure_gradient')
+    Call c3d_check(c3d_compute_rhs(ctx, Int(slot, c_int)), 'compute_rhs')
+  End Subroutine compute_rhs
+End Module equations
+
+!------------------------------------------------------------------------------!
+Module mass_flow
+  Use c3d_global
+  Use c3d_flow
+  Implicit None
+Contains
+  ! compute_mean_mass_flow_U(U, Qflow), src/mass_flow.f90:24-53
+  Subroutine compute_mean_mass_flow_U(Qflow)
+    Real(Int64), Intent(Out) :: Qflow
+    Call c3d_check(c3d_mean_mass_flow_u(ctx, Qflow), 'compute_mean_mass_flow_U')
+  End Subroutine compute_mean_mass_flow_U
+
+  ! compute_dPx_for_constant_mass_flow(U, dPdx), src/mass_flow.f90:144-175
+  Subroutine compute_dPx_for_constant_mass_flow(dPdx_out)
+    Real(Int64), Intent(Out) :: dPdx_out
+    Real(Int64) :: q
+    Call c3d_check(c3d_mean_mass_flow_u(ctx, q), 'compute_dPx_for_constant_mass_flow')
+    dPdx_out = Qflow_x_0 - q
+  End Subroutine compute_dPx_for_constant_mass_flow
+End Module mass_flow
+
+!------------------------------------------------------------------------------!
+Module time_integration
+  Use c3d_global
+  Use c3d_flow
+  Use equations
+  Use mass_flow
+  Use boundary_conditions
+  Use projection
+  Implicit None
+Contains
+  ! compute_time_step_RK3, src/time_integration.f90:86-213 (no body)
+  Subroutine compute_time_step_RK3
+    Real(Int64) :: to
+    Integer(Int32) :: stage
+    to = t
+    Call c3d_check(c3d_save_state(ctx), 'save_state')                   ! Uo = U
+    Do stage = 1, 3
+       rk_step = stage
+       Call compute_rhs(stage)                                         ! F_stage
+       Call c3d_check(c3d_rk3_update(ctx, Int(stage, c_int), dt), 'rk3_update')
+       t = to + rk_t(stage)*dt
+       Call apply_boundary_conditions_resident
+       Call compute_non_IB_projection
+       Call apply_boundary_conditions_resident
+    End Do
+    If (x_mass_cte == 1) Then                                          ! :187-193
+       dPdx_prev = dPdx
+       Call compute_dPx_for_constant_mass_flow(dPdx)
+       Call c3d_check(c3d_add_to_u(ctx, dPdx), 'add_to_u')
+       dPdx = 0.5d0*(dPdx_prev + dPdx/dt)
+       Call apply_boundary_conditions_resident
+    End If
+  End Subroutine compute_time_step_RK3
+
+  ! compute_dt, src/time_integration.f90:219-267
+  Subroutine compute_dt
+    Call c3d_check(c3d_compute_dt(ctx, CFL, dt), 'compute_dt')
+  End Subroutine compute_dt
+End Module time_integration

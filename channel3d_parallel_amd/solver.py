@@ -134,7 +134,56 @@ class Channel:
         self._chk(self.L.c3d_check_divergence(self.h, C.byref(v)))
         return v.value
 
+    # RK3 time integration (src/time_integration.f90, src/equations.f90) -------
+    def set_pressure_gradient(self, dPdx=0.0, dPdy=0.0, dPdz=0.0):
+        self._chk(self.L.c3d_set_pressure_gradient(self.h, dPdx, dPdy, dPdz))
+
+    def get_pressure_gradient(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.c3d_get_pressure_gradient(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def save_state(self):
+        self._chk(self.L.c3d_save_state(self.h))
+
+    def compute_rhs(self, slot=1):
+        """compute_rhs_u/v/w of the resident velocity into stage slot 1..3."""
+        self._chk(self.L.c3d_compute_rhs(self.h, slot))
+
+    def get_rhs(self, slot=1):
+        fu = np.zeros((self.nx - 2, self.nyg - 2, self.nzg - 2), order="F")
+        fv = np.zeros((self.nxg - 2, self.ny - 2, self.nzg - 2), order="F")
+        fw = np.zeros((self.nxg - 2, self.nyg - 2, self.nz - 2), order="F")
+        self._chk(self.L.c3d_get_rhs(self.h, slot, ptr(fu), ptr(fv), ptr(fw)))
+        return fu, fv, fw
+
+    def rk3_update(self, rk_step, dt):
+        self._chk(self.L.c3d_rk3_update(self.h, rk_step, dt))
+
+    def compute_dt(self, cfl):
+        v = C.c_double()
+        self._chk(self.L.c3d_compute_dt(self.h, cfl, C.byref(v)))
+        return v.value
+
+    def mean_mass_flow_u(self):
+        v = C.c_double()
+        self._chk(self.L.c3d_mean_mass_flow_u(self.h, C.byref(v)))
+        return v.value
+
+    def add_to_u(self, d):
+        self._chk(self.L.c3d_add_to_u(self.h, d))
+
+    def compute_time_step_RK3(self, dt, x_mass_cte=0, q0=0.0):
+        """One compute_time_step_RK3 (src/time_integration.f90:86-193)."""
+        self._chk(self.L.c3d_time_step_rk3(self.h, dt, x_mass_cte, q0))
+
     # measurement ---------------------------------------------------------------
+    def copy_bandwidth(self, nbytes, iters=20):
+        """Average ms of a device streaming copy of nbytes (calibration)."""
+        ms = C.c_double()
+        self._chk(self.L.c3d_copy_bandwidth(self.h, nbytes, iters, C.byref(ms)))
+        return ms.value
+
     def time_projection(self, iters, dt=1.0):
         ns = self.L.c3d_stage_count()
         tot = C.c_double()

@@ -24,6 +24,8 @@
 #ifndef C3D_H
 #define C3D_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -101,6 +103,45 @@ int c3d_compute_non_ib_projection(c3d_ctx *ctx, double dt, int rk_step);
  * (src/monitor.f90:210-221). */
 int c3d_check_divergence(c3d_ctx *ctx, double *max_div);
 
+/* ---- RK3 time integration ------------------------------------------------ */
+
+/* Mean pressure gradients dPdx, dPdy, dPdz used by the right-hand sides
+ * (module variables of src/global.f90:176 in the reference). */
+int c3d_set_pressure_gradient(c3d_ctx *ctx, double dPdx, double dPdy, double dPdz);
+int c3d_get_pressure_gradient(const c3d_ctx *ctx, double *dPdx, double *dPdy, double *dPdz);
+
+/* Uo = U, Vo = V, Wo = W (src/time_integration.f90:92-94). */
+int c3d_save_state(c3d_ctx *ctx);
+
+/* compute_rhs_u/v/w(U,V,W, F) of the resident velocity into the stage slot
+ * (1..3) Fu<slot>, Fv<slot>, Fw<slot> (src/equations.f90:24-423). */
+int c3d_compute_rhs(c3d_ctx *ctx, int slot);
+/* Copies stage slot right-hand sides to host arrays with the interior
+ * shapes (nx-2,nyg-2,nzg-2), (nxg-2,ny-2,nzg-2), (nxg-2,nyg-2,nz-2). */
+int c3d_get_rhs(c3d_ctx *ctx, int slot, double *Fu, double *Fv, double *Fw);
+
+/* Stage update U(int) = Uo(int) + dt*sum_l rk_coef(rk_step,l)*F_l with the
+ * Wray coefficients (src/time_integration.f90:105-107,134-136,163-168). */
+int c3d_rk3_update(c3d_ctx *ctx, int rk_step, double dt);
+
+/* compute_dt (src/time_integration.f90:219-267), including the reference's
+ * maximum of dx/|u| and the 0.1 cap; cfl < 0 gives dt = -cfl. */
+int c3d_compute_dt(c3d_ctx *ctx, double cfl, double *dt);
+
+/* compute_mean_mass_flow_U: H-masked trapezoid bulk velocity of the
+ * resident U (src/mass_flow.f90:24-53). */
+int c3d_mean_mass_flow_u(c3d_ctx *ctx, double *qflow);
+
+/* U(2:nx-1,2:nyg-1,2:nzg-1) += d (src/time_integration.f90:190). */
+int c3d_add_to_u(c3d_ctx *ctx, double d);
+
+/* One compute_time_step_RK3 (src/time_integration.f90:86-193) on the
+ * resident state: three stages of right-hand sides, stage update,
+ * boundary conditions, non-IB projection, boundary conditions; then, if
+ * x_mass_cte == 1, the constant-flow correction toward q0 (Qflow_x_0),
+ * which also updates dPdx. */
+int c3d_time_step_rk3(c3d_ctx *ctx, double dt, int x_mass_cte, double q0);
+
 /* Timing of `iters` back-to-back resident projections with HIP events on
  * the context stream.  total_ms: average ms per projection;
  * kernel_ms[0..nk): average ms per kernel stage (order from
@@ -109,6 +150,10 @@ int c3d_check_divergence(c3d_ctx *ctx, double *max_div);
 int c3d_time_projection(c3d_ctx *ctx, int iters, double dt, double *total_ms, double *kernel_ms,
                         int nk);
 int c3d_stage_count(void);
+/* Streaming device-to-device copy of nbytes (multiple of 16), `iters`
+ * times; *ms = average ms per copy.  No reference counterpart: calibrates
+ * achievable HBM bandwidth and the traffic counters. */
+int c3d_copy_bandwidth(c3d_ctx *ctx, size_t nbytes, int iters, double *ms);
 const char *c3d_stage_name(int i);
 
 #ifdef __cplusplus

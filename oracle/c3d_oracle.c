@@ -11,6 +11,7 @@
  * c3d_oracle.h.
  */
 #include "c3d_oracle.h"
+#include "c3d_oracle_impl.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -57,13 +58,7 @@ static inline double cabs1(cplx a) { return fabs(a.re) + fabs(a.im); }
 /* ------------------------------------------------------------------------ */
 /* grid                                                                     */
 /* ------------------------------------------------------------------------ */
-struct orc_grid {
-  int nx, ny, nz, nxg, nyg, nzg, nxp, nzp, nym;
-  double *x, *y, *z, *xg, *yg, *zg;
-  double *kxx, *kzz;
-  double *dyy_d, *dyy_l, *dyy_u; /* D(j), DL(j)=Dyy(j+1,j), DU(j)=Dyy(j,j+1) */
-  double dx, dz, dxmin, dymin, dzmin, Lx, Ly, Lz, Lxp, Lzp;
-};
+
 
 static double min_diff(const double *a, int n) {
   double m = a[1] - a[0];

@@ -76,6 +76,30 @@ void orc_projection(const orc_grid *g, double *U, double *V, double *W,
 double orc_max_abs_div(const orc_grid *g, const double *U, const double *V,
                        const double *W);
 
+/* ---- RK3 advection-diffusion (c3d_oracle_rk.c) ---- */
+/* compute_rhs_u/v/w (src/equations.f90): F in the interior bounds
+ * (2:nx-1,2:nyg-1,2:nzg-1) / (2:nxg-1,2:ny-1,2:nzg-1) / (2:nxg-1,2:nyg-1,2:nz-1).
+ * H (interior Heaviside mask, field shape) may be NULL for H = 1. */
+void orc_rhs_u(const orc_grid *g, const double *U, const double *V, const double *W,
+               const double *Hu, double nu, double dPdx, double *F);
+void orc_rhs_v(const orc_grid *g, const double *U, const double *V, const double *W,
+               const double *Hv, double nu, double dPdy, double *F);
+void orc_rhs_w(const orc_grid *g, const double *U, const double *V, const double *W,
+               const double *Hw, double nu, double dPdz, double *F);
+/* RK3 stage update of one component (which 0 U, 1 V, 2 W), stage 1..3. */
+void orc_rk_update(const orc_grid *g, int which, int stage, double dt, const double *F1,
+                   const double *F2, const double *F3, const double *Xo, double *Xn);
+/* compute_dt (src/time_integration.f90:219-267). */
+double orc_compute_dt(const orc_grid *g, const double *U, const double *V, const double *W,
+                      double cfl, double nu);
+/* H-masked bulk velocity <U> (src/mass_flow.f90:24-53); Hu may be NULL. */
+double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu);
+/* One compute_time_step_RK3 without a body (src/time_integration.f90:86-193);
+ * q0 = NAN disables the constant-flow correction. */
+void orc_time_step_rk3(const orc_grid *g, double *U, double *V, double *W, double *P,
+                       double *Pint, double dt, double nu, double *dPdx, double q0,
+                       int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,15 @@ def lib():
                                      C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_max_abs_div.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_max_abs_div.restype = C.c_double
+        for f in (L.orc_rhs_u, L.orc_rhs_v, L.orc_rhs_w):
+            f.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, _dp]
+        L.orc_rk_update.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.orc_compute_dt.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.orc_compute_dt.restype = C.c_double
+        L.orc_mean_mass_flow_u.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_mean_mass_flow_u.restype = C.c_double
+        L.orc_time_step_rk3.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                        C.c_double, _dp, C.c_double, C.c_int]
         _lib = L
     return _lib
 
@@ -168,3 +177,43 @@ class Grid:
 
     def max_abs_div(self, U, V, W) -> float:
         return lib().orc_max_abs_div(self.h, _p(U), _p(V), _p(W))
+
+    # RK3 (c3d_oracle_rk.c) ---------------------------------------------------
+    @property
+    def shape_fu(self):
+        return (self.nx - 2, self.nyg - 2, self.nzg - 2)
+
+    @property
+    def shape_fv(self):
+        return (self.nxg - 2, self.ny - 2, self.nzg - 2)
+
+    @property
+    def shape_fw(self):
+        return (self.nxg - 2, self.nyg - 2, self.nz - 2)
+
+    def rhs(self, U, V, W, nu, dPdx=0.0, dPdy=0.0, dPdz=0.0, H=(None, None, None)):
+        """compute_rhs_u/v/w (src/equations.f90) -> (Fu, Fv, Fw)."""
+        fu, fv, fw = (np.zeros(s, order="F") for s in (self.shape_fu, self.shape_fv, self.shape_fw))
+        lib().orc_rhs_u(self.h, _p(U), _p(V), _p(W), _p(H[0]), nu, dPdx, _p(fu))
+        lib().orc_rhs_v(self.h, _p(U), _p(V), _p(W), _p(H[1]), nu, dPdy, _p(fv))
+        lib().orc_rhs_w(self.h, _p(U), _p(V), _p(W), _p(H[2]), nu, dPdz, _p(fw))
+        return fu, fv, fw
+
+    def rk_update(self, which, stage, dt, F1, F2, F3, Xo, X):
+        z = F1
+        lib().orc_rk_update(self.h, which, stage, dt, _p(F1), _p(F2 if F2 is not None else z),
+                            _p(F3 if F3 is not None else z), _p(Xo), _p(X))
+
+    def compute_dt(self, U, V, W, cfl, nu) -> float:
+        return lib().orc_compute_dt(self.h, _p(U), _p(V), _p(W), cfl, nu)
+
+    def mean_mass_flow_u(self, U, Hu=None) -> float:
+        return lib().orc_mean_mass_flow_u(self.h, _p(U), _p(Hu))
+
+    def time_step_rk3(self, U, V, W, P, dt, nu, dPdx, q0=float("nan"), nthreads=0):
+        """One compute_time_step_RK3 in place; returns (new dPdx, P_interim)."""
+        d = np.array([dPdx], dtype=np.float64)
+        Pint = self.zeros("rhs")
+        lib().orc_time_step_rk3(self.h, _p(U), _p(V), _p(W), _p(P), _p(Pint), dt, nu, _p(d), q0,
+                                nthreads)
+        return float(d[0]), Pint

@@ -15,6 +15,50 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "projection_driver")
 
 
+CHANNEL_FD = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "channel_fd")
+
+
+def test_fortran_rk3_time_loop(tmp_path):
+    """The Fortran time loop (compute_dt + compute_time_step_RK3 with
+    constant mass flow, as app/main.f90 runs it) on config 1 for 5 steps."""
+    if not os.path.exists(CHANNEL_FD):
+        pytest.fail("Fortran channel_fd not built (make -C channel3d_parallel_amd/fortran)")
+    nx, ny, nz, nsteps = 34, 65, 34, 5
+    nu, cfl = 1.0 / 2800.0, 0.5
+    dpdx0 = (180 * nu) ** 2
+    print(f"\n[case] fortran channel_fd {nx}x{ny}x{nz} steps={nsteps}", flush=True)
+    U, V, W = fields.channel_ic(nx, ny, nz, seed=1)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
+    r = subprocess.run([CHANNEL_FD, str(nx), str(ny), str(nz), "1", "2.0", repr(nu), repr(cfl),
+                        repr(dpdx0), str(nsteps), "1", str(fin), str(fout)],
+                       check=True, timeout=120, capture_output=True, text=True)
+    print(r.stdout)
+    raw = np.fromfile(fout)
+    g = orc.Grid(nx, ny, nz)
+    outs, off = [], 0
+    for s in (g.shape_u, g.shape_v, g.shape_w, g.shape_p):
+        n = int(np.prod(s))
+        outs.append(raw[off:off + n].reshape(s, order="F"))
+        off += n
+    dpdx_gpu, t_gpu, div_gpu = raw[off:off + 3]
+    g.apply_bc(U, V, W)
+    q0 = g.mean_mass_flow_u(U)
+    P = g.zeros("p")
+    dpdx, t = dpdx0, 0.0
+    for _ in range(nsteps):
+        dt = g.compute_dt(U, V, W, cfl, nu)
+        dpdx, _ = g.time_step_rk3(U, V, W, P, dt, nu, dpdx, q0)
+        t += dt
+    for name, a, b in (("U", outs[0], U), ("V", outs[1], V), ("W", outs[2], W), ("P", outs[3], P)):
+        rr = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"  {name}: rel L2 = {rr:.3e}")
+        assert rr <= 1e-11, name
+    assert abs(dpdx_gpu - dpdx) <= 1e-9 * abs(dpdx)
+    assert abs(t_gpu - t) <= 1e-14 * t
+    assert div_gpu <= 1e-10
+
+
 @pytest.mark.parametrize("nx,ny,nz,gt,rk", [(34, 65, 34, 1, 1), (66, 33, 34, 0, 2)])
 def test_fortran_projection_driver(tmp_path, nx, ny, nz, gt, rk):
     if not os.path.exists(DRIVER):

@@ -1,0 +1,126 @@
+"""GPU parity of the RK3 row (stencils, stage update, compute_dt, mass flow,
+full time step) against the oracle.
+
+Tolerances: right-hand sides and stage updates are bitwise (same operations
+in the same order, FMA contraction off); the bulk-flow reduction sums in a
+different order (relative 1e-14); full time steps include the projection
+(relative L2 1e-12 after one step, 1e-10 after 100 steps of config 1).
+"""
+import numpy as np
+import pytest
+
+from channel3d_parallel_amd import Channel, fields
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+NU = 1.0 / 2800.0
+DPDX = (180.0 * NU) ** 2  # u_tau^2 for Re_tau = 180
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def state(nx, ny, nz, gt=1, seed=5, kind="random"):
+    print(f"\n[case] {nx}x{ny}x{nz} gt={gt} seed={seed} kind={kind}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=gt)
+    if kind == "random":
+        U, V, W = fields.white_faces(nx, ny, nz, seed)
+    else:
+        U, V, W = fields.channel_ic(nx, ny, nz, seed=seed, grid_type=gt)
+    g.apply_bc(U, V, W)
+    return g, U, V, W
+
+
+@pytest.mark.parametrize("nx,ny,nz,gt", [(34, 33, 18, 1), (18, 17, 34, 0)])
+def test_rhs_bitwise(nx, ny, nz, gt):
+    g, U, V, W = state(nx, ny, nz, gt)
+    with Channel(nx, ny, nz, grid_type=gt, nu=NU) as ch:
+        ch.set_fields(U, V, W)
+        ch.set_pressure_gradient(DPDX, 0.0, 0.0)
+        ch.compute_rhs(2)
+        gu, gv, gw = ch.get_rhs(2)
+    ru, rv, rw = g.rhs(U, V, W, NU, DPDX, 0.0, 0.0)
+    assert np.array_equal(gu, ru) and np.array_equal(gv, rv) and np.array_equal(gw, rw)
+
+
+def test_rk3_update_bitwise():
+    nx, ny, nz = 34, 33, 18
+    g, U, V, W = state(nx, ny, nz)
+    dt = 0.0123
+    with Channel(nx, ny, nz, nu=NU) as ch:
+        ch.set_fields(U, V, W)
+        ch.save_state()
+        ch.compute_rhs(1)
+        ch.rk3_update(1, dt)
+        ch.compute_rhs(2)
+        ch.rk3_update(2, dt)
+        ch.compute_rhs(3)
+        ch.rk3_update(3, dt)
+        out = ch.get_fields()
+    F = [g.rhs(U, V, W, NU)]
+    Us = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F")]
+    cur = [a.copy(order="F") for a in Us]
+    for st in (1, 2, 3):
+        if st > 1:
+            F.append(g.rhs(*cur, NU))
+        for q in range(3):
+            g.rk_update(q, st, dt, F[0][q], F[1][q] if st > 1 else None,
+                        F[2][q] if st > 2 else None, Us[q], cur[q])
+    assert np.array_equal(out["U"], cur[0]) and np.array_equal(out["V"], cur[1])
+    assert np.array_equal(out["W"], cur[2])
+
+
+def test_compute_dt_and_mass_flow():
+    nx, ny, nz = 34, 65, 34
+    g, U, V, W = state(nx, ny, nz, kind="ic", seed=1)
+    with Channel(nx, ny, nz, nu=NU) as ch:
+        ch.set_fields(U, V, W)
+        for cfl in (0.5, -0.002):
+            assert ch.compute_dt(cfl) == g.compute_dt(U, V, W, cfl, NU)
+        q = ch.mean_mass_flow_u()
+        # ~1e5-term sum in a different order than the oracle's sequential loop
+        assert abs(q - g.mean_mass_flow_u(U)) <= 1e-13 * abs(q)
+        ch.add_to_u(0.25)
+        U2 = ch.get_fields()["U"]
+    ref = U.copy(order="F")
+    ref[1:-1, 1:-1, 1:-1] += 0.25
+    assert np.array_equal(U2, ref)
+
+
+@pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (100, 1e-10)])
+def test_channel_config1_time_steps(nsteps, tol):
+    """Config 1: 32x64x32 cells (34x65x34 faces), tanh alpha 2, Re_b 2800,
+    laminar parabola + 10% divergence-free perturbation, constant flow rate,
+    dt from compute_dt at CFL 0.5."""
+    nx, ny, nz = 34, 65, 34
+    g, U, V, W = state(nx, ny, nz, kind="ic", seed=1)
+    q0 = g.mean_mass_flow_u(U)
+    Ur, Vr, Wr = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    Pr = g.zeros("p")
+    dpdx_r = DPDX
+    with Channel(nx, ny, nz, nu=NU) as ch:
+        ch.set_fields(U, V, W, P=np.zeros(g.shape_p, order="F"))
+        ch.set_pressure_gradient(DPDX, 0.0, 0.0)
+        for n in range(nsteps):
+            dt = ch.compute_dt(0.5)
+            dt_r = g.compute_dt(Ur, Vr, Wr, 0.5, NU)
+            assert abs(dt - dt_r) <= 1e-15 * dt_r
+            ch.compute_time_step_RK3(dt, x_mass_cte=1, q0=q0)
+            dpdx_r, _ = g.time_step_rk3(Ur, Vr, Wr, Pr, dt_r, NU, dpdx_r, q0)
+        out = ch.get_fields()
+        dpdx = ch.get_pressure_gradient()[0]
+        div = ch.check_divergence()
+        q = ch.mean_mass_flow_u()
+    for name, a, b in (("U", out["U"], Ur), ("V", out["V"], Vr), ("W", out["W"], Wr),
+                       ("P", out["P"], Pr)):
+        r = rel(a, b)
+        print(f"  after {nsteps} steps {name}: rel L2 = {r:.3e}")
+        assert r <= tol, name
+    print(f"  dPdx gpu={dpdx:.15e} oracle={dpdx_r:.15e}; max|div|={div:.2e}; Q={q:.15f} Q0={q0:.15f}")
+    # dPdx = (dPdx_prev + (Q0 - <U>)/dt)/2: the bulk-flow round-off of the
+    # reduction (~1e-14) is amplified by 1/dt, so the bound is absolute
+    assert abs(dpdx - dpdx_r) <= 1e-11
+    assert div <= 1e-10
+    assert abs(q - q0) <= 1e-12

@@ -1,0 +1,135 @@
+"""Pins the oracle's RK3 row (CPU): right-hand sides against the reference's
+array-pass formulation and closed forms, the RK3 coefficients against the
+third-order amplification polynomial, compute_dt and the mass-flow rule."""
+import numpy as np
+import pytest
+
+from channel3d_parallel_amd import fields
+from oracle import orc
+from tests import refimpl
+
+RNG = np.random.default_rng(7)
+NU = 1.0 / 2800.0
+
+
+def rand_state(g):
+    U = np.asfortranarray(RNG.standard_normal(g.shape_u))
+    V = np.asfortranarray(RNG.standard_normal(g.shape_v))
+    W = np.asfortranarray(RNG.standard_normal(g.shape_w))
+    g.apply_bc(U, V, W)
+    return U, V, W
+
+
+@pytest.mark.parametrize("nx,ny,nz,gt", [(10, 9, 12, 1), (12, 11, 8, 0)])
+def test_rhs_equals_array_pass_formulation(nx, ny, nz, gt):
+    """Pointwise oracle == the reference's term_1/term_2 array passes (bitwise)."""
+    g = orc.Grid(nx, ny, nz, grid_type=gt)
+    U, V, W = rand_state(g)
+    fu, fv, fw = g.rhs(U, V, W, NU, dPdx=0.3, dPdy=0.2, dPdz=0.1)
+    geo = [g.get(w) for w in range(6)]
+    ru = refimpl.rhs_arraypass("u", U, V, W, *geo, NU, 0.3)
+    rv = refimpl.rhs_arraypass("v", U, V, W, *geo, NU, 0.2)
+    rw = refimpl.rhs_arraypass("w", U, V, W, *geo, NU, 0.1)
+    assert np.array_equal(fu, ru)
+    assert np.array_equal(fv, rv)
+    assert np.array_equal(fw, rw)
+
+
+def test_rhs_linear_shear_is_pressure_gradient_only():
+    """U linear in y (V = W = 0): no convection, no viscous term."""
+    g = orc.Grid(10, 17, 10)
+    yg = g.get(4)
+    U = np.zeros(g.shape_u, order="F")
+    U[:] = (0.3 * yg - 0.1)[None, :, None]
+    V = np.zeros(g.shape_v, order="F")
+    W = np.zeros(g.shape_w, order="F")
+    fu, fv, fw = g.rhs(U, V, W, NU, dPdx=2.5e-3)
+    assert np.abs(fu - 2.5e-3).max() < 1e-15
+    assert np.abs(fv).max() == 0.0 and np.abs(fw).max() == 0.0
+
+
+def _wave_rhs_error(n):
+    """max error of rhs_u for u = sin(x) cos(z), v = w = 0 on a uniform grid
+    against the exact -(u^2)_x + nu (u_xx + u_zz)."""
+    g = orc.Grid(n + 2, 9, n + 2, Lxp=2 * np.pi, Lzp=2 * np.pi, grid_type=0)
+    x, xg, zg = g.get(0), g.get(3), g.get(5)
+    U = np.zeros(g.shape_u, order="F")
+    U[:] = (np.sin(x)[:, None, None] * np.cos(zg)[None, None, :])
+    V = np.zeros(g.shape_v, order="F")
+    W = np.zeros(g.shape_w, order="F")
+    nu = 0.05
+    fu, _, _ = g.rhs(U, V, W, nu)
+    X, Z = x[1:-1][:, None], zg[1:-1][None, :]
+    exact = -2 * np.sin(X) * np.cos(X) * np.cos(Z) ** 2 - 2 * nu * np.sin(X) * np.cos(Z)
+    return np.abs(fu[:, 3, :] - exact).max()
+
+
+def test_rhs_second_order():
+    e1, e2 = _wave_rhs_error(32), _wave_rhs_error(64)
+    assert 3.6 < e1 / e2 < 4.4
+
+
+def test_rk3_amplification_polynomial():
+    """Stages with F = lam*X reproduce R(z) = 1 + z + z^2/2 + z^3/6 (Wray RK3)."""
+    g = orc.Grid(8, 5, 8)
+    lam, dt = -0.7, 0.3
+    X0 = np.asfortranarray(RNG.standard_normal(g.shape_u))
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+    Fs = []
+    X = X0.copy(order="F")
+    for st in (1, 2, 3):
+        Fs.append(np.asfortranarray(lam * X[inner]))
+        Xn = X0.copy(order="F")
+        g.rk_update(0, st, dt, Fs[0], Fs[1] if st > 1 else None, Fs[2] if st > 2 else None, X0, Xn)
+        X = Xn
+    z = lam * dt
+    R = 1 + z + z * z / 2 + z ** 3 / 6
+    assert np.allclose(X[inner], R * X0[inner], rtol=1e-14, atol=0)
+    assert np.array_equal(X[0], X0[0])  # ghosts untouched
+
+
+def test_compute_dt_closed_form():
+    g = orc.Grid(34, 65, 34)
+    U, V, W = rand_state(g)
+    s = g.scalars
+    cfl = 0.5
+    ref = min(cfl * s["dxmin"] ** 2 / NU, cfl * s["dymin"] ** 2 / NU, cfl * s["dzmin"] ** 2 / NU, 0.1)
+    assert g.compute_dt(U, V, W, cfl, NU) == ref
+    assert g.compute_dt(U, V, W, -0.01, NU) == 0.01
+    # small viscosity: the 0.1 cap
+    assert g.compute_dt(U, V, W, cfl, 1e-9) == 0.1
+
+
+def test_mass_flow_constant_and_correction():
+    g = orc.Grid(18, 17, 10)
+    U = np.full(g.shape_u, 1.25, order="F")
+    assert abs(g.mean_mass_flow_u(U) - 1.25) < 1e-13
+    U, V, W = rand_state(g)
+    q = g.mean_mass_flow_u(U)
+    U2 = U.copy(order="F")
+    U2[1:-1, 1:-1, 1:-1] += 0.5
+    assert abs(g.mean_mass_flow_u(U2) - (q + 0.5)) < 1e-13
+
+
+def test_channel_ic_divergence_free_and_bulk():
+    nx, ny, nz = 34, 65, 34
+    g = orc.Grid(nx, ny, nz)
+    U, V, W = fields.channel_ic(nx, ny, nz, seed=1)
+    g.apply_bc(U, V, W)
+    assert g.max_abs_div(U, V, W) < 1e-12
+    assert abs(g.mean_mass_flow_u(U) - 1.0) < 0.02
+
+
+def test_time_step_rk3_keeps_flow_rate_and_divergence():
+    nx, ny, nz = 18, 33, 18
+    g = orc.Grid(nx, ny, nz)
+    U, V, W = fields.channel_ic(nx, ny, nz, seed=1)
+    g.apply_bc(U, V, W)
+    q0 = g.mean_mass_flow_u(U)
+    P = g.zeros("p")
+    dpdx = (180 * NU) ** 2
+    dt = g.compute_dt(U, V, W, 0.5, NU)
+    for _ in range(3):
+        dpdx, _ = g.time_step_rk3(U, V, W, P, dt, NU, dpdx, q0)
+    assert abs(g.mean_mass_flow_u(U) - q0) < 1e-12
+    assert g.max_abs_div(U, V, W) < 1e-10
