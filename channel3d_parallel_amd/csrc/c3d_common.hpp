@@ -8,8 +8,10 @@
 
 namespace c3d {
 
+// Complex value aligned to its full size, so that LDS and global accesses
+// of one element are single 16-byte (FP64) or 8-byte (FP32) operations.
 template <typename real>
-struct cpx {
+struct alignas(2 * sizeof(real)) cpx {
   real x, y;
 };
 
@@ -36,6 +38,22 @@ __device__ __forceinline__ cpx<real> cmul(cpx<real> a, cpx<real> b) {
 template <typename real>
 __device__ __forceinline__ cpx<real> cmulc(cpx<real> a, cpx<real> b) {
   return cmk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// XCD-aware logical workgroup coordinates.  MI355X dispatches workgroups
+// round-robin over its 8 XCDs (hardware id b runs on XCD b % 8), each with
+// its own L2.  The remap gives every XCD one contiguous range of logical
+// ids, so tiles that are neighbours in logical order (and share halo rows)
+// run on the same XCD.  A bijection on [0, n) for any grid size.
+__device__ __forceinline__ void xcd_tile(unsigned& bx, unsigned& by, unsigned& bz) {
+  constexpr unsigned kXcd = 8;
+  const unsigned n = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned q = n / kXcd, r = n % kXcd, x = b % kXcd, slot = b / kXcd;
+  const unsigned lid = x * q + (x < r ? x : r) + slot;
+  bx = lid % gridDim.x;
+  by = (lid / gridDim.x) % gridDim.y;
+  bz = lid / (gridDim.x * gridDim.y);
 }
 
 // Fortran column-major offset of 1-based (i,j,k) in an (n1,n2,*) array.

@@ -64,47 +64,68 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   const int j = jj + 2;
   const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
 
-  // 1) rows -> LDS (x contiguous, coalesced).  All loads of a batch are
-  //    issued before any is used so that many are in flight per lane.
-  constexpr int ITEMS = (TK * N + kBlock - 1) / kBlock;
-  constexpr int BATCH = ITEMS < 4 ? ITEMS : 4;
-  for (int q0 = 0; q0 < ITEMS; q0 += BATCH) {
-    double v[BATCH][6];
+  // 1) rows -> LDS.  Rows are walked with uniform (scalar) row bases; each
+  //    lane owns the x positions n = tid, tid + kBlock, ...  Loads of a
+  //    batch of RB rows are all issued before any is used.
+  constexpr int NPL = (N + kBlock - 1) / kBlock;  // x positions per lane
+  constexpr int RB = TK < 4 ? TK : 4;             // rows per load batch
+  const double ry = g.rdyf[j];
 #pragma unroll
-    for (int q = 0; q < BATCH; q++) {
-      const int e = tid + (q0 + q) * kBlock;
-      if ((TK * N) % kBlock == 0 || e < TK * N) {
-        const int n = e % N, t = e / N;
-        const int i = n + 2, k = k0 + t + 2;
-        if constexpr (FROM_UVW) {
-          v[q][0] = U[fidx(i, j, k, nx, nyg)];
-          v[q][1] = U[fidx(i - 1, j, k, nx, nyg)];
-          v[q][2] = V[fidx(i, j, k, nxg, ny)];
-          v[q][3] = V[fidx(i, j - 1, k, nxg, ny)];
-          v[q][4] = W[fidx(i, j, k, nxg, nyg)];
-          v[q][5] = W[fidx(i, j, k - 1, nxg, nyg)];
-        } else {
-          v[q][0] = rhs[ridx(i, j, k, nxg, nyg)];
+  for (int t0 = 0; t0 < TK; t0 += RB) {
+    double v[RB][NPL][6];
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int k = k0 + t0 + r + 2;
+      if constexpr (FROM_UVW) {
+        const double* Ur = U + fidx(1, j, k, nx, nyg);
+        const double* Vr = V + fidx(1, j, k, nxg, ny);
+        const double* Wr = W + fidx(1, j, k, nxg, nyg);
+        const double* Vm = Vr - nxg;
+        const double* Wm = Wr - (size_t)nxg * nyg;
+#pragma unroll
+        for (int p = 0; p < NPL; p++) {
+          const int n = tid + p * kBlock;
+          if (N % kBlock == 0 || n < N) {
+            // x index i = n + 2 -> offset n + 1 in the row
+            v[r][p][0] = Ur[n + 1];
+            v[r][p][1] = Ur[n];
+            v[r][p][2] = Vr[n + 1];
+            v[r][p][3] = Vm[n + 1];
+            v[r][p][4] = Wr[n + 1];
+            v[r][p][5] = Wm[n + 1];
+          }
+        }
+      } else {
+        const double* Rr = rhs + ridx(2, j, k, nxg, nyg);
+#pragma unroll
+        for (int p = 0; p < NPL; p++) {
+          const int n = tid + p * kBlock;
+          if (N % kBlock == 0 || n < N) v[r][p][0] = Rr[n];
         }
       }
     }
 #pragma unroll
-    for (int q = 0; q < BATCH; q++) {
-      const int e = tid + (q0 + q) * kBlock;
-      if ((TK * N) % kBlock == 0 || e < TK * N) {
-        const int n = e % N, t = e / N;
-        const int i = n + 2, k = k0 + t + 2;
-        double d;
-        if constexpr (FROM_UVW) {
-          // divergence, src/differential_operators.f90:64-72, with the
-          // face-spacing reciprocals tabulated
-          d = (v[q][0] - v[q][1]) * g.rdxf[i] + (v[q][2] - v[q][3]) * g.rdyf[j] +
-              (v[q][4] - v[q][5]) * g.rdzf[k];
-        } else {
-          d = v[q][0];
+    for (int p = 0; p < NPL; p++) {
+      const int n = tid + p * kBlock;
+      if (N % kBlock == 0 || n < N) {
+        const double rx = g.rdxf[n + 2];
+#pragma unroll
+        for (int r = 0; r < RB; r += 2) {  // rows 2q, 2q+1 -> one complex
+          double d[2];
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            if constexpr (FROM_UVW) {
+              // divergence, src/differential_operators.f90:64-72, with the
+              // face-spacing reciprocals tabulated
+              const double rz = g.rdzf[k0 + t0 + r + h + 2];
+              d[h] = (v[r + h][p][0] - v[r + h][p][1]) * rx +
+                     (v[r + h][p][2] - v[r + h][p][3]) * ry + (v[r + h][p][4] - v[r + h][p][5]) * rz;
+            } else {
+              d[h] = v[r + h][p][0];
+            }
+          }
+          buf[((t0 + r) >> 1) * LP + lp(n)] = cmk((real)d[0], (real)d[1]);
         }
-        real* bp = reinterpret_cast<real*>(&buf[(t >> 1) * LP + lp(n)]);
-        bp[t & 1] = (real)d;
       }
     }
   }
@@ -403,27 +424,45 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
   const double cP = dt * (8.0 / 15.0);
   const int idup = nxg - 1;     // x duplicate of i = 2
   const int kdup = nzp + 2;     // z duplicate of k = 2
-  for (int e = tid; e < TK * N; e += kBlock) {
-    const int t = e / N, n = e % N;
-    const cpx<real> z = buf[(t >> 1) * LP + lp(n)];
-    const double phi = (double)((t & 1) ? z.y : z.x) / norm;
-    const double o = (MODE == 1) ? phi / dt : phi;
-    const int i = n + 2, k = k0 + t + 2;
-    out[ridx(i, j, k, nxg, nyg)] = o;
-    if (rk1) P[fidx(i, j, k, nxg, nyg)] = phi / cP;
-    if (n == 0) {
-      out[ridx(idup, j, k, nxg, nyg)] = o;
-      if (rk1) P[fidx(idup, j, k, nxg, nyg)] = phi / cP;
-    }
-    if (k0 == 0 && t == 0) {
-      out[ridx(i, j, kdup, nxg, nyg)] = o;
-      if (rk1) P[fidx(i, j, kdup, nxg, nyg)] = phi / cP;
-      if (n == 0) {
-        out[ridx(idup, j, kdup, nxg, nyg)] = o;
-        if (rk1) P[fidx(idup, j, kdup, nxg, nyg)] = phi / cP;
+  // rows walked with uniform row bases; lane owns x positions tid + p*kBlock
+  constexpr int NPL = (N + kBlock - 1) / kBlock;
+  cpx<real> zpair[NPL];
+#pragma unroll
+  for (int t = 0; t < TK; t++) {
+    const int k = k0 + t + 2;
+    double* orow = out + ridx(2, j, k, nxg, nyg);      // x index 2 at offset 0
+    double* prow = P + fidx(2, j, k, nxg, nyg);
+    const bool dupz = (k0 == 0 && t == 0);             // row k = 2 also fills k = nzp+2
+    double* orow2 = out + ridx(2, j, kdup, nxg, nyg);
+    double* prow2 = P + fidx(2, j, kdup, nxg, nyg);
+#pragma unroll
+    for (int p = 0; p < NPL; p++) {
+      const int n = tid + p * kBlock;
+      if (N % kBlock == 0 || n < N) {
+        // rows 2q and 2q+1 share one complex; the even row reads it
+        cpx<real> z;
+        if ((t & 1) == 0) zpair[p] = buf[(t >> 1) * LP + lp(n)];
+        z = zpair[p];
+        const double phi = (double)((t & 1) ? z.y : z.x) / norm;
+        const double o = (MODE == 1) ? phi / dt : phi;
+        orow[n] = o;
+        if (rk1) prow[n] = phi / cP;
+        if (n == 0) {  // x duplicate i = nxg-1 (offset nxp)
+          orow[N] = o;
+          if (rk1) prow[N] = phi / cP;
+        }
+        if (dupz) {
+          orow2[n] = o;
+          if (rk1) prow2[n] = phi / cP;
+          if (n == 0) {
+            orow2[N] = o;
+            if (rk1) prow2[N] = phi / cP;
+          }
+        }
       }
     }
   }
+  (void)idup;
   if (MODE == 1 && k0 == 0) {
     // rhs_p plane nzg is zero after the divergence; P_interim = 0/dt there.
     for (int i = 2 + tid; i <= nxg - 1; i += kBlock) out[ridx(i, j, g.nzg, nxg, nyg)] = 0.0;
@@ -467,10 +506,13 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
                                                          const double* __restrict__ Pint,
                                                          double dt) {
   // one lane per (i, j), walking kGradKC consecutive z planes so that
-  // p(k+1) of one plane is p(k) of the next
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int kb = blockIdx.z * kGradKC + 2;
+  // p(k+1) of one plane is p(k) of the next; XCD-aware tile order so the
+  // p(j+1) halo row is usually in the same L2
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int kb = bz * kGradKC + 2;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   if (i > nx - 1 || j > nyg - 1) return;
   const double hx = g.dxg[i];
@@ -594,11 +636,13 @@ __global__ __launch_bounds__(kBlock) void k_max_div(DevGrid g, const double* __r
                                                     const double* __restrict__ W,
                                                     unsigned long long* __restrict__ red) {
   __shared__ double sm[kBlock / 64];
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
   double v = 0.0;
   if (i <= g.nxg - 1 && j <= g.nyg - 1)
-    for (int k = blockIdx.z + 2; k <= g.nzg - 1; k += gridDim.z)
+    for (int k = bz + 2; k <= g.nzg - 1; k += gridDim.z)
       v = fmax(v, fabs(div_at(g, U, V, W, i, j, k)));
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
@@ -626,10 +670,19 @@ __global__ __launch_bounds__(kBlock) void k_gradient(DevGrid g, const double* __
 }
 
 // Plain streaming copy (calibration of bandwidth and traffic counters).
+// Each lane moves 4 x 16 bytes per iteration (all loads issued first).
 __global__ __launch_bounds__(kBlock) void k_copy(const double2* __restrict__ src,
                                                  double2* __restrict__ dst, size_t n) {
-  for (size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (size_t)gridDim.x * kBlock)
-    dst[q] = src[q];
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; q + 3 * stride < n; q += 4 * stride) {
+    const double2 a = src[q], b = src[q + stride], c = src[q + 2 * stride], d = src[q + 3 * stride];
+    dst[q] = a;
+    dst[q + stride] = b;
+    dst[q + 2 * stride] = c;
+    dst[q + 3 * stride] = d;
+  }
+  for (; q < n; q += stride) dst[q] = src[q];
 }
 
 // Dispatch a compile-time FFT length.
@@ -762,7 +815,7 @@ hipError_t launch_max_div(const DevGrid& g, const double* U, const double* V, co
 
 hipError_t launch_copy(const void* src, void* dst, size_t nbytes, hipStream_t s) {
   const size_t n = nbytes / sizeof(double2);
-  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(kBlock), 0, s, (const double2*)src, (double2*)dst, n);
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(kBlock), 0, s, (const double2*)src, (double2*)dst, n);
   return hipGetLastError();
 }
 

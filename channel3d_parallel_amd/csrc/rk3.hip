@@ -25,9 +25,11 @@ __global__ __launch_bounds__(kB) void k_rhs_u(DevGrid g, const double* __restric
                                               const double* __restrict__ H, double nu,
                                               double dPdx, double* __restrict__ F) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);  // neighbouring tiles on one XCD (shared halo rows)
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int k = bz + 2;
   if (i > g.nx - 1 || j > g.nyg - 1) return;
   const double uc1 = 0.5 * (U_(i - 1, j, k) + U_(i, j, k));
   const double uc2 = 0.5 * (U_(i, j, k) + U_(i + 1, j, k));
@@ -66,9 +68,11 @@ __global__ __launch_bounds__(kB) void k_rhs_v(DevGrid g, const double* __restric
                                               const double* __restrict__ H, double nu,
                                               double dPdy, double* __restrict__ F) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);  // neighbouring tiles on one XCD (shared halo rows)
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int k = bz + 2;
   if (i > g.nxg - 1 || j > g.ny - 1) return;
   const double vc1 = 0.5 * (V_(i, j - 1, k) + V_(i, j, k));
   const double vc2 = 0.5 * (V_(i, j, k) + V_(i, j + 1, k));
@@ -106,9 +110,11 @@ __global__ __launch_bounds__(kB) void k_rhs_w(DevGrid g, const double* __restric
                                               const double* __restrict__ H, double nu,
                                               double dPdz, double* __restrict__ F) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);  // neighbouring tiles on one XCD (shared halo rows)
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int k = bz + 2;
   if (i > g.nxg - 1 || j > g.nyg - 1) return;
   const double wc1 = 0.5 * (W_(i, j, k - 1) + W_(i, j, k));
   const double wc2 = 0.5 * (W_(i, j, k) + W_(i, j, k + 1));
@@ -149,9 +155,11 @@ __global__ __launch_bounds__(kB) void k_rk_update(int n1, int n2, int i1, int j1
                                                   const double* __restrict__ Xo,
                                                   double* __restrict__ X) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);  // neighbouring tiles on one XCD (shared halo rows)
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int k = bz + 2;
   if (i > i1 || j > j1) return;
   const double c11 = 8.0 / 15.0, c21 = 1.0 / 4.0, c22 = 5.0 / 12.0, c31 = 1.0 / 4.0, c32 = 0.0,
                c33 = 3.0 / 4.0;
@@ -167,9 +175,11 @@ __global__ __launch_bounds__(kB) void k_rk_update(int n1, int n2, int i1, int j1
 
 // U(2:nx-1, 2:nyg-1, 2:nzg-1) += d  (src/time_integration.f90:190)
 __global__ __launch_bounds__(kB) void k_add_u(DevGrid g, double* __restrict__ U, double d) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
-  const int k = blockIdx.z + 2;
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);  // neighbouring tiles on one XCD (shared halo rows)
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int k = bz + 2;
   if (i > g.nx - 1 || j > g.nyg - 1) return;
   const size_t o = fidx(i, j, k, g.nx, g.nyg);
   U[o] = U[o] + d;
