@@ -100,6 +100,52 @@ void orc_time_step_rk3(const orc_grid *g, double *U, double *V, double *W, doubl
                        double *Pint, double dt, double nu, double *dPdx, double q0,
                        int nthreads);
 
+/* ---- Immersed boundary (c3d_oracle_ib.c) ----
+ * Marker arrays: xb, yb, zb, sb [nb]; normals, ub [3nb] component-major
+ * (x block, y block, z block).  Requires a uniform y grid (y_ref_index = 1).
+ * nzb z slabs of nxb markers; the first nxb1 of each slab are body 1 and the
+ * rest body 2 (remove_mean_per_body grouping). */
+typedef struct orc_ib orc_ib;
+/* deltafnc (src/immersed_boundary_operators.f90:270-294). */
+double orc_deltafnc(double x1, double x2, double dr);
+/* setup_IB_operators (src/immersed_boundary_operators.f90:19-256). */
+orc_ib *orc_ib_new(const orc_grid *g, int nb, const double *xb, const double *yb,
+                   const double *zb, const double *normals, const double *sb,
+                   const double *ub, int nzb, int nxb, int nxb1);
+void orc_ib_free(orc_ib *ib);
+/* stencil s (0 u, 1 v, 2 w, 3 centre): 125 entries per marker, marker-major,
+ * Fortran indices of the field (centre: of P). */
+void orc_ib_get_stencil(const orc_ib *ib, int s, int *xi, int *yi, int *zi, double *w,
+                        double *dxn);
+/* E (with_dxn 0) / E1n (with_dxn 1): fields -> out[3nb]. */
+void orc_ib_regT(const orc_ib *ib, const double *U, const double *V, const double *W,
+                 int with_dxn, double *out);
+/* E1n of a centre field in rhs_p shape -> out[nb]. */
+void orc_ib_regTc_1n(const orc_ib *ib, const double *P, double *out);
+/* R of component comp (0 u, 1 v, 2 w): f[nb] -> full field F. */
+void orc_ib_reg(const orc_ib *ib, int comp, const double *f, double *F);
+/* remove_mean_per_body (src/immersed_boundary_geometry.f90:400-442), f[nb]. */
+void orc_ib_remove_mean(const orc_ib *ib, double *f);
+/* compute_heaviside (src/heaviside.f90:20-54); outputs may be NULL. */
+void orc_ib_heaviside(orc_ib *ib, double *Hc_ext, double *Hu_ext, double *Hv_ext,
+                      double *Hw_ext);
+void orc_ib_get_heaviside(const orc_ib *ib, double *E1nHc, double *E1nH, double *Hu_int,
+                          double *Hv_int, double *Hw_int);
+/* schur (src/projection.f90:98-144), f, Sf [4nb]; rk_step != 1. */
+void orc_ib_schur(const orc_ib *ib, const double *f, double *Sf, double dt, double nu,
+                  int nthreads);
+/* bicgstab (src/projection.f90:146-186); x [4nb] in/out; returns iterations. */
+int orc_ib_bicgstab(const orc_ib *ib, double *x, const double *b, double tol, int maxit,
+                    double dt, double nu, int rk_step, double *P, double *err, int nthreads);
+/* compute_IB_projection (src/projection.f90:48-96) after orc_projection. */
+int orc_ib_projection(const orc_ib *ib, double *U, double *V, double *W, const double *Uint,
+                      const double *Vint, const double *Wint, double *Pint, double *P,
+                      double *fb, double dt, double nu, double tol, int maxit, int rk_step,
+                      double *err, int nthreads);
+/* check_slip (src/monitor.f90:223-231). */
+double orc_ib_max_slip(const orc_ib *ib, const double *U, const double *V, const double *W,
+                       const double *fb);
+
 #ifdef __cplusplus
 }
 #endif

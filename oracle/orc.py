@@ -217,3 +217,137 @@ class Grid:
         lib().orc_time_step_rk3(self.h, _p(U), _p(V), _p(W), _p(P), _p(Pint), dt, nu, _p(d), q0,
                                 nthreads)
         return float(d[0]), Pint
+
+
+_ip = C.POINTER(C.c_int)
+
+
+def _ib_sigs(L):
+    if getattr(L, "_ib_ready", False):
+        return
+    L.orc_deltafnc.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.orc_deltafnc.restype = C.c_double
+    L.orc_ib_new.restype = C.c_void_p
+    L.orc_ib_new.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                             C.c_int, C.c_int, C.c_int]
+    L.orc_ib_free.argtypes = [C.c_void_p]
+    L.orc_ib_get_stencil.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]
+    L.orc_ib_regT.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp]
+    L.orc_ib_regTc_1n.argtypes = [C.c_void_p, _dp, _dp]
+    L.orc_ib_reg.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.orc_ib_remove_mean.argtypes = [C.c_void_p, _dp]
+    L.orc_ib_heaviside.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.orc_ib_get_heaviside.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    L.orc_ib_schur.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int]
+    L.orc_ib_bicgstab.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_double,
+                                  C.c_double, C.c_int, _dp, _dp, C.c_int]
+    L.orc_ib_bicgstab.restype = C.c_int
+    L.orc_ib_projection.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                    C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _dp,
+                                    C.c_int]
+    L.orc_ib_projection.restype = C.c_int
+    L.orc_ib_max_slip.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.orc_ib_max_slip.restype = C.c_double
+    L._ib_ready = True
+
+
+def deltafnc(x1, x2, dr) -> float:
+    L = lib()
+    _ib_sigs(L)
+    return L.orc_deltafnc(x1, x2, dr)
+
+
+class IB:
+    """Immersed-boundary operators on a uniform-y oracle grid.
+
+    ``m`` is a marker set with attributes ``xb, yb, zb, normals, sb, ub,
+    nzb, nxb, nxb1`` (see channel3d_parallel_amd.ib_geometry).
+    """
+
+    def __init__(self, grid: Grid, m):
+        L = lib()
+        _ib_sigs(L)
+        self.g, self.m = grid, m
+        self.nb = int(m.xb.size)
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64)
+                      for a in (m.xb, m.yb, m.zb, m.normals, m.sb, m.ub)]
+        self.h = L.orc_ib_new(grid.h, self.nb, *[_p(a) for a in self._keep],
+                              m.nzb, m.nxb, m.nxb1)
+        self.heaviside_done = False
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_ib_free(self.h)
+            self.h = None
+
+    def stencil(self, s):
+        n = self.nb * 125
+        xi, yi, zi = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        w, dxn = np.zeros(n), np.zeros(n)
+        lib().orc_ib_get_stencil(self.h, s, xi.ctypes.data_as(_ip), yi.ctypes.data_as(_ip),
+                                 zi.ctypes.data_as(_ip), _p(w), _p(dxn))
+        sh = (self.nb, 125)
+        return xi.reshape(sh), yi.reshape(sh), zi.reshape(sh), w.reshape(sh), dxn.reshape(sh)
+
+    def E(self, U, V, W, with_dxn=False):
+        out = np.zeros(3 * self.nb)
+        lib().orc_ib_regT(self.h, _p(U), _p(V), _p(W), int(with_dxn), _p(out))
+        return out
+
+    def E1n_c(self, P):
+        out = np.zeros(self.nb)
+        lib().orc_ib_regTc_1n(self.h, _p(P), _p(out))
+        return out
+
+    def R(self, comp, f):
+        F = self.g.zeros("uvw"[comp])
+        lib().orc_ib_reg(self.h, comp, _p(np.ascontiguousarray(f, dtype=np.float64)), _p(F))
+        return F
+
+    def remove_mean(self, f):
+        f = np.array(f, dtype=np.float64)
+        lib().orc_ib_remove_mean(self.h, _p(f))
+        return f
+
+    def heaviside(self):
+        g = self.g
+        Hc, Hu, Hv, Hw = g.zeros("rhs"), g.zeros("u"), g.zeros("v"), g.zeros("w")
+        lib().orc_ib_heaviside(self.h, _p(Hc), _p(Hu), _p(Hv), _p(Hw))
+        self.heaviside_done = True
+        return Hc, Hu, Hv, Hw
+
+    def heaviside_coeffs(self):
+        g = self.g
+        a, b = np.zeros(self.nb), np.zeros(3 * self.nb)
+        Hu, Hv, Hw = g.zeros("u"), g.zeros("v"), g.zeros("w")
+        lib().orc_ib_get_heaviside(self.h, _p(a), _p(b), _p(Hu), _p(Hv), _p(Hw))
+        return dict(E1nHc=a, E1nH=b, Hu_int=Hu, Hv_int=Hv, Hw_int=Hw)
+
+    def schur(self, f, dt, nu, nthreads=0):
+        assert self.heaviside_done
+        Sf = np.zeros(4 * self.nb)
+        lib().orc_ib_schur(self.h, _p(np.ascontiguousarray(f, dtype=np.float64)), _p(Sf),
+                           dt, nu, nthreads)
+        return Sf
+
+    def bicgstab(self, b, x0=None, tol=1e-10, maxit=1000, dt=1.0, nu=1.0, rk_step=2, P=None,
+                 nthreads=0):
+        assert self.heaviside_done
+        x = np.zeros(4 * self.nb) if x0 is None else np.array(x0, dtype=np.float64)
+        err = np.zeros(1)
+        it = lib().orc_ib_bicgstab(self.h, _p(x), _p(np.ascontiguousarray(b, dtype=np.float64)),
+                                   tol, maxit, dt, nu, rk_step, _p(P), _p(err), nthreads)
+        return x, it, float(err[0])
+
+    def projection(self, U, V, W, Uint, Vint, Wint, Pint, fb, dt, nu, tol=1e-10, maxit=1000,
+                   rk_step=2, P=None, nthreads=0):
+        """compute_IB_projection in place on U, V, W and fb; returns (iters, ||r||^2)."""
+        assert self.heaviside_done
+        err = np.zeros(1)
+        it = lib().orc_ib_projection(self.h, _p(U), _p(V), _p(W), _p(Uint), _p(Vint), _p(Wint),
+                                     _p(Pint), _p(P), _p(fb), dt, nu, tol, maxit, rk_step,
+                                     _p(err), nthreads)
+        return it, float(err[0])
+
+    def max_slip(self, U, V, W, fb) -> float:
+        return lib().orc_ib_max_slip(self.h, _p(U), _p(V), _p(W), _p(fb))
