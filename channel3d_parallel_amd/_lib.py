@@ -56,6 +56,24 @@ SIGNATURES = [
     ("c3d_copy_bandwidth", C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _dp]),
     ("c3d_stage_count", C.c_int, []),
     ("c3d_stage_name", C.c_char_p, [C.c_int]),
+    ("c3d_ib_setup", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
+                               C.c_int]),
+    ("c3d_ib_get_stencil", C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]),
+    ("c3d_ib_get_heaviside", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("c3d_ib_interpolate", C.c_int, [C.c_void_p, C.c_int, _dp]),
+    ("c3d_ib_interpolate_c", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("c3d_ib_spread", C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    ("c3d_ib_remove_mean", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_ib_schur", C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_int]),
+    ("c3d_ib_bicgstab", C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_int,
+                                  _ip, _dp]),
+    ("c3d_ib_set_solver", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int]),
+    ("c3d_ib_set_force", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_ib_get_force", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_compute_ib_projection", C.c_int, [C.c_void_p, C.c_double, C.c_int, _ip, _dp]),
+    ("c3d_check_slip", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_ib_stats", C.c_int, [C.c_void_p, _ip, C.POINTER(C.c_longlong), _dp]),
+    ("c3d_ib_set_exterior_pressure_gradient", C.c_int, [C.c_void_p, C.c_int]),
 ]
 
 _lib = None
@@ -87,3 +105,10 @@ def ptr(a):
     if not (a.flags["F_CONTIGUOUS"] or a.ndim == 1):
         raise ValueError("arrays must be Fortran-contiguous")
     return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    """Pointer to an int32 numpy array."""
+    if a.dtype != "int32" or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("contiguous int32 array required")
+    return a.ctypes.data_as(_ip)

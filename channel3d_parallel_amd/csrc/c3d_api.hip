@@ -11,10 +11,34 @@
 
 #include "../../include/c3d.h"
 #include "fft_lds.hpp"
+#include "ib.hpp"
 #include "kernels.hpp"
 
 using c3d::cpx;
 using c3d::DevGrid;
+
+// Immersed-boundary state of a context (allocated by c3d_ib_setup).
+struct IBState {
+  c3d::ib::Markers m{};
+  c3d::ib::Spread sp[3]{};
+  // spread targets: zero except at support nodes and their ghost copies
+  double *Ureg = nullptr, *Vreg = nullptr, *Wreg = nullptr;
+  double* phi = nullptr;     // rhs_p-shaped Poisson solution / dt
+  double* Hc_ext = nullptr;  // rhs_p-shaped exterior Heaviside
+  double *Hu_int = nullptr, *Hv_int = nullptr, *Hw_int = nullptr;
+  double *E1nHc = nullptr, *E1nH = nullptr;
+  double *Uint = nullptr, *Vint = nullptr, *Wint = nullptr;  // U_interim ...
+  double* aux = nullptr;  // 3nb
+  double* vec[8] = {nullptr};  // x r rhat p v h s t  (4nb each)
+  double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr;
+  double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
+  double tol = 1e-10;
+  int maxit = 1000, guess = 0;
+  int last_iters = 0;
+  long long accum_iters = 0;
+  double last_err = 0.0;
+  bool ext_pg = false;
+};
 
 struct c3d_ctx {
   c3d_desc desc{};
@@ -37,8 +61,10 @@ struct c3d_ctx {
   double* F[3][3] = {{nullptr}};
   double* part = nullptr;  // mass-flow row partials, then 2 result slots
   double dPdx = 0.0, dPdy = 0.0, dPdz = 0.0;
-  // interior Heaviside masks (null: no body, H = 1)
-  const double *Hu = nullptr, *Hv = nullptr, *Hw = nullptr;
+  // interior Heaviside masks of the right-hand sides (null: H = 1) and of
+  // the bulk-flow integral
+  const double *Hu = nullptr, *Hv = nullptr, *Hw = nullptr, *Hm = nullptr;
+  IBState* ib = nullptr;
   std::string err;
 };
 
@@ -304,6 +330,7 @@ int c3d_destroy(c3d_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c->ib;
   delete c;
   return C3D_OK;
 }
@@ -427,10 +454,22 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   return 0;
 }
 
+// U_interim = U ... (src/projection.f90:26-28), kept only when a body needs it.
+static int save_interim(c3d_ctx* c) {
+  if (!c->ib) return 0;
+  HIPCHK(c, hipMemcpyAsync(c->ib->Uint, c->U, c->nu * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ib->Vint, c->V, c->nv * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ib->Wint, c->W, c->nw * 8, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step);
+
 int c3d_compute_non_ib_projection(c3d_ctx* c, double dt, int rk_step) {
   if (!c) return C3D_EINVAL;
   if (!(dt != 0.0)) return fail(c, C3D_EINVAL, "dt must be nonzero");
-  int rc = projection_launch(c, dt, rk_step, nullptr);
+  int rc = save_interim(c);
+  if (!rc) rc = projection_launch(c, dt, rk_step, nullptr);
   if (rc) return rc;
   return sync(c);
 }
@@ -527,7 +566,7 @@ static int rk_update_launch(c3d_ctx* c, int rk_step, double dt) {
 
 static int mass_flow(c3d_ctx* c, double* q) {
   double* out = c->part + 2 * (size_t)c3d::massflow_rows(c->g);
-  HIPCHK(c, c3d::launch_massflow(c->g, c->U, c->Hu, c->part, out, c->stream));
+  HIPCHK(c, c3d::launch_massflow(c->g, c->U, c->Hm, c->part, out, c->stream));
   double h[2];
   HIPCHK(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   int rc = sync(c);
@@ -636,7 +675,14 @@ int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
     if (!rc) rc = rk_update_launch(c, st, dt);                  // :105-107
     if (!rc && c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
       rc = fail(c, C3D_EHIP, "boundary conditions");            // :117
+    if (!rc) rc = save_interim(c);
     if (!rc) rc = projection_launch(c, dt, st, nullptr);        // :118
+    if (!rc && c->ib) {                                         // :119-122
+      if (c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
+        rc = fail(c, C3D_EHIP, "boundary conditions");
+      if (!rc) rc = ib_projection_launch(c, dt, st);
+      if (rc == C3D_ENOCONV) rc = 0;  // the reference warns and continues
+    }
     if (!rc && c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
       rc = fail(c, C3D_EHIP, "boundary conditions");            // :123
   }
@@ -692,6 +738,467 @@ int c3d_time_projection(c3d_ctx* c, int iters, double dt, double* total_ms, doub
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return rc;
+}
+
+// ----------------------------------------------------------------------------
+// Immersed boundary (src/immersed_boundary_operators.f90, src/heaviside.f90,
+// src/projection.f90:48-186, src/immersed_boundary_geometry.f90:400-442)
+// ----------------------------------------------------------------------------
+
+// Poisson solve of div(Ureg, Vreg, Wreg) into ib->phi = solution / dt, with
+// the rk_step 1 pressure side effect (src/poisson.f90:91-99).
+static int ib_poisson_reg(c3d_ctx* c, double dt, int rk_step) {
+  IBState* ib = c->ib;
+  const bool rk1 = rk_step == 1;
+  HIPCHK(c, c3d::launch_fwd_x(c->g, true, ib->Ureg, ib->Vreg, ib->Wreg, nullptr, c->S, c->stream));
+  int rc = spectral_core(c);
+  if (rc) return rc;
+  HIPCHK(c, c3d::launch_inv_x(c->g, 1, c->S, ib->phi, c->P, dt, rk1, c->stream));
+  if (rk1) HIPCHK(c, c3d::launch_p_ghosts(c->g, c->P, c->stream));
+  return 0;
+}
+
+// U_reg = R f (src/projection.f90:66-73): aux, three spreads, boundary conditions.
+static int ib_spread_force(c3d_ctx* c, const double* f, double dt) {
+  IBState* ib = c->ib;
+  const int nb = ib->m.nb;
+  HIPCHK(c, c3d::ib::launch_aux(ib->m, f, dt, c->desc.nu, ib->aux, c->stream));
+  double* F[3] = {ib->Ureg, ib->Vreg, ib->Wreg};
+  for (int q = 0; q < 3; q++)
+    HIPCHK(c, c3d::ib::launch_spread(ib->m, ib->sp[q], q, ib->aux + (size_t)q * nb, F[q], c->stream));
+  HIPCHK(c, c3d::launch_apply_bc(c->g, ib->Ureg, ib->Vreg, ib->Wreg, c->stream));
+  return 0;
+}
+
+// schur (src/projection.f90:98-144) on device vectors.
+static int ib_schur_launch(c3d_ctx* c, const double* f, double* Sf, double dt, int rk_step) {
+  IBState* ib = c->ib;
+  const size_t nb = ib->m.nb;
+  int rc = ib_spread_force(c, f, dt);
+  if (!rc) rc = ib_poisson_reg(c, dt, rk_step);
+  if (rc) return rc;
+  HIPCHK(c, c3d::ib::launch_interp_c(c->g, ib->m, ib->phi, -1.0, Sf + 3 * nb, c->stream));
+  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, Sf + 3 * nb, ib->E1nHc, f + 3 * nb, c->stream));
+  HIPCHK(c, c3d::ib::launch_schur_u(c->g, ib->m, ib->phi, dt, ib->Ureg, ib->Vreg, ib->Wreg,
+                                    ib->E1nH, f, Sf, c->stream));
+  return 0;
+}
+
+// bicgstab (src/projection.f90:146-186) on device vectors x (in/out), b.
+static int ib_bicgstab_launch(c3d_ctx* c, double* x, const double* b, double tol, int maxit,
+                              double dt, int rk_step, int* iters_out, double* err_out) {
+  IBState* ib = c->ib;
+  namespace B = c3d::ib;
+  const size_t n = 4 * (size_t)ib->m.nb;
+  double *r = ib->vec[1], *rhat = ib->vec[2], *p = ib->vec[3], *v = ib->vec[4], *h = ib->vec[5],
+         *sv = ib->vec[6], *t = ib->vec[7];
+  int rc = ib_schur_launch(c, x, r, dt, rk_step);
+  if (rc) return rc;
+  HIPCHK(c, B::launch_bicg_init(b, r, rhat, p, v, n, c->stream));
+  HIPCHK(c, B::launch_bicg_scalar(0, ib->sc, ib->part, nullptr, c->stream));
+  double error = 1.0;
+  const double eps = tol * tol;
+  int iter = 0;
+  while (iter < maxit && error >= eps) {
+    HIPCHK(c, B::launch_dot(rhat, r, nullptr, nullptr, n, ib->part, 0, c->stream));
+    HIPCHK(c, B::launch_bicg_scalar(1, ib->sc, ib->part, nullptr, c->stream));
+    HIPCHK(c, B::launch_bicg_p(r, v, ib->sc, p, n, c->stream));
+    if ((rc = ib_schur_launch(c, p, v, dt, rk_step))) return rc;
+    HIPCHK(c, B::launch_dot(rhat, v, nullptr, nullptr, n, ib->part, 1, c->stream));
+    HIPCHK(c, B::launch_bicg_scalar(2, ib->sc, ib->part, nullptr, c->stream));
+    HIPCHK(c, B::launch_bicg_hs(x, p, r, v, ib->sc, h, sv, n, c->stream));
+    if ((rc = ib_schur_launch(c, sv, t, dt, rk_step))) return rc;
+    HIPCHK(c, B::launch_dot(t, sv, t, t, n, ib->part, 2, c->stream));
+    HIPCHK(c, B::launch_bicg_scalar(3, ib->sc, ib->part, nullptr, c->stream));
+    HIPCHK(c, B::launch_bicg_xr(h, sv, t, ib->sc, x, r, ib->part, n, c->stream));
+    HIPCHK(c, B::launch_bicg_scalar(4, ib->sc, ib->part, nullptr, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&error, ib->sc + 5, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    iter++;
+  }
+  ib->last_iters = iter;
+  ib->accum_iters += iter;
+  ib->last_err = error;
+  if (iters_out) *iters_out = iter;
+  if (err_out) *err_out = error;
+  if (iter >= maxit && error >= eps)
+    return fail(c, C3D_ENOCONV, "BiCGSTAB reached %d iterations with ||r|| = %.3e > %.3e", iter,
+                std::sqrt(error), tol);
+  return 0;
+}
+
+// compute_IB_projection (src/projection.f90:48-96).
+static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
+  IBState* ib = c->ib;
+  const size_t nb = ib->m.nb;
+  HIPCHK(c, c3d::ib::launch_rhs_u(c->g, ib->m, c->U, c->V, c->W, ib->rhs_ib, c->stream));  // :51-52
+  HIPCHK(c, c3d::ib::launch_interp_c(c->g, ib->m, c->Pint, -1.0, ib->rhs_ib + 3 * nb,
+                                     c->stream));                                        // :53-54
+  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, ib->rhs_ib + 3 * nb, nullptr, nullptr,
+                                        c->stream));                                     // :57
+  if (ib->guess == 1) HIPCHK(c, hipMemsetAsync(ib->fb, 0, 4 * nb * 8, c->stream));
+  if (ib->guess == 2) ib->guess = 0;
+  int rc = ib_bicgstab_launch(c, ib->fb, ib->rhs_ib, ib->tol, ib->maxit, dt, rk_step, nullptr,
+                              nullptr);                                                  // :60
+  if (rc && rc != C3D_ENOCONV) return rc;
+  const int conv = rc;
+  if ((rc = ib_spread_force(c, ib->fb, dt))) return rc;                                  // :66-73
+  if ((rc = ib_poisson_reg(c, dt, rk_step))) return rc;                                  // :76-80
+  HIPCHK(c, c3d::ib::launch_ib_correct(c->g, ib->Uint, ib->Vint, ib->Wint, ib->Ureg, ib->Vreg,
+                                       ib->Wreg, c->Pint, ib->phi, dt, c->U, c->V, c->W,
+                                       c->stream));                                      // :83-92
+  HIPCHK(c, c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream));                  // :94
+  return conv;  // C3D_ENOCONV keeps the message set by the solve
+}
+
+static int need_ib(c3d_ctx* c) {
+  if (!c) return C3D_EINVAL;
+  if (!c->ib) return fail(c, C3D_EINVAL, "no immersed body set up (c3d_ib_setup)");
+  return 0;
+}
+
+int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const double* zb,
+                 const double* normals, const double* sb, const double* ub, int nzb, int nxb,
+                 int nxb1) {
+  if (!c || !xb || !yb || !zb || !normals || !sb || !ub) return C3D_EINVAL;
+  if (c->ib) return fail(c, C3D_EINVAL, "an immersed body is already set up on this context");
+  if (c->desc.grid_type != 0)
+    return fail(c, C3D_EINVAL, "immersed bodies need a uniform y grid (grid_type 0)");
+  if (nb <= 0 || nzb <= 0 || nxb <= 0 || (long long)nzb * nxb != nb || nxb1 <= 0 || nxb1 >= nxb)
+    return fail(c, C3D_EINVAL, "marker counts: nb = nzb*nxb and 0 < nxb1 < nxb required");
+  if ((long long)nb * 125 > 0x7fffffff) return fail(c, C3D_EINVAL, "too many markers");
+  for (int l = 0; l < nb; l++)
+    if (!(xb[l] >= 0.0 && xb[l] < c->Lxp && zb[l] >= 0.0 && zb[l] < c->Lzp && yb[l] > 0.0 &&
+          yb[l] < c->Ly))
+      return fail(c, C3D_EINVAL, "marker %d outside the periodic box", l);
+  IBState* ib = new IBState();
+  c->ib = ib;
+  auto bail = [&](int rc) {
+    c->ib = nullptr;
+    delete ib;  // device memory stays in c->allocs until c3d_destroy
+    return rc;
+  };
+  namespace B = c3d::ib;
+  B::Markers& m = ib->m;
+  m.nb = nb; m.nzb = nzb; m.nxb = nxb; m.nxb1 = nxb1;
+  m.A = c->dx * c->dymin * c->dz;
+  m.factor = 1.0 / (c->dx * c->dymin * c->dz);
+  const size_t n4 = 4 * (size_t)nb;
+  int rc = 0;
+  std::vector<double> hv;
+  auto up = [&](const double** dst, const double* src, size_t n) {
+    hv.assign(src, src + n);
+    return upload(c, dst, hv);
+  };
+  if ((rc = up(&m.xb, xb, nb)) || (rc = up(&m.yb, yb, nb)) || (rc = up(&m.zb, zb, nb)) ||
+      (rc = up(&m.normals, normals, 3 * (size_t)nb)) || (rc = up(&m.sb, sb, nb)) ||
+      (rc = up(&m.ub, ub, 3 * (size_t)nb)))
+    return bail(rc);
+  for (int s = 0; s < 4; s++)
+    if ((rc = dalloc(c, &m.st[s].idx, 15 * (size_t)nb)) || (rc = dalloc(c, &m.st[s].phi, 15 * (size_t)nb)) ||
+        (rc = dalloc(c, &m.st[s].dd, 15 * (size_t)nb)))
+      return bail(rc);
+  B::SetupGrid sg{};
+  if ((rc = upload(c, &sg.x, c->x)) || (rc = upload(c, &sg.y, c->y)) || (rc = upload(c, &sg.z, c->z)))
+    return bail(rc);
+  sg.dx = c->dx; sg.dz = c->dz; sg.dymin = c->dymin; sg.Lxp = c->Lxp; sg.Lzp = c->Lzp;
+  if (B::launch_setup(c->g, sg, m, c->stream) != hipSuccess)
+    return bail(fail(c, C3D_EHIP, "IB stencil set-up launch failed"));
+  std::vector<int> idx[4];
+  for (int s = 0; s < 4; s++) {
+    idx[s].resize(15 * (size_t)nb);
+    if (hipMemcpyAsync(idx[s].data(), m.st[s].idx, idx[s].size() * 4, hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess)
+      return bail(fail(c, C3D_EHIP, "stencil copy-back failed"));
+  }
+  if ((rc = sync(c))) return bail(rc);
+  // every node must be a canonical interior node inside the gradient's
+  // ranges (the Schur kernels evaluate G at the nodes and skip the boundary
+  // conditions there): x in [2, nx-1], z in [2, nz-1], y in [2, nyg-1]
+  // (v: [2, ny-1])
+  const int ylim[4] = {c->nyg - 1, c->ny - 1, c->nyg - 1, c->nyg - 1};
+  for (int s = 0; s < 4; s++)
+    for (int t = 0; t < 5; t++)
+      for (int l = 0; l < nb; l++) {
+        const int X = idx[s][(0 * 5 + t) * (size_t)nb + l], Y = idx[s][(1 * 5 + t) * (size_t)nb + l],
+                  Z = idx[s][(2 * 5 + t) * (size_t)nb + l];
+        if (X < 2 || X > c->nx - 1 || Z < 2 || Z > c->nz - 1)
+          return bail(fail(c, C3D_EINVAL, "stencil %d of marker %d: x/z index out of range", s, l));
+        if (Y < 2 || Y > ylim[s])
+          return bail(fail(c, C3D_EINVAL, "marker %d: stencil reaches a wall ghost row", l));
+      }
+  // node -> entry maps of the three spreading operators (counting sort by
+  // node offset; entries within a node stay in ascending marker order)
+  const int n1[3] = {c->nx, c->nxg, c->nxg}, n2[3] = {c->nyg, c->ny, c->nyg};
+  const size_t nf[3] = {c->nu, c->nv, c->nw};
+  for (int q = 0; q < 3; q++) {
+    const std::vector<int>& I = idx[q];
+    std::vector<int> cnt(nf[q] + 1, 0), off((size_t)nb * 125);
+    for (int l = 0; l < nb; l++)
+      for (int p = 0; p < 125; p++) {
+        const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+        const size_t o = c3d::fidx(I[(0 * 5 + i) * (size_t)nb + l], I[(1 * 5 + j) * (size_t)nb + l],
+                                   I[(2 * 5 + k) * (size_t)nb + l], n1[q], n2[q]);
+        off[(size_t)l * 125 + p] = (int)o;
+        cnt[o + 1]++;
+      }
+    std::vector<int> node, ptr(1, 0);
+    for (size_t o = 0; o < nf[q]; o++)
+      if (cnt[o + 1]) {
+        node.push_back((int)o);
+        ptr.push_back(ptr.back() + cnt[o + 1]);
+      }
+    for (size_t o = 0; o < nf[q]; o++) cnt[o + 1] += cnt[o];
+    std::vector<int> qq((size_t)nb * 125);
+    for (size_t e = 0; e < qq.size(); e++) qq[cnt[off[e]]++] = (int)e;
+    ib->sp[q].nnodes = (int)node.size();
+    if ((rc = upload(c, &ib->sp[q].node, node)) || (rc = upload(c, &ib->sp[q].ptr, ptr)) ||
+        (rc = upload(c, &ib->sp[q].q, qq)))
+      return bail(rc);
+  }
+  // resident buffers
+  if ((rc = dalloc(c, &ib->Ureg, c->nu)) || (rc = dalloc(c, &ib->Vreg, c->nv)) ||
+      (rc = dalloc(c, &ib->Wreg, c->nw)) || (rc = dalloc(c, &ib->phi, c->nr)) ||
+      (rc = dalloc(c, &ib->Hc_ext, c->nr)) || (rc = dalloc(c, &ib->Hu_int, c->nu)) ||
+      (rc = dalloc(c, &ib->Hv_int, c->nv)) || (rc = dalloc(c, &ib->Hw_int, c->nw)) ||
+      (rc = dalloc(c, &ib->Uint, c->nu)) || (rc = dalloc(c, &ib->Vint, c->nv)) ||
+      (rc = dalloc(c, &ib->Wint, c->nw)) || (rc = dalloc(c, &ib->E1nHc, (size_t)nb)) ||
+      (rc = dalloc(c, &ib->E1nH, 3 * (size_t)nb)) || (rc = dalloc(c, &ib->aux, 3 * (size_t)nb)) ||
+      (rc = dalloc(c, &ib->rhs_ib, n4)) || (rc = dalloc(c, &ib->fb, n4)) ||
+      (rc = dalloc(c, &ib->fin, n4)) || (rc = dalloc(c, &ib->fout, n4)) ||
+      (rc = dalloc(c, &ib->sc, 8)) || (rc = dalloc(c, &ib->part, 5 * (size_t)B::kDotBlocks)))
+    return bail(rc);
+  for (int v = 0; v < 8; v++)
+    if ((rc = dalloc(c, &ib->vec[v], n4))) return bail(rc);
+  if (hipMemsetAsync(ib->Ureg, 0, c->nu * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(ib->Vreg, 0, c->nv * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(ib->Wreg, 0, c->nw * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(ib->fb, 0, n4 * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(ib->phi, 0, c->nr * 8, c->stream) != hipSuccess)
+    return bail(fail(c, C3D_EHIP, "IB buffer clear failed"));
+  // compute_heaviside (src/heaviside.f90:20-54); the exterior face fields
+  // are built in the U_interim buffers
+  double *Hu = ib->Uint, *Hv = ib->Vint, *Hw = ib->Wint;
+  double* Freg[3] = {ib->Ureg, ib->Vreg, ib->Wreg};
+  bool ok = true;
+  for (int q = 0; q < 3 && ok; q++)
+    ok = B::launch_spread(m, ib->sp[q], q, m.normals + (size_t)q * nb, Freg[q], c->stream) == hipSuccess;
+  ok = ok && c3d::launch_apply_bc(c->g, ib->Ureg, ib->Vreg, ib->Wreg, c->stream) == hipSuccess;
+  ok = ok && c3d::launch_divergence(c->g, ib->Ureg, ib->Vreg, ib->Wreg, ib->Hc_ext, c->stream) == hipSuccess;
+  ok = ok && B::launch_affine(ib->Hc_ext, -0.0, -1.0, ib->Hc_ext, c->nr, c->stream) == hipSuccess;
+  ok = ok && c3d::launch_fwd_x(c->g, false, nullptr, nullptr, nullptr, ib->Hc_ext, c->S, c->stream) == hipSuccess;
+  if (!ok) return bail(fail(c, C3D_EHIP, "Heaviside set-up failed"));
+  if ((rc = spectral_core(c))) return bail(rc);
+  ok = c3d::launch_inv_x(c->g, 0, c->S, ib->Hc_ext, c->P, 1.0, false, c->stream) == hipSuccess;
+  ok = ok && B::launch_affine(ib->Hc_ext, 1.0, -1.0, ib->Hc_ext, c->nr, c->stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(Hu, 0, c->nu * 8, c->stream) == hipSuccess &&
+       hipMemsetAsync(Hv, 0, c->nv * 8, c->stream) == hipSuccess &&
+       hipMemsetAsync(Hw, 0, c->nw * 8, c->stream) == hipSuccess;
+  ok = ok && B::launch_h_interp(c->g, ib->Hc_ext, Hu, Hv, Hw, c->stream) == hipSuccess;
+  ok = ok && c3d::launch_apply_bc(c->g, Hu, Hv, Hw, c->stream) == hipSuccess;
+  ok = ok && B::launch_affine(Hu, 1.0, -1.0, ib->Hu_int, c->nu, c->stream) == hipSuccess &&
+       B::launch_affine(Hv, 1.0, -1.0, ib->Hv_int, c->nv, c->stream) == hipSuccess &&
+       B::launch_affine(Hw, 1.0, -1.0, ib->Hw_int, c->nw, c->stream) == hipSuccess;
+  ok = ok && c3d::launch_apply_bc(c->g, ib->Hu_int, ib->Hv_int, ib->Hw_int, c->stream) == hipSuccess;
+  ok = ok && B::launch_interp_c(c->g, m, ib->Hc_ext, 1.0, ib->E1nHc, c->stream) == hipSuccess;
+  ok = ok && B::launch_interp_uvw(c->g, m, Hu, Hv, Hw, true, ib->E1nH, c->stream) == hipSuccess;
+  if (!ok) return bail(fail(c, C3D_EHIP, "Heaviside set-up failed"));
+  if ((rc = sync(c))) return bail(rc);
+  c->Hm = ib->Hu_int;
+  return C3D_OK;
+}
+
+int c3d_ib_get_stencil(c3d_ctx* c, int s, int* xi, int* yi, int* zi, double* w, double* dxn) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (s < 0 || s > 3) return fail(c, C3D_EINVAL, "stencil must be 0..3");
+  IBState* ib = c->ib;
+  const size_t nb = ib->m.nb, n = 15 * nb;
+  std::vector<int> I(n);
+  std::vector<double> ph(n), dd(n), nrm(3 * nb);
+  HIPCHK(c, hipMemcpyAsync(I.data(), ib->m.st[s].idx, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ph.data(), ib->m.st[s].phi, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dd.data(), ib->m.st[s].dd, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(nrm.data(), ib->m.normals, 3 * nb * 8, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = sync(c))) return rc;
+  for (size_t l = 0; l < nb; l++)
+    for (int p = 0; p < 125; p++) {
+      const size_t a = (size_t)(p % 5) * nb + l, b = (size_t)(5 + (p / 5) % 5) * nb + l,
+                   d = (size_t)(10 + p / 25) * nb + l, o = l * 125 + p;
+      if (xi) xi[o] = I[a];
+      if (yi) yi[o] = I[b];
+      if (zi) zi[o] = I[d];
+      if (w) w[o] = ((ib->m.A * ph[a]) * ph[b]) * ph[d];
+      if (dxn) {
+#pragma clang fp contract(off)
+        dxn[o] = (nrm[l] * dd[a] + nrm[nb + l] * dd[b]) + nrm[2 * nb + l] * dd[d];
+      }
+    }
+  return C3D_OK;
+}
+
+int c3d_ib_get_heaviside(c3d_ctx* c, double* E1nHc, double* E1nH, double* Hu_int, double* Hv_int,
+                         double* Hw_int, double* Hc_ext) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  IBState* ib = c->ib;
+  const size_t nb = ib->m.nb;
+  if (E1nHc) HIPCHK(c, hipMemcpyAsync(E1nHc, ib->E1nHc, nb * 8, hipMemcpyDeviceToHost, c->stream));
+  if (E1nH) HIPCHK(c, hipMemcpyAsync(E1nH, ib->E1nH, 3 * nb * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Hu_int) HIPCHK(c, hipMemcpyAsync(Hu_int, ib->Hu_int, c->nu * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Hv_int) HIPCHK(c, hipMemcpyAsync(Hv_int, ib->Hv_int, c->nv * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Hw_int) HIPCHK(c, hipMemcpyAsync(Hw_int, ib->Hw_int, c->nw * 8, hipMemcpyDeviceToHost, c->stream));
+  if (Hc_ext) HIPCHK(c, hipMemcpyAsync(Hc_ext, ib->Hc_ext, c->nr * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_interpolate(c3d_ctx* c, int with_dxn, double* out) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!out) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  HIPCHK(c, c3d::ib::launch_interp_uvw(c->g, ib->m, c->U, c->V, c->W, with_dxn != 0, ib->fout,
+                                       c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, ib->fout, 3 * (size_t)ib->m.nb * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_interpolate_c(c3d_ctx* c, const double* p, double* out) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!p || !out) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  HIPCHK(c, hipMemcpyAsync(ib->phi, p, c->nr * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, c3d::ib::launch_interp_c(c->g, ib->m, ib->phi, 1.0, ib->fout, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, ib->fout, (size_t)ib->m.nb * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_spread(c3d_ctx* c, int comp, const double* f, double* F) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (comp < 0 || comp > 2 || !f || !F) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  const size_t nf[3] = {c->nu, c->nv, c->nw};
+  double* T[3] = {ib->Ureg, ib->Vreg, ib->Wreg};
+  HIPCHK(c, hipMemcpyAsync(ib->fin, f, (size_t)ib->m.nb * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(T[comp], 0, nf[comp] * 8, c->stream));
+  HIPCHK(c, c3d::ib::launch_spread(ib->m, ib->sp[comp], comp, ib->fin, T[comp], c->stream));
+  HIPCHK(c, hipMemcpyAsync(F, T[comp], nf[comp] * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_remove_mean(c3d_ctx* c, double* f) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!f) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  const size_t nb = ib->m.nb;
+  HIPCHK(c, hipMemcpyAsync(ib->fin, f, nb * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, ib->fin, nullptr, nullptr, c->stream));
+  HIPCHK(c, hipMemcpyAsync(f, ib->fin, nb * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_schur(c3d_ctx* c, const double* f, double* Sf, double dt, int rk_step) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!f || !Sf || !(dt != 0.0)) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  const size_t n4 = 4 * (size_t)ib->m.nb;
+  HIPCHK(c, hipMemcpyAsync(ib->fin, f, n4 * 8, hipMemcpyHostToDevice, c->stream));
+  if ((rc = ib_schur_launch(c, ib->fin, ib->fout, dt, rk_step))) return rc;
+  HIPCHK(c, hipMemcpyAsync(Sf, ib->fout, n4 * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_bicgstab(c3d_ctx* c, double* x, const double* b, double tol, int maxit, double dt,
+                    int rk_step, int* iters, double* err) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!x || !b || maxit < 0 || !(dt != 0.0)) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  const size_t n4 = 4 * (size_t)ib->m.nb;
+  HIPCHK(c, hipMemcpyAsync(ib->vec[0], x, n4 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ib->fin, b, n4 * 8, hipMemcpyHostToDevice, c->stream));
+  rc = ib_bicgstab_launch(c, ib->vec[0], ib->fin, tol, maxit, dt, rk_step, iters, err);
+  if (rc && rc != C3D_ENOCONV) return rc;
+  HIPCHK(c, hipMemcpyAsync(x, ib->vec[0], n4 * 8, hipMemcpyDeviceToHost, c->stream));
+  const int s = sync(c);
+  return s ? s : rc;
+}
+
+int c3d_ib_set_solver(c3d_ctx* c, double tol, int maxit, int guess_mode) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 2) return C3D_EINVAL;
+  c->ib->tol = tol;
+  c->ib->maxit = maxit;
+  c->ib->guess = guess_mode;
+  return C3D_OK;
+}
+
+int c3d_ib_set_force(c3d_ctx* c, const double* fb) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!fb) return C3D_EINVAL;
+  HIPCHK(c, hipMemcpyAsync(c->ib->fb, fb, 4 * (size_t)c->ib->m.nb * 8, hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_get_force(c3d_ctx* c, double* fb) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!fb) return C3D_EINVAL;
+  HIPCHK(c, hipMemcpyAsync(fb, c->ib->fb, 4 * (size_t)c->ib->m.nb * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_compute_ib_projection(c3d_ctx* c, double dt, int rk_step, int* iters, double* err) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!(dt != 0.0)) return fail(c, C3D_EINVAL, "dt must be nonzero");
+  rc = ib_projection_launch(c, dt, rk_step);
+  if (rc && rc != C3D_ENOCONV) return rc;
+  if (iters) *iters = c->ib->last_iters;
+  if (err) *err = c->ib->last_err;
+  const int s = sync(c);
+  return s ? s : rc;
+}
+
+int c3d_check_slip(c3d_ctx* c, double* max_slip) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!max_slip) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  HIPCHK(c, hipMemsetAsync(c->red, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, c3d::ib::launch_max_slip(c->g, ib->m, c->U, c->V, c->W, ib->E1nH, ib->fb, c->red,
+                                     c->stream));
+  unsigned long long bits = 0;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->red, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = sync(c))) return rc;
+  std::memcpy(max_slip, &bits, sizeof(double));
+  return C3D_OK;
+}
+
+int c3d_ib_stats(c3d_ctx* c, int* last_iters, long long* accum_iters, double* last_err) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (last_iters) *last_iters = c->ib->last_iters;
+  if (accum_iters) *accum_iters = c->ib->accum_iters;
+  if (last_err) *last_err = c->ib->last_err;
+  return C3D_OK;
+}
+
+int c3d_ib_set_exterior_pressure_gradient(c3d_ctx* c, int flag) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  IBState* ib = c->ib;
+  ib->ext_pg = flag != 0;
+  c->Hu = ib->ext_pg ? ib->Hu_int : nullptr;
+  c->Hv = ib->ext_pg ? ib->Hv_int : nullptr;
+  c->Hw = ib->ext_pg ? ib->Hw_int : nullptr;
+  return C3D_OK;
 }
 
 }  // extern "C"

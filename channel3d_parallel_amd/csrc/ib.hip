@@ -1,0 +1,666 @@
+// Immersed-boundary kernels for gfx950: stencil set-up, interpolation,
+// spreading, Schur-complement pieces and BiCGSTAB vector steps.
+//
+// Interpolation runs one wavefront per marker: the 125 stencil points are
+// spread over the 64 lanes and reduced with a fixed butterfly, so results are
+// deterministic.  Spreading is a gather over a node -> entry map built once
+// per static body, summing each node's contributions in ascending marker
+// order (the order of the reference's serial scatter), with no atomics.
+#include <hip/hip_runtime.h>
+
+#include "ib.hpp"
+
+#pragma clang fp contract(off)
+
+namespace c3d {
+namespace ib {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// deltafnc, src/immersed_boundary_operators.f90:270-294.
+__device__ double deltafnc(double r, double dr) {
+  const double pi = 4.0 * atan(1.0);
+  const double s3 = sqrt(3.0);
+  const double r1 = fabs(r) / dr;
+  const double r2 = r1 * r1;
+  double d;
+  if (r1 <= 1.0) {
+    d = 17.0 / 48.0 + s3 * pi / 108.0 + r1 / 4.0 - r2 / 4.0 +
+        (1.0 - 2.0 * r1) / 16.0 * sqrt(-12.0 * r2 + 12.0 * r1 + 1.0) -
+        s3 / 12.0 * asin(s3 / 2.0 * (2.0 * r1 - 1.0));
+  } else if (r1 <= 2.0) {
+    d = 55.0 / 48.0 - s3 * pi / 108.0 - 13.0 * r1 / 12.0 + r2 / 4.0 +
+        (2.0 * r1 - 3.0) / 48.0 * sqrt(-12.0 * r2 + 36.0 * r1 - 23.0) +
+        s3 / 36.0 * asin(s3 / 2.0 * (2.0 * r1 - 3.0));
+  } else {
+    d = 0.0;
+  }
+  return d / dr;
+}
+
+__device__ __forceinline__ int floor_div(int a, int b) {
+  return (int)floor((double)a / (double)b);
+}
+
+// One thread per (marker, stencil): pivots (src/immersed_boundary_operators.f90:28-42)
+// and the per-axis windows of the u (:46-98), v (:101-148), w (:151-194) and
+// centre (:197-244) stencils on one periodic domain, centre z index nzg-1
+// stored as its canonical plane 2.
+__global__ void k_setup(DevGrid g, SetupGrid sg, Markers m) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = m.nb;
+  if (tid >= 4 * nb) return;
+  const int s = tid / nb, l = tid - s * nb;
+  const int nx = g.nx, nz = g.nz, nzg = g.nzg;
+  const double dx = sg.dx, dz = sg.dz, dymin = sg.dymin;
+  const double xb = m.xb[l], yb = m.yb[l], zb = m.zb[l];
+  const double* x = sg.x;
+  const double* y = sg.y;
+  const double* z = sg.z;
+  const int x_piv = (int)floor(xb / dx) + 1;
+  const int xm_piv = (int)floor((xb + 0.5 * dx) / dx);
+  const int y_piv = (int)floor((yb - y[0]) / dymin) + 1;
+  const int ym_piv = (int)floor((yb - (y[0] + 0.5 * dymin)) / dymin) + 1;
+  const int z_piv = (int)floor(zb / dz) + 1;
+  const int zm_piv = (int)floor((zb + 0.5 * dz) / dz) + 1;
+  Stencil st = m.st[s];
+  for (int t = 0; t < 5; t++) {
+    const int o = t - 2;
+    // x
+    int X;
+    double xgd;
+    if (s == 0) {
+      const int ii = x_piv + o;
+      const int xs = floor_div(ii - 1, nx - 2);
+      const int iip = ii - xs * (nx - 2);
+      X = (iip == 1) ? nx - 1 : iip;
+      xgd = x[iip - 1] + xs * sg.Lxp;
+    } else {
+      const int ii = xm_piv + o - (s == 2 ? 1 : 0);
+      const int xs = floor_div(ii - 1, nx - 2);
+      const int iip = ii - xs * (nx - 2);
+      X = iip + 1;
+      xgd = 0.5 * (x[iip - 1] + x[iip]) + xs * sg.Lxp;
+    }
+    // y (a window reaching past the walls is flagged with Y = -1 and
+    // rejected by the host before any other IB kernel runs)
+    int Y = -1;
+    double ygd = 0.0;
+    if (s == 1) {
+      const int jj = y_piv + o;
+      if (jj >= 1 && jj <= g.ny) {
+        Y = jj;
+        ygd = y[jj - 1];
+      }
+    } else {
+      const int jj = ym_piv + o;
+      if (jj >= 1 && jj <= g.ny - 1) {
+        Y = jj + 1;
+        ygd = 0.5 * (y[jj - 1] + y[jj]);
+      }
+    }
+    // z
+    int Z;
+    double zgd;
+    if (s == 2) {
+      const int kk = z_piv + o;
+      const int zs = floor_div(kk - 2, nz - 2);
+      const int kkp = kk - zs * (nz - 2);
+      Z = kkp;
+      zgd = z[kkp - 1] + zs * sg.Lzp;
+    } else {
+      const int kk = zm_piv + o;
+      const int zs = floor_div(kk - 2, nz - 1);
+      const int kkp = kk - zs * (nz - 2);
+      Z = (kkp == nzg - 1) ? 2 : kkp;
+      zgd = 0.5 * (z[kkp - 2] + z[kkp - 1]) + zs * sg.Lzp;
+    }
+    const double ddx = xgd - xb, ddy = ygd - yb, ddz = zgd - zb;
+    st.idx[(0 * 5 + t) * nb + l] = X;
+    st.idx[(1 * 5 + t) * nb + l] = Y;
+    st.idx[(2 * 5 + t) * nb + l] = Z;
+    st.dd[(0 * 5 + t) * nb + l] = ddx;
+    st.dd[(1 * 5 + t) * nb + l] = ddy;
+    st.dd[(2 * 5 + t) * nb + l] = ddz;
+    st.phi[(0 * 5 + t) * nb + l] = deltafnc(ddx, dx);
+    st.phi[(1 * 5 + t) * nb + l] = deltafnc(ddy, dymin);
+    st.phi[(2 * 5 + t) * nb + l] = deltafnc(ddz, dz);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Point p = i + 5*(j + 5*k) of marker l in stencil st: node indices, weight
+// and (optionally) the normal distance n . (x_node - xb).
+struct Pt {
+  int X, Y, Z;
+  double w, dxn;
+};
+
+template <bool DXN>
+__device__ __forceinline__ Pt stencil_point(const Markers& m, const Stencil& st, int l, int p) {
+  const int nb = m.nb;
+  const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+  Pt r;
+  r.X = st.idx[(0 * 5 + i) * nb + l];
+  r.Y = st.idx[(1 * 5 + j) * nb + l];
+  r.Z = st.idx[(2 * 5 + k) * nb + l];
+  r.w = ((m.A * st.phi[(0 * 5 + i) * nb + l]) * st.phi[(1 * 5 + j) * nb + l]) *
+        st.phi[(2 * 5 + k) * nb + l];
+  r.dxn = 0.0;
+  if (DXN)
+    r.dxn = (m.normals[l] * st.dd[(0 * 5 + i) * nb + l] +
+             m.normals[nb + l] * st.dd[(1 * 5 + j) * nb + l]) +
+            m.normals[2 * nb + l] * st.dd[(2 * 5 + k) * nb + l];
+  return r;
+}
+
+// Marker handled by this wavefront (XCD-aware: neighbouring markers, whose
+// stencils share nodes, run on the same XCD and L2).
+__device__ __forceinline__ int wave_marker(int nb) {
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
+  return (int)bx * (kBlock / 64) + (int)(threadIdx.x >> 6);
+}
+
+// E / E1n of U, V, W (src/immersed_boundary_operators.f90:409-501).
+template <bool DXN>
+__global__ __launch_bounds__(kBlock) void k_interp_uvw(DevGrid g, Markers m, const double* __restrict__ U,
+                                                       const double* __restrict__ V,
+                                                       const double* __restrict__ W,
+                                                       double* __restrict__ out) {
+  const int l = wave_marker(m.nb);
+  const int lane = threadIdx.x & 63;
+  if (l >= m.nb) return;
+  const double* F[3] = {U, V, W};
+  const int n1[3] = {g.nx, g.nxg, g.nxg}, n2[3] = {g.nyg, g.ny, g.nyg};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double acc = 0.0;
+    for (int p = lane; p < 125; p += 64) {
+      const Pt q = stencil_point<DXN>(m, m.st[c], l, p);
+      double v = q.w * F[c][fidx(q.X, q.Y, q.Z, n1[c], n2[c])];
+      if (DXN) v = v * q.dxn;
+      acc += v;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[c * m.nb + l] = acc;
+  }
+}
+
+// E1n of a centre field in rhs_p layout (:503-526).
+__global__ __launch_bounds__(kBlock) void k_interp_c(DevGrid g, Markers m, const double* __restrict__ P,
+                                                     double sign, double* __restrict__ out) {
+  const int l = wave_marker(m.nb);
+  const int lane = threadIdx.x & 63;
+  if (l >= m.nb) return;
+  double acc = 0.0;
+  for (int p = lane; p < 125; p += 64) {
+    const Pt q = stencil_point<true>(m, m.st[3], l, p);
+    acc += (q.w * P[ridx(q.X, q.Y, q.Z, g.nxg, g.nyg)]) * q.dxn;
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) out[l] = sign * acc;
+}
+
+// Schur velocity rows (src/projection.f90:131-142) with the gradient of
+// dt*phi evaluated at the stencil nodes; every node lies in the gradient's
+// index ranges (checked at set-up), where apply_boundary_conditions does not
+// write.
+__global__ __launch_bounds__(kBlock) void k_schur_u(DevGrid g, Markers m, const double* __restrict__ phi,
+                                                    double dt, const double* __restrict__ Ureg,
+                                                    const double* __restrict__ Vreg,
+                                                    const double* __restrict__ Wreg,
+                                                    const double* __restrict__ E1nH,
+                                                    const double* __restrict__ dudn,
+                                                    double* __restrict__ Sf) {
+  const int l = wave_marker(m.nb);
+  const int lane = threadIdx.x & 63;
+  if (l >= m.nb) return;
+  const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double acc = 0.0;
+    for (int p = lane; p < 125; p += 64) {
+      const Pt q = stencil_point<false>(m, m.st[c], l, p);
+      const double p0 = dt * phi[ridx(q.X, q.Y, q.Z, nxg, nyg)];
+      double G, R;
+      if (c == 0) {
+        G = (dt * phi[ridx(q.X + 1, q.Y, q.Z, nxg, nyg)] - p0) / g.dxg[q.X];
+        R = Ureg[fidx(q.X, q.Y, q.Z, nx, nyg)];
+      } else if (c == 1) {
+        G = (dt * phi[ridx(q.X, q.Y + 1, q.Z, nxg, nyg)] - p0) / g.dyg[q.Y];
+        R = Vreg[fidx(q.X, q.Y, q.Z, nxg, ny)];
+      } else {
+        G = (dt * phi[ridx(q.X, q.Y, q.Z + 1, nxg, nyg)] - p0) / g.dzg[q.Z];
+        R = Wreg[fidx(q.X, q.Y, q.Z, nxg, nyg)];
+      }
+      acc += q.w * (G - R);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) Sf[c * m.nb + l] = acc - E1nH[c * m.nb + l] * dudn[c * m.nb + l];
+  }
+}
+
+// -E u + ub (src/projection.f90:51-52).
+__global__ __launch_bounds__(kBlock) void k_rhs_u(DevGrid g, Markers m, const double* __restrict__ U,
+                                                  const double* __restrict__ V,
+                                                  const double* __restrict__ W,
+                                                  double* __restrict__ out) {
+  const int l = wave_marker(m.nb);
+  const int lane = threadIdx.x & 63;
+  if (l >= m.nb) return;
+  const double* F[3] = {U, V, W};
+  const int n1[3] = {g.nx, g.nxg, g.nxg}, n2[3] = {g.nyg, g.ny, g.nyg};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double acc = 0.0;
+    for (int p = lane; p < 125; p += 64) {
+      const Pt q = stencil_point<false>(m, m.st[c], l, p);
+      acc += q.w * F[c][fidx(q.X, q.Y, q.Z, n1[c], n2[c])];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[c * m.nb + l] = -acc + m.ub[c * m.nb + l];
+  }
+}
+
+__global__ void k_aux(Markers m, const double* __restrict__ f, double dt, double nu,
+                      double* __restrict__ aux) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = m.nb;
+  if (t >= 3 * nb) return;
+  const int l = t % nb;
+  aux[t] = dt * nu * f[t] - dt * f[3 * nb + l] * m.normals[t];
+}
+
+// R, src/immersed_boundary_operators.f90:544-603: one thread per support node.
+__global__ __launch_bounds__(kBlock) void k_spread(int nb, double A, double factor,
+                                                   const double* __restrict__ phi,
+                                                   const double* __restrict__ sb, Spread sp,
+                                                   const double* __restrict__ f,
+                                                   double* __restrict__ F) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= sp.nnodes) return;
+  double acc = 0.0;
+  for (int e = sp.ptr[t]; e < sp.ptr[t + 1]; e++) {
+    const int q = sp.q[e];
+    const int l = q / 125, p = q - 125 * l;
+    const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+    const double w = ((A * phi[(0 * 5 + i) * nb + l]) * phi[(1 * 5 + j) * nb + l]) *
+                     phi[(2 * 5 + k) * nb + l];
+    acc = acc + w * factor * sb[l] * f[l];
+  }
+  F[sp.node[t]] = acc;
+}
+
+// remove_mean_per_body for the two-group slab layout
+// (src/immersed_boundary_geometry.f90:400-442), then f -= coef*pj.  One block.
+__global__ __launch_bounds__(1024) void k_remove_mean(Markers m, double* __restrict__ f,
+                                                      const double* __restrict__ coef,
+                                                      const double* __restrict__ pj) {
+  __shared__ double s1[1024], s2[1024];
+  const int nb = m.nb, nxb = m.nxb, nxb1 = m.nxb1;
+  double a = 0.0, b = 0.0;
+  for (int l = threadIdx.x; l < nb; l += 1024) {
+    if (l % nxb < nxb1)
+      a += f[l];
+    else
+      b += f[l];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double avg1 = s1[0] / (double)(nxb1 * m.nzb);
+  const double avg2 = s2[0] / (double)((nxb - nxb1) * m.nzb);
+  for (int l = threadIdx.x; l < nb; l += 1024) {
+    double v = f[l] - ((l % nxb < nxb1) ? avg1 : avg2);
+    if (coef) v = v - coef[l] * pj[l];
+    f[l] = v;
+  }
+}
+
+// U = U_interim - U_reg - G(dt*(P_interim - phi))  (src/projection.f90:83-92)
+__global__ __launch_bounds__(kBlock) void k_ib_correct(DevGrid g, const double* __restrict__ Uint,
+                                                       const double* __restrict__ Vint,
+                                                       const double* __restrict__ Wint,
+                                                       const double* __restrict__ Ureg,
+                                                       const double* __restrict__ Vreg,
+                                                       const double* __restrict__ Wreg,
+                                                       const double* __restrict__ Pint,
+                                                       const double* __restrict__ phi, double dt,
+                                                       double* __restrict__ U,
+                                                       double* __restrict__ V,
+                                                       double* __restrict__ W) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 1;
+  const int k = blockIdx.z + 1;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
+  if (i > nxg || j > nyg) return;
+  auto q = [&](int a, int b, int c) {
+    const size_t o = ridx(a, b, c, nxg, nyg);
+    return dt * (Pint[o] - phi[o]);
+  };
+  const bool core = i >= 2 && i <= nx - 1 && j >= 2 && k >= 2;
+  if (i <= nx) {  // U(nx, nyg, nzg)
+    const size_t o = fidx(i, j, k, nx, nyg);
+    double G = 0.0;
+    if (core && j <= nyg - 1 && k <= nzg - 1) G = (q(i + 1, j, k) - q(i, j, k)) / g.dxg[i];
+    U[o] = Uint[o] - Ureg[o] - G;
+  }
+  if (j <= ny) {  // V(nxg, ny, nzg)
+    const size_t o = fidx(i, j, k, nxg, ny);
+    double G = 0.0;
+    if (core && j <= ny - 1 && k <= nzg - 1) G = (q(i, j + 1, k) - q(i, j, k)) / g.dyg[j];
+    V[o] = Vint[o] - Vreg[o] - G;
+  }
+  if (k <= nz) {  // W(nxg, nyg, nz)
+    const size_t o = fidx(i, j, k, nxg, nyg);
+    double G = 0.0;
+    if (core && j <= nyg - 1 && k <= nz - 1) G = (q(i, j, k + 1) - q(i, j, k)) / g.dzg[k];
+    W[o] = Wint[o] - Wreg[o] - G;
+  }
+}
+
+__global__ void k_affine(const double* __restrict__ x, double a, double b, double* __restrict__ y,
+                         size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = a + b * x[t];
+}
+
+// interpolate_x/y/z of src/interpolation.f90 into the heaviside ranges.
+__global__ void k_h_interp(DevGrid g, const double* __restrict__ Hc, double* __restrict__ Hu,
+                           double* __restrict__ Hv, double* __restrict__ Hw) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 2;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 2;
+  const int k = blockIdx.z + 2;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  if (i > nxg - 1 || j > nyg - 1) return;
+  const double c = Hc[ridx(i, j, k, nxg, nyg)];
+  if (i <= nx - 1) Hu[fidx(i, j, k, nx, nyg)] = 0.5 * (c + Hc[ridx(i + 1, j, k, nxg, nyg)]);
+  if (j <= ny - 1) Hv[fidx(i, j, k, nxg, ny)] = 0.5 * (c + Hc[ridx(i, j + 1, k, nxg, nyg)]);
+  if (k <= nz) Hw[fidx(i, j, k, nxg, nyg)] = 0.5 * (c + Hc[ridx(i, j, k + 1, nxg, nyg)]);
+}
+
+// ---- BiCGSTAB ----
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kBlock / 64; q++) r += sh[q];
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a, const double* __restrict__ b,
+                                                const double* __restrict__ a2,
+                                                const double* __restrict__ b2, size_t n,
+                                                double* __restrict__ part, int slot) {
+  __shared__ double sh[2][kBlock / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)kDotBlocks * kBlock) {
+    s1 += a[t] * b[t];
+    if (a2) s2 += a2[t] * b2[t];
+  }
+  const double r1 = block_sum(s1, sh[0]);
+  if (threadIdx.x == 0) part[slot * kDotBlocks + blockIdx.x] = r1;
+  if (a2) {
+    const double r2 = block_sum(s2, sh[1]);
+    if (threadIdx.x == 0) part[(slot + 1) * kDotBlocks + blockIdx.x] = r2;
+  }
+}
+
+__device__ __forceinline__ double sum_part(const double* part, int slot) {
+  double r = 0.0;
+  for (int b = 0; b < kDotBlocks; b++) r += part[slot * kDotBlocks + b];
+  return r;
+}
+
+// Scalar slots: 0 rho_o, 1 alpha, 2 om, 3 rho_n, 4 bta, 5 err.
+__global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ part,
+                              double* host_err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  switch (step) {
+    case 0:
+      sc[0] = 1.0;
+      sc[1] = 1.0;
+      sc[2] = 1.0;
+      break;
+    case 1: {  // rho_n = rhat.r ; bta = (rho_n/rho_o)*(alpha/om) ; rho_o = rho_n
+      const double rho_n = sum_part(part, 0);
+      sc[4] = (rho_n / sc[0]) * (sc[1] / sc[2]);
+      sc[3] = rho_n;
+      sc[0] = rho_n;
+      break;
+    }
+    case 2:  // alpha = rho_n / rhat.nu
+      sc[1] = sc[3] / sum_part(part, 1);
+      break;
+    case 3:  // om = tv.sv / tv.tv
+      sc[2] = sum_part(part, 2) / sum_part(part, 3);
+      break;
+    case 4: {
+      const double e = sum_part(part, 4);
+      sc[5] = e;
+      if (host_err) *host_err = e;
+      break;
+    }
+  }
+}
+
+__global__ void k_bicg_p(const double* __restrict__ r, const double* __restrict__ v,
+                         const double* __restrict__ sc, double* __restrict__ p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double bta = sc[4], om = sc[2];
+  p[t] = r[t] + bta * (p[t] - om * v[t]);
+}
+
+__global__ void k_bicg_hs(const double* __restrict__ x, const double* __restrict__ p,
+                          const double* __restrict__ r, const double* __restrict__ v,
+                          const double* __restrict__ sc, double* __restrict__ h,
+                          double* __restrict__ sv, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double alpha = sc[1];
+  h[t] = x[t] + alpha * p[t];
+  sv[t] = r[t] - alpha * v[t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bicg_xr(const double* __restrict__ h,
+                                                    const double* __restrict__ sv,
+                                                    const double* __restrict__ tv,
+                                                    const double* __restrict__ sc,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ part, size_t n) {
+  __shared__ double sh[kBlock / 64];
+  const double om = sc[2];
+  double s = 0.0;
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)kDotBlocks * kBlock) {
+    x[t] = h[t] + om * sv[t];
+    const double rr = sv[t] - om * tv[t];
+    r[t] = rr;
+    s += rr * rr;
+  }
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0) part[4 * kDotBlocks + blockIdx.x] = tot;
+}
+
+__global__ void k_bicg_init(const double* __restrict__ b, double* __restrict__ r,
+                            double* __restrict__ rhat, double* __restrict__ p,
+                            double* __restrict__ v, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double rr = b[t] - r[t];
+  r[t] = rr;
+  rhat[t] = rr;
+  p[t] = 0.0;
+  v[t] = 0.0;
+}
+
+// check_slip, src/monitor.f90:223-231
+__global__ __launch_bounds__(kBlock) void k_max_slip(DevGrid g, Markers m, const double* __restrict__ U,
+                                                     const double* __restrict__ V,
+                                                     const double* __restrict__ W,
+                                                     const double* __restrict__ E1nH,
+                                                     const double* __restrict__ fb,
+                                                     unsigned long long* red) {
+  const int l = wave_marker(m.nb);
+  const int lane = threadIdx.x & 63;
+  if (l >= m.nb) return;
+  const double* F[3] = {U, V, W};
+  const int n1[3] = {g.nx, g.nxg, g.nxg}, n2[3] = {g.nyg, g.ny, g.nyg};
+  double mx = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double acc = 0.0;
+    for (int p = lane; p < 125; p += 64) {
+      const Pt q = stencil_point<false>(m, m.st[c], l, p);
+      acc += q.w * F[c][fidx(q.X, q.Y, q.Z, n1[c], n2[c])];
+    }
+    acc = wave_sum(acc);
+    const int o = c * m.nb + l;
+    mx = fmax(mx, fabs(acc - E1nH[o] * fb[o] - m.ub[o]));
+  }
+  if (lane == 0) atomicMax(red, (unsigned long long)__double_as_longlong(mx));
+}
+
+inline dim3 blocks_for(size_t n, int b = kBlock) { return dim3((unsigned)((n + b - 1) / b)); }
+inline dim3 wave_blocks(int nb) { return dim3((unsigned)((nb + kBlock / 64 - 1) / (kBlock / 64))); }
+
+}  // namespace
+
+hipError_t launch_setup(const DevGrid& g, const SetupGrid& sg, const Markers& m, hipStream_t s) {
+  hipLaunchKernelGGL(k_setup, blocks_for((size_t)4 * m.nb), dim3(kBlock), 0, s, g, sg, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_uvw(const DevGrid& g, const Markers& m, const double* U, const double* V,
+                             const double* W, bool with_dxn, double* out, hipStream_t s) {
+  if (with_dxn)
+    hipLaunchKernelGGL(k_interp_uvw<true>, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, U, V, W, out);
+  else
+    hipLaunchKernelGGL(k_interp_uvw<false>, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, U, V, W, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_c(const DevGrid& g, const Markers& m, const double* P, double sign,
+                           double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_interp_c, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, P, sign, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_schur_u(const DevGrid& g, const Markers& m, const double* phi, double dt,
+                          const double* Ureg, const double* Vreg, const double* Wreg,
+                          const double* E1nH, const double* dudn, double* Sf, hipStream_t s) {
+  hipLaunchKernelGGL(k_schur_u, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, phi, dt, Ureg, Vreg,
+                     Wreg, E1nH, dudn, Sf);
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs_u(const DevGrid& g, const Markers& m, const double* U, const double* V,
+                        const double* W, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_rhs_u, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, U, V, W, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, double* aux,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_aux, blocks_for((size_t)3 * m.nb), dim3(kBlock), 0, s, m, f, dt, nu, aux);
+  return hipGetLastError();
+}
+
+hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const double* f, double* F,
+                         hipStream_t s) {
+  if (sp.nnodes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spread, blocks_for(sp.nnodes), dim3(kBlock), 0, s, m.nb, m.A, m.factor,
+                     (const double*)m.st[comp].phi, m.sb, sp, f, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_remove_mean(const Markers& m, double* f, const double* coef, const double* pj,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_remove_mean, dim3(1), dim3(1024), 0, s, m, f, coef, pj);
+  return hipGetLastError();
+}
+
+hipError_t launch_ib_correct(const DevGrid& g, const double* Uint, const double* Vint,
+                             const double* Wint, const double* Ureg, const double* Vreg,
+                             const double* Wreg, const double* Pint, const double* phi, double dt,
+                             double* U, double* V, double* W, hipStream_t s) {
+  hipLaunchKernelGGL(k_ib_correct, dim3((g.nxg + 63) / 64, (g.nyg + 3) / 4, g.nzg), dim3(kBlock), 0,
+                     s, g, Uint, Vint, Wint, Ureg, Vreg, Wreg, Pint, phi, dt, U, V, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_affine(const double* x, double a, double b, double* y, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_affine, blocks_for(n), dim3(kBlock), 0, s, x, a, b, y, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_h_interp(const DevGrid& g, const double* Hc, double* Hu, double* Hv, double* Hw,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_h_interp, dim3((g.nxg - 2 + 63) / 64, (g.nyg - 2 + 3) / 4, g.nzg - 1),
+                     dim3(kBlock), 0, s, g, Hc, Hu, Hv, Hw);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot(const double* a, const double* b, const double* a2, const double* b2,
+                      size_t n, double* part, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot, dim3(kDotBlocks), dim3(kBlock), 0, s, a, b, a2, b2, n, part, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_scalar(int step, double* sc, const double* part, double* host_err,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(64), 0, s, step, sc, part, host_err);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_p(const double* r, const double* v, const double* sc, double* p, size_t n,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_p, blocks_for(n), dim3(kBlock), 0, s, r, v, sc, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_hs(const double* x, const double* p, const double* r, const double* v,
+                          const double* sc, double* h, double* sv, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_hs, blocks_for(n), dim3(kBlock), 0, s, x, p, r, v, sc, h, sv, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* sc,
+                          double* x, double* r, double* part, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3(kDotBlocks), dim3(kBlock), 0, s, h, sv, t, sc, x, r, part, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p, double* v,
+                            size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_init, blocks_for(n), dim3(kBlock), 0, s, b, r, rhat, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_slip(const DevGrid& g, const Markers& m, const double* U, const double* V,
+                           const double* W, const double* E1nH, const double* fb,
+                           unsigned long long* red, hipStream_t s) {
+  hipLaunchKernelGGL(k_max_slip, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, U, V, W, E1nH, fb, red);
+  return hipGetLastError();
+}
+
+}  // namespace ib
+}  // namespace c3d

@@ -1,0 +1,119 @@
+// Immersed-boundary operators on the device: delta-kernel stencils,
+// interpolation E / E1n, spreading R, per-body mean removal, Schur-complement
+// pieces and BiCGSTAB vector kernels (ib.hip).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace c3d {
+namespace ib {
+
+// Stencil of one staggered location (u, v, w or cell centre) for all
+// markers, separable per axis: entry (a, t, l) at [(a*5 + t)*nb + l] for axis
+// a = 0 x, 1 y, 2 z and window position t = 0..4.  idx holds the canonical
+// Fortran index of the node, phi the 1-D kernel value deltafnc(grid, xb, d),
+// dd the signed distance grid - xb.  The 125-point weight of (i, j, k) is
+// ((dx*dymin*dz * phi_x[i]) * phi_y[j]) * phi_z[k], the reference's product
+// order (src/immersed_boundary_operators.f90:86-91).
+struct Stencil {
+  int* idx;
+  double* phi;
+  double* dd;
+};
+
+// Marker data and stencils resident on the device.
+struct Markers {
+  int nb, nzb, nxb, nxb1;
+  const double *xb, *yb, *zb, *normals, *sb, *ub;  // normals, ub: [3nb]
+  Stencil st[4];  // 0 u, 1 v, 2 w, 3 centre
+  double A;       // dx*dymin*dz
+  double factor;  // 1/(dx*dymin*dz)
+};
+
+// Node -> entry map of the spreading operator of one component: sorted
+// support node offsets, CSR row pointers and entries q = l*125 + point, each
+// row in ascending marker order.
+struct Spread {
+  int nnodes;
+  const int* node;
+  const int* ptr;
+  const int* q;
+};
+
+// Grid data the set-up kernel needs besides DevGrid.
+struct SetupGrid {
+  const double *x, *y, *z;  // face coordinates, Fortran index i at [i-1]
+  double dx, dz, dymin, Lxp, Lzp;
+};
+
+hipError_t launch_setup(const DevGrid& g, const SetupGrid& sg, const Markers& m, hipStream_t s);
+
+// E (with_dxn false) or E1n (true) of three staggered fields -> out[3nb].
+hipError_t launch_interp_uvw(const DevGrid& g, const Markers& m, const double* U, const double* V,
+                             const double* W, bool with_dxn, double* out, hipStream_t s);
+// E1n of a centre field in rhs_p layout -> out[nb], times sign.
+hipError_t launch_interp_c(const DevGrid& g, const Markers& m, const double* P, double sign,
+                           double* out, hipStream_t s);
+// Schur velocity rows: Sf[c*nb+l] = E_c(G(dt*phi) - Reg_c)[l] - E1nH[c*nb+l]*dudn[c*nb+l],
+// G the gradient of the rhs_p-shaped phi evaluated at the stencil nodes.
+hipError_t launch_schur_u(const DevGrid& g, const Markers& m, const double* phi, double dt,
+                          const double* Ureg, const double* Vreg, const double* Wreg,
+                          const double* E1nH, const double* dudn, double* Sf, hipStream_t s);
+// Velocity-rhs rows: out[c*nb+l] = -E_c(U)[l] + ub[c*nb+l].
+hipError_t launch_rhs_u(const DevGrid& g, const Markers& m, const double* U, const double* V,
+                        const double* W, double* out, hipStream_t s);
+// aux[c*nb+l] = dt*nu*f[c*nb+l] - dt*f[3nb+l]*n[c*nb+l]   (c = 0..2)
+hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, double* aux,
+                      hipStream_t s);
+// R: Reg_c[node] = sum over entries of w*factor*sb*f_c, written at support
+// nodes only (other nodes keep the zeros they were created with).
+hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const double* f,
+                         double* F, hipStream_t s);
+// f[nb] -= per-body mean; then f -= coef*pj when coef != nullptr.
+hipError_t launch_remove_mean(const Markers& m, double* f, const double* coef, const double* pj,
+                              hipStream_t s);
+// U = (Uint - Ureg) - G(dt*(Pint - phi)) on the gradient's index ranges,
+// (Uint - Ureg) elsewhere (src/projection.f90:83-92).
+hipError_t launch_ib_correct(const DevGrid& g, const double* Uint, const double* Vint,
+                             const double* Wint, const double* Ureg, const double* Vreg,
+                             const double* Wreg, const double* Pint, const double* phi, double dt,
+                             double* U, double* V, double* W, hipStream_t s);
+
+// Element-wise helpers.
+// y = a + b*x over n entries
+hipError_t launch_affine(const double* x, double a, double b, double* y, size_t n, hipStream_t s);
+// Heaviside face interpolation of the rhs_p-shaped Hc into zeroed fields
+// (src/heaviside.f90:40-42).
+hipError_t launch_h_interp(const DevGrid& g, const double* Hc, double* Hu, double* Hv, double* Hw,
+                           hipStream_t s);
+
+// BiCGSTAB (src/projection.f90:146-186) device scalars and vector kernels.
+// Scalar slots: 0 rho_o, 1 alpha, 2 om, 3 rho_n, 4 bta, 5 err.
+constexpr int kDotBlocks = 128;
+// part[slot*kDotBlocks + b] = block b's partial of sum a*b (slot selects one of
+// the partial arrays); two dots at once when a2 != nullptr (slot + 1).
+hipError_t launch_dot(const double* a, const double* b, const double* a2, const double* b2,
+                      size_t n, double* part, int slot, hipStream_t s);
+// Scalar steps: 0 initialise, 1 rho_n/bta, 2 alpha, 3 om, 4 err (also to
+// *host_err when not null).
+hipError_t launch_bicg_scalar(int step, double* sc, const double* part, double* host_err,
+                              hipStream_t s);
+// p = r + bta*(p - om*v)
+hipError_t launch_bicg_p(const double* r, const double* v, const double* sc, double* p, size_t n,
+                         hipStream_t s);
+// h = x + alpha*p;  s = r - alpha*v
+hipError_t launch_bicg_hs(const double* x, const double* p, const double* r, const double* v,
+                          const double* sc, double* h, double* sv, size_t n, hipStream_t s);
+// x = h + om*s;  r = s - om*t; partials of r.r into part slot 4
+hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* sc,
+                          double* x, double* r, double* part, size_t n, hipStream_t s);
+// r = b - r; rhat = r; p = v = 0
+hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p, double* v,
+                            size_t n, hipStream_t s);
+// max |E u - E1nH*dudn - ub| as double bits into *red (zeroed by the caller).
+hipError_t launch_max_slip(const DevGrid& g, const Markers& m, const double* U, const double* V,
+                           const double* W, const double* E1nH, const double* fb,
+                           unsigned long long* red, hipStream_t s);
+
+}  // namespace ib
+}  // namespace c3d

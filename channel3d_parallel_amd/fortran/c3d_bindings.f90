@@ -140,6 +140,88 @@ Module c3d_bindings
        Real(c_double), Value :: d
      End Function c3d_add_to_u
 
+     ! ---- immersed boundary ----
+     Integer(c_int) Function c3d_ib_setup(ctx, nb, xb, yb, zb, normals, sb, ub, nzb, nxb, nxb1) &
+          Bind(C, name="c3d_ib_setup")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: nb, nzb, nxb, nxb1
+       Real(c_double), Intent(In) :: xb(*), yb(*), zb(*), normals(*), sb(*), ub(*)
+     End Function c3d_ib_setup
+
+     Integer(c_int) Function c3d_ib_set_solver(ctx, tol, maxit, guess_mode) &
+          Bind(C, name="c3d_ib_set_solver")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: tol
+       Integer(c_int), Value :: maxit, guess_mode
+     End Function c3d_ib_set_solver
+
+     Integer(c_int) Function c3d_ib_set_force(ctx, fb) Bind(C, name="c3d_ib_set_force")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: fb(*)
+     End Function c3d_ib_set_force
+
+     Integer(c_int) Function c3d_ib_get_force(ctx, fb) Bind(C, name="c3d_ib_get_force")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: fb(*)
+     End Function c3d_ib_get_force
+
+     Integer(c_int) Function c3d_compute_ib_projection(ctx, dt, rk_step, iters, err) &
+          Bind(C, name="c3d_compute_ib_projection")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: dt
+       Integer(c_int), Value :: rk_step
+       Integer(c_int), Intent(Out) :: iters
+       Real(c_double), Intent(Out) :: err
+     End Function c3d_compute_ib_projection
+
+     Integer(c_int) Function c3d_ib_schur(ctx, f, Sf, dt, rk_step) Bind(C, name="c3d_ib_schur")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: f(*)
+       Real(c_double) :: Sf(*)
+       Real(c_double), Value :: dt
+       Integer(c_int), Value :: rk_step
+     End Function c3d_ib_schur
+
+     Integer(c_int) Function c3d_ib_bicgstab(ctx, x, b, tol, maxit, dt, rk_step, iters, err) &
+          Bind(C, name="c3d_ib_bicgstab")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: x(*)
+       Real(c_double), Intent(In) :: b(*)
+       Real(c_double), Value :: tol, dt
+       Integer(c_int), Value :: maxit, rk_step
+       Integer(c_int), Intent(Out) :: iters
+       Real(c_double), Intent(Out) :: err
+     End Function c3d_ib_bicgstab
+
+     Integer(c_int) Function c3d_check_slip(ctx, max_slip) Bind(C, name="c3d_check_slip")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: max_slip
+     End Function c3d_check_slip
+
+     Integer(c_int) Function c3d_ib_stats(ctx, last_iters, accum_iters, last_err) &
+          Bind(C, name="c3d_ib_stats")
+       Import :: c_int, c_ptr, c_double, c_long_long
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Intent(Out) :: last_iters
+       Integer(c_long_long), Intent(Out) :: accum_iters
+       Real(c_double), Intent(Out) :: last_err
+     End Function c3d_ib_stats
+
+     Integer(c_int) Function c3d_ib_set_exterior_pressure_gradient(ctx, flag) &
+          Bind(C, name="c3d_ib_set_exterior_pressure_gradient")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: flag
+     End Function c3d_ib_set_exterior_pressure_gradient
+
   End Interface
 
 Contains

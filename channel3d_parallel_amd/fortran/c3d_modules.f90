@@ -5,7 +5,9 @@
 !   projection              :: compute_non_IB_projection (src/projection.f90:22)!
 !   differential_operators  :: divergence, gradient (src/differential_operators.f90) !
 !   boundary_conditions     :: apply_boundary_conditions (src/boundary_conditions.f90:20) !
-!   monitor                 :: check_divergence (src/monitor.f90:210)          !
+!   monitor                 :: check_divergence, check_slip (src/monitor.f90:210-231) !
+!   projection              :: compute_IB_projection, schur, bicgstab          !
+!                              (src/projection.f90:48-186)                     !
 ! The reference keeps its state in module `global`; here c3d_global holds    !
 ! the library context, the sizes and the time-step scalars, and the velocity  !
 ! stays resident on the GPU between calls.                                   !
@@ -21,6 +23,14 @@ Module c3d_global
   Integer(Int32) :: nx, ny, nz, nxg, nyg, nzg
   Integer(Int32) :: rk_step = 1
   Real(Int64)    :: dt = 0d0, nu = 0d0
+  ! immersed boundary: a body is set up (body_type /= 'none'), BiCGSTAB
+  ! settings (src/global.f90, namelist &params) and iteration counters
+  Logical        :: ib_active = .False.
+  Real(Int64)    :: cg_tol = 1d-10
+  Integer(Int32) :: cg_max_iter = 1000
+  Integer(Int32) :: cg_guess = 0   ! 0 previous force (reference), 1 zero
+  Integer(Int32) :: cg_last_iter = 0
+  Integer(Int64) :: cg_accum_iter = 0
 
 Contains
 
@@ -151,7 +161,64 @@ Contains
     Call c3d_check(c3d_compute_non_ib_projection(ctx, dt, Int(rk_step, c_int)), &
                    'compute_non_IB_projection')
   End Subroutine compute_non_IB_projection
+
+  ! compute_IB_projection, src/projection.f90:48-96, on the resident state.
+  ! Non-convergence is a warning as in the reference (:182-185).
+  Subroutine compute_IB_projection
+    Integer(c_int) :: rc, it
+    Real(c_double) :: err
+    Integer(c_long_long) :: acc
+    rc = c3d_compute_ib_projection(ctx, dt, Int(rk_step, c_int), it, err)
+    If (rc == -4) Then
+       Write(error_unit,'(A,I5,A,ES10.3,A,ES10.3)') 'WARNING: BiCGSTAB reached max iterations (', &
+            it, ') but residual norm (', Sqrt(err), ') exceeds requested tolerance ', cg_tol
+    Else
+       Call c3d_check(rc, 'compute_IB_projection')
+    End If
+    Call c3d_check(c3d_ib_stats(ctx, it, acc, err), 'ib_stats')
+    cg_last_iter = it
+    cg_accum_iter = acc
+  End Subroutine compute_IB_projection
+
+  ! schur(Sf_, f_), src/projection.f90:98-144 (host arrays).
+  Subroutine schur(Sf_, f_)
+    Real(Int64), Contiguous, Intent(In)  :: f_(:)
+    Real(Int64), Contiguous, Intent(Out) :: Sf_(:)
+    Call c3d_check(c3d_ib_schur(ctx, f_, Sf_, dt, Int(rk_step, c_int)), 'schur')
+  End Subroutine schur
+
+  ! bicgstab(bcg_x, bcg_b), src/projection.f90:146-186 (host arrays).
+  Subroutine bicgstab(bcg_x, bcg_b)
+    Real(Int64), Contiguous, Intent(InOut) :: bcg_x(:)
+    Real(Int64), Contiguous, Intent(In)    :: bcg_b(:)
+    Integer(c_int) :: rc, it
+    Real(c_double) :: err
+    rc = c3d_ib_bicgstab(ctx, bcg_x, bcg_b, cg_tol, Int(cg_max_iter, c_int), dt, &
+                         Int(rk_step, c_int), it, err)
+    If (rc /= -4) Call c3d_check(rc, 'bicgstab')
+    cg_accum_iter = cg_accum_iter + it
+  End Subroutine bicgstab
 End Module projection
+
+!------------------------------------------------------------------------------!
+Module immersed_boundary_operators
+  Use c3d_global
+  Implicit None
+Contains
+  ! setup_IB_operators (src/immersed_boundary_operators.f90:19-256) with the
+  ! Heaviside fields of compute_heaviside (src/heaviside.f90:20-54) for a
+  ! static body; markers as in src/global.f90 (normals, ub: 3*nb).
+  Subroutine setup_IB_operators(xb, yb, zb, normals, sb, ub, nzb, nxb, nxb1)
+    Real(Int64), Contiguous, Intent(In) :: xb(:), yb(:), zb(:), normals(:), sb(:), ub(:)
+    Integer(Int32), Intent(In) :: nzb, nxb, nxb1
+    Call c3d_check(c3d_ib_setup(ctx, Int(Size(xb), c_int), xb, yb, zb, normals, sb, ub, &
+                                Int(nzb, c_int), Int(nxb, c_int), Int(nxb1, c_int)), &
+                   'setup_IB_operators')
+    Call c3d_check(c3d_ib_set_solver(ctx, cg_tol, Int(cg_max_iter, c_int), Int(cg_guess, c_int)), &
+                   'ib_set_solver')
+    ib_active = .True.
+  End Subroutine setup_IB_operators
+End Module immersed_boundary_operators
 
 !------------------------------------------------------------------------------!
 Module monitor
@@ -163,4 +230,10 @@ Contains
     Real(Int64), Intent(Out) :: max_divergence
     Call c3d_check(c3d_check_divergence(ctx, max_divergence), 'check_divergence')
   End Subroutine check_divergence
+
+  ! check_slip(max_slip), src/monitor.f90:223-231 (resident state).
+  Subroutine check_slip(max_slip)
+    Real(Int64), Intent(Out) :: max_slip
+    Call c3d_check(c3d_check_slip(ctx, max_slip), 'check_slip')
+  End Subroutine check_slip
 End Module monitor

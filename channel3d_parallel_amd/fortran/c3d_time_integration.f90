@@ -67,7 +67,7 @@ Module time_integration
   Use projection
   Implicit None
 Contains
-  ! compute_time_step_RK3, src/time_integration.f90:86-213 (no body)
+  ! compute_time_step_RK3, src/time_integration.f90:86-213 (static body)
   Subroutine compute_time_step_RK3
     Real(Int64) :: to
     Integer(Int32) :: stage
@@ -80,6 +80,10 @@ Contains
        t = to + rk_t(stage)*dt
        Call apply_boundary_conditions_resident
        Call compute_non_IB_projection
+       If (ib_active) Then                                             ! :119-122
+          Call apply_boundary_conditions_resident
+          Call compute_IB_projection
+       End If
        Call apply_boundary_conditions_resident
     End Do
     If (x_mass_cte == 1) Then                                          ! :187-193

@@ -31,7 +31,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import ptr
+from ._lib import iptr, ptr
 
 
 class C3DError(RuntimeError):
@@ -176,6 +176,101 @@ class Channel:
     def compute_time_step_RK3(self, dt, x_mass_cte=0, q0=0.0):
         """One compute_time_step_RK3 (src/time_integration.f90:86-193)."""
         self._chk(self.L.c3d_time_step_rk3(self.h, dt, x_mass_cte, q0))
+
+    # immersed boundary (src/immersed_boundary_operators.f90, src/heaviside.f90,
+    # src/projection.f90:48-186) ------------------------------------------------
+    def setup_IB(self, m):
+        """setup_IB_operators + compute_heaviside for a marker set ``m``
+        (attributes xb, yb, zb, normals, sb, ub, nzb, nxb, nxb1)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64)
+                for a in (m.xb, m.yb, m.zb, m.normals, m.sb, m.ub)]
+        self.nb = int(arrs[0].size)
+        self._chk(self.L.c3d_ib_setup(self.h, self.nb, *[ptr(a) for a in arrs],
+                                      int(m.nzb), int(m.nxb), int(m.nxb1)))
+
+    def ib_stencil(self, s):
+        n = self.nb * 125
+        xi, yi, zi = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        w, dxn = np.zeros(n), np.zeros(n)
+        self._chk(self.L.c3d_ib_get_stencil(self.h, s, iptr(xi), iptr(yi), iptr(zi), ptr(w),
+                                            ptr(dxn)))
+        sh = (self.nb, 125)
+        return xi.reshape(sh), yi.reshape(sh), zi.reshape(sh), w.reshape(sh), dxn.reshape(sh)
+
+    def ib_heaviside(self):
+        a, b = np.zeros(self.nb), np.zeros(3 * self.nb)
+        Hu, Hv, Hw, Hc = self.zeros("u"), self.zeros("v"), self.zeros("w"), self.zeros("rhs")
+        self._chk(self.L.c3d_ib_get_heaviside(self.h, ptr(a), ptr(b), ptr(Hu), ptr(Hv), ptr(Hw),
+                                              ptr(Hc)))
+        return dict(E1nHc=a, E1nH=b, Hu_int=Hu, Hv_int=Hv, Hw_int=Hw, Hc_ext=Hc)
+
+    def regT(self, with_dxn=False):
+        """E (regT) or E1n (regT_1n) of the resident velocity -> [3nb]."""
+        out = np.zeros(3 * self.nb)
+        self._chk(self.L.c3d_ib_interpolate(self.h, int(with_dxn), ptr(out)))
+        return out
+
+    def regTc_1n(self, p):
+        p = np.asfortranarray(p, dtype=np.float64)
+        out = np.zeros(self.nb)
+        self._chk(self.L.c3d_ib_interpolate_c(self.h, ptr(p), ptr(out)))
+        return out
+
+    def reg(self, comp, f):
+        """regu / regv / regw (comp 0, 1, 2) of f[nb] -> field."""
+        F = self.zeros("uvw"[comp])
+        self._chk(self.L.c3d_ib_spread(self.h, comp, ptr(np.ascontiguousarray(f, dtype=np.float64)),
+                                       ptr(F)))
+        return F
+
+    def remove_mean_per_body(self, f):
+        f = np.array(f, dtype=np.float64)
+        self._chk(self.L.c3d_ib_remove_mean(self.h, ptr(f)))
+        return f
+
+    def schur(self, f, dt, rk_step=2):
+        Sf = np.zeros(4 * self.nb)
+        self._chk(self.L.c3d_ib_schur(self.h, ptr(np.ascontiguousarray(f, dtype=np.float64)),
+                                      ptr(Sf), dt, rk_step))
+        return Sf
+
+    def bicgstab(self, b, x0=None, tol=1e-10, maxit=1000, dt=1.0, rk_step=2):
+        """Returns (x, iterations, ||r||^2); raises C3DError on no convergence."""
+        x = np.zeros(4 * self.nb) if x0 is None else np.array(x0, dtype=np.float64)
+        it, err = C.c_int(), C.c_double()
+        self._chk(self.L.c3d_ib_bicgstab(self.h, ptr(x), ptr(np.ascontiguousarray(b, dtype=np.float64)),
+                                         tol, maxit, dt, rk_step, C.byref(it), C.byref(err)))
+        return x, it.value, err.value
+
+    def set_IB_solver(self, tol=1e-10, maxit=1000, guess_mode=0):
+        self._chk(self.L.c3d_ib_set_solver(self.h, tol, maxit, guess_mode))
+
+    def set_IB_force(self, fb):
+        self._chk(self.L.c3d_ib_set_force(self.h, ptr(np.ascontiguousarray(fb, dtype=np.float64))))
+
+    def get_IB_force(self):
+        fb = np.zeros(4 * self.nb)
+        self._chk(self.L.c3d_ib_get_force(self.h, ptr(fb)))
+        return fb
+
+    def compute_IB_projection(self, dt, rk_step=2):
+        """Returns (iterations, ||r||^2)."""
+        it, err = C.c_int(), C.c_double()
+        self._chk(self.L.c3d_compute_ib_projection(self.h, dt, rk_step, C.byref(it), C.byref(err)))
+        return it.value, err.value
+
+    def check_slip(self) -> float:
+        v = C.c_double()
+        self._chk(self.L.c3d_check_slip(self.h, C.byref(v)))
+        return v.value
+
+    def ib_stats(self):
+        a, b, e = C.c_int(), C.c_longlong(), C.c_double()
+        self._chk(self.L.c3d_ib_stats(self.h, C.byref(a), C.byref(b), C.byref(e)))
+        return a.value, b.value, e.value
+
+    def set_exterior_pressure_gradient(self, flag):
+        self._chk(self.L.c3d_ib_set_exterior_pressure_gradient(self.h, int(flag)))
 
     # measurement ---------------------------------------------------------------
     def copy_bandwidth(self, nbytes, iters=20):

@@ -137,7 +137,8 @@ int c3d_add_to_u(c3d_ctx *ctx, double d);
 
 /* One compute_time_step_RK3 (src/time_integration.f90:86-193) on the
  * resident state: three stages of right-hand sides, stage update,
- * boundary conditions, non-IB projection, boundary conditions; then, if
+ * boundary conditions, non-IB projection, boundary conditions (with a body
+ * set up: then the IB projection and boundary conditions, :118-123); then, if
  * x_mass_cte == 1, the constant-flow correction toward q0 (Qflow_x_0),
  * which also updates dPdx. */
 int c3d_time_step_rk3(c3d_ctx *ctx, double dt, int x_mass_cte, double q0);
@@ -155,6 +156,77 @@ int c3d_stage_count(void);
  * achievable HBM bandwidth and the traffic counters. */
 int c3d_copy_bandwidth(c3d_ctx *ctx, size_t nbytes, int iters, double *ms);
 const char *c3d_stage_name(int i);
+
+/* ---- Immersed boundary ----------------------------------------------
+ * Marker arrays follow src/global.f90:228-247: xb, yb, zb, sb [nb];
+ * normals, ub [3nb] component-major (x block, y block, z block).  Markers
+ * are grouped in nzb z slabs of nxb; the first nxb1 of a slab belong to
+ * body 1, the rest to body 2 (the remove_mean_per_body grouping of
+ * src/immersed_boundary_geometry.f90:417-441).  Static bodies on a uniform
+ * y grid (grid_type 0, y_ref_index = 1), whose 5x5x5 stencils stay clear of
+ * the wall ghost rows. */
+
+/* setup_IB_operators (src/immersed_boundary_operators.f90:19-256) followed
+ * by compute_heaviside (src/heaviside.f90:20-54): builds the four delta
+ * stencils on the device, the spreading map and the Heaviside fields and
+ * coefficients.  Once per context. */
+int c3d_ib_setup(c3d_ctx *ctx, int nb, const double *xb, const double *yb, const double *zb,
+                 const double *normals, const double *sb, const double *ub, int nzb, int nxb,
+                 int nxb1);
+/* Stencil s (0 u, 1 v, 2 w, 3 centre) as the reference stores it
+ * (src/immersed_boundary_operators.f90:46-244): for marker l and point
+ * p = i + 5*(j + 5*k), entry [l*125 + p] of the Fortran node indices, the
+ * weight dx*dymin*dz*phi_x*phi_y*phi_z and dxn = n . (x_node - xb).
+ * Arrays of nb*125; any may be NULL. */
+int c3d_ib_get_stencil(c3d_ctx *ctx, int s, int *xi, int *yi, int *zi, double *w, double *dxn);
+/* Heaviside outputs: E1nHc_exterior [nb], E1nH_exterior [3nb], the
+ * interior masks Hu/Hv/Hw_interior (field shapes) and Hc_exterior (rhs_p
+ * shape).  Any may be NULL. */
+int c3d_ib_get_heaviside(c3d_ctx *ctx, double *E1nHc, double *E1nH, double *Hu_int,
+                         double *Hv_int, double *Hw_int, double *Hc_ext);
+/* regT (with_dxn 0) / regT_1n (1) of the resident U, V, W -> out[3nb]
+ * (src/immersed_boundary_operators.f90:296-342). */
+int c3d_ib_interpolate(c3d_ctx *ctx, int with_dxn, double *out);
+/* regTc_1n of a host centre field p (rhs_p shape) -> out[nb] (:344-357). */
+int c3d_ib_interpolate_c(c3d_ctx *ctx, const double *p, double *out);
+/* regu / regv / regw (comp 0, 1, 2) of f[nb] into the host field F of that
+ * component's shape, zero outside the support (:359-400, :544-603). */
+int c3d_ib_spread(c3d_ctx *ctx, int comp, const double *f, double *F);
+/* remove_mean_per_body of f[nb] in place
+ * (src/immersed_boundary_geometry.f90:400-442). */
+int c3d_ib_remove_mean(c3d_ctx *ctx, double *f);
+/* schur (src/projection.f90:98-144): Sf = S f for host f, Sf [4nb].  The
+ * resident velocity is not used as scratch (the reference overwrites it). */
+int c3d_ib_schur(c3d_ctx *ctx, const double *f, double *Sf, double dt, int rk_step);
+/* bicgstab (src/projection.f90:146-186) on host arrays: x [4nb] is the
+ * initial guess and the solution.  *iters, *err (= ||r||^2) may be NULL.
+ * Returns C3D_ENOCONV if maxit was reached above tolerance (x is still
+ * written back). */
+int c3d_ib_bicgstab(c3d_ctx *ctx, double *x, const double *b, double tol, int maxit, double dt,
+                    int rk_step, int *iters, double *err);
+/* Solver settings for the IB projection: absolute tolerance cg_tol,
+ * cg_max_iter, and the initial guess: guess_mode 0 = the previous force
+ * fb (the reference's warm start), 1 = zero, 2 = the force last set by
+ * c3d_ib_set_force for the next solve only, then 0. */
+int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
+/* Resident IB force fb = (dudn_jump[3nb], p_jump[nb]) [4nb]. */
+int c3d_ib_set_force(c3d_ctx *ctx, const double *fb);
+int c3d_ib_get_force(c3d_ctx *ctx, double *fb);
+/* compute_IB_projection (src/projection.f90:48-96) on the resident state,
+ * after c3d_compute_non_ib_projection (which keeps U_interim when a body is
+ * set up) and boundary conditions.  Returns C3D_ENOCONV with the fields
+ * updated if BiCGSTAB stopped at its iteration limit. */
+int c3d_compute_ib_projection(c3d_ctx *ctx, double dt, int rk_step, int *iters, double *err);
+/* check_slip (src/monitor.f90:223-231): max|E u - E1nH_exterior*dudn - ub|. */
+int c3d_check_slip(c3d_ctx *ctx, double *max_slip);
+/* BiCGSTAB statistics: iterations of the last solve, iterations summed over
+ * all solves (cg_accum_iter, src/projection.f90:181), last ||r||^2. */
+int c3d_ib_stats(c3d_ctx *ctx, int *last_iters, long long *accum_iters, double *last_err);
+/* exterior_pressure_gradient (src/input_output.f90:52, src/equations.f90:147-151):
+ * 1 multiplies the driving pressure gradient by H_interior, 0 applies it
+ * everywhere.  The bulk-flow integral is H_interior-masked once a body is
+ * set up (src/mass_flow.f90:24-53). */
+int c3d_ib_set_exterior_pressure_gradient(c3d_ctx *ctx, int flag);
 
 #ifdef __cplusplus
 }

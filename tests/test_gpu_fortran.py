@@ -91,3 +91,54 @@ def test_fortran_projection_driver(tmp_path, nx, ny, nz, gt, rk):
     if rk == 1:
         assert np.linalg.norm(outs[3] - P) <= 1e-12 * np.linalg.norm(P)
     assert max_div <= max(1e-12, 4 * g.max_abs_div(U, V, W))
+
+
+CYL_FD = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "cylinders_fd")
+
+
+def test_fortran_cylinders_matches_c_abi(tmp_path):
+    """The Fortran IB time loop (setup_IB_operators, compute_time_step_RK3 with
+    compute_IB_projection) gives bitwise the fields of the same steps driven
+    through the C ABI's compute_time_step_RK3: same kernels, same order."""
+    from channel3d_parallel_amd import Channel, ib_geometry
+    if not os.path.exists(CYL_FD):
+        pytest.fail("Fortran cylinders_fd not built (make -C channel3d_parallel_amd/fortran)")
+    nx, ny, nz, nsteps = 34, 33, 34, 3
+    nu, dt, dpdx = 1.0 / 2800.0, 0.004, 3.0 / 2800.0
+    print(f"\n[case] fortran cylinders_fd {nx}x{ny}x{nz} steps={nsteps}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=0)
+    m = ib_geometry.two_cylinders(g.scalars["Lzp"], nzb=16, nth=24)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.0, grid_type=0)
+    fmk, fin, fout = tmp_path / "mk.bin", tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fmk, "wb") as f:
+        np.array([m.nb, m.nzb, m.nxb, m.nxb1], dtype=np.int32).tofile(f)
+        np.concatenate([m.xb, m.yb, m.zb, m.normals, m.sb, m.ub]).tofile(f)
+    np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
+    r = subprocess.run([CYL_FD, str(nx), str(ny), str(nz), repr(nu), repr(dt), repr(dpdx),
+                        str(nsteps), "1", "0", str(fmk), str(fin), str(fout)],
+                       check=True, timeout=120, capture_output=True, text=True)
+    print(r.stdout)
+    raw = np.fromfile(fout)
+    outs, off = [], 0
+    for s in (g.shape_u, g.shape_v, g.shape_w):
+        n = int(np.prod(s))
+        outs.append(raw[off:off + n].reshape(s, order="F"))
+        off += n
+    fb_f = raw[off:off + 4 * m.nb]
+    div_f, slip_f, iters_f = raw[off + 4 * m.nb:off + 4 * m.nb + 3]
+    g.apply_bc(U, V, W)
+    with Channel(nx, ny, nz, grid_type=0, nu=nu) as ch:
+        ch.set_fields(U, V, W)
+        ch.setup_IB(m)
+        ch.set_pressure_gradient(dpdx, 0.0, 0.0)
+        ch.set_IB_solver(tol=1e-10, maxit=1000, guess_mode=0)
+        for _ in range(nsteps):
+            ch.compute_time_step_RK3(dt, x_mass_cte=0)
+        ref = ch.get_fields()
+        fb = ch.get_IB_force()
+        _, accum, _ = ch.ib_stats()
+    for name, a, b in (("U", outs[0], ref["U"]), ("V", outs[1], ref["V"]), ("W", outs[2], ref["W"])):
+        assert np.array_equal(a, b), name
+    assert np.array_equal(fb_f, fb)
+    assert int(iters_f) == accum
+    assert div_f < 1e-12 and slip_f < 1e-9
