@@ -48,8 +48,105 @@ def stage_bytes(nx, ny, nz, word=8):
         "tri_y": 2 * c * nspec,
         "fft_z_inv": 2 * c * nspec,
         "inv_x": c * nspec + word * nrhs,            # spectrum read; P_interim written
+        "refine": 0,                                 # FP64 path: no refinement sweeps
         "grad_correct": word * ngrad + 6 * word * ngrad,  # P_interim + U,V,W read; U,V,W written
     }
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def mixed_section(nx, ny, nz, U, V, W, dt, steps, ref, fp64_ms):
+    """Mixed precision (FP32 spectra + FP64 refinement) against the FP64 GPU
+    result of the same projection: speed ratio and error per sweep count."""
+    from channel3d_parallel_amd import Channel
+    from channel3d_parallel_amd._lib import PREC_MIXED
+
+    res = {}
+    with Channel(nx, ny, nz, grid_type=1, alpha=2.0, precision=PREC_MIXED) as ch:
+        for nref in (0, 1, 2, 3):
+            print(f"[bench] mixed precision, {nref} refinement sweeps", file=sys.stderr, flush=True)
+            ch.set_refinement(nref)
+            ch.set_fields(U, V, W)
+            ch.compute_non_IB_projection(dt)
+            rr = ch.refinement_residuals()
+            ch.apply_boundary_conditions()
+            div = ch.check_divergence()
+            f = ch.get_fields()
+            err_u = max(rel(f["U"], ref["U"]), rel(f["V"], ref["V"]), rel(f["W"], ref["W"]))
+            err_p = rel(f["P_interim"], ref["P_interim"])
+            del f
+            for _ in range(3):
+                ch.compute_non_IB_projection(dt)
+            ms, stages = ch.time_projection(steps, dt)
+            res[str(nref)] = {"solves_per_s": 1e3 / ms, "ms": ms, "speed_vs_fp64": fp64_ms / ms,
+                              "post_projection_max_div": div, "rel_err_velocity_vs_fp64": err_u,
+                              "rel_err_P_interim_vs_fp64": err_p,
+                              "sweep_residual_max": [float(x) for x in rr], "stages_ms": stages}
+    return res
+
+
+def rk3_section(nx, ny, nz, U, V, W, steps):
+    """ns per grid point per RK3 substep: compute_time_step_RK3 without a body."""
+    from channel3d_parallel_amd import Channel
+
+    nu = 1.0 / 2800.0
+    with Channel(nx, ny, nz, grid_type=1, alpha=2.0, nu=nu) as ch:
+        ch.set_fields(U, V, W)
+        ch.apply_boundary_conditions()
+        ch.set_pressure_gradient((180.0 * nu) ** 2, 0.0, 0.0)
+        dt = 1e-4
+        ch.compute_time_step_RK3(dt)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ch.compute_time_step_RK3(dt)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    ncell = (nx - 2) * (ny - 1) * (nz - 2)
+    return {"ms_per_step": ms, "ns_per_point_per_substep": ms * 1e6 / (3 * ncell),
+            "steps": steps, "note": "host-timed whole steps (3 substeps: rhs, update, BCs, projection)"}
+
+
+def ib_section(nsteps=3):
+    """Two-cylinder IB case (130x129x130 uniform, nb = 24,576): plain
+    BiCGSTAB from a zero guess at tolerance 1e-10; iterations, slip and time."""
+    from channel3d_parallel_amd import Channel, fields, ib_geometry
+
+    nx, ny, nz = 130, 129, 130
+    nu, dt = 1.0 / 2800.0, 0.01
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.0, grid_type=0)
+    with Channel(nx, ny, nz, grid_type=0, nu=nu) as ch:
+        t0 = time.perf_counter()
+        m = ib_geometry.two_cylinders(ch.grid(11)[9])
+        ch.setup_IB(m)
+        setup_ms = (time.perf_counter() - t0) * 1e3
+        ch.set_fields(U, V, W)
+        ch.apply_boundary_conditions()
+        ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=1)
+        ch.compute_non_IB_projection(dt, 2)
+        ch.apply_boundary_conditions()
+        t0 = time.perf_counter()
+        it, err = ch.compute_IB_projection(dt, 2)
+        proj_ms = (time.perf_counter() - t0) * 1e3
+        slip, div = ch.check_slip(), ch.check_divergence()
+        ch.set_pressure_gradient(3.0 * nu, 0.0, 0.0)
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=1)
+            _, before, _ = ch.ib_stats()
+            ch.compute_time_step_RK3(dt)
+            _, after, _ = ch.ib_stats()
+            iters.append(after - before)
+        step_ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        slip_step = ch.check_slip()
+    return {"workload": "two counter-rotating cylinders r=0.2 at (2.0,0.7),(4.0,1.3), 96x128 "
+                        "markers each, 128x128x128 cells uniform y, dt 0.01",
+            "nb": m.nb, "setup_ms": setup_ms, "bicgstab_iterations_zero_guess": it,
+            "residual_norm": float(np.sqrt(err)), "ib_projection_ms": proj_ms,
+            "ms_per_iteration": proj_ms / max(it, 1), "max_slip": slip, "max_div": div,
+            "rk3_step_ms": step_ms, "iterations_per_step_zero_guess": iters,
+            "max_slip_after_steps": slip_step}
 
 
 def read_traffic(kernel, workload):
@@ -86,6 +183,8 @@ def main():
     ap.add_argument("--workload", default="proj384", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-solves", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-precision, RK3 and IB sections")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported; this build measures one GPU "
@@ -115,6 +214,7 @@ def main():
         ch.compute_non_IB_projection(dt)
         ch.apply_boundary_conditions()
         max_div = ch.check_divergence()
+        ref = None if a.no_extras else ch.get_fields()
         copy_bytes = 1 << 30
         copy_ms = ch.copy_bandwidth(copy_bytes, iters=10)
         print("[bench] warm-up", file=sys.stderr, flush=True)
@@ -130,7 +230,7 @@ def main():
     achieved = sb[dom] / (stages[dom] * 1e-3) / 1e9
     total_bytes = sum(sb.values())
     out = {
-        "metric": "projection solves/s (FP64, 384x193x384)",
+        "metric": "projection solves/s (FP64, 384x193x384); mixed vs FP64 at equal div err and IB iterations reported beside it",
         "value": 1e3 / tot_ms,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -158,6 +258,13 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(dom, a.workload)},
     }
+    if not a.no_extras:
+        out["mixed"] = mixed_section(nx, ny, nz, U, V, W, dt, a.steps, ref, tot_ms)
+        del ref
+        print("[bench] RK3 steps", file=sys.stderr, flush=True)
+        out["rk3"] = rk3_section(nx, ny, nz, U, V, W, 5)
+        print("[bench] IB two-cylinder case", file=sys.stderr, flush=True)
+        out["ib"] = ib_section()
     if not a.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         print(f"[bench] CPU baseline: oracle, {threads} threads, {a.cpu_solves} solves",

@@ -56,6 +56,8 @@ SIGNATURES = [
     ("c3d_copy_bandwidth", C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _dp]),
     ("c3d_stage_count", C.c_int, []),
     ("c3d_stage_name", C.c_char_p, [C.c_int]),
+    ("c3d_set_refinement", C.c_int, [C.c_void_p, C.c_int]),
+    ("c3d_refinement_residuals", C.c_int, [C.c_void_p, _dp, C.c_int]),
     ("c3d_ib_setup", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
                                C.c_int]),
     ("c3d_ib_get_stencil", C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]),

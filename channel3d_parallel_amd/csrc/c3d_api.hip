@@ -65,6 +65,12 @@ struct c3d_ctx {
   // the bulk-flow integral
   const double *Hu = nullptr, *Hv = nullptr, *Hw = nullptr, *Hm = nullptr;
   IBState* ib = nullptr;
+  // mixed precision (C3D_PREC_MIXED): FP32 spectrum, FP64 divergence b and
+  // potential phi, FP32 residual; nref FP64 refinement sweeps per projection
+  cpx<float>* S32 = nullptr;
+  double *b64 = nullptr, *phi64 = nullptr, *gauge = nullptr;
+  float* r32 = nullptr;
+  int nref = 2;
   std::string err;
 };
 
@@ -243,8 +249,10 @@ int spectral_core(c3d_ctx* c) {
 }
 
 const char* kStageNames[] = {"fwd_x_div", "fft_z_fwd", "tri_y", "fft_z_inv", "inv_x",
-                             "grad_correct"};
-constexpr int kNumStages = 6;
+                             "refine", "grad_correct"};
+constexpr int kNumStages = 7;
+constexpr int kMaxRef = 8;          // refinement sweeps per mixed projection
+constexpr int kRedRefine = 4;       // first reduction slot of the sweep residual maxima
 
 }  // namespace
 
@@ -259,7 +267,9 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   if (!c3d::fft_length_supported(d.nx - 2) || !c3d::fft_length_supported(d.nz - 2))
     return C3D_EINVAL;
   if (d.grid_type != 0 && d.grid_type != 1) return C3D_EINVAL;
-  if (d.precision != C3D_PREC_FP64) return C3D_EINVAL;
+  if (d.precision != C3D_PREC_FP64 && d.precision != C3D_PREC_MIXED) return C3D_EINVAL;
+  if (d.precision == C3D_PREC_MIXED && d.ny - 1 != 64 && d.ny - 1 != 128 && d.ny - 1 != 192)
+    return C3D_EINVAL;  // FP32-storage tridiagonal lengths
   if (!(d.Lxp > 0) || !(d.Lzp > 0) || !(d.Ly > 0)) return C3D_EINVAL;
 
   c3d_ctx* c = new c3d_ctx();
@@ -312,7 +322,12 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
       (rc = dalloc(c, &c->W, c->nw)) || (rc = dalloc(c, &c->P, c->np)) ||
       (rc = dalloc(c, &c->Pint, c->nr)) || (rc = dalloc(c, &c->rhs, c->nr)) ||
       (rc = dalloc(c, &c->S, c->nspec)) || (rc = dalloc(c, &c->CP, c->nspec)) ||
-      (rc = dalloc(c, &c->red, 4)))
+      (rc = dalloc(c, &c->red, kRedRefine + kMaxRef)))
+    return cleanup(rc);
+  if (d.precision == C3D_PREC_MIXED &&
+      ((rc = dalloc(c, &c->S32, c->nspec)) || (rc = dalloc(c, &c->b64, c->nr)) ||
+       (rc = dalloc(c, &c->phi64, c->nr)) || (rc = dalloc(c, &c->r32, c->nr)) ||
+       (rc = dalloc(c, &c->gauge, 1))))
     return cleanup(rc);
   if (hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->V, 0, c->nv * 8, c->stream) != hipSuccess ||
@@ -435,22 +450,54 @@ int c3d_solve_poisson_equation(c3d_ctx* c, double* rhs, int rk_step, double dt) 
   return sync(c);
 }
 
+// compute_non_IB_projection (src/projection.f90:22-46) on the resident
+// velocity.  FP64: one spectral solve.  Mixed: the FP32-storage solve of the
+// divergence, then nref sweeps of FP64 residual r = b - D G phi and an FP32
+// correction solve, so that the potential converges to the FP64 solution of
+// the projection's own discrete operator.
 static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev) {
   const bool rk1 = rk_step == 1;
-  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-  HIPCHK(c, c3d::launch_fwd_x(c->g, true, c->U, c->V, c->W, nullptr, c->S, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  HIPCHK(c, c3d::launch_fft_z(c->g, -1, c->S, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  HIPCHK(c, c3d::launch_tri(c->g, c->S, c->CP, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
-  HIPCHK(c, c3d::launch_fft_z(c->g, +1, c->S, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
-  HIPCHK(c, c3d::launch_inv_x(c->g, 1, c->S, c->Pint, c->P, dt, rk1, c->stream));
-  if (rk1) HIPCHK(c, c3d::launch_p_ghosts(c->g, c->P, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[5], c->stream));
-  HIPCHK(c, c3d::launch_grad_correct(c->g, c->U, c->V, c->W, c->Pint, dt, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[6], c->stream));
+  const bool mixed = c->desc.precision == C3D_PREC_MIXED;
+  const DevGrid& g = c->g;
+  hipStream_t st = c->stream;
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], st));
+  if (mixed) {
+    HIPCHK(c, hipMemsetAsync(c->red + kRedRefine, 0, kMaxRef * sizeof(unsigned long long), st));
+    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, c->b64, nullptr, c->S32, st));
+  } else {
+    HIPCHK(c, c3d::launch_fwd_x(g, true, c->U, c->V, c->W, nullptr, c->S, st));
+  }
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], st));
+  HIPCHK(c, mixed ? c3d::launch_fft_z32(g, -1, c->S32, st) : c3d::launch_fft_z(g, -1, c->S, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], st));
+  HIPCHK(c, mixed ? c3d::launch_tri32(g, c->S32, c->CP, st) : c3d::launch_tri(g, c->S, c->CP, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[3], st));
+  HIPCHK(c, mixed ? c3d::launch_fft_z32(g, +1, c->S32, st) : c3d::launch_fft_z(g, +1, c->S, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[4], st));
+  if (!mixed)
+    HIPCHK(c, c3d::launch_inv_x(g, 1, c->S, c->Pint, c->P, dt, rk1, st));
+  else if (c->nref == 0)
+    HIPCHK(c, c3d::launch_inv_x32(g, 1, c->S32, nullptr, c->Pint, c->P, dt, rk1, st));
+  else
+    HIPCHK(c, c3d::launch_inv_x32(g, 0, c->S32, nullptr, c->phi64, c->P, dt, false, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[5], st));
+  for (int s = 1; mixed && s <= c->nref; s++) {
+    const bool last = s == c->nref;
+    HIPCHK(c, c3d::launch_residual(g, c->b64, c->phi64, c->gauge, c->r32,
+                                   c->red + kRedRefine + s - 1, st));
+    HIPCHK(c, c3d::launch_fwd_x32(g, nullptr, nullptr, nullptr, nullptr, c->r32, c->S32, st));
+    HIPCHK(c, c3d::launch_fft_z32(g, -1, c->S32, st));
+    HIPCHK(c, c3d::launch_tri32(g, c->S32, c->CP, st));
+    HIPCHK(c, c3d::launch_fft_z32(g, +1, c->S32, st));
+    if (last)
+      HIPCHK(c, c3d::launch_inv_x32(g, 4, c->S32, c->phi64, c->Pint, c->P, dt, rk1, st));
+    else
+      HIPCHK(c, c3d::launch_inv_x32(g, 3, c->S32, c->phi64, c->phi64, c->P, dt, false, st));
+  }
+  if (rk1) HIPCHK(c, c3d::launch_p_ghosts(g, c->P, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[6], st));
+  HIPCHK(c, c3d::launch_grad_correct(g, c->U, c->V, c->W, c->Pint, dt, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[7], st));
   return 0;
 }
 
@@ -700,6 +747,30 @@ int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
   }
   if (!rc) rc = sync(c);
   return rc;
+}
+
+int c3d_set_refinement(c3d_ctx* c, int nref) {
+  if (!c) return C3D_EINVAL;
+  if (c->desc.precision != C3D_PREC_MIXED)
+    return fail(c, C3D_EINVAL, "refinement sweeps apply to C3D_PREC_MIXED contexts");
+  if (nref < 0 || nref > kMaxRef) return fail(c, C3D_EINVAL, "nref must be 0..%d", kMaxRef);
+  c->nref = nref;
+  return C3D_OK;
+}
+
+int c3d_refinement_residuals(c3d_ctx* c, double* rmax, int n) {
+  if (!c || !rmax || n < 0) return C3D_EINVAL;
+  if (c->desc.precision != C3D_PREC_MIXED)
+    return fail(c, C3D_EINVAL, "refinement sweeps apply to C3D_PREC_MIXED contexts");
+  const int m = n < c->nref ? n : c->nref;
+  unsigned long long bits[kMaxRef] = {0};
+  if (m > 0)
+    HIPCHK(c, hipMemcpyAsync(bits, c->red + kRedRefine, m * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  std::memcpy(rmax, bits, m * sizeof(double));
+  return m;
 }
 
 int c3d_stage_count(void) { return kNumStages; }

@@ -43,13 +43,21 @@ int tile_rows(int nxp, int nzp) {
 // k = blockIdx.x*TK + 2 .. +TK-1.  Rows 2p and 2p+1 are transformed together
 // as the real and imaginary parts of one complex row and separated after.
 // ---------------------------------------------------------------------------
-template <typename real, int N, int TK, bool FROM_UVW>
+// Input sources of the forward x transform.
+constexpr int kSrcRhs = 0;    // rhs_p-shaped FP64 array
+constexpr int kSrcUVW = 1;    // divergence of U, V, W computed on the fly
+constexpr int kSrcUVWB = 2;   // same, and the FP64 divergence also stored to bout
+constexpr int kSrcRhs32 = 3;  // rhs_p-shaped FP32 array (refinement residual)
+
+template <typename real, int N, int TK, int SRC>
 __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __restrict__ U,
                                                   const double* __restrict__ V,
                                                   const double* __restrict__ W,
-                                                  const double* __restrict__ rhs,
+                                                  const void* __restrict__ rhs_in,
                                                   const cpx<real>* __restrict__ tw,
-                                                  cpx<real>* __restrict__ S) {
+                                                  cpx<real>* __restrict__ S,
+                                                  double* __restrict__ bout) {
+  constexpr bool FROM_UVW = (SRC == kSrcUVW || SRC == kSrcUVWB);
   constexpr int NP = TK / 2;
   constexpr int NT = kBlock / NP;
   constexpr int NH = N / 2 + 1;
@@ -95,8 +103,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
             v[r][p][5] = Wm[n + 1];
           }
         }
+      } else if constexpr (SRC == kSrcRhs32) {
+        const float* Rr = static_cast<const float*>(rhs_in) + ridx(2, j, k, nxg, nyg);
+#pragma unroll
+        for (int p = 0; p < NPL; p++) {
+          const int n = tid + p * kBlock;
+          if (N % kBlock == 0 || n < N) v[r][p][0] = (double)Rr[n];
+        }
       } else {
-        const double* Rr = rhs + ridx(2, j, k, nxg, nyg);
+        const double* Rr = static_cast<const double*>(rhs_in) + ridx(2, j, k, nxg, nyg);
 #pragma unroll
         for (int p = 0; p < NPL; p++) {
           const int n = tid + p * kBlock;
@@ -123,6 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
             } else {
               d[h] = v[r + h][p][0];
             }
+            if constexpr (SRC == kSrcUVWB) bout[ridx(n + 2, j, k0 + t0 + r + h + 2, nxg, nyg)] = d[h];
           }
           buf[((t0 + r) >> 1) * LP + lp(n)] = cmk((real)d[0], (real)d[1]);
         }
@@ -266,9 +282,9 @@ __global__ __launch_bounds__(kBlock) void k_tri64(DevGrid g, cpx<double>* __rest
 // chunk in CK), so the memory traffic is the rhs read twice, the solution
 // written once and the small checkpoint array.
 // ---------------------------------------------------------------------------
-template <int NCH, int C>
-__global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>* __restrict__ S,
-                                                          double* __restrict__ CK) {
+template <typename real, int NCH, int C>
+__global__ __launch_bounds__(kBlock) void k_tri_chunked(DevGrid g, cpx<real>* __restrict__ S,
+                                                        double* __restrict__ CK) {
   const int nzp = g.nzp;
   const size_t stride = (size_t)g.nxh * nzp;
   const size_t m = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -277,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>
   const double lamx = g.kxx[kx];
   const double lamz = g.kzz[kz];
   const bool zero_mode = (kx == 0 && kz == 0);
-  cpx<double>* s = S + m;
+  cpx<real>* s = S + m;
   double* ck = CK + m;
   constexpr int N = NCH * C;
 
@@ -290,7 +306,10 @@ __global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>
     ck[(size_t)(3 * c + 2) * stride] = yprev.y;
     cpx<double> b[C];
 #pragma unroll
-    for (int r = 0; r < C; r++) b[r] = s[(size_t)(c * C + r) * stride];
+    for (int r = 0; r < C; r++) {
+      const cpx<real> t = s[(size_t)(c * C + r) * stride];
+      b[r] = cmk((double)t.x, (double)t.y);
+    }
 #pragma unroll
     for (int r = 0; r < C; r++) {
       const int jj = c * C + r;
@@ -309,7 +328,10 @@ __global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>
     cpx<double> y[C];
     double cpv[C];
 #pragma unroll
-    for (int r = 0; r < C; r++) y[r] = s[(size_t)(c * C + r) * stride];
+    for (int r = 0; r < C; r++) {
+      const cpx<real> t = s[(size_t)(c * C + r) * stride];
+      y[r] = cmk((double)t.x, (double)t.y);
+    }
     double cp = ck[(size_t)(3 * c) * stride];
     cpx<double> yp = cmk(ck[(size_t)(3 * c + 1) * stride], ck[(size_t)(3 * c + 2) * stride]);
 #pragma unroll
@@ -330,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>
       cpx<double> x = y[r];
       if (jj < N - 1) x = cmk(x.x - cpv[r] * xnext.x, x.y - cpv[r] * xnext.y);
       xnext = x;
-      s[(size_t)jj * stride] = x;
+      s[(size_t)jj * stride] = cmk((real)x.x, (real)x.y);
     }
   }
 }
@@ -342,11 +364,14 @@ __global__ __launch_bounds__(kBlock) void k_tri64_chunked(DevGrid g, cpx<double>
 // MODE 0: out = phi (rhs_p), MODE 1: out = phi/dt (P_interim, with the zero
 // plane nzg that the divergence leaves), rk1: P = phi/(dt*8/15).
 // ---------------------------------------------------------------------------
+// Mixed-precision refinement modes: MODE 3: out = phi_in + phi (correction
+// added to the FP64 potential), MODE 4: out = (phi_in + phi)/dt (P_interim
+// from the corrected potential), rk1: P = (phi_in + phi)/(dt*8/15).
 template <typename real, int N, int TK, int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __restrict__ S,
                                                   const cpx<real>* __restrict__ tw,
-                                                  double* __restrict__ out, double* __restrict__ P,
-                                                  double dt, int rk1) {
+                                                  double* out, double* __restrict__ P,
+                                                  double dt, int rk1, const double* phi_in) {
   constexpr int NP = TK / 2;
   constexpr int NT = kBlock / NP;
   constexpr int NH = N / 2 + 1;
@@ -443,8 +468,9 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
         cpx<real> z;
         if ((t & 1) == 0) zpair[p] = buf[(t >> 1) * LP + lp(n)];
         z = zpair[p];
-        const double phi = (double)((t & 1) ? z.y : z.x) / norm;
-        const double o = (MODE == 1) ? phi / dt : phi;
+        double phi = (double)((t & 1) ? z.y : z.x) / norm;
+        if constexpr (MODE == 3 || MODE == 4) phi = phi_in[ridx(2, j, k, nxg, nyg) + n] + phi;
+        const double o = (MODE == 1 || MODE == 4) ? phi / dt : phi;
         orow[n] = o;
         if (rk1) prow[n] = phi / cP;
         if (n == 0) {  // x duplicate i = nxg-1 (offset nxp)
@@ -463,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
     }
   }
   (void)idup;
-  if (MODE == 1 && k0 == 0) {
+  if ((MODE == 1 || MODE == 4) && k0 == 0) {
     // rhs_p plane nzg is zero after the divergence; P_interim = 0/dt there.
     for (int i = 2 + tid; i <= nxg - 1; i += kBlock) out[ridx(i, j, g.nzg, nxg, nyg)] = 0.0;
   }
@@ -472,6 +498,79 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
     // also covers it: rhs(nxg-1, j, nzg) = rhs(2, j, nzg).
     out[ridx(idup, j, g.nzg, nxg, nyg)] = out[ridx(2, j, g.nzg, nxg, nyg)];
   }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual of the projection's discrete Poisson problem for the mixed
+// path: r = b - D(G(phi)) on the unique cells (i = 2..nxp+1, j = 2..nyg-1,
+// k = 2..nzp+1), stored in FP32.  b is the FP64 divergence of the interim
+// velocity and D, G are the staggered divergence and gradient with the
+// projection's index ranges: periodic in x and z (the face left of cell 2 is
+// the periodic image of face nx-1, resp. nz-1), no gradient on the wall
+// faces (j = 1, ny).  phi carries its periodic duplicates (i = nxg-1,
+// k = nzp+2).  |r| maxima per block go to rmax as double bits.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __restrict__ b,
+                                                     const double* __restrict__ phi,
+                                                     const double* __restrict__ gauge,
+                                                     float* __restrict__ r,
+                                                     unsigned long long* rmax) {
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int i = bx * 64 + (threadIdx.x & 63) + 2;
+  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  const int kb = bz * kGradKC + 2;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
+  const int nzp = g.nzp;
+  double amax = 0.0;
+  if (i <= nx - 1 && j <= nyg - 1) {
+    const int im = (i == 2) ? nx - 1 : i - 1;  // periodic image of cell 1
+    const double rx = g.rdxf[i], ry = g.rdyf[j];
+    const double hxm = g.dxg[(i == 2) ? nx - 1 : i - 1], hx = g.dxg[i];
+    const double hym = g.dyg[j - 1], hy = g.dyg[j];
+    for (int kk = 0; kk < kGradKC; kk++) {
+      const int k = kb + kk;
+      if (k > nzp + 1) break;
+      const int km = (k == 2) ? nz - 1 : k - 1;
+      const double p0 = phi[ridx(i, j, k, nxg, nyg)];
+      const double gxp = (phi[ridx(i + 1, j, k, nxg, nyg)] - p0) / hx;
+      const double gxm = (p0 - phi[ridx(im, j, k, nxg, nyg)]) / hxm;
+      const double gyp = (j <= ny - 1) ? (phi[ridx(i, j + 1, k, nxg, nyg)] - p0) / hy : 0.0;
+      const double gym = (j >= 3) ? (p0 - phi[ridx(i, j - 1, k, nxg, nyg)]) / hym : 0.0;
+      const double gzp = (phi[ridx(i, j, k + 1, nxg, nyg)] - p0) / g.dzg[k];
+      const double gzm = (p0 - phi[ridx(i, j, km, nxg, nyg)]) / g.dzg[km];
+      const double lap = (gxp - gxm) * rx + (gyp - gym) * ry + (gzp - gzm) * g.rdzf[k];
+      double res = b[ridx(i, j, k, nxg, nyg)] - lap;
+      if (j == 2) res -= *gauge;  // (0,0)-mode row of the spectral operator
+      r[ridx(i, j, k, nxg, nyg)] = (float)res;
+      amax = fmax(amax, fabs(res));
+    }
+  }
+  // wavefront max, one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
+  if ((threadIdx.x & 63) == 0 && rmax)
+    atomicMax(rmax, (unsigned long long)__double_as_longlong(amax));
+}
+
+// gauge = 0.5 * Dyy(2,2) * mean over the unique cells of phi(:, 2, :): the
+// spectral solver's (0,0) mode uses D(2)*3/2 (src/poisson.f90:69), i.e. the
+// operator L + 0.5*Dyy(2,2)*(plane mean at j = 2); a residual of that
+// operator makes refinement converge to the FP64 solver's gauge.  One block,
+// fixed-order reduction.
+__global__ __launch_bounds__(1024) void k_gauge(DevGrid g, const double* __restrict__ phi,
+                                                double* __restrict__ gauge) {
+  __shared__ double sh[1024];
+  const int nxp = g.nxp, nzp = g.nzp, nxg = g.nxg, nyg = g.nyg;
+  const int n = nxp * nzp;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < n; e += 1024) acc += phi[ridx(2 + e % nxp, 2, 2 + e / nxp, nxg, nyg)];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *gauge = 0.5 * g.dyy_d[0] * (sh[0] / (double)n);
 }
 
 // ---------------------------------------------------------------------------
@@ -722,35 +821,73 @@ hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const 
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
     if (from_uvw)
-      hipLaunchKernelGGL((k_fwd_x<double, NN, TK, true>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         rhs, g.twx64, S);
+      hipLaunchKernelGGL((k_fwd_x<double, NN, TK, kSrcUVW>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         (const void*)rhs, g.twx64, S, nullptr);
     else
-      hipLaunchKernelGGL((k_fwd_x<double, NN, TK, false>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         rhs, g.twx64, S);
+      hipLaunchKernelGGL((k_fwd_x<double, NN, TK, kSrcRhs>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         (const void*)rhs, g.twx64, S, nullptr);
   }));
   return hipGetLastError();
 }
 
-hipError_t launch_fft_z(const DevGrid& g, int sign, cpx<double>* S, hipStream_t s) {
+hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
+                          double* b, const float* r, cpx<float>* S, hipStream_t s) {
+  const int tk = tile_rows(g.nxp, g.nzp);
+  dim3 grid(g.nzp / tk, g.nym);
+  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
+    if (U)
+      hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcUVWB>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         (const void*)nullptr, g.twx32, S, b);
+    else
+      hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcRhs32>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         (const void*)r, g.twx32, S, nullptr);
+  }));
+  return hipGetLastError();
+}
+
+template <typename real>
+static hipError_t fft_z_any(const DevGrid& g, int sign, cpx<real>* S, const cpx<real>* tw,
+                            hipStream_t s) {
   const int ncol = g.nym * g.nxh;
   dim3 grid((ncol + 3) / 4);
   C3D_DISPATCH_N(g.nzp, {
     if (sign < 0)
-      hipLaunchKernelGGL((k_fft_z<double, NN, -1>), grid, dim3(kBlock), 0, s, S, g.twz64, ncol);
+      hipLaunchKernelGGL((k_fft_z<real, NN, -1>), grid, dim3(kBlock), 0, s, S, tw, ncol);
     else
-      hipLaunchKernelGGL((k_fft_z<double, NN, +1>), grid, dim3(kBlock), 0, s, S, g.twz64, ncol);
+      hipLaunchKernelGGL((k_fft_z<real, NN, +1>), grid, dim3(kBlock), 0, s, S, tw, ncol);
   });
   return hipGetLastError();
+}
+
+hipError_t launch_fft_z(const DevGrid& g, int sign, cpx<double>* S, hipStream_t s) {
+  return fft_z_any<double>(g, sign, S, g.twz64, s);
+}
+hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t s) {
+  return fft_z_any<float>(g, sign, S, g.twz32, s);
 }
 
 hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
   const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
   switch (g.nym) {
-    case 64: hipLaunchKernelGGL((k_tri64_chunked<4, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
-    case 128: hipLaunchKernelGGL((k_tri64_chunked<8, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
-    case 192: hipLaunchKernelGGL((k_tri64_chunked<12, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 64: hipLaunchKernelGGL((k_tri_chunked<double, 4, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 128: hipLaunchKernelGGL((k_tri_chunked<double, 8, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 192: hipLaunchKernelGGL((k_tri_chunked<double, 12, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
     default: hipLaunchKernelGGL(k_tri64, grid, dim3(kBlock), 0, s, g, S, CP); break;
+  }
+  return hipGetLastError();
+}
+
+bool tri32_supported(const DevGrid& g) { return g.nym == 64 || g.nym == 128 || g.nym == 192; }
+
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, double* CP, hipStream_t s) {
+  const size_t nm = (size_t)g.nxh * g.nzp;
+  const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
+  switch (g.nym) {
+    case 64: hipLaunchKernelGGL((k_tri_chunked<float, 4, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 128: hipLaunchKernelGGL((k_tri_chunked<float, 8, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 192: hipLaunchKernelGGL((k_tri_chunked<float, 12, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -762,11 +899,48 @@ hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
     if (mode == 0)
       hipLaunchKernelGGL((k_inv_x<double, NN, TK, 0>), grid, dim3(kBlock), 0, s, g, S, g.twx64,
-                         out, P, dt, rk1 ? 1 : 0);
+                         out, P, dt, rk1 ? 1 : 0, (const double*)nullptr);
     else
       hipLaunchKernelGGL((k_inv_x<double, NN, TK, 1>), grid, dim3(kBlock), 0, s, g, S, g.twx64,
-                         out, P, dt, rk1 ? 1 : 0);
+                         out, P, dt, rk1 ? 1 : 0, (const double*)nullptr);
   }));
+  return hipGetLastError();
+}
+
+hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
+                          double* out, double* P, double dt, bool rk1, hipStream_t s) {
+  const int tk = tile_rows(g.nxp, g.nzp);
+  dim3 grid(g.nzp / tk, g.nym);
+  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
+    switch (mode) {
+      case 0:
+        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 0>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
+                           out, P, dt, rk1 ? 1 : 0, phi_in);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 1>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
+                           out, P, dt, rk1 ? 1 : 0, phi_in);
+        break;
+      case 3:
+        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 3>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
+                           out, P, dt, rk1 ? 1 : 0, phi_in);
+        break;
+      case 4:
+        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 4>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
+                           out, P, dt, rk1 ? 1 : 0, phi_in);
+        break;
+      default: return hipErrorInvalidValue;
+    }
+  }));
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                           float* r, unsigned long long* rmax, hipStream_t s) {
+  hipLaunchKernelGGL(k_gauge, dim3(1), dim3(1024), 0, s, g, phi, gauge);
+  hipLaunchKernelGGL(k_residual,
+                     grid_pts(g, g.nxp, g.nyg - 2, (g.nzp + kGradKC - 1) / kGradKC),
+                     dim3(kBlock), 0, s, g, b, phi, (const double*)gauge, r, rmax);
   return hipGetLastError();
 }
 

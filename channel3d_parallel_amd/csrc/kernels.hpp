@@ -48,6 +48,24 @@ hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t 
 // mode 1: out = P_interim = solution/dt (projection).  rk1: also write P.
 hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
                         double dt, bool rk1, hipStream_t s);
+// Mixed-precision projection pieces (FP32 spectral storage, FP64 tridiagonal
+// arithmetic, FP64 refinement).  launch_fwd_x32 with U != null: divergence of
+// U, V, W (FP64, also stored to b) -> FP32 spectrum; with U == null: the FP32
+// residual r -> FP32 spectrum.
+hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
+                          double* b, const float* r, cpx<float>* S, hipStream_t s);
+hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t s);
+bool tri32_supported(const DevGrid& g);
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, double* CP, hipStream_t s);
+// mode 0: out = phi, 1: out = phi/dt, 3: out = phi_in + phi,
+// 4: out = (phi_in + phi)/dt; rk1 writes P from the same potential.
+hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
+                          double* out, double* P, double dt, bool rk1, hipStream_t s);
+// r = b - M phi (FP64, stored FP32) on the unique cells, M = D G plus the
+// spectral solver's (0,0)-mode gauge row; gauge is one double of scratch;
+// max |r| as double bits into *rmax (may be null; zeroed by the caller).
+hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                           float* r, unsigned long long* rmax, hipStream_t s);
 // Ghost copies of P (src/poisson.f90:93-98), in the reference order.
 hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s);
 // U -= G(dt * Pint) on the gradient's index ranges.

@@ -272,6 +272,18 @@ class Channel:
     def set_exterior_pressure_gradient(self, flag):
         self._chk(self.L.c3d_ib_set_exterior_pressure_gradient(self.h, int(flag)))
 
+    # mixed precision -----------------------------------------------------------
+    def set_refinement(self, nref):
+        """FP64 refinement sweeps per projection of a precision='mixed' channel."""
+        self._chk(self.L.c3d_set_refinement(self.h, nref))
+
+    def refinement_residuals(self):
+        out = np.zeros(8)
+        n = self.L.c3d_refinement_residuals(self.h, ptr(out), out.size)
+        if n < 0:
+            self._chk(n)
+        return out[:n].copy()
+
     # measurement ---------------------------------------------------------------
     def copy_bandwidth(self, nbytes, iters=20):
         """Average ms of a device streaming copy of nbytes (calibration)."""

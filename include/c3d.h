@@ -38,7 +38,8 @@ extern "C" {
 #define C3D_ENAN (-5)     /* non-finite value detected */
 
 #define C3D_PREC_FP64 0  /* plain FP64 path */
-#define C3D_PREC_MIXED 1 /* FP32 spectral solve + FP64 iterative refinement */
+#define C3D_PREC_MIXED 1 /* FP32 spectral solve + FP64 iterative refinement;
+                            wall-normal length ny-1 in {64, 128, 192} */
 
 typedef struct c3d_ctx c3d_ctx;
 
@@ -151,6 +152,16 @@ int c3d_time_step_rk3(c3d_ctx *ctx, double dt, int x_mass_cte, double q0);
 int c3d_time_projection(c3d_ctx *ctx, int iters, double dt, double *total_ms, double *kernel_ms,
                         int nk);
 int c3d_stage_count(void);
+/* Mixed precision (C3D_PREC_MIXED contexts): number of FP64 refinement
+ * sweeps per projection (0..8, default 2).  The first solve and every
+ * correction run the x/z transforms on FP32 spectra with FP64 tridiagonal
+ * arithmetic; each sweep forms the FP64 residual b - D G phi of the
+ * projection's own staggered operator.  No reference counterpart: the
+ * reference solves in FP64 only (src/poisson.f90:33-101). */
+int c3d_set_refinement(c3d_ctx *ctx, int nref);
+/* max |b - D G phi| entering each sweep of the last mixed projection,
+ * rmax[0..min(n, nref)); returns the count written. */
+int c3d_refinement_residuals(c3d_ctx *ctx, double *rmax, int n);
 /* Streaming device-to-device copy of nbytes (multiple of 16), `iters`
  * times; *ms = average ms per copy.  No reference counterpart: calibrates
  * achievable HBM bandwidth and the traffic counters. */
