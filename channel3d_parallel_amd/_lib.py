@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libc3d.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 C3D_OK = 0
 ERRORS = {-1: "C3D_EINVAL", -2: "C3D_EHIP", -3: "C3D_ENOMEM", -4: "C3D_ENOCONV", -5: "C3D_ENAN"}
@@ -76,6 +77,18 @@ SIGNATURES = [
     ("c3d_check_slip", C.c_int, [C.c_void_p, _dp]),
     ("c3d_ib_stats", C.c_int, [C.c_void_p, _ip, C.POINTER(C.c_longlong), _dp]),
     ("c3d_ib_set_exterior_pressure_gradient", C.c_int, [C.c_void_p, C.c_int]),
+    ("c3d_mlp_param_count", C.c_int, []),
+    ("c3d_mlp_init", C.c_int, [C.c_void_p, C.c_ulonglong]),
+    ("c3d_mlp_set_scales", C.c_int, [C.c_void_p, _fp, _fp]),
+    ("c3d_mlp_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("c3d_mlp_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("c3d_mlp_forward", C.c_int, [C.c_void_p, _fp, C.c_int, _fp]),
+    ("c3d_mlp_gradients", C.c_int, [C.c_void_p, _fp, _fp, C.c_int, _fp, _fp]),
+    ("c3d_mlp_train", C.c_int, [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                C.c_ulonglong, _fp]),
+    ("c3d_mlp_gemm_test", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp,
+                                    _fp, _fp]),
+    ("c3d_ib_last_pair", C.c_int, [C.c_void_p, _fp, _dp]),
 ]
 
 _lib = None
@@ -114,3 +127,12 @@ def iptr(a):
     if a.dtype != "int32" or not a.flags["C_CONTIGUOUS"]:
         raise TypeError("contiguous int32 array required")
     return a.ctypes.data_as(_ip)
+
+
+def fptr(a):
+    """Pointer to a contiguous float32 numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    if a.dtype != "float32" or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("contiguous float32 array required")
+    return a.ctypes.data_as(_fp)

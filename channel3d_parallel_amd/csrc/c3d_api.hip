@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -13,9 +15,27 @@
 #include "fft_lds.hpp"
 #include "ib.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 using c3d::cpx;
 using c3d::DevGrid;
+
+// Learned-guess network of a context (allocated by c3d_mlp_init): FP32
+// parameters, gradients and Adam moments, per-channel input/output scales,
+// activation buffers sized for `cap` rows.
+struct MLPState {
+  c3d::mlp::Layout L{};
+  float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
+  float *iscale = nullptr, *oscale = nullptr, *ones = nullptr;
+  int t = 0;
+  int cap = 0;
+  float *X = nullptr, *T = nullptr, *dY = nullptr, *part = nullptr;
+  float* act[c3d::mlp::kLayers] = {nullptr};
+  float* dH[c3d::mlp::kLayers - 1] = {nullptr};
+  std::vector<void*> bufs;  // activation buffers (freed on regrow)
+  int* nb9 = nullptr;       // IB marker neighbourhoods
+  bool ready = false;       // weights set by training or c3d_mlp_set_params
+};
 
 // Immersed-boundary state of a context (allocated by c3d_ib_setup).
 struct IBState {
@@ -32,6 +52,7 @@ struct IBState {
   double* vec[8] = {nullptr};  // x r rhat p v h s t  (4nb each)
   double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr;
   double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
+  double* fprev = nullptr;  // force before the last solve (learned-guess pairs)
   double tol = 1e-10;
   int maxit = 1000, guess = 0;
   int last_iters = 0;
@@ -65,6 +86,7 @@ struct c3d_ctx {
   // the bulk-flow integral
   const double *Hu = nullptr, *Hv = nullptr, *Hw = nullptr, *Hm = nullptr;
   IBState* ib = nullptr;
+  struct MLPState* mlp = nullptr;
   // mixed precision (C3D_PREC_MIXED): FP32 spectrum, FP64 divergence b and
   // potential phi, FP32 residual; nref FP64 refinement sweeps per projection
   cpx<float>* S32 = nullptr;
@@ -327,7 +349,7 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   if (d.precision == C3D_PREC_MIXED &&
       ((rc = dalloc(c, &c->S32, c->nspec)) || (rc = dalloc(c, &c->b64, c->nr)) ||
        (rc = dalloc(c, &c->phi64, c->nr)) || (rc = dalloc(c, &c->r32, c->nr)) ||
-       (rc = dalloc(c, &c->gauge, 1))))
+       (rc = dalloc(c, &c->gauge, (size_t)c->nzp + 1))))
     return cleanup(rc);
   if (hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->V, 0, c->nv * 8, c->stream) != hipSuccess ||
@@ -345,6 +367,9 @@ int c3d_destroy(c3d_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->mlp)
+    for (void* p : c->mlp->bufs) (void)hipFree(p);
+  delete c->mlp;
   delete c->ib;
   delete c;
   return C3D_OK;
@@ -511,6 +536,7 @@ static int save_interim(c3d_ctx* c) {
 }
 
 static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step);
+static int mlp_guess(c3d_ctx* c);
 
 int c3d_compute_non_ib_projection(c3d_ctx* c, double dt, int rk_step) {
   if (!c) return C3D_EINVAL;
@@ -764,9 +790,8 @@ int c3d_refinement_residuals(c3d_ctx* c, double* rmax, int n) {
     return fail(c, C3D_EINVAL, "refinement sweeps apply to C3D_PREC_MIXED contexts");
   const int m = n < c->nref ? n : c->nref;
   unsigned long long bits[kMaxRef] = {0};
-  if (m > 0)
-    HIPCHK(c, hipMemcpyAsync(bits, c->red + kRedRefine, m * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bits, c->red + kRedRefine, sizeof(bits), hipMemcpyDeviceToHost,
+                           c->stream));
   int rc = sync(c);
   if (rc) return rc;
   std::memcpy(rmax, bits, m * sizeof(double));
@@ -909,6 +934,11 @@ static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
                                         c->stream));                                     // :57
   if (ib->guess == 1) HIPCHK(c, hipMemsetAsync(ib->fb, 0, 4 * nb * 8, c->stream));
   if (ib->guess == 2) ib->guess = 0;
+  HIPCHK(c, hipMemcpyAsync(ib->fprev, ib->fb, 4 * nb * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (ib->guess == 3) {  // learned guess: fb = fprev + network(features)
+    int rc0 = mlp_guess(c);
+    if (rc0) return rc0;
+  }
   int rc = ib_bicgstab_launch(c, ib->fb, ib->rhs_ib, ib->tol, ib->maxit, dt, rk_step, nullptr,
                               nullptr);                                                  // :60
   if (rc && rc != C3D_ENOCONV) return rc;
@@ -1037,6 +1067,7 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
       (rc = dalloc(c, &ib->E1nH, 3 * (size_t)nb)) || (rc = dalloc(c, &ib->aux, 3 * (size_t)nb)) ||
       (rc = dalloc(c, &ib->rhs_ib, n4)) || (rc = dalloc(c, &ib->fb, n4)) ||
       (rc = dalloc(c, &ib->fin, n4)) || (rc = dalloc(c, &ib->fout, n4)) ||
+      (rc = dalloc(c, &ib->fprev, n4)) ||
       (rc = dalloc(c, &ib->sc, 8)) || (rc = dalloc(c, &ib->part, 5 * (size_t)B::kDotBlocks)))
     return bail(rc);
   for (int v = 0; v < 8; v++)
@@ -1202,7 +1233,7 @@ int c3d_ib_bicgstab(c3d_ctx* c, double* x, const double* b, double tol, int maxi
 int c3d_ib_set_solver(c3d_ctx* c, double tol, int maxit, int guess_mode) {
   int rc = need_ib(c);
   if (rc) return rc;
-  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 2) return C3D_EINVAL;
+  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 3) return C3D_EINVAL;
   c->ib->tol = tol;
   c->ib->maxit = maxit;
   c->ib->guess = guess_mode;
@@ -1270,6 +1301,323 @@ int c3d_ib_set_exterior_pressure_gradient(c3d_ctx* c, int flag) {
   c->Hv = ib->ext_pg ? ib->Hv_int : nullptr;
   c->Hw = ib->ext_pg ? ib->Hw_int : nullptr;
   return C3D_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Learned initial guess for the Schur solve (no reference counterpart: the
+// reference warm-starts BiCGSTAB from the previous force, src/projection.f90:60)
+// ----------------------------------------------------------------------------
+static uint64_t splitmix(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static int mlp_reserve(c3d_ctx* c, int rows) {
+  MLPState* m = c->mlp;
+  if (rows <= m->cap) return 0;
+  for (void* p : m->bufs) (void)hipFree(p);
+  m->bufs.clear();
+  auto get = [&](float** p, size_t n) -> int {
+    void* q = nullptr;
+    if (hipMalloc(&q, n * sizeof(float)) != hipSuccess)
+      return fail(c, C3D_ENOMEM, "network buffers (%zu floats)", n);
+    m->bufs.push_back(q);
+    *p = static_cast<float*>(q);
+    return 0;
+  };
+  using namespace c3d::mlp;
+  int rc = 0;
+  if ((rc = get(&m->X, (size_t)rows * kIn)) || (rc = get(&m->T, (size_t)rows * kOut)) ||
+      (rc = get(&m->dY, (size_t)rows * kOut)))
+    return rc;
+  for (int i = 0; i < kLayers; i++)
+    if ((rc = get(&m->act[i], (size_t)rows * m->L.out[i]))) return rc;
+  for (int i = 0; i < kLayers - 1; i++)
+    if ((rc = get(&m->dH[i], (size_t)rows * m->L.out[i]))) return rc;
+  m->cap = rows;
+  return 0;
+}
+
+static int need_mlp(c3d_ctx* c) {
+  if (!c) return C3D_EINVAL;
+  if (!c->mlp) return fail(c, C3D_EINVAL, "no network (c3d_mlp_init)");
+  return 0;
+}
+
+// Marker neighbourhoods on the slab layout: marker i of slab j of a ring of
+// n markers has neighbours (i + di mod n, j + dj mod nzb), di, dj in -1..1.
+static int mlp_neighbours(c3d_ctx* c) {
+  MLPState* m = c->mlp;
+  IBState* ib = c->ib;
+  if (m->nb9) return 0;
+  const int nb = ib->m.nb, nxb = ib->m.nxb, nxb1 = ib->m.nxb1, nzb = ib->m.nzb;
+  std::vector<int> h((size_t)9 * nb);
+  for (int j = 0; j < nzb; j++)
+    for (int i = 0; i < nxb; i++) {
+      const int lo = i < nxb1 ? 0 : nxb1, n = i < nxb1 ? nxb1 : nxb - nxb1, ii = i - lo;
+      int s = 0;
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          const int jj = (j + dj + nzb) % nzb, i2 = lo + (ii + di + n) % n;
+          h[(size_t)9 * (j * nxb + i) + s++] = jj * nxb + i2;
+        }
+    }
+  int* d = nullptr;
+  int rc = dalloc(c, &d, h.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(d, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  m->nb9 = d;
+  return 0;
+}
+
+static int mlp_guess(c3d_ctx* c) {
+  MLPState* m = c->mlp;
+  IBState* ib = c->ib;
+  if (!m || !m->ready) return fail(c, C3D_EINVAL, "learned guess requested without a trained network");
+  const int nb = ib->m.nb;
+  int rc = mlp_neighbours(c);
+  if (!rc) rc = mlp_reserve(c, nb);
+  if (rc) return rc;
+  namespace N = c3d::mlp;
+  HIPCHK(c, N::features(nb, m->nb9, ib->rhs_ib, ib->fprev, m->iscale, m->X, c->stream));
+  HIPCHK(c, N::forward(m->L, m->P, m->X, nb, m->act, c->stream));
+  HIPCHK(c, N::apply(nb, m->act[N::kLayers - 1], m->oscale, ib->fprev, ib->fb, c->stream));
+  return 0;
+}
+
+int c3d_mlp_param_count(void) { return c3d::mlp::layout().total; }
+
+int c3d_mlp_init(c3d_ctx* c, unsigned long long seed) {
+  if (!c) return C3D_EINVAL;
+  if (!c->mlp) {
+    MLPState* m = new MLPState();
+    m->L = c3d::mlp::layout();
+    const size_t n = m->L.total;
+    int rc = 0;
+    c->mlp = m;
+    if ((rc = dalloc(c, &m->P, n)) || (rc = dalloc(c, &m->G, n)) || (rc = dalloc(c, &m->M, n)) ||
+        (rc = dalloc(c, &m->V, n)) || (rc = dalloc(c, &m->iscale, c3d::mlp::kCh)) ||
+        (rc = dalloc(c, &m->oscale, c3d::mlp::kOut)) || (rc = dalloc(c, &m->ones, c3d::mlp::kCh)) ||
+        (rc = dalloc(c, &m->part, c3d::mlp::kLossBlocks)))
+      return rc;
+  }
+  MLPState* m = c->mlp;
+  // He-uniform weights (ReLU layers), zero biases
+  std::vector<float> P(m->L.total, 0.f);
+  uint64_t st = seed;
+  for (int i = 0; i < c3d::mlp::kLayers; i++) {
+    const float a = std::sqrt(6.f / (float)m->L.in[i]);
+    for (int e = 0; e < m->L.out[i] * m->L.in[i]; e++) {
+      const double u = (double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+      P[m->L.w[i] + e] = (float)((2.0 * u - 1.0) * a);
+    }
+  }
+  const std::vector<float> one(8, 1.f);
+  HIPCHK(c, hipMemcpy(m->P, P.data(), P.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(m->M, 0, P.size() * 4));
+  HIPCHK(c, hipMemset(m->V, 0, P.size() * 4));
+  HIPCHK(c, hipMemcpy(m->iscale, one.data(), c3d::mlp::kCh * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(m->oscale, one.data(), c3d::mlp::kOut * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(m->ones, one.data(), c3d::mlp::kCh * 4, hipMemcpyHostToDevice));
+  m->t = 0;
+  m->ready = false;
+  return C3D_OK;
+}
+
+int c3d_mlp_set_scales(c3d_ctx* c, const float* iscale, const float* oscale) {
+  int rc = need_mlp(c);
+  if (rc) return rc;
+  if (!iscale || !oscale) return C3D_EINVAL;
+  HIPCHK(c, hipMemcpy(c->mlp->iscale, iscale, c3d::mlp::kCh * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->mlp->oscale, oscale, c3d::mlp::kOut * 4, hipMemcpyHostToDevice));
+  return C3D_OK;
+}
+
+int c3d_mlp_get_params(c3d_ctx* c, float* P) {
+  int rc = need_mlp(c);
+  if (rc) return rc;
+  if (!P) return C3D_EINVAL;
+  HIPCHK(c, hipMemcpy(P, c->mlp->P, (size_t)c->mlp->L.total * 4, hipMemcpyDeviceToHost));
+  return C3D_OK;
+}
+
+int c3d_mlp_set_params(c3d_ctx* c, const float* P) {
+  int rc = need_mlp(c);
+  if (rc) return rc;
+  if (!P) return C3D_EINVAL;
+  HIPCHK(c, hipMemcpy(c->mlp->P, P, (size_t)c->mlp->L.total * 4, hipMemcpyHostToDevice));
+  c->mlp->ready = true;
+  return C3D_OK;
+}
+
+int c3d_mlp_forward(c3d_ctx* c, const float* X, int n, float* Y) {
+  int rc = need_mlp(c);
+  if (rc) return rc;
+  if (!X || !Y || n <= 0) return C3D_EINVAL;
+  MLPState* m = c->mlp;
+  namespace N = c3d::mlp;
+  if ((rc = mlp_reserve(c, n))) return rc;
+  HIPCHK(c, hipMemcpyAsync(m->X, X, (size_t)n * N::kIn * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, N::forward(m->L, m->P, m->X, n, m->act, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Y, m->act[N::kLayers - 1], (size_t)n * N::kOut * 4, hipMemcpyDeviceToHost,
+                           c->stream));
+  return sync(c);
+}
+
+static int mlp_loss_grad_launch(c3d_ctx* c, int n, float* loss) {
+  MLPState* m = c->mlp;
+  namespace N = c3d::mlp;
+  HIPCHK(c, N::forward(m->L, m->P, m->X, n, m->act, c->stream));
+  HIPCHK(c, N::loss_grad(m->act[N::kLayers - 1], m->T, n, m->dY, m->part, c->stream));
+  HIPCHK(c, N::backward(m->L, m->P, m->X, n, m->act, m->dY, m->dH, m->G, c->stream));
+  if (loss) {
+    float h[N::kLossBlocks];
+    HIPCHK(c, hipMemcpyAsync(h, m->part, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    int rc = sync(c);
+    if (rc) return rc;
+    double s = 0.0;
+    for (int b = 0; b < N::kLossBlocks; b++) s += h[b];
+    *loss = (float)s;
+  }
+  return 0;
+}
+
+int c3d_mlp_gradients(c3d_ctx* c, const float* X, const float* T, int n, float* G, float* loss) {
+  int rc = need_mlp(c);
+  if (rc) return rc;
+  if (!X || !T || !G || n <= 0) return C3D_EINVAL;
+  MLPState* m = c->mlp;
+  namespace N = c3d::mlp;
+  if ((rc = mlp_reserve(c, n))) return rc;
+  HIPCHK(c, hipMemcpyAsync(m->X, X, (size_t)n * N::kIn * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(m->T, T, (size_t)n * N::kOut * 4, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mlp_loss_grad_launch(c, n, loss))) return rc;
+  HIPCHK(c, hipMemcpyAsync(G, m->G, (size_t)m->L.total * 4, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_mlp_train(c3d_ctx* c, const float* X, const float* T, int n, int epochs, int batch,
+                  float lr, unsigned long long seed, float* loss_hist) {
+  int rc = need_mlp(c);
+  if (rc) return rc;
+  if (!X || !T || n <= 0 || epochs <= 0 || batch <= 0 || batch % 16 || !(lr > 0.f))
+    return fail(c, C3D_EINVAL, "training arguments (batch must be a positive multiple of 16)");
+  MLPState* m = c->mlp;
+  namespace N = c3d::mlp;
+  const int nbatch = n / batch;
+  if (nbatch == 0) return fail(c, C3D_EINVAL, "fewer samples than one batch");
+  if ((rc = mlp_reserve(c, n > batch ? n : batch))) return rc;
+  // shuffled copies of the data set, uploaded per epoch into the staging rows
+  float *DX = nullptr, *DT = nullptr;
+  if (hipMalloc(&DX, (size_t)n * N::kIn * 4) != hipSuccess ||
+      hipMalloc(&DT, (size_t)n * N::kOut * 4) != hipSuccess) {
+    if (DX) (void)hipFree(DX);
+    return fail(c, C3D_ENOMEM, "training set buffers");
+  }
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  std::vector<float> hx((size_t)n * N::kIn), ht((size_t)n * N::kOut);
+  uint64_t st = seed;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int e = 0; e < epochs && !rc; e++) {
+    for (int i = n - 1; i > 0; i--) std::swap(perm[i], perm[(int)(splitmix(st) % (uint64_t)(i + 1))]);
+    for (int i = 0; i < n; i++) {
+      std::memcpy(&hx[(size_t)i * N::kIn], &X[(size_t)perm[i] * N::kIn], N::kIn * 4);
+      std::memcpy(&ht[(size_t)i * N::kOut], &T[(size_t)perm[i] * N::kOut], N::kOut * 4);
+    }
+    if (hipMemcpyAsync(DX, hx.data(), hx.size() * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(DT, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+      rc = fail(c, C3D_EHIP, "training upload");
+      break;
+    }
+    for (int b = 0; b < nbatch && !rc; b++) {
+      if (hipMemcpyAsync(m->X, DX + (size_t)b * batch * N::kIn, (size_t)batch * N::kIn * 4,
+                         hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+          hipMemcpyAsync(m->T, DT + (size_t)b * batch * N::kOut, (size_t)batch * N::kOut * 4,
+                         hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+        rc = fail(c, C3D_EHIP, "batch copy");
+        break;
+      }
+      rc = mlp_loss_grad_launch(c, batch, nullptr);
+      if (rc) break;
+      m->t++;
+      const float lr_t = lr * std::sqrt(1.f - std::pow(b2, (float)m->t)) / (1.f - std::pow(b1, (float)m->t));
+      if (N::adam(m->P, m->G, m->M, m->V, m->L.total, lr_t, b1, b2, eps, c->stream) != hipSuccess)
+        rc = fail(c, C3D_EHIP, "adam");
+    }
+    if (!rc && loss_hist) {  // full-set loss after the epoch
+      if (hipMemcpyAsync(m->X, DX, (size_t)n * N::kIn * 4, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+          hipMemcpyAsync(m->T, DT, (size_t)n * N::kOut * 4, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+          N::forward(m->L, m->P, m->X, n, m->act, c->stream) != hipSuccess ||
+          N::loss_grad(m->act[N::kLayers - 1], m->T, n, m->dY, m->part, c->stream) != hipSuccess) {
+        rc = fail(c, C3D_EHIP, "epoch loss");
+        break;
+      }
+      float h[N::kLossBlocks];
+      if (hipMemcpyAsync(h, m->part, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess) {
+        rc = fail(c, C3D_EHIP, "epoch loss read-back");
+        break;
+      }
+      double s = 0.0;
+      for (int q = 0; q < N::kLossBlocks; q++) s += h[q];
+      loss_hist[e] = (float)s;
+    }
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(DX);
+  (void)hipFree(DT);
+  if (!rc) m->ready = true;
+  return rc;
+}
+
+int c3d_mlp_gemm_test(c3d_ctx* c, int M, int N, int K, int ta, int tb, const float* A, const float* B,
+                      float* C) {
+  if (!c || !A || !B || !C || M <= 0 || N <= 0 || K <= 0) return C3D_EINVAL;
+  float *a = nullptr, *b = nullptr, *d = nullptr;
+  int rc = 0;
+  const size_t na = (size_t)M * K, nbv = (size_t)K * N, nc = (size_t)M * N;
+  if (hipMalloc(&a, na * 4) != hipSuccess || hipMalloc(&b, nbv * 4) != hipSuccess ||
+      hipMalloc(&d, nc * 4) != hipSuccess) {
+    rc = fail(c, C3D_ENOMEM, "gemm test buffers");
+  } else {
+    const int lda = ta ? M : K, ldb = tb ? K : N;
+    if (hipMemcpy(a, A, na * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b, B, nbv * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        c3d::mlp::gemm_test(M, N, K, ta, tb, a, lda, b, ldb, d, N, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(C, d, nc * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(c, C3D_EHIP, "gemm test");
+  }
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (d) (void)hipFree(d);
+  return rc;
+}
+
+int c3d_ib_last_pair(c3d_ctx* c, float* X, double* dF) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!X && !dF) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  const int nb = ib->m.nb;
+  namespace N = c3d::mlp;
+  if (X) {
+    if (!c->mlp && (rc = c3d_mlp_init(c, 1))) return rc;
+    if ((rc = mlp_neighbours(c)) || (rc = mlp_reserve(c, nb))) return rc;
+    HIPCHK(c, N::features(nb, c->mlp->nb9, ib->rhs_ib, ib->fprev, c->mlp->ones, c->mlp->X, c->stream));
+    HIPCHK(c, hipMemcpyAsync(X, c->mlp->X, (size_t)nb * N::kIn * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (dF) {
+    std::vector<double> a(4 * (size_t)nb), b(4 * (size_t)nb);
+    HIPCHK(c, hipMemcpyAsync(a.data(), ib->fb, a.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.data(), ib->fprev, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = sync(c))) return rc;
+    for (size_t q = 0; q < a.size(); q++) dF[q] = a[q] - b[q];
+  }
+  return sync(c);
 }
 
 }  // extern "C"

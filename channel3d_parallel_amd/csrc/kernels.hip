@@ -515,6 +515,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
                                                      const double* __restrict__ gauge,
                                                      float* __restrict__ r,
                                                      unsigned long long* rmax) {
+  // one lane per (i, j) walking kGradKC z planes with the z neighbours of
+  // phi kept in registers
   unsigned bx, by, bz;
   xcd_tile(bx, by, bz);
   const int i = bx * 64 + (threadIdx.x & 63) + 2;
@@ -528,22 +530,28 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
     const double rx = g.rdxf[i], ry = g.rdyf[j];
     const double hxm = g.dxg[(i == 2) ? nx - 1 : i - 1], hx = g.dxg[i];
     const double hym = g.dyg[j - 1], hy = g.dyg[j];
+    const double gj = (j == 2) ? *gauge : 0.0;  // (0,0)-mode row of the spectral operator
+    const int km0 = (kb == 2) ? nz - 1 : kb - 1;
+    double pm = phi[ridx(i, j, km0, nxg, nyg)];
+    double p0 = phi[ridx(i, j, kb, nxg, nyg)];
+#pragma unroll
     for (int kk = 0; kk < kGradKC; kk++) {
       const int k = kb + kk;
       if (k > nzp + 1) break;
       const int km = (k == 2) ? nz - 1 : k - 1;
-      const double p0 = phi[ridx(i, j, k, nxg, nyg)];
+      const double pp = phi[ridx(i, j, k + 1, nxg, nyg)];
       const double gxp = (phi[ridx(i + 1, j, k, nxg, nyg)] - p0) / hx;
       const double gxm = (p0 - phi[ridx(im, j, k, nxg, nyg)]) / hxm;
       const double gyp = (j <= ny - 1) ? (phi[ridx(i, j + 1, k, nxg, nyg)] - p0) / hy : 0.0;
       const double gym = (j >= 3) ? (p0 - phi[ridx(i, j - 1, k, nxg, nyg)]) / hym : 0.0;
-      const double gzp = (phi[ridx(i, j, k + 1, nxg, nyg)] - p0) / g.dzg[k];
-      const double gzm = (p0 - phi[ridx(i, j, km, nxg, nyg)]) / g.dzg[km];
+      const double gzp = (pp - p0) / g.dzg[k];
+      const double gzm = (p0 - pm) / g.dzg[km];
       const double lap = (gxp - gxm) * rx + (gyp - gym) * ry + (gzp - gzm) * g.rdzf[k];
-      double res = b[ridx(i, j, k, nxg, nyg)] - lap;
-      if (j == 2) res -= *gauge;  // (0,0)-mode row of the spectral operator
+      const double res = (b[ridx(i, j, k, nxg, nyg)] - lap) - gj;
       r[ridx(i, j, k, nxg, nyg)] = (float)res;
       amax = fmax(amax, fabs(res));
+      pm = p0;
+      p0 = pp;
     }
   }
   // wavefront max, one atomic per wavefront
@@ -555,22 +563,32 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
 // gauge = 0.5 * Dyy(2,2) * mean over the unique cells of phi(:, 2, :): the
 // spectral solver's (0,0) mode uses D(2)*3/2 (src/poisson.f90:69), i.e. the
 // operator L + 0.5*Dyy(2,2)*(plane mean at j = 2); a residual of that
-// operator makes refinement converge to the FP64 solver's gauge.  One block,
-// fixed-order reduction.
-__global__ __launch_bounds__(1024) void k_gauge(DevGrid g, const double* __restrict__ phi,
-                                                double* __restrict__ gauge) {
-  __shared__ double sh[1024];
-  const int nxp = g.nxp, nzp = g.nzp, nxg = g.nxg, nyg = g.nyg;
-  const int n = nxp * nzp;
-  double acc = 0.0;
-  for (int e = threadIdx.x; e < n; e += 1024) acc += phi[ridx(2 + e % nxp, 2, 2 + e / nxp, nxg, nyg)];
-  sh[threadIdx.x] = acc;
+// operator makes refinement converge to the FP64 solver's gauge.  Row sums
+// per z (one block each), then one block sums the rows: fixed order.
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *gauge = 0.5 * g.dyy_d[0] * (sh[0] / (double)n);
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gauge_rows(DevGrid g, const double* __restrict__ phi,
+                                                       double* __restrict__ rows) {
+  __shared__ double sh[kBlock / 64];
+  const int k = blockIdx.x + 2;
+  double acc = 0.0;
+  for (int i = 2 + threadIdx.x; i <= g.nxp + 1; i += kBlock) acc += phi[ridx(i, 2, k, g.nxg, g.nyg)];
+  const double t = block_sum256(acc, sh);
+  if (threadIdx.x == 0) rows[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gauge_final(DevGrid g, const double* __restrict__ rows,
+                                                        double* __restrict__ gauge) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < g.nzp; k += kBlock) acc += rows[k];
+  const double t = block_sum256(acc, sh);
+  if (threadIdx.x == 0) *gauge = 0.5 * g.dyy_d[0] * (t / ((double)g.nxp * (double)g.nzp));
 }
 
 // ---------------------------------------------------------------------------
@@ -937,7 +955,9 @@ hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const
 
 hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
                            float* r, unsigned long long* rmax, hipStream_t s) {
-  hipLaunchKernelGGL(k_gauge, dim3(1), dim3(1024), 0, s, g, phi, gauge);
+  // gauge[0] = result, gauge[1 .. nzp] = row sums
+  hipLaunchKernelGGL(k_gauge_rows, dim3(g.nzp), dim3(kBlock), 0, s, g, phi, gauge + 1);
+  hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(kBlock), 0, s, g, (const double*)(gauge + 1), gauge);
   hipLaunchKernelGGL(k_residual,
                      grid_pts(g, g.nxp, g.nyg - 2, (g.nzp + kGradKC - 1) / kGradKC),
                      dim3(kBlock), 0, s, g, b, phi, (const double*)gauge, r, rmax);

@@ -62,7 +62,7 @@ hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, double* CP, hipStream_t
 hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
                           double* out, double* P, double dt, bool rk1, hipStream_t s);
 // r = b - M phi (FP64, stored FP32) on the unique cells, M = D G plus the
-// spectral solver's (0,0)-mode gauge row; gauge is one double of scratch;
+// spectral solver's (0,0)-mode gauge row; gauge is nzp + 1 doubles of scratch;
 // max |r| as double bits into *rmax (may be null; zeroed by the caller).
 hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
                            float* r, unsigned long long* rmax, hipStream_t s);

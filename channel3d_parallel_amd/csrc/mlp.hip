@@ -1,0 +1,258 @@
+// Learned initial guess for the IB Schur solve: a 54 -> 256 -> 256 -> 256 -> 4
+// ReLU MLP with hand-written training and inference kernels for gfx950.
+//
+// GEMMs run on the matrix cores: v_mfma_f32_16x16x16_bf16 with BF16 operands
+// (round-to-nearest-even from the FP32 activations and master weights) and
+// FP32 accumulation.  One wavefront computes one 16x16 output tile, taking
+// its operands straight from global memory (the matrices are at most
+// 24,576 x 256, so the GEMMs are latency- rather than throughput-bound).
+//
+// MFMA 16x16x16 operand layout (lane l, r = l % 16, q = l / 16):
+//   A: 4 values A(row r, k = 4q .. 4q+3)      B: 4 values B(k = 4q .. 4q+3, col r)
+//   D: 4 values D(row 4q .. 4q+3, col r)
+#include <hip/hip_runtime.h>
+
+#include "mlp.hpp"
+
+namespace c3d {
+namespace mlp {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ short bf16_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// C(M x N) = op(A)(M x K) * op(B)(K x N), FP32 in memory, BF16 MFMA.
+//   op(A)(m,k) = TA ? A[k*lda + m] : A[m*lda + k]
+//   op(B)(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n]
+// Epilogues: kEpiNone, kEpiBias (+bias[n]), kEpiBiasRelu, kEpiMask
+// (C *= (H(m,n) > 0), the ReLU derivative of the forward activation H).
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(kBlock) void k_gemm(int M, int N, int K, const float* __restrict__ A,
+                                                 int lda, const float* __restrict__ B, int ldb,
+                                                 float* __restrict__ C, int ldc,
+                                                 const float* __restrict__ bias,
+                                                 const float* __restrict__ H, int ldh) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tilesN = (N + 15) / 16;
+  const int tile = blockIdx.x * (kBlock / 64) + wave;
+  const int tm = tile / tilesN, tn = tile % tilesN;
+  if (tm * 16 >= M) return;  // whole wavefront leaves together
+  const int r = lane & 15, q = lane >> 4;
+  const int am = tm * 16 + r;  // row of A this lane feeds
+  const int bn = tn * 16 + r;  // column of B this lane feeds
+  float4_t acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    short4_t a, b;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int k = k0 + 4 * q + e;
+      float av = 0.f, bv = 0.f;
+      if (am < M && k < K) av = TA ? A[(size_t)k * lda + am] : A[(size_t)am * lda + k];
+      if (bn < N && k < K) bv = TB ? B[(size_t)bn * ldb + k] : B[(size_t)k * ldb + bn];
+      a[e] = bf16_bits(av);
+      b[e] = bf16_bits(bv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int m = tm * 16 + 4 * q + e, n = tn * 16 + r;
+    if (m < M && n < N) {
+      float v = acc[e];
+      if (EPI == kEpiBias || EPI == kEpiBiasRelu) v += bias[n];
+      if (EPI == kEpiBiasRelu) v = fmaxf(v, 0.f);
+      if (EPI == kEpiMask) v = (H[(size_t)m * ldh + n] > 0.f) ? v : 0.f;
+      C[(size_t)m * ldc + n] = v;
+    }
+  }
+}
+
+// Mean-squared error over n x 4 outputs: dY = 2 (Y - T) / (4n); per-block
+// partial sums of (Y - T)^2 / (4n) in part.
+__global__ __launch_bounds__(kBlock) void k_mse(const float* __restrict__ Y, const float* __restrict__ T,
+                                                int count, float* __restrict__ dY,
+                                                float* __restrict__ part) {
+  __shared__ float sh[kBlock];
+  const float inv = 1.f / (float)count;
+  float acc = 0.f;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < count; e += gridDim.x * kBlock) {
+    const float d = Y[e] - T[e];
+    dY[e] = 2.f * d * inv;
+    acc += d * d * inv;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// db[n] = sum over rows of dY(m, n), fixed order.
+__global__ void k_colsum(const float* __restrict__ dY, int M, int N, float* __restrict__ db) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int m = 0; m < M; m++) s += dY[(size_t)m * N + n];
+  db[n] = s;
+}
+
+// Adam (FP32), bias-corrected step size lr_t passed in.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr_t, float b1, float b2, float eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
+// Feature rows of the IB markers: for neighbour slot s (0..8) of marker l,
+// channels (b_u x, b_u y, b_u z, dudn_x, dudn_y, p_jump) of marker nb9[9l+s],
+// divided by the channel scale.
+__global__ void k_features(int nb, const int* __restrict__ nb9, const double* __restrict__ rhs,
+                           const double* __restrict__ fprev, const float* __restrict__ iscale,
+                           float* __restrict__ X) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * kIn) return;
+  const int l = t / kIn, f = t % kIn, s = f / kCh, c = f % kCh;
+  const int o = nb9[9 * l + s];
+  double v;
+  if (c < 3)
+    v = rhs[(size_t)c * nb + o];
+  else if (c == 3)
+    v = fprev[o];
+  else if (c == 4)
+    v = fprev[(size_t)nb + o];
+  else
+    v = fprev[(size_t)3 * nb + o];
+  X[t] = (float)v / iscale[c];
+}
+
+// fb[c*nb + l] = fprev[c*nb + l] + Y[l][c] * oscale[c]   (c = 0..3)
+__global__ void k_apply(int nb, const float* __restrict__ Y, const float* __restrict__ oscale,
+                        const double* __restrict__ fprev, double* __restrict__ fb) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * nb) return;
+  const int c = t / nb, l = t % nb;
+  fb[t] = fprev[t] + (double)Y[(size_t)l * kOut + c] * (double)oscale[c];
+}
+
+template <bool TA, bool TB, int EPI>
+hipError_t gemm(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
+                int ldc, const float* bias, const float* H, int ldh, hipStream_t s) {
+  const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+  const int blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(blocks), dim3(kBlock), 0, s, M, N, K, A, lda, B,
+                     ldb, C, ldc, bias, H, ldh);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+Layout layout() {
+  Layout L{};
+  const int dims[kLayers + 1] = {kIn, kHidden, kHidden, kHidden, kOut};
+  int off = 0;
+  for (int i = 0; i < kLayers; i++) {
+    L.w[i] = off;
+    off += dims[i + 1] * dims[i];
+    L.b[i] = off;
+    off += dims[i + 1];
+    L.in[i] = dims[i];
+    L.out[i] = dims[i + 1];
+  }
+  L.total = off;
+  return L;
+}
+
+hipError_t forward(const Layout& L, const float* P, const float* X, int n, float* const act[kLayers],
+                   hipStream_t s) {
+  const float* in = X;
+  for (int i = 0; i < kLayers; i++) {
+    hipError_t e;
+    if (i < kLayers - 1)
+      e = gemm<false, true, kEpiBiasRelu>(n, L.out[i], L.in[i], in, L.in[i], P + L.w[i], L.in[i],
+                                          act[i], L.out[i], P + L.b[i], nullptr, 0, s);
+    else
+      e = gemm<false, true, kEpiBias>(n, L.out[i], L.in[i], in, L.in[i], P + L.w[i], L.in[i],
+                                      act[i], L.out[i], P + L.b[i], nullptr, 0, s);
+    if (e != hipSuccess) return e;
+    in = act[i];
+  }
+  return hipSuccess;
+}
+
+hipError_t loss_grad(const float* Y, const float* T, int n, float* dY, float* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_mse, dim3(kLossBlocks), dim3(kBlock), 0, s, Y, T, n * kOut, dY, part);
+  return hipGetLastError();
+}
+
+hipError_t backward(const Layout& L, const float* P, const float* X, int n,
+                    float* const act[kLayers], float* dY, float* const dH[kLayers - 1], float* G,
+                    hipStream_t s) {
+  // layer i maps act[i-1] (X for i = 0) to act[i]; d holds dL/d(pre-activation)
+  const float* d = dY;
+  for (int i = kLayers - 1; i >= 0; i--) {
+    const float* in = (i == 0) ? X : act[i - 1];
+    hipError_t e = gemm<true, false, kEpiNone>(L.out[i], L.in[i], n, d, L.out[i], in, L.in[i],
+                                               G + L.w[i], L.in[i], nullptr, nullptr, 0, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_colsum, dim3((L.out[i] + 63) / 64), dim3(64), 0, s, d, n, L.out[i],
+                       G + L.b[i]);
+    if (i > 0) {
+      e = gemm<false, false, kEpiMask>(n, L.in[i], L.out[i], d, L.out[i], P + L.w[i], L.in[i],
+                                       dH[i - 1], L.in[i], nullptr, act[i - 1], L.in[i], s);
+      if (e != hipSuccess) return e;
+      d = dH[i - 1];
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t adam(float* P, const float* G, float* M, float* V, int n, float lr_t, float b1, float b2,
+                float eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, G, M, V, n,
+                     lr_t, b1, b2, eps);
+  return hipGetLastError();
+}
+
+hipError_t features(int nb, const int* nb9, const double* rhs, const double* fprev,
+                    const float* iscale, float* X, hipStream_t s) {
+  const size_t n = (size_t)nb * kIn;
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nb,
+                     nb9, rhs, fprev, iscale, X);
+  return hipGetLastError();
+}
+
+hipError_t apply(int nb, const float* Y, const float* oscale, const double* fprev, double* fb,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)((4 * (size_t)nb + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, s, nb, Y, oscale, fprev, fb);
+  return hipGetLastError();
+}
+
+hipError_t gemm_test(int M, int N, int K, int ta, int tb, const float* A, int lda, const float* B,
+                     int ldb, float* C, int ldc, hipStream_t s) {
+  if (!ta && !tb) return gemm<false, false, kEpiNone>(M, N, K, A, lda, B, ldb, C, ldc, nullptr, nullptr, 0, s);
+  if (!ta && tb) return gemm<false, true, kEpiNone>(M, N, K, A, lda, B, ldb, C, ldc, nullptr, nullptr, 0, s);
+  if (ta && !tb) return gemm<true, false, kEpiNone>(M, N, K, A, lda, B, ldb, C, ldc, nullptr, nullptr, 0, s);
+  return gemm<true, true, kEpiNone>(M, N, K, A, lda, B, ldb, C, ldc, nullptr, nullptr, 0, s);
+}
+
+}  // namespace mlp
+}  // namespace c3d

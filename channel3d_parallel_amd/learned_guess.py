@@ -1,0 +1,130 @@
+"""Learned initial guess for the IB Schur solve: data generation, training
+and the iteration A/B comparison.
+
+Training pairs come from the IB solver itself: for every IB projection of a
+two-cylinder run with seeded rotation speeds, the features the network sees
+(per marker and 3x3 neighbourhood: the velocity block of the Schur
+right-hand side and the previous force) and the converged force increment.
+The network is trained on the GPU by the library's own kernels
+(c3d_mlp_train).  Plain BiCGSTAB from a zero guess stays the judge: every
+variant is solved to the same absolute tolerance.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import fields, ib_geometry
+from .solver import Channel
+
+NU = 1.0 / 2800.0
+
+
+def speeds(seed, run):
+    """Rotation speeds of the two cylinders, uniform in [0.3, 0.7] U_b."""
+    u = fields.uniform(seed, 2, offset=2 * run)
+    return 0.3 + 0.4 * float(u[0]), -(0.3 + 0.4 * float(u[1]))
+
+
+def make_channel(nx, ny, nz, nth, nzb, s1, s2, dt_guess_mode=0, tol=1e-10, maxit=2000):
+    ch = Channel(nx, ny, nz, grid_type=0, nu=NU)
+    m = ib_geometry.two_cylinders(ch.grid(11)[9], nzb=nzb, nth=nth, speed1=s1, speed2=s2)
+    ch.setup_IB(m)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.0, grid_type=0)
+    ch.set_fields(U, V, W, P=np.zeros((nx + 1, ny + 1, nz + 1), order="F"))
+    ch.apply_boundary_conditions()
+    ch.set_pressure_gradient(3.0 * NU, 0.0, 0.0)
+    ch.set_IB_solver(tol=tol, maxit=maxit, guess_mode=dt_guess_mode)
+    return ch, m
+
+
+def stage(ch, st, dt, before_ib=None):
+    """One RK3 stage of compute_time_step_RK3 (src/time_integration.f90:98-123)
+    through the per-procedure entry points; before_ib(ch, st) runs right
+    before the IB projection and may run it itself (returning True)."""
+    if st == 1:
+        ch.save_state()
+    ch.compute_rhs(st)
+    ch.rk3_update(st, dt)
+    ch.apply_boundary_conditions()
+    ch.compute_non_IB_projection(dt, st)
+    ch.apply_boundary_conditions()
+    done = before_ib(ch, st) if before_ib else False
+    if not done:
+        ch.compute_IB_projection(dt, st)
+    ch.apply_boundary_conditions()
+
+
+def generate_pairs(nx, ny, nz, nth, nzb, runs, steps, dt, seed=11, max_pairs=None, rng_seed=5):
+    """Features X [n, 54] (unscaled) and force increments T [n, 4] from
+    `runs` runs of `steps` RK3 steps (reference warm start)."""
+    Xs, Ts = [], []
+    for run in range(runs):
+        s1, s2 = speeds(seed, run)
+        ch, m = make_channel(nx, ny, nz, nth, nzb, s1, s2)
+        with ch:
+            for _ in range(steps):
+                for st in (1, 2, 3):
+                    stage(ch, st, dt)
+                    X, dF = ch.ib_last_pair()
+                    Xs.append(X)
+                    Ts.append(dF.reshape(4, m.nb).T.astype(np.float64))
+    X = np.concatenate(Xs)
+    T = np.concatenate(Ts)
+    if max_pairs and X.shape[0] > max_pairs:
+        idx = np.random.default_rng(rng_seed).choice(X.shape[0], max_pairs, replace=False)
+        X, T = X[idx], T[idx]
+    return X, T
+
+
+def scales(X, T):
+    """Per-channel input scales (6 channels shared by the 9 neighbours) and
+    output scales; a channel without spread (dudn_z of z-invariant bodies)
+    gets a scale 1e-6 of the largest so its targets stay small."""
+    xc = X.reshape(-1, 9, 6)
+    isc = np.sqrt(np.mean(xc.astype(np.float64) ** 2, axis=(0, 1)))
+    isc = np.maximum(isc, 1e-6 * isc.max())
+    osc = np.sqrt(np.mean(T ** 2, axis=0))
+    osc = np.maximum(osc, 1e-6 * osc.max())
+    return isc.astype(np.float32), osc.astype(np.float32)
+
+
+def train(ch, X, T, epochs=40, batch=256, lr=1e-3, seed=1):
+    """Initialises and trains the network of channel `ch`; returns the
+    per-epoch loss history and the scales."""
+    isc, osc = scales(X, T)
+    ch.mlp_init(seed)
+    ch.mlp_set_scales(isc, osc)
+    Xs = (X.reshape(-1, 9, 6) / isc).reshape(-1, 54).astype(np.float32)
+    Ts = (T / osc).astype(np.float32)
+    hist = ch.mlp_train(Xs, Ts, epochs, batch=batch, lr=lr, seed=seed + 100)
+    return hist, isc, osc
+
+
+def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000):
+    """At every IB projection of `steps` RK3 steps, solve from the zero guess,
+    from the previous force (the reference's warm start) and from the
+    learned guess, all from the same state; continue with the zero-guess
+    solution.  Returns per-solve iteration counts and force differences."""
+    rec = {"zero": [], "previous": [], "learned": [], "max_force_diff_learned_vs_zero": [],
+           "max_force_diff_previous_vs_zero": []}
+
+    def variants(c, st):
+        snap = c.get_fields()
+        fb0 = c.get_IB_force()
+        out = {}
+        for name, mode in (("previous", 0), ("learned", 3), ("zero", 1)):
+            c.set_fields(snap["U"], snap["V"], snap["W"], P=snap["P"])
+            c.set_IB_force(fb0)
+            c.set_IB_solver(tol=tol, maxit=maxit, guess_mode=mode)
+            it, _ = c.compute_IB_projection(dt, st)
+            rec[name].append(it)
+            out[name] = c.get_IB_force()
+        scale = max(np.max(np.abs(out["zero"])), 1e-300)
+        rec["max_force_diff_learned_vs_zero"].append(float(np.max(np.abs(out["learned"] - out["zero"])) / scale))
+        rec["max_force_diff_previous_vs_zero"].append(float(np.max(np.abs(out["previous"] - out["zero"])) / scale))
+        return True  # the zero-guess solve ran last: the state continues from it
+
+    for _ in range(steps):
+        for st in (1, 2, 3):
+            stage(ch, st, dt, before_ib=variants)
+    return rec

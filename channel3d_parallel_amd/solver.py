@@ -31,7 +31,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import iptr, ptr
+from ._lib import fptr, iptr, ptr
 
 
 class C3DError(RuntimeError):
@@ -271,6 +271,62 @@ class Channel:
 
     def set_exterior_pressure_gradient(self, flag):
         self._chk(self.L.c3d_ib_set_exterior_pressure_gradient(self.h, int(flag)))
+
+    # learned initial guess for the Schur solve -------------------------------------
+    def mlp_init(self, seed=1):
+        self._chk(self.L.c3d_mlp_init(self.h, seed))
+
+    def mlp_set_scales(self, iscale, oscale):
+        self._chk(self.L.c3d_mlp_set_scales(self.h, fptr(np.asarray(iscale, dtype=np.float32)),
+                                            fptr(np.asarray(oscale, dtype=np.float32))))
+
+    def mlp_params(self):
+        p = np.zeros(self.L.c3d_mlp_param_count(), dtype=np.float32)
+        self._chk(self.L.c3d_mlp_get_params(self.h, fptr(p)))
+        return p
+
+    def mlp_set_params(self, p):
+        self._chk(self.L.c3d_mlp_set_params(self.h, fptr(np.ascontiguousarray(p, dtype=np.float32))))
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 4), dtype=np.float32)
+        self._chk(self.L.c3d_mlp_forward(self.h, fptr(X), X.shape[0], fptr(Y)))
+        return Y
+
+    def mlp_gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        G = np.zeros(self.L.c3d_mlp_param_count(), dtype=np.float32)
+        loss = C.c_float()
+        self._chk(self.L.c3d_mlp_gradients(self.h, fptr(X), fptr(T), X.shape[0], fptr(G),
+                                           C.byref(loss)))
+        return G, loss.value
+
+    def mlp_train(self, X, T, epochs, batch=256, lr=1e-3, seed=7):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.zeros(epochs, dtype=np.float32)
+        self._chk(self.L.c3d_mlp_train(self.h, fptr(X), fptr(T), X.shape[0], epochs, batch, lr,
+                                       seed, fptr(hist)))
+        return hist
+
+    def mlp_gemm(self, A, B, ta=False, tb=False):
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        M, K = (A.shape[1], A.shape[0]) if ta else A.shape
+        N = B.shape[0] if tb else B.shape[1]
+        Cm = np.zeros((M, N), dtype=np.float32)
+        self._chk(self.L.c3d_mlp_gemm_test(self.h, M, N, K, int(ta), int(tb), fptr(A), fptr(B),
+                                           fptr(Cm)))
+        return Cm
+
+    def ib_last_pair(self):
+        """(features [nb, 54] unscaled, force increment [4nb]) of the last IB solve."""
+        X = np.zeros((self.nb, 54), dtype=np.float32)
+        dF = np.zeros(4 * self.nb)
+        self._chk(self.L.c3d_ib_last_pair(self.h, fptr(X), ptr(dF)))
+        return X, dF
 
     # mixed precision -----------------------------------------------------------
     def set_refinement(self, nref):

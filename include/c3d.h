@@ -218,7 +218,8 @@ int c3d_ib_bicgstab(c3d_ctx *ctx, double *x, const double *b, double tol, int ma
 /* Solver settings for the IB projection: absolute tolerance cg_tol,
  * cg_max_iter, and the initial guess: guess_mode 0 = the previous force
  * fb (the reference's warm start), 1 = zero, 2 = the force last set by
- * c3d_ib_set_force for the next solve only, then 0. */
+ * c3d_ib_set_force for the next solve only, then 0, 3 = the previous force
+ * plus the learned increment (c3d_mlp_*). */
 int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
 /* Resident IB force fb = (dudn_jump[3nb], p_jump[nb]) [4nb]. */
 int c3d_ib_set_force(c3d_ctx *ctx, const double *fb);
@@ -238,6 +239,44 @@ int c3d_ib_stats(c3d_ctx *ctx, int *last_iters, long long *accum_iters, double *
  * everywhere.  The bulk-flow integral is H_interior-masked once a body is
  * set up (src/mass_flow.f90:24-53). */
 int c3d_ib_set_exterior_pressure_gradient(c3d_ctx *ctx, int flag);
+
+/* ---- Learned initial guess for the Schur solve --------------------------
+ * No reference counterpart: the reference starts BiCGSTAB from the previous
+ * force (src/projection.f90:60).  A 54 -> 256 -> 256 -> 256 -> 4 ReLU network
+ * (BF16 matrix-core GEMMs, FP32 accumulation and master weights) maps, for
+ * each marker, 6 channels over its 3x3 (azimuth x span) neighbourhood -- the
+ * velocity block of the Schur right-hand side (ub - E u*, 3 components) and
+ * the previous force (dudn_x, dudn_y, p_jump) -- to the force increment
+ * (dudn_x, dudn_y, dudn_z, p_jump).  Inputs are divided by iscale[6],
+ * outputs multiplied by oscale[4].  c3d_ib_set_solver guess_mode 3 uses it:
+ * fb = previous fb + increment, then BiCGSTAB to the same tolerance. */
+int c3d_mlp_param_count(void);
+/* (Re)initialises the weights (He-uniform, SplitMix64 seed), zero biases,
+ * unit scales, and clears the Adam state. */
+int c3d_mlp_init(c3d_ctx *ctx, unsigned long long seed);
+int c3d_mlp_set_scales(c3d_ctx *ctx, const float *iscale, const float *oscale);
+/* Flat FP32 parameters: per layer W[out][in] then b[out]. */
+int c3d_mlp_get_params(c3d_ctx *ctx, float *params);
+int c3d_mlp_set_params(c3d_ctx *ctx, const float *params);
+/* Y[n][4] = network(X[n][54]) (scaled units). */
+int c3d_mlp_forward(c3d_ctx *ctx, const float *X, int n, float *Y);
+/* Mean-squared-error gradient of the parameters on (X, T[n][4]) without an
+ * update; *loss (may be NULL) receives the loss. */
+int c3d_mlp_gradients(c3d_ctx *ctx, const float *X, const float *T, int n, float *grad,
+                      float *loss);
+/* Adam training on the GPU: `epochs` passes over the n samples, shuffled
+ * per epoch (SplitMix64 seed), minibatches of `batch` (multiple of 16);
+ * loss_hist[epochs] (may be NULL) receives the full-set loss per epoch. */
+int c3d_mlp_train(c3d_ctx *ctx, const float *X, const float *T, int n, int epochs, int batch,
+                  float lr, unsigned long long seed, float *loss_hist);
+/* C[M][N] = op(A) op(B) with the network's BF16 matrix-core GEMM; A is
+ * M x K (K x M if ta), B is K x N (N x K if tb), row-major.  Test hook. */
+int c3d_mlp_gemm_test(c3d_ctx *ctx, int M, int N, int K, int ta, int tb, const float *A,
+                      const float *B, float *C);
+/* Training pair of the last IB solve: X[nb][54] unscaled features (inputs
+ * as the network sees them before scaling) and dF[4nb] = converged force
+ * minus the force the solve started from (component-major). */
+int c3d_ib_last_pair(c3d_ctx *ctx, float *X, double *dF);
 
 #ifdef __cplusplus
 }
