@@ -140,13 +140,40 @@ def ib_section(nsteps=3):
             iters.append(after - before)
         step_ms = (time.perf_counter() - t0) * 1e3 / nsteps
         slip_step = ch.check_slip()
+    learned = learned_section(nx, ny, nz, dt)
     return {"workload": "two counter-rotating cylinders r=0.2 at (2.0,0.7),(4.0,1.3), 96x128 "
                         "markers each, 128x128x128 cells uniform y, dt 0.01",
+            "learned_guess": learned,
             "nb": m.nb, "setup_ms": setup_ms, "bicgstab_iterations_zero_guess": it,
             "residual_norm": float(np.sqrt(err)), "ib_projection_ms": proj_ms,
             "ms_per_iteration": proj_ms / max(it, 1), "max_slip": slip, "max_div": div,
             "rk3_step_ms": step_ms, "iterations_per_step_zero_guess": iters,
             "max_slip_after_steps": slip_step}
+
+
+def learned_section(nx, ny, nz, dt):
+    """Learned Schur guess on the two-cylinder case: 20k training pairs from
+    two runs with seeded rotation speeds in [0.3, 0.7] U_b, GPU training,
+    then zero / previous-force / learned starts compared on a held-out run."""
+    from channel3d_parallel_amd import learned_guess as lg
+
+    t0 = time.perf_counter()
+    X, T = lg.generate_pairs(nx, ny, nz, 96, 128, runs=2, steps=2, dt=dt, seed=11, max_pairs=20000)
+    gen_s = time.perf_counter() - t0
+    s1, s2 = lg.speeds(99, 0)
+    ch, _ = lg.make_channel(nx, ny, nz, 96, 128, s1, s2)
+    with ch:
+        t0 = time.perf_counter()
+        hist, _, _ = lg.train(ch, X, T, epochs=40, batch=256, lr=1e-3)
+        train_s = time.perf_counter() - t0
+        rec = lg.ab_compare(ch, dt, steps=2)
+    return {"pairs": int(X.shape[0]), "generation_s": gen_s, "training_s": train_s,
+            "loss_first_last": [float(hist[0]), float(hist[-1])],
+            "held_out_speeds": [s1, s2],
+            "iterations_zero_guess": rec["zero"], "iterations_previous_force": rec["previous"],
+            "iterations_learned": rec["learned"],
+            "max_rel_force_diff_learned_vs_zero": max(rec["max_force_diff_learned_vs_zero"]),
+            "max_rel_force_diff_previous_vs_zero": max(rec["max_force_diff_previous_vs_zero"])}
 
 
 def read_traffic(kernel, workload):

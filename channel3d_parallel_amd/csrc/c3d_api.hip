@@ -53,6 +53,7 @@ struct IBState {
   double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr;
   double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
   double* fprev = nullptr;  // force before the last solve (learned-guess pairs)
+  double last_dt = 1.0;     // dt of the last IB projection
   double tol = 1e-10;
   int maxit = 1000, guess = 0;
   int last_iters = 0;
@@ -536,7 +537,7 @@ static int save_interim(c3d_ctx* c) {
 }
 
 static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step);
-static int mlp_guess(c3d_ctx* c);
+static int mlp_guess(c3d_ctx* c, double dt, int rk_step);
 
 int c3d_compute_non_ib_projection(c3d_ctx* c, double dt, int rk_step) {
   if (!c) return C3D_EINVAL;
@@ -881,16 +882,18 @@ static int ib_schur_launch(c3d_ctx* c, const double* f, double* Sf, double dt, i
 }
 
 // bicgstab (src/projection.f90:146-186) on device vectors x (in/out), b.
+// r_ready: vec[1] already holds b - S x (the learned start computes it).
 static int ib_bicgstab_launch(c3d_ctx* c, double* x, const double* b, double tol, int maxit,
-                              double dt, int rk_step, int* iters_out, double* err_out) {
+                              double dt, int rk_step, int* iters_out, double* err_out,
+                              bool r_ready = false) {
   IBState* ib = c->ib;
   namespace B = c3d::ib;
   const size_t n = 4 * (size_t)ib->m.nb;
   double *r = ib->vec[1], *rhat = ib->vec[2], *p = ib->vec[3], *v = ib->vec[4], *h = ib->vec[5],
          *sv = ib->vec[6], *t = ib->vec[7];
-  int rc = ib_schur_launch(c, x, r, dt, rk_step);
-  if (rc) return rc;
-  HIPCHK(c, B::launch_bicg_init(b, r, rhat, p, v, n, c->stream));
+  int rc = 0;
+  if (!r_ready && (rc = ib_schur_launch(c, x, r, dt, rk_step))) return rc;
+  HIPCHK(c, B::launch_bicg_init(r_ready ? nullptr : b, r, rhat, p, v, n, c->stream));
   HIPCHK(c, B::launch_bicg_scalar(0, ib->sc, ib->part, nullptr, c->stream));
   double error = 1.0;
   const double eps = tol * tol;
@@ -935,12 +938,15 @@ static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
   if (ib->guess == 1) HIPCHK(c, hipMemsetAsync(ib->fb, 0, 4 * nb * 8, c->stream));
   if (ib->guess == 2) ib->guess = 0;
   HIPCHK(c, hipMemcpyAsync(ib->fprev, ib->fb, 4 * nb * 8, hipMemcpyDeviceToDevice, c->stream));
-  if (ib->guess == 3) {  // learned guess: fb = fprev + network(features)
-    int rc0 = mlp_guess(c);
+  ib->last_dt = dt;
+  bool r_ready = false;
+  if (ib->guess == 3) {  // learned start: fb = fprev + a d, d from the network
+    int rc0 = mlp_guess(c, dt, rk_step);
     if (rc0) return rc0;
+    r_ready = true;
   }
   int rc = ib_bicgstab_launch(c, ib->fb, ib->rhs_ib, ib->tol, ib->maxit, dt, rk_step, nullptr,
-                              nullptr);                                                  // :60
+                              nullptr, r_ready);                                         // :60
   if (rc && rc != C3D_ENOCONV) return rc;
   const int conv = rc;
   if ((rc = ib_spread_force(c, ib->fb, dt))) return rc;                                  // :66-73
@@ -1372,18 +1378,32 @@ static int mlp_neighbours(c3d_ctx* c) {
   return 0;
 }
 
-static int mlp_guess(c3d_ctx* c) {
+// Learned start of the Schur solve.  The network proposes an increment d of
+// the force; the start is fb = fprev + a d with the step a minimising
+// |b - S(fprev + a d)| (one Schur product S d beyond the warm start's
+// S fprev), so its residual is never larger than the warm start's.  Leaves
+// the residual b - S fb in vec[1] for BiCGSTAB.
+static int mlp_guess(c3d_ctx* c, double dt, int rk_step) {
   MLPState* m = c->mlp;
   IBState* ib = c->ib;
   if (!m || !m->ready) return fail(c, C3D_EINVAL, "learned guess requested without a trained network");
   const int nb = ib->m.nb;
+  const size_t n4 = 4 * (size_t)nb;
   int rc = mlp_neighbours(c);
   if (!rc) rc = mlp_reserve(c, nb);
   if (rc) return rc;
   namespace N = c3d::mlp;
-  HIPCHK(c, N::features(nb, m->nb9, ib->rhs_ib, ib->fprev, m->iscale, m->X, c->stream));
+  namespace B = c3d::ib;
+  double *r = ib->vec[1], *d = ib->vec[5], *Sd = ib->vec[6];
+  if ((rc = ib_schur_launch(c, ib->fb, r, dt, rk_step))) return rc;  // S fprev
+  HIPCHK(c, B::launch_bicg_init(ib->rhs_ib, r, ib->vec[2], ib->vec[3], ib->vec[4], n4, c->stream));
+  HIPCHK(c, N::features(nb, m->nb9, r, ib->fprev, m->iscale, m->X, c->stream));
   HIPCHK(c, N::forward(m->L, m->P, m->X, nb, m->act, c->stream));
-  HIPCHK(c, N::apply(nb, m->act[N::kLayers - 1], m->oscale, ib->fprev, ib->fb, c->stream));
+  HIPCHK(c, N::apply(nb, m->act[N::kLayers - 1], m->oscale, nullptr, d, c->stream));
+  if ((rc = ib_schur_launch(c, d, Sd, dt, rk_step))) return rc;      // S d
+  HIPCHK(c, B::launch_dot(r, Sd, Sd, Sd, n4, ib->part, 2, c->stream));
+  HIPCHK(c, B::launch_bicg_scalar(5, ib->sc, ib->part, nullptr, c->stream));
+  HIPCHK(c, B::launch_line_step(ib->fb, d, r, Sd, ib->sc, n4, c->stream));
   return 0;
 }
 
@@ -1607,7 +1627,13 @@ int c3d_ib_last_pair(c3d_ctx* c, float* X, double* dF) {
   if (X) {
     if (!c->mlp && (rc = c3d_mlp_init(c, 1))) return rc;
     if ((rc = mlp_neighbours(c)) || (rc = mlp_reserve(c, nb))) return rc;
-    HIPCHK(c, N::features(nb, c->mlp->nb9, ib->rhs_ib, ib->fprev, c->mlp->ones, c->mlp->X, c->stream));
+    // residual of the solve's warm start, b - S fprev (rk_step 2: the
+    // operator without the pressure side effect)
+    double* r = ib->vec[1];
+    if ((rc = ib_schur_launch(c, ib->fprev, r, ib->last_dt, 2))) return rc;
+    HIPCHK(c, c3d::ib::launch_bicg_init(ib->rhs_ib, r, ib->vec[2], ib->vec[3], ib->vec[4],
+                                        4 * (size_t)nb, c->stream));
+    HIPCHK(c, N::features(nb, c->mlp->nb9, r, ib->fprev, c->mlp->ones, c->mlp->X, c->stream));
     HIPCHK(c, hipMemcpyAsync(X, c->mlp->X, (size_t)nb * N::kIn * 4, hipMemcpyDeviceToHost, c->stream));
   }
   if (dF) {

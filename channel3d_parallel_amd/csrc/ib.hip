@@ -459,6 +459,11 @@ __global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ p
       if (host_err) *host_err = e;
       break;
     }
+    case 5: {  // minimal-residual step along d: a = <r, S d> / <S d, S d>
+      const double den = sum_part(part, 3);
+      sc[6] = den > 0.0 ? sum_part(part, 2) / den : 0.0;
+      break;
+    }
   }
 }
 
@@ -505,11 +510,22 @@ __global__ void k_bicg_init(const double* __restrict__ b, double* __restrict__ r
                             double* __restrict__ v, size_t n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double rr = b[t] - r[t];
+  const double rr = b ? b[t] - r[t] : r[t];  // b == null: r already holds the residual
   r[t] = rr;
   rhat[t] = rr;
   p[t] = 0.0;
   v[t] = 0.0;
+}
+
+// x = x + a d;  r = r - a S d  (a = sc[6])
+__global__ void k_line_step(double* __restrict__ x, const double* __restrict__ d,
+                            double* __restrict__ r, const double* __restrict__ Sd,
+                            const double* __restrict__ sc, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double a = sc[6];
+  x[t] = x[t] + a * d[t];
+  r[t] = r[t] - a * Sd[t];
 }
 
 // check_slip, src/monitor.f90:223-231
@@ -652,6 +668,12 @@ hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, co
 hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p, double* v,
                             size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_bicg_init, blocks_for(n), dim3(kBlock), 0, s, b, r, rhat, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_step(double* x, const double* d, double* r, const double* Sd,
+                            const double* sc, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_line_step, blocks_for(n), dim3(kBlock), 0, s, x, d, r, Sd, sc, n);
   return hipGetLastError();
 }
 

@@ -95,7 +95,8 @@ constexpr int kDotBlocks = 128;
 hipError_t launch_dot(const double* a, const double* b, const double* a2, const double* b2,
                       size_t n, double* part, int slot, hipStream_t s);
 // Scalar steps: 0 initialise, 1 rho_n/bta, 2 alpha, 3 om, 4 err (also to
-// *host_err when not null).
+// *host_err when not null), 5 minimal-residual step sc[6] = <r,Sd>/<Sd,Sd>
+// from partial slots 2 and 3.
 hipError_t launch_bicg_scalar(int step, double* sc, const double* part, double* host_err,
                               hipStream_t s);
 // p = r + bta*(p - om*v)
@@ -107,9 +108,12 @@ hipError_t launch_bicg_hs(const double* x, const double* p, const double* r, con
 // x = h + om*s;  r = s - om*t; partials of r.r into part slot 4
 hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* sc,
                           double* x, double* r, double* part, size_t n, hipStream_t s);
-// r = b - r; rhat = r; p = v = 0
+// r = b - r (b == null: r unchanged); rhat = r; p = v = 0
 hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p, double* v,
                             size_t n, hipStream_t s);
+// x += a d; r -= a Sd with a = sc[6]
+hipError_t launch_line_step(double* x, const double* d, double* r, const double* Sd,
+                            const double* sc, size_t n, hipStream_t s);
 // max |E u - E1nH*dudn - ub| as double bits into *red (zeroed by the caller).
 hipError_t launch_max_slip(const DevGrid& g, const Markers& m, const double* U, const double* V,
                            const double* W, const double* E1nH, const double* fb,

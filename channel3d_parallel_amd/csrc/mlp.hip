@@ -122,9 +122,10 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // Feature rows of the IB markers: for neighbour slot s (0..8) of marker l,
-// channels (b_u x, b_u y, b_u z, dudn_x, dudn_y, p_jump) of marker nb9[9l+s],
+// channels (r_u x, r_u y, r_u z, r_p, dudn_x, dudn_y) of marker nb9[9l+s] --
+// the residual b - S fprev of the warm start and the previous force --
 // divided by the channel scale.
-__global__ void k_features(int nb, const int* __restrict__ nb9, const double* __restrict__ rhs,
+__global__ void k_features(int nb, const int* __restrict__ nb9, const double* __restrict__ res,
                            const double* __restrict__ fprev, const float* __restrict__ iscale,
                            float* __restrict__ X) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -132,14 +133,10 @@ __global__ void k_features(int nb, const int* __restrict__ nb9, const double* __
   const int l = t / kIn, f = t % kIn, s = f / kCh, c = f % kCh;
   const int o = nb9[9 * l + s];
   double v;
-  if (c < 3)
-    v = rhs[(size_t)c * nb + o];
-  else if (c == 3)
-    v = fprev[o];
-  else if (c == 4)
-    v = fprev[(size_t)nb + o];
+  if (c < 4)
+    v = res[(size_t)c * nb + o];
   else
-    v = fprev[(size_t)3 * nb + o];
+    v = fprev[(size_t)(c - 4) * nb + o];
   X[t] = (float)v / iscale[c];
 }
 
@@ -149,7 +146,8 @@ __global__ void k_apply(int nb, const float* __restrict__ Y, const float* __rest
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * nb) return;
   const int c = t / nb, l = t % nb;
-  fb[t] = fprev[t] + (double)Y[(size_t)l * kOut + c] * (double)oscale[c];
+  const double d = (double)Y[(size_t)l * kOut + c] * (double)oscale[c];
+  fb[t] = fprev ? fprev[t] + d : d;
 }
 
 template <bool TA, bool TB, int EPI>
@@ -231,11 +229,11 @@ hipError_t adam(float* P, const float* G, float* M, float* V, int n, float lr_t,
   return hipGetLastError();
 }
 
-hipError_t features(int nb, const int* nb9, const double* rhs, const double* fprev,
+hipError_t features(int nb, const int* nb9, const double* res, const double* fprev,
                     const float* iscale, float* X, hipStream_t s) {
   const size_t n = (size_t)nb * kIn;
   hipLaunchKernelGGL(k_features, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nb,
-                     nb9, rhs, fprev, iscale, X);
+                     nb9, res, fprev, iscale, X);
   return hipGetLastError();
 }
 

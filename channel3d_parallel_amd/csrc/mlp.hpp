@@ -34,11 +34,12 @@ hipError_t backward(const Layout& L, const float* P, const float* X, int n,
                     hipStream_t s);
 hipError_t adam(float* P, const float* G, float* M, float* V, int n, float lr_t, float b1, float b2,
                 float eps, hipStream_t s);
-// IB feature rows X (nb x 54) from the Schur right-hand side and the
-// previous force; nb9 holds 9 neighbour indices per marker.
-hipError_t features(int nb, const int* nb9, const double* rhs, const double* fprev,
+// IB feature rows X (nb x 54) from the warm start's Schur residual
+// res = b - S fprev (4nb) and the previous force; nb9 holds 9 neighbour
+// indices per marker.
+hipError_t features(int nb, const int* nb9, const double* res, const double* fprev,
                     const float* iscale, float* X, hipStream_t s);
-// fb = fprev + Y * oscale per output channel.
+// fb = fprev + Y * oscale per output channel (fprev == null: fb = Y * oscale).
 hipError_t apply(int nb, const float* Y, const float* oscale, const double* fprev, double* fb,
                  hipStream_t s);
 // Plain GEMM with the MFMA kernel (tests of the operand layout).

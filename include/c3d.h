@@ -245,11 +245,12 @@ int c3d_ib_set_exterior_pressure_gradient(c3d_ctx *ctx, int flag);
  * force (src/projection.f90:60).  A 54 -> 256 -> 256 -> 256 -> 4 ReLU network
  * (BF16 matrix-core GEMMs, FP32 accumulation and master weights) maps, for
  * each marker, 6 channels over its 3x3 (azimuth x span) neighbourhood -- the
- * velocity block of the Schur right-hand side (ub - E u*, 3 components) and
- * the previous force (dudn_x, dudn_y, p_jump) -- to the force increment
- * (dudn_x, dudn_y, dudn_z, p_jump).  Inputs are divided by iscale[6],
- * outputs multiplied by oscale[4].  c3d_ib_set_solver guess_mode 3 uses it:
- * fb = previous fb + increment, then BiCGSTAB to the same tolerance. */
+ * residual of the warm start r = b - S fprev (velocity rows x, y, z and the
+ * pressure row) and the previous force (dudn_x, dudn_y) -- to a force
+ * increment d (dudn_x, dudn_y, dudn_z, p_jump).  Inputs are divided by
+ * iscale[6], outputs multiplied by oscale[4].  c3d_ib_set_solver guess_mode
+ * 3 starts BiCGSTAB from fb = fprev + a d with the step a minimising
+ * |b - S(fprev + a d)|, then iterates to the same tolerance. */
 int c3d_mlp_param_count(void);
 /* (Re)initialises the weights (He-uniform, SplitMix64 seed), zero biases,
  * unit scales, and clears the Adam state. */
@@ -273,9 +274,10 @@ int c3d_mlp_train(c3d_ctx *ctx, const float *X, const float *T, int n, int epoch
  * M x K (K x M if ta), B is K x N (N x K if tb), row-major.  Test hook. */
 int c3d_mlp_gemm_test(c3d_ctx *ctx, int M, int N, int K, int ta, int tb, const float *A,
                       const float *B, float *C);
-/* Training pair of the last IB solve: X[nb][54] unscaled features (inputs
- * as the network sees them before scaling) and dF[4nb] = converged force
- * minus the force the solve started from (component-major). */
+/* Training pair of the last IB solve: X[nb][54] unscaled features (the
+ * residual of the previous force and that force, as the network sees them
+ * before scaling) and dF[4nb] = converged force minus the previous force
+ * (component-major).  Costs one Schur product. */
 int c3d_ib_last_pair(c3d_ctx *ctx, float *X, double *dF);
 
 #ifdef __cplusplus
