@@ -107,7 +107,7 @@ def rk3_section(nx, ny, nz, U, V, W, steps):
             "steps": steps, "note": "host-timed whole steps (3 substeps: rhs, update, BCs, projection)"}
 
 
-def ib_section(nsteps=3):
+def ib_section(nsteps=3, learned=True):
     """Two-cylinder IB case (130x129x130 uniform, nb = 24,576): plain
     BiCGSTAB from a zero guess at tolerance 1e-10; iterations, slip and time."""
     from channel3d_parallel_amd import Channel, fields, ib_geometry
@@ -140,7 +140,7 @@ def ib_section(nsteps=3):
             iters.append(after - before)
         step_ms = (time.perf_counter() - t0) * 1e3 / nsteps
         slip_step = ch.check_slip()
-    learned = learned_section(nx, ny, nz, dt)
+    learned = learned_section(nx, ny, nz, dt) if learned else None
     return {"workload": "two counter-rotating cylinders r=0.2 at (2.0,0.7),(4.0,1.3), 96x128 "
                         "markers each, 128x128x128 cells uniform y, dt 0.01",
             "learned_guess": learned,
@@ -212,7 +212,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, RK3 and IB sections")
+    ap.add_argument("--only", choices=["ib", "mixed"],
+                    help="profiling runs: only the IB case (no learned guess) or only the "
+                         "mixed-precision projection with 2 refinement sweeps")
     a = ap.parse_args()
+    if a.only == "ib":
+        print(json.dumps({"ib": ib_section(nsteps=1, learned=False)}))
+        return
+    if a.only == "mixed":
+        from channel3d_parallel_amd import Channel, fields
+        from channel3d_parallel_amd._lib import PREC_MIXED
+        nx, ny, nz, _, seed = WORKLOADS["proj384"]
+        U, V, W = fields.spectral_faces(nx, ny, nz, seed)
+        with Channel(nx, ny, nz, grid_type=1, alpha=2.0, precision=PREC_MIXED) as ch:
+            ch.set_refinement(2)
+            ch.set_fields(U, V, W)
+            ch.apply_boundary_conditions()
+            ms, stages = ch.time_projection(a.steps, 1e-3)
+        print(json.dumps({"mixed_nref2_ms": ms, "stages_ms": stages}))
+        return
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported; this build measures one GPU "
               "(the multi-GPU design is documented, not built).", file=sys.stderr)

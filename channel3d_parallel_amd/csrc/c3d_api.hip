@@ -50,7 +50,7 @@ struct IBState {
   double *Uint = nullptr, *Vint = nullptr, *Wint = nullptr;  // U_interim ...
   double* aux = nullptr;  // 3nb
   double* vec[8] = {nullptr};  // x r rhat p v h s t  (4nb each)
-  double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr;
+  double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr, *mpart = nullptr;
   double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
   double* fprev = nullptr;  // force before the last solve (learned-guess pairs)
   double last_dt = 1.0;     // dt of the last IB projection
@@ -875,7 +875,7 @@ static int ib_schur_launch(c3d_ctx* c, const double* f, double* Sf, double dt, i
   if (!rc) rc = ib_poisson_reg(c, dt, rk_step);
   if (rc) return rc;
   HIPCHK(c, c3d::ib::launch_interp_c(c->g, ib->m, ib->phi, -1.0, Sf + 3 * nb, c->stream));
-  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, Sf + 3 * nb, ib->E1nHc, f + 3 * nb, c->stream));
+  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, Sf + 3 * nb, ib->E1nHc, f + 3 * nb, ib->mpart, c->stream));
   HIPCHK(c, c3d::ib::launch_schur_u(c->g, ib->m, ib->phi, dt, ib->Ureg, ib->Vreg, ib->Wreg,
                                     ib->E1nH, f, Sf, c->stream));
   return 0;
@@ -934,7 +934,7 @@ static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
   HIPCHK(c, c3d::ib::launch_interp_c(c->g, ib->m, c->Pint, -1.0, ib->rhs_ib + 3 * nb,
                                      c->stream));                                        // :53-54
   HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, ib->rhs_ib + 3 * nb, nullptr, nullptr,
-                                        c->stream));                                     // :57
+                                        ib->mpart, c->stream));                                     // :57
   if (ib->guess == 1) HIPCHK(c, hipMemsetAsync(ib->fb, 0, 4 * nb * 8, c->stream));
   if (ib->guess == 2) ib->guess = 0;
   HIPCHK(c, hipMemcpyAsync(ib->fprev, ib->fb, 4 * nb * 8, hipMemcpyDeviceToDevice, c->stream));
@@ -1049,19 +1049,47 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
         off[(size_t)l * 125 + p] = (int)o;
         cnt[o + 1]++;
       }
-    std::vector<int> node, ptr(1, 0);
+    // CSR by node offset first (entries in ascending e = l*125 + p)
+    std::vector<int> node, ncnt;
     for (size_t o = 0; o < nf[q]; o++)
       if (cnt[o + 1]) {
         node.push_back((int)o);
-        ptr.push_back(ptr.back() + cnt[o + 1]);
+        ncnt.push_back(cnt[o + 1]);
       }
     for (size_t o = 0; o < nf[q]; o++) cnt[o + 1] += cnt[o];
-    std::vector<int> qq((size_t)nb * 125);
-    for (size_t e = 0; e < qq.size(); e++) qq[cnt[off[e]]++] = (int)e;
-    ib->sp[q].nnodes = (int)node.size();
-    if ((rc = upload(c, &ib->sp[q].node, node)) || (rc = upload(c, &ib->sp[q].ptr, ptr)) ||
-        (rc = upload(c, &ib->sp[q].q, qq)))
+    std::vector<int> csr((size_t)nb * 125);
+    for (size_t e = 0; e < csr.size(); e++) csr[cnt[off[e]]++] = (int)e;
+    // jagged diagonal order: ranks by descending count (stable in offset)
+    const size_t nn = node.size();
+    std::vector<int> start(nn), rank(nn);
+    for (size_t r = 0, acc = 0; r < nn; r++) {
+      start[r] = (int)acc;
+      acc += ncnt[r];
+      rank[r] = (int)r;
+    }
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b2) { return ncnt[a] > ncnt[b2]; });
+    const int maxc = nn ? ncnt[rank[0]] : 0;
+    std::vector<int> slot(maxc + 1, 0), jnode(nn), jcnt(nn), qq(csr.size());
+    {
+      std::vector<int> per(maxc, 0);  // ranks with more than i entries
+      for (size_t r = 0; r < nn; r++)
+        for (int i = 0; i < ncnt[rank[r]]; i++) per[i]++;
+      for (int i = 0; i < maxc; i++) slot[i + 1] = slot[i] + per[i];
+    }
+    for (size_t r = 0; r < nn; r++) {
+      const int src = rank[r];
+      jnode[r] = node[src];
+      jcnt[r] = ncnt[src];
+      for (int i = 0; i < ncnt[src]; i++) qq[(size_t)slot[i] + r] = csr[(size_t)start[src] + i];
+    }
+    ib->sp[q].nnodes = (int)nn;
+    ib->sp[q].nentries = (int)qq.size();
+    if ((rc = upload(c, &ib->sp[q].node, jnode)) || (rc = upload(c, &ib->sp[q].cnt, jcnt)) ||
+        (rc = upload(c, &ib->sp[q].slot, slot)) || (rc = upload(c, &ib->sp[q].q, qq)) ||
+        (rc = dalloc(c, &ib->sp[q].ml, qq.size())) || (rc = dalloc(c, &ib->sp[q].wfs, qq.size())))
       return bail(rc);
+    if (B::launch_spread_coeffs(m, ib->sp[q], q, c->stream) != hipSuccess)
+      return bail(fail(c, C3D_EHIP, "spreading coefficients"));
   }
   // resident buffers
   if ((rc = dalloc(c, &ib->Ureg, c->nu)) || (rc = dalloc(c, &ib->Vreg, c->nv)) ||
@@ -1074,7 +1102,8 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
       (rc = dalloc(c, &ib->rhs_ib, n4)) || (rc = dalloc(c, &ib->fb, n4)) ||
       (rc = dalloc(c, &ib->fin, n4)) || (rc = dalloc(c, &ib->fout, n4)) ||
       (rc = dalloc(c, &ib->fprev, n4)) ||
-      (rc = dalloc(c, &ib->sc, 8)) || (rc = dalloc(c, &ib->part, 5 * (size_t)B::kDotBlocks)))
+      (rc = dalloc(c, &ib->sc, 8)) || (rc = dalloc(c, &ib->part, 5 * (size_t)B::kDotBlocks)) ||
+      (rc = dalloc(c, &ib->mpart, 128)))
     return bail(rc);
   for (int v = 0; v < 8; v++)
     if ((rc = dalloc(c, &ib->vec[v], n4))) return bail(rc);
@@ -1203,7 +1232,7 @@ int c3d_ib_remove_mean(c3d_ctx* c, double* f) {
   IBState* ib = c->ib;
   const size_t nb = ib->m.nb;
   HIPCHK(c, hipMemcpyAsync(ib->fin, f, nb * 8, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, ib->fin, nullptr, nullptr, c->stream));
+  HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, ib->fin, nullptr, nullptr, ib->mpart, c->stream));
   HIPCHK(c, hipMemcpyAsync(f, ib->fin, nb * 8, hipMemcpyDeviceToHost, c->stream));
   return sync(c);
 }

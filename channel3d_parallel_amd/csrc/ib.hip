@@ -169,6 +169,73 @@ __device__ __forceinline__ int wave_marker(int nb) {
   return (int)bx * (kBlock / 64) + (int)(threadIdx.x >> 6);
 }
 
+// Sums over the u, v and w stencils of marker l at once: the 375 points are
+// spread over the 64 lanes (up to 6 per lane, all loads in flight together)
+// and term(c, point) is accumulated per component; the three sums are then
+// reduced across the wavefront.
+// Per-axis stencil values of marker l held one per lane: lane 15c + 5a + t
+// (c < ncomp) holds entry (axis a, position t) of stencil st[c0 + c]; the
+// 125 points then fetch theirs by cross-lane reads instead of 6 loads each.
+struct LaneStencil {
+  int idx;
+  double phi, dd;
+};
+
+template <bool DXN>
+__device__ __forceinline__ LaneStencil load_lane_stencil(const Markers& m, int l, int lane, int c0,
+                                                         int ncomp) {
+  LaneStencil r{0, 0.0, 0.0};
+  if (lane < 15 * ncomp) {
+    const int c = c0 + lane / 15, e = lane % 15;  // e = 5a + t
+    const Stencil st = c == 0 ? m.st[0] : (c == 1 ? m.st[1] : (c == 2 ? m.st[2] : m.st[3]));
+    const size_t o = (size_t)e * m.nb + l;
+    r.idx = st.idx[o];
+    r.phi = st.phi[o];
+    if (DXN) r.dd = st.dd[o];
+  }
+  return r;
+}
+
+// Point p = i + 5*(j + 5*k) of the c-th loaded stencil, from the lanes.
+template <bool DXN>
+__device__ __forceinline__ Pt lane_point(const Markers& m, const LaneStencil& ls, int l, int c, int p) {
+  const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+  const int b = 15 * c;
+  Pt r;
+  r.X = __shfl(ls.idx, b + i, 64);
+  r.Y = __shfl(ls.idx, b + 5 + j, 64);
+  r.Z = __shfl(ls.idx, b + 10 + k, 64);
+  r.w = ((m.A * __shfl(ls.phi, b + i, 64)) * __shfl(ls.phi, b + 5 + j, 64)) *
+        __shfl(ls.phi, b + 10 + k, 64);
+  r.dxn = 0.0;
+  if (DXN) {
+    r.dxn = (m.normals[l] * __shfl(ls.dd, b + i, 64) + m.normals[m.nb + l] * __shfl(ls.dd, b + 5 + j, 64)) +
+            m.normals[2 * m.nb + l] * __shfl(ls.dd, b + 10 + k, 64);
+  }
+  return r;
+}
+
+template <bool DXN, typename Term>
+__device__ __forceinline__ void gather3(const Markers& m, int l, int lane, Term term, double out[3]) {
+  const LaneStencil ls = load_lane_stencil<DXN>(m, l, lane, 0, 3);
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const int p = lane + 64 * q;
+    const int c = p >= 250 ? 2 : (p >= 125 ? 1 : 0);
+    // all lanes take part in the cross-lane reads; out-of-range points add 0
+    const Pt pt = lane_point<DXN>(m, ls, l, c, (p < 375) ? p - 125 * c : 0);
+    if (p < 375) {
+      const double v = term(c, pt);
+      acc[0] += c == 0 ? v : 0.0;
+      acc[1] += c == 1 ? v : 0.0;
+      acc[2] += c == 2 ? v : 0.0;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) out[c] = wave_sum(acc[c]);
+}
+
 // E / E1n of U, V, W (src/immersed_boundary_operators.f90:409-501).
 template <bool DXN>
 __global__ __launch_bounds__(kBlock) void k_interp_uvw(DevGrid g, Markers m, const double* __restrict__ U,
@@ -178,20 +245,17 @@ __global__ __launch_bounds__(kBlock) void k_interp_uvw(DevGrid g, Markers m, con
   const int l = wave_marker(m.nb);
   const int lane = threadIdx.x & 63;
   if (l >= m.nb) return;
-  const double* F[3] = {U, V, W};
-  const int n1[3] = {g.nx, g.nxg, g.nxg}, n2[3] = {g.nyg, g.ny, g.nyg};
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double acc = 0.0;
-    for (int p = lane; p < 125; p += 64) {
-      const Pt q = stencil_point<DXN>(m, m.st[c], l, p);
-      double v = q.w * F[c][fidx(q.X, q.Y, q.Z, n1[c], n2[c])];
-      if (DXN) v = v * q.dxn;
-      acc += v;
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) out[c * m.nb + l] = acc;
-  }
+  const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
+  double r[3];
+  gather3<DXN>(m, l, lane, [&](int c, const Pt& q) {
+    const double f = c == 0 ? U[fidx(q.X, q.Y, q.Z, nx, nyg)]
+                            : (c == 1 ? V[fidx(q.X, q.Y, q.Z, nxg, ny)] : W[fidx(q.X, q.Y, q.Z, nxg, nyg)]);
+    double v = q.w * f;
+    if (DXN) v = v * q.dxn;
+    return v;
+  }, r);
+  if (lane == 0)
+    for (int c = 0; c < 3; c++) out[c * m.nb + l] = r[c];
 }
 
 // E1n of a centre field in rhs_p layout (:503-526).
@@ -200,10 +264,13 @@ __global__ __launch_bounds__(kBlock) void k_interp_c(DevGrid g, Markers m, const
   const int l = wave_marker(m.nb);
   const int lane = threadIdx.x & 63;
   if (l >= m.nb) return;
+  const LaneStencil ls = load_lane_stencil<true>(m, l, lane, 3, 1);
   double acc = 0.0;
-  for (int p = lane; p < 125; p += 64) {
-    const Pt q = stencil_point<true>(m, m.st[3], l, p);
-    acc += (q.w * P[ridx(q.X, q.Y, q.Z, g.nxg, g.nyg)]) * q.dxn;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int p = lane + 64 * q;
+    const Pt pt = lane_point<true>(m, ls, l, 0, p < 125 ? p : 0);
+    if (p < 125) acc += (pt.w * P[ridx(pt.X, pt.Y, pt.Z, g.nxg, g.nyg)]) * pt.dxn;
   }
   acc = wave_sum(acc);
   if (lane == 0) out[l] = sign * acc;
@@ -224,28 +291,25 @@ __global__ __launch_bounds__(kBlock) void k_schur_u(DevGrid g, Markers m, const 
   const int lane = threadIdx.x & 63;
   if (l >= m.nb) return;
   const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double acc = 0.0;
-    for (int p = lane; p < 125; p += 64) {
-      const Pt q = stencil_point<false>(m, m.st[c], l, p);
-      const double p0 = dt * phi[ridx(q.X, q.Y, q.Z, nxg, nyg)];
-      double G, R;
-      if (c == 0) {
-        G = (dt * phi[ridx(q.X + 1, q.Y, q.Z, nxg, nyg)] - p0) / g.dxg[q.X];
-        R = Ureg[fidx(q.X, q.Y, q.Z, nx, nyg)];
-      } else if (c == 1) {
-        G = (dt * phi[ridx(q.X, q.Y + 1, q.Z, nxg, nyg)] - p0) / g.dyg[q.Y];
-        R = Vreg[fidx(q.X, q.Y, q.Z, nxg, ny)];
-      } else {
-        G = (dt * phi[ridx(q.X, q.Y, q.Z + 1, nxg, nyg)] - p0) / g.dzg[q.Z];
-        R = Wreg[fidx(q.X, q.Y, q.Z, nxg, nyg)];
-      }
-      acc += q.w * (G - R);
+  double r[3];
+  gather3<false>(m, l, lane, [&](int c, const Pt& q) {
+    const double p0 = dt * phi[ridx(q.X, q.Y, q.Z, nxg, nyg)];
+    double G, R;
+    if (c == 0) {
+      G = (dt * phi[ridx(q.X + 1, q.Y, q.Z, nxg, nyg)] - p0) / g.dxg[q.X];
+      R = Ureg[fidx(q.X, q.Y, q.Z, nx, nyg)];
+    } else if (c == 1) {
+      G = (dt * phi[ridx(q.X, q.Y + 1, q.Z, nxg, nyg)] - p0) / g.dyg[q.Y];
+      R = Vreg[fidx(q.X, q.Y, q.Z, nxg, ny)];
+    } else {
+      G = (dt * phi[ridx(q.X, q.Y, q.Z + 1, nxg, nyg)] - p0) / g.dzg[q.Z];
+      R = Wreg[fidx(q.X, q.Y, q.Z, nxg, nyg)];
     }
-    acc = wave_sum(acc);
-    if (lane == 0) Sf[c * m.nb + l] = acc - E1nH[c * m.nb + l] * dudn[c * m.nb + l];
-  }
+    return q.w * (G - R);
+  }, r);
+  if (lane == 0)
+    for (int c = 0; c < 3; c++)
+      Sf[c * m.nb + l] = r[c] - E1nH[c * m.nb + l] * dudn[c * m.nb + l];
 }
 
 // -E u + ub (src/projection.f90:51-52).
@@ -256,18 +320,15 @@ __global__ __launch_bounds__(kBlock) void k_rhs_u(DevGrid g, Markers m, const do
   const int l = wave_marker(m.nb);
   const int lane = threadIdx.x & 63;
   if (l >= m.nb) return;
-  const double* F[3] = {U, V, W};
-  const int n1[3] = {g.nx, g.nxg, g.nxg}, n2[3] = {g.nyg, g.ny, g.nyg};
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double acc = 0.0;
-    for (int p = lane; p < 125; p += 64) {
-      const Pt q = stencil_point<false>(m, m.st[c], l, p);
-      acc += q.w * F[c][fidx(q.X, q.Y, q.Z, n1[c], n2[c])];
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) out[c * m.nb + l] = -acc + m.ub[c * m.nb + l];
-  }
+  const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
+  double r[3];
+  gather3<false>(m, l, lane, [&](int c, const Pt& q) {
+    const double f = c == 0 ? U[fidx(q.X, q.Y, q.Z, nx, nyg)]
+                            : (c == 1 ? V[fidx(q.X, q.Y, q.Z, nxg, ny)] : W[fidx(q.X, q.Y, q.Z, nxg, nyg)]);
+    return q.w * f;
+  }, r);
+  if (lane == 0)
+    for (int c = 0; c < 3; c++) out[c * m.nb + l] = -r[c] + m.ub[c * m.nb + l];
 }
 
 __global__ void k_aux(Markers m, const double* __restrict__ f, double dt, double nu,
@@ -279,35 +340,49 @@ __global__ void k_aux(Markers m, const double* __restrict__ f, double dt, double
   aux[t] = dt * nu * f[t] - dt * f[3 * nb + l] * m.normals[t];
 }
 
-// R, src/immersed_boundary_operators.f90:544-603: one thread per support node.
-__global__ __launch_bounds__(kBlock) void k_spread(int nb, double A, double factor,
-                                                   const double* __restrict__ phi,
-                                                   const double* __restrict__ sb, Spread sp,
-                                                   const double* __restrict__ f,
+// Static coefficients of the spreading entries: w*factor*sb[l] per entry.
+__global__ void k_spread_coeffs(int nb, double A, double factor, const double* __restrict__ phi,
+                                const double* __restrict__ sb, Spread sp) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= sp.nentries) return;
+  const int q = sp.q[e];
+  const int l = q / 125, p = q - 125 * l;
+  const int i = p % 5, j = (p / 5) % 5, k = p / 25;
+  const double w = ((A * phi[(0 * 5 + i) * nb + l]) * phi[(1 * 5 + j) * nb + l]) *
+                   phi[(2 * 5 + k) * nb + l];
+  sp.ml[e] = l;
+  sp.wfs[e] = w * factor * sb[l];
+}
+
+// R, src/immersed_boundary_operators.f90:544-603: one thread per support node,
+// contributions in ascending marker order.
+__global__ __launch_bounds__(kBlock) void k_spread(Spread sp, const double* __restrict__ f,
                                                    double* __restrict__ F) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= sp.nnodes) return;
   double acc = 0.0;
-  for (int e = sp.ptr[t]; e < sp.ptr[t + 1]; e++) {
-    const int q = sp.q[e];
-    const int l = q / 125, p = q - 125 * l;
-    const int i = p % 5, j = (p / 5) % 5, k = p / 25;
-    const double w = ((A * phi[(0 * 5 + i) * nb + l]) * phi[(1 * 5 + j) * nb + l]) *
-                     phi[(2 * 5 + k) * nb + l];
-    acc = acc + w * factor * sb[l] * f[l];
+  const int n = sp.cnt[t];
+  // loads of several entries in flight; the sum stays in entry order
+#pragma unroll 4
+  for (int i = 0; i < n; i++) {
+    const int e = sp.slot[i] + t;
+    acc = acc + sp.wfs[e] * f[sp.ml[e]];
   }
   F[sp.node[t]] = acc;
 }
 
 // remove_mean_per_body for the two-group slab layout
-// (src/immersed_boundary_geometry.f90:400-442), then f -= coef*pj.  One block.
-__global__ __launch_bounds__(1024) void k_remove_mean(Markers m, double* __restrict__ f,
-                                                      const double* __restrict__ coef,
-                                                      const double* __restrict__ pj) {
-  __shared__ double s1[1024], s2[1024];
+// (src/immersed_boundary_geometry.f90:400-442), then f -= coef*pj.  Block
+// partial sums of the two bodies, then every block re-reduces the partials
+// in the same fixed order and applies.
+constexpr int kMeanBlocks = 64;
+
+__global__ __launch_bounds__(kBlock) void k_body_sums(Markers m, const double* __restrict__ f,
+                                                      double* __restrict__ part) {
+  __shared__ double s1[kBlock], s2[kBlock];
   const int nb = m.nb, nxb = m.nxb, nxb1 = m.nxb1;
   double a = 0.0, b = 0.0;
-  for (int l = threadIdx.x; l < nb; l += 1024) {
+  for (int l = blockIdx.x * kBlock + threadIdx.x; l < nb; l += kMeanBlocks * kBlock) {
     if (l % nxb < nxb1)
       a += f[l];
     else
@@ -316,20 +391,36 @@ __global__ __launch_bounds__(1024) void k_remove_mean(Markers m, double* __restr
   s1[threadIdx.x] = a;
   s2[threadIdx.x] = b;
   __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       s1[threadIdx.x] += s1[threadIdx.x + o];
       s2[threadIdx.x] += s2[threadIdx.x + o];
     }
     __syncthreads();
   }
-  const double avg1 = s1[0] / (double)(nxb1 * m.nzb);
-  const double avg2 = s2[0] / (double)((nxb - nxb1) * m.nzb);
-  for (int l = threadIdx.x; l < nb; l += 1024) {
-    double v = f[l] - ((l % nxb < nxb1) ? avg1 : avg2);
-    if (coef) v = v - coef[l] * pj[l];
-    f[l] = v;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1[0];
+    part[kMeanBlocks + blockIdx.x] = s2[0];
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_remove_mean(Markers m, double* __restrict__ f,
+                                                        const double* __restrict__ part,
+                                                        const double* __restrict__ coef,
+                                                        const double* __restrict__ pj) {
+  const int nb = m.nb, nxb = m.nxb, nxb1 = m.nxb1;
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < kMeanBlocks; q++) {
+    a += part[q];
+    b += part[kMeanBlocks + q];
+  }
+  const double avg1 = a / (double)(nxb1 * m.nzb);
+  const double avg2 = b / (double)((nxb - nxb1) * m.nzb);
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= nb) return;
+  double v = f[l] - ((l % nxb < nxb1) ? avg1 : avg2);
+  if (coef) v = v - coef[l] * pj[l];
+  f[l] = v;
 }
 
 // U = U_interim - U_reg - G(dt*(P_interim - phi))  (src/projection.f90:83-92)
@@ -538,19 +629,17 @@ __global__ __launch_bounds__(kBlock) void k_max_slip(DevGrid g, Markers m, const
   const int l = wave_marker(m.nb);
   const int lane = threadIdx.x & 63;
   if (l >= m.nb) return;
-  const double* F[3] = {U, V, W};
-  const int n1[3] = {g.nx, g.nxg, g.nxg}, n2[3] = {g.nyg, g.ny, g.nyg};
+  const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
+  double r[3];
+  gather3<false>(m, l, lane, [&](int c, const Pt& q) {
+    const double f = c == 0 ? U[fidx(q.X, q.Y, q.Z, nx, nyg)]
+                            : (c == 1 ? V[fidx(q.X, q.Y, q.Z, nxg, ny)] : W[fidx(q.X, q.Y, q.Z, nxg, nyg)]);
+    return q.w * f;
+  }, r);
   double mx = 0.0;
-#pragma unroll
   for (int c = 0; c < 3; c++) {
-    double acc = 0.0;
-    for (int p = lane; p < 125; p += 64) {
-      const Pt q = stencil_point<false>(m, m.st[c], l, p);
-      acc += q.w * F[c][fidx(q.X, q.Y, q.Z, n1[c], n2[c])];
-    }
-    acc = wave_sum(acc);
     const int o = c * m.nb + l;
-    mx = fmax(mx, fabs(acc - E1nH[o] * fb[o] - m.ub[o]));
+    mx = fmax(mx, fabs(r[c] - E1nH[o] * fb[o] - m.ub[o]));
   }
   if (lane == 0) atomicMax(red, (unsigned long long)__double_as_longlong(mx));
 }
@@ -600,17 +689,27 @@ hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, d
   return hipGetLastError();
 }
 
+hipError_t launch_spread_coeffs(const Markers& m, const Spread& sp, int comp, hipStream_t s) {
+  if (sp.nentries == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spread_coeffs, blocks_for(sp.nentries), dim3(kBlock), 0, s, m.nb, m.A,
+                     m.factor, (const double*)m.st[comp].phi, m.sb, sp);
+  return hipGetLastError();
+}
+
 hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const double* f, double* F,
                          hipStream_t s) {
+  (void)m;
+  (void)comp;
   if (sp.nnodes == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_spread, blocks_for(sp.nnodes), dim3(kBlock), 0, s, m.nb, m.A, m.factor,
-                     (const double*)m.st[comp].phi, m.sb, sp, f, F);
+  hipLaunchKernelGGL(k_spread, blocks_for(sp.nnodes), dim3(kBlock), 0, s, sp, f, F);
   return hipGetLastError();
 }
 
 hipError_t launch_remove_mean(const Markers& m, double* f, const double* coef, const double* pj,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_remove_mean, dim3(1), dim3(1024), 0, s, m, f, coef, pj);
+                              double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_body_sums, dim3(kMeanBlocks), dim3(kBlock), 0, s, m, (const double*)f, part);
+  hipLaunchKernelGGL(k_remove_mean, blocks_for(m.nb), dim3(kBlock), 0, s, m, f, (const double*)part,
+                     coef, pj);
   return hipGetLastError();
 }
 

@@ -30,14 +30,22 @@ struct Markers {
   double factor;  // 1/(dx*dymin*dz)
 };
 
-// Node -> entry map of the spreading operator of one component: sorted
-// support node offsets, CSR row pointers and entries q = l*125 + point, each
-// row in ascending marker order.
+// Node -> entry map of the spreading operator of one component in jagged
+// diagonal order: support nodes ranked by descending entry count (node[r]
+// its field offset, cnt[r] its entry count), entry i of rank r stored at
+// slot[i] + r, so that lanes of consecutive ranks read consecutive
+// addresses.  Each node's entries are in ascending marker order.  Per entry:
+// q = l*125 + point, the marker ml and the static coefficient
+// wfs = (w*factor)*sb[ml] (the reference's product order, so wfs*f[ml]
+// rounds as w*factor*sb*f does).
 struct Spread {
-  int nnodes;
+  int nnodes, nentries;
   const int* node;
-  const int* ptr;
+  const int* cnt;
+  const int* slot;
   const int* q;
+  int* ml;
+  double* wfs;
 };
 
 // Grid data the set-up kernel needs besides DevGrid.
@@ -69,9 +77,12 @@ hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, d
 // nodes only (other nodes keep the zeros they were created with).
 hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const double* f,
                          double* F, hipStream_t s);
-// f[nb] -= per-body mean; then f -= coef*pj when coef != nullptr.
+// Fills sp.ml and sp.wfs of component comp from the stencils (once per body).
+hipError_t launch_spread_coeffs(const Markers& m, const Spread& sp, int comp, hipStream_t s);
+// f[nb] -= per-body mean; then f -= coef*pj when coef != nullptr.  part
+// holds 128 doubles of scratch.
 hipError_t launch_remove_mean(const Markers& m, double* f, const double* coef, const double* pj,
-                              hipStream_t s);
+                              double* part, hipStream_t s);
 // U = (Uint - Ureg) - G(dt*(Pint - phi)) on the gradient's index ranges,
 // (Uint - Ureg) elsewhere (src/projection.f90:83-92).
 hipError_t launch_ib_correct(const DevGrid& g, const double* Uint, const double* Vint,

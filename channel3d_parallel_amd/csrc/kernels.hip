@@ -554,10 +554,16 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
       p0 = pp;
     }
   }
-  // wavefront max, one atomic per wavefront
+  // block max, one atomic per block
+  __shared__ double sm[kBlock / 64];
   for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
-  if ((threadIdx.x & 63) == 0 && rmax)
-    atomicMax(rmax, (unsigned long long)__double_as_longlong(amax));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = amax;
+  __syncthreads();
+  if (threadIdx.x == 0 && rmax) {
+    double m = sm[0];
+    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, sm[w]);
+    atomicMax(rmax, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // gauge = 0.5 * Dyy(2,2) * mean over the unique cells of phi(:, 2, :): the
