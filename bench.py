@@ -107,9 +107,11 @@ def rk3_section(nx, ny, nz, U, V, W, steps):
             "steps": steps, "note": "host-timed whole steps (3 substeps: rhs, update, BCs, projection)"}
 
 
-def ib_section(nsteps=3, learned=True):
+def ib_section(nsteps=200, learned=True):
     """Two-cylinder IB case (130x129x130 uniform, nb = 24,576): plain
-    BiCGSTAB from a zero guess at tolerance 1e-10; iterations, slip and time."""
+    BiCGSTAB from a zero guess at tolerance 1e-10 -- one IB projection, then
+    `nsteps` RK3 steps from laminar with a constant driving pressure
+    gradient; iterations, slip, divergence and time."""
     from channel3d_parallel_amd import Channel, fields, ib_geometry
 
     nx, ny, nz = 130, 129, 130
@@ -130,16 +132,21 @@ def ib_section(nsteps=3, learned=True):
         proj_ms = (time.perf_counter() - t0) * 1e3
         slip, div = ch.check_slip(), ch.check_divergence()
         ch.set_pressure_gradient(3.0 * nu, 0.0, 0.0)
+        ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=1)
         iters = []
         t0 = time.perf_counter()
-        for _ in range(nsteps):
-            ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=1)
+        for n in range(nsteps):
             _, before, _ = ch.ib_stats()
             ch.compute_time_step_RK3(dt)
             _, after, _ = ch.ib_stats()
             iters.append(after - before)
+            if n % 50 == 49:
+                print(f"[bench] IB step {n + 1}: {iters[-1]} BiCGSTAB iterations", file=sys.stderr,
+                      flush=True)
         step_ms = (time.perf_counter() - t0) * 1e3 / nsteps
         slip_step = ch.check_slip()
+        div_step = ch.check_divergence()
+        umax = float(np.max(np.abs(ch.get_fields()["U"])))
     learned = learned_section(nx, ny, nz, dt) if learned else None
     return {"workload": "two counter-rotating cylinders r=0.2 at (2.0,0.7),(4.0,1.3), 96x128 "
                         "markers each, 128x128x128 cells uniform y, dt 0.01",
@@ -147,8 +154,11 @@ def ib_section(nsteps=3, learned=True):
             "nb": m.nb, "setup_ms": setup_ms, "bicgstab_iterations_zero_guess": it,
             "residual_norm": float(np.sqrt(err)), "ib_projection_ms": proj_ms,
             "ms_per_iteration": proj_ms / max(it, 1), "max_slip": slip, "max_div": div,
-            "rk3_step_ms": step_ms, "iterations_per_step_zero_guess": iters,
-            "max_slip_after_steps": slip_step}
+            "rk3_steps": nsteps, "rk3_step_ms": step_ms,
+            "iterations_per_step_zero_guess": {"first": iters[:3], "last": iters[-3:],
+                                               "mean": float(np.mean(iters)),
+                                               "per_solve_mean": float(np.mean(iters)) / 3},
+            "after_steps": {"max_slip": slip_step, "max_div": div_step, "max_abs_U": umax}}
 
 
 def learned_section(nx, ny, nz, dt):
@@ -217,7 +227,7 @@ def main():
                          "mixed-precision projection with 2 refinement sweeps")
     a = ap.parse_args()
     if a.only == "ib":
-        print(json.dumps({"ib": ib_section(nsteps=1, learned=False)}))
+        print(json.dumps({"ib": ib_section(nsteps=3, learned=False)}))
         return
     if a.only == "mixed":
         from channel3d_parallel_amd import Channel, fields

@@ -18,7 +18,8 @@ namespace c3d {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGradKC = 8;  // z planes per lane in the gradient correction
+constexpr int kGradKC = 8;  // z planes per lane in the residual kernel
+constexpr int kGradKCDefault = 2;  // z planes per lane in the gradient correction
 
 // Rows of the z tile handled by one x-transform workgroup (row pairs are
 // packed into one complex transform).  Default: 16 rows (256-byte kz runs in
@@ -623,6 +624,7 @@ __global__ void k_pghost_z(DevGrid g, double* __restrict__ P) {
 // Gradient correction: U = U - G(dt*Pint) on the index ranges of
 // src/differential_operators.f90:28-50 (faces outside them keep U).
 // ---------------------------------------------------------------------------
+template <int KC>
 __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __restrict__ U,
                                                          double* __restrict__ V,
                                                          double* __restrict__ W,
@@ -635,14 +637,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
   xcd_tile(bx, by, bz);
   const int i = bx * 64 + (threadIdx.x & 63) + 2;
   const int j = by * 4 + (threadIdx.x >> 6) + 2;
-  const int kb = bz * kGradKC + 2;
+  const int kb = bz * KC + 2;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   if (i > nx - 1 || j > nyg - 1) return;
   const double hx = g.dxg[i];
   const double hy = (j <= ny - 1) ? g.dyg[j] : 1.0;
   double p = dt * Pint[ridx(i, j, kb, nxg, nyg)];
 #pragma unroll
-  for (int kk = 0; kk < kGradKC; kk++) {
+  for (int kk = 0; kk < KC; kk++) {
     const int k = kb + kk;
     if (k > nzg - 1) break;
     const double px = dt * Pint[ridx(i + 1, j, k, nxg, nyg)];
@@ -977,11 +979,28 @@ hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s) {
   return hipGetLastError();
 }
 
+// z planes per lane of the gradient correction (2 measured fastest at
+// 384x193x384: more workgroups in flight); C3D_GRAD_KC (2, 4, 8, 16)
+// overrides it (tuning knob).
+static int grad_kc() {
+  static const int kc = [] {
+    const char* e = getenv("C3D_GRAD_KC");
+    const int v = e ? atoi(e) : 0;
+    return (v == 2 || v == 4 || v == 8 || v == 16) ? v : kGradKCDefault;
+  }();
+  return kc;
+}
+
 hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
                                double dt, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad_correct,
-                     grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kGradKC - 1) / kGradKC),
-                     dim3(kBlock), 0, s, g, U, V, W, Pint, dt);
+  const int kc = grad_kc();
+  const dim3 grid = grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kc - 1) / kc);
+  switch (kc) {
+    case 4: hipLaunchKernelGGL(k_grad_correct<4>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
+    case 8: hipLaunchKernelGGL(k_grad_correct<8>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
+    case 16: hipLaunchKernelGGL(k_grad_correct<16>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
+    default: hipLaunchKernelGGL(k_grad_correct<2>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
+  }
   return hipGetLastError();
 }
 
