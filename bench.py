@@ -222,10 +222,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, RK3 and IB sections")
-    ap.add_argument("--only", choices=["ib", "mixed"],
-                    help="profiling runs: only the IB case (no learned guess) or only the "
-                         "mixed-precision projection with 2 refinement sweeps")
+    ap.add_argument("--only", choices=["ib", "mixed", "rk3"],
+                    help="profiling runs: only the IB case (no learned guess), only the "
+                         "mixed-precision projection with 2 refinement sweeps, or only RK3 steps")
     a = ap.parse_args()
+    if a.only == "rk3":
+        from channel3d_parallel_amd import fields
+        nx, ny, nz, _, seed = WORKLOADS["proj384"]
+        U, V, W = fields.spectral_faces(nx, ny, nz, seed)
+        print(json.dumps({"rk3": rk3_section(nx, ny, nz, U, V, W, a.steps)}))
+        return
     if a.only == "ib":
         print(json.dumps({"ib": ib_section(nsteps=3, learned=False)}))
         return

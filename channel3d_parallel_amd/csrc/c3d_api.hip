@@ -318,9 +318,22 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
     for (double& e : v) e = (e != 0.0) ? 1.0 / e : 0.0;
     return v;
   };
+  // nu/h with the quotient rounded once, as nu*1.0/h evaluates
+  auto nu_over = [&](std::vector<double> v) {
+    for (double& e : v) e = (e != 0.0) ? (d.nu * 1.0) / e : 0.0;
+    return v;
+  };
   if ((rc = upload(c, &g.rdxf, recip(face_spacing(c->x)))) ||
       (rc = upload(c, &g.rdyf, recip(face_spacing(c->y)))) ||
-      (rc = upload(c, &g.rdzf, recip(face_spacing(c->z)))))
+      (rc = upload(c, &g.rdzf, recip(face_spacing(c->z)))) ||
+      (rc = upload(c, &g.rdxg, recip(centre_spacing(c->xg)))) ||
+      (rc = upload(c, &g.rdyg, recip(centre_spacing(c->yg)))) ||
+      (rc = upload(c, &g.rdzg, recip(centre_spacing(c->zg)))) ||
+      (rc = upload(c, &g.nudxf, nu_over(face_spacing(c->x)))) ||
+      (rc = upload(c, &g.nudyf, nu_over(face_spacing(c->y)))) ||
+      (rc = upload(c, &g.nudzf, nu_over(face_spacing(c->z)))) ||
+      (rc = upload(c, &g.nudxg, nu_over(centre_spacing(c->xg)))) ||
+      (rc = upload(c, &g.nudzg, nu_over(centre_spacing(c->zg)))))
     return cleanup(rc);
   if ((rc = upload(c, &g.dxf, face_spacing(c->x))) || (rc = upload(c, &g.dyf, face_spacing(c->y))) ||
       (rc = upload(c, &g.dzf, face_spacing(c->z))) ||

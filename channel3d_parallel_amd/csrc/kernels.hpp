@@ -21,6 +21,15 @@ struct DevGrid {
   const double* rdxf;  // 1/dxf (same indexing)
   const double* rdyf;  // 1/dyf
   const double* rdzf;  // 1/dzf
+  const double* rdxg;  // 1/dxg (same indexing as dxg)
+  const double* rdyg;  // 1/dyg
+  const double* rdzg;  // 1/dzg
+  // nu/spacing factors of the viscous terms (nu of the context)
+  const double* nudxf;  // nu/dxf
+  const double* nudyf;  // nu/dyf
+  const double* nudzf;  // nu/dzf
+  const double* nudxg;  // nu/dxg
+  const double* nudzg;  // nu/dzg
   const double* kxx;  // [nxp] modified wavenumbers
   const double* kzz;  // [nzp]
   const double* dyy_d;  // [nym]   Dyy(j,j)
