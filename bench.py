@@ -291,7 +291,7 @@ def main():
     achieved = sb[dom] / (stages[dom] * 1e-3) / 1e9
     total_bytes = sum(sb.values())
     out = {
-        "metric": "projection solves/s (FP64, 384x193x384); mixed vs FP64 at equal div err and IB iterations reported beside it",
+        "metric": "projection solves/s + %HBM roofline, mixed vs FP64 at equal div err; IB iters",
         "value": 1e3 / tot_ms,
         "unit": "solves/s",
         "n_gpus": 1,

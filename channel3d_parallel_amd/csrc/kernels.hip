@@ -182,7 +182,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   const int nzp = g.nzp;
   for (int e = tid; e < NH * TK; e += kBlock) {
     const int kx = e / TK, t = e % TK;
-    S[((size_t)jj * NH + kx) * nzp + k0 + t] = buf[kx * TKP + t];
+    // streaming store: the spectrum is next read by another kernel
+    const cpx<real> v = buf[kx * TKP + t];
+    cpx<real>* d = &S[((size_t)jj * NH + kx) * nzp + k0 + t];
+    __builtin_nontemporal_store(v.x, &d->x);
+    __builtin_nontemporal_store(v.y, &d->y);
   }
 }
 
@@ -397,7 +401,10 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
       vy[q] = (real)0;
       if (e < NH * TK) {
         const int kx = e / TK, t = e % TK;
-        const cpx<real> v = S[((size_t)jj * NH + kx) * nzp + k0 + t];
+        const cpx<real>* sp = &S[((size_t)jj * NH + kx) * nzp + k0 + t];
+        cpx<real> v;
+        v.x = __builtin_nontemporal_load(&sp->x);  // read once
+        v.y = __builtin_nontemporal_load(&sp->y);
         vx[q] = v.x;
         vy[q] = v.y;
       }
@@ -472,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
         double phi = (double)((t & 1) ? z.y : z.x) / norm;
         if constexpr (MODE == 3 || MODE == 4) phi = phi_in[ridx(2, j, k, nxg, nyg) + n] + phi;
         const double o = (MODE == 1 || MODE == 4) ? phi / dt : phi;
-        orow[n] = o;
+        __builtin_nontemporal_store(o, &orow[n]);
         if (rk1) prow[n] = phi / cP;
         if (n == 0) {  // x duplicate i = nxg-1 (offset nxp)
           orow[N] = o;
