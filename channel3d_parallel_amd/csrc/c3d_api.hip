@@ -357,7 +357,8 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   if ((rc = dalloc(c, &c->U, c->nu)) || (rc = dalloc(c, &c->V, c->nv)) ||
       (rc = dalloc(c, &c->W, c->nw)) || (rc = dalloc(c, &c->P, c->np)) ||
       (rc = dalloc(c, &c->Pint, c->nr)) || (rc = dalloc(c, &c->rhs, c->nr)) ||
-      (rc = dalloc(c, &c->S, c->nspec)) || (rc = dalloc(c, &c->CP, c->nspec)) ||
+      (rc = dalloc(c, &c->S, c->nspec)) ||
+      (!c3d::tri_part_supported(c->g) && (rc = dalloc(c, &c->CP, c->nspec))) ||
       (rc = dalloc(c, &c->red, kRedRefine + kMaxRef)))
     return cleanup(rc);
   if (d.precision == C3D_PREC_MIXED &&
@@ -509,7 +510,7 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   if (ev) HIPCHK(c, hipEventRecord(ev[1], st));
   HIPCHK(c, mixed ? c3d::launch_fft_z32(g, -1, c->S32, st) : c3d::launch_fft_z(g, -1, c->S, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[2], st));
-  HIPCHK(c, mixed ? c3d::launch_tri32(g, c->S32, c->CP, st) : c3d::launch_tri(g, c->S, c->CP, st));
+  HIPCHK(c, mixed ? c3d::launch_tri32(g, c->S32, st) : c3d::launch_tri(g, c->S, c->CP, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], st));
   HIPCHK(c, mixed ? c3d::launch_fft_z32(g, +1, c->S32, st) : c3d::launch_fft_z(g, +1, c->S, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[4], st));
@@ -526,7 +527,7 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
                                    c->red + kRedRefine + s - 1, st));
     HIPCHK(c, c3d::launch_fwd_x32(g, nullptr, nullptr, nullptr, nullptr, c->r32, c->S32, st));
     HIPCHK(c, c3d::launch_fft_z32(g, -1, c->S32, st));
-    HIPCHK(c, c3d::launch_tri32(g, c->S32, c->CP, st));
+    HIPCHK(c, c3d::launch_tri32(g, c->S32, st));
     HIPCHK(c, c3d::launch_fft_z32(g, +1, c->S32, st));
     if (last)
       HIPCHK(c, c3d::launch_inv_x32(g, 4, c->S32, c->phi64, c->Pint, c->P, dt, rk1, st));

@@ -280,85 +280,160 @@ __global__ __launch_bounds__(kBlock) void k_tri64(DevGrid g, cpx<double>* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Same Thomas solve, chunked for a compile-time length NYM = NCH*C: rows are
-// loaded C at a time (C independent loads in flight per lane) and the
-// forward sweep is re-run chunk by chunk during the back substitution from
-// checkpoints (cp and the eliminated rhs entering each chunk, 3 doubles per
-// chunk in CK), so the memory traffic is the rhs read twice, the solution
-// written once and the small checkpoint array.
+// Same system for NYM = 8*M rows (64, 128, 192), partitioned: each mode's rows are split into 8
+// parts of M rows, one lane per part, so every row is read once and written
+// once and the whole part stays in registers.  A wave holds 8 modes (lane =
+// part*8 + mode) so that each row access is a 128-byte run of 8 modes.
+//   pass 1  forward elimination inside the part with the unknown x_{a-1}
+//           above it carried as a spike column sp:
+//             x_i = y_i - sp_i x_{a-1} - cp_i x_{i+1}
+//   pass 2  back-substituted forms of the part's first and last rows,
+//             x_i = G_i - V_i x_{a-1} - W_i x_b   (i = a, b-1)
+//   reduced the 2x8 interface equations of a mode, solved by its part-0
+//           lane through LDS: L_q = x(first row of q), R_q = x(last row);
+//           forward L_q = lam_q + mu_q L_{q+1}, R_q = rho_q + sig_q L_{q+1},
+//           back L_q, then R_{q-1} = rho_{q-1} + sig_{q-1} L_q
+//   pass 3  back substitution with the boundary values x_{a-1} = R_{q-1},
+//           x_b = L_{q+1}.
+// Part 0 (l = 0 in its first row) and the last part (u = 0 in its last row)
+// have no outside coupling, so the reduced system has no boundary terms.
 // ---------------------------------------------------------------------------
-template <typename real, int NCH, int C>
-__global__ __launch_bounds__(kBlock) void k_tri_chunked(DevGrid g, cpx<real>* __restrict__ S,
-                                                        double* __restrict__ CK) {
+template <typename real, int M>
+__global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __restrict__ S) {
+  constexpr int P = 8;            // parts per mode
+  constexpr int N = P * M;        // rows
+  constexpr int MW = 64 / P;      // modes per wave
+  constexpr int MB = kBlock / P;  // modes per workgroup
+  constexpr int CPT = M + 1;      // padded pitch of the row tables
+  constexpr int RP = 8 * P + 1;   // padded pitch of a mode's interface rows
+  __shared__ double td[P * CPT], tl[P * CPT], tu[P * CPT];
+  __shared__ double red[MB * RP];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N; e += kBlock) {
+    const int q = e / M, r = e % M;
+    td[q * CPT + r] = g.dyy_d[e];
+    tl[q * CPT + r] = (e == 0) ? 0.0 : g.dyy_l[e - 1];
+    tu[q * CPT + r] = (e == N - 1) ? 0.0 : g.dyy_u[e];
+  }
+  const int lane = tid & 63;
+  const int p = lane / MW;
+  const int mb = (tid >> 6) * MW + lane % MW;
   const int nzp = g.nzp;
   const size_t stride = (size_t)g.nxh * nzp;
-  const size_t m = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  if (m >= stride) return;
-  const int kx = (int)(m / nzp), kz = (int)(m % nzp);
+  const size_t m = (size_t)blockIdx.x * MB + mb;
+  const bool valid = m < stride;
+  const size_t mv = valid ? m : 0;
+  const int kx = (int)(mv / nzp), kz = (int)(mv % nzp);
   const double lamx = g.kxx[kx];
   const double lamz = g.kzz[kz];
-  const bool zero_mode = (kx == 0 && kz == 0);
-  cpx<real>* s = S + m;
-  double* ck = CK + m;
-  constexpr int N = NCH * C;
+  const bool zero_first = (kx == 0 && kz == 0 && p == 0);
+  cpx<real>* s = S + mv + (size_t)(p * M) * stride;
 
-  // forward sweep: y_j = (b_j - l_{j-1} y_{j-1}) / den_j, cp_j = u_j / den_j
-  double cprev = 0.0;
-  cpx<double> yprev = cmk(0.0, 0.0);
-  for (int c = 0; c < NCH; c++) {
-    ck[(size_t)(3 * c) * stride] = cprev;
-    ck[(size_t)(3 * c + 1) * stride] = yprev.x;
-    ck[(size_t)(3 * c + 2) * stride] = yprev.y;
-    cpx<double> b[C];
+  double yr[M], yi[M], cp[M], sp[M];
 #pragma unroll
-    for (int r = 0; r < C; r++) {
-      const cpx<real> t = s[(size_t)(c * C + r) * stride];
-      b[r] = cmk((double)t.x, (double)t.y);
-    }
-#pragma unroll
-    for (int r = 0; r < C; r++) {
-      const int jj = c * C + r;
-      double d = (g.dyy_d[jj] + lamx) + lamz;
-      if (jj == 0 && zero_mode) d = 1.5 * d;
-      const double l = (jj == 0) ? 0.0 : g.dyy_l[jj - 1];
-      const double inv = 1.0 / (d - l * cprev);
-      yprev = cmk((b[r].x - l * yprev.x) * inv, (b[r].y - l * yprev.y) * inv);
-      cprev = (jj < N - 1) ? g.dyy_u[jj] * inv : 0.0;
+  for (int r = 0; r < M; r++) {
+    yr[r] = 0.0;
+    yi[r] = 0.0;
+    if (valid) {
+      const cpx<real> t = s[(size_t)r * stride];
+      yr[r] = (double)t.x;
+      yi[r] = (double)t.y;
     }
   }
-  // back substitution x_j = y_j - cp_j x_{j+1}, chunk by chunk from the top,
-  // re-running the forward sweep inside each chunk from its checkpoint
-  cpx<double> xnext = cmk(0.0, 0.0);
-  for (int c = NCH - 1; c >= 0; c--) {
-    cpx<double> y[C];
-    double cpv[C];
+  __syncthreads();  // row tables
+
+  // pass 1
+  const double* rd = td + p * CPT;
+  const double* rl = tl + p * CPT;
+  const double* ru = tu + p * CPT;
 #pragma unroll
-    for (int r = 0; r < C; r++) {
-      const cpx<real> t = s[(size_t)(c * C + r) * stride];
-      y[r] = cmk((double)t.x, (double)t.y);
+  for (int r = 0; r < M; r++) {
+    double d = (rd[r] + lamx) + lamz;
+    const double l = rl[r];
+    if (r == 0) {
+      if (zero_first) d = 1.5 * d;
+      const double inv = 1.0 / d;
+      yr[0] = yr[0] * inv;
+      yi[0] = yi[0] * inv;
+      sp[0] = l * inv;
+      cp[0] = ru[0] * inv;
+    } else {
+      const double inv = 1.0 / (d - l * cp[r - 1]);
+      yr[r] = (yr[r] - l * yr[r - 1]) * inv;
+      yi[r] = (yi[r] - l * yi[r - 1]) * inv;
+      sp[r] = -(l * sp[r - 1]) * inv;
+      cp[r] = ru[r] * inv;
     }
-    double cp = ck[(size_t)(3 * c) * stride];
-    cpx<double> yp = cmk(ck[(size_t)(3 * c + 1) * stride], ck[(size_t)(3 * c + 2) * stride]);
+  }
+  // pass 2
+  double* my = red + mb * RP + p * 8;
+  {
+    double gr = yr[M - 1], gi = yi[M - 1], v = sp[M - 1], w = cp[M - 1];
+    my[4] = gr;
+    my[5] = gi;
+    my[6] = v;
+    my[7] = w;
 #pragma unroll
-    for (int r = 0; r < C; r++) {
-      const int jj = c * C + r;
-      double d = (g.dyy_d[jj] + lamx) + lamz;
-      if (jj == 0 && zero_mode) d = 1.5 * d;
-      const double l = (jj == 0) ? 0.0 : g.dyy_l[jj - 1];
-      const double inv = 1.0 / (d - l * cp);
-      yp = cmk((y[r].x - l * yp.x) * inv, (y[r].y - l * yp.y) * inv);
-      cp = (jj < N - 1) ? g.dyy_u[jj] * inv : 0.0;
-      cpv[r] = cp;
-      y[r] = yp;
+    for (int r = M - 2; r >= 0; r--) {
+      gr = yr[r] - cp[r] * gr;
+      gi = yi[r] - cp[r] * gi;
+      v = sp[r] - cp[r] * v;
+      w = -(cp[r] * w);
     }
+    my[0] = gr;
+    my[1] = gi;
+    my[2] = v;
+    my[3] = w;
+  }
+  __syncthreads();
+  // reduced system: slot q holds GL(2) VL WL GR(2) VR WR on entry; the
+  // forward sweep leaves lam(2) mu sig_{q-1} rho_{q-1}(2) in it and the
+  // back sweep x_{a-1}(2) x_b(2) in positions 4..7
+  if (p == 0) {
+    double* rs = red + mb * RP;
+    double rhr = 0.0, rhi = 0.0, sig = 0.0;
+    for (int q = 0; q < P; q++) {
+      double* e = rs + q * 8;
+      const double glr = e[0], gli = e[1], vl = e[2], wl = e[3];
+      const double grr = e[4], gri = e[5], vr = e[6], wr = e[7];
+      const double iden = 1.0 / (1.0 + vl * sig);
+      const double lr = (glr - vl * rhr) * iden, li = (gli - vl * rhi) * iden;
+      const double mu = -wl * iden;
+      e[0] = lr;
+      e[1] = li;
+      e[2] = mu;
+      e[3] = sig;
+      e[4] = rhr;
+      e[5] = rhi;
+      const double nr = grr - vr * (rhr + sig * lr), ni = gri - vr * (rhi + sig * li);
+      sig = -(vr * sig * mu + wr);
+      rhr = nr;
+      rhi = ni;
+    }
+    double lnr = 0.0, lni = 0.0;  // L_{q+1}
+    for (int q = P - 1; q >= 0; q--) {
+      double* e = rs + q * 8;
+      const double lr = e[0] + e[2] * lnr, li = e[1] + e[2] * lni;  // L_q
+      const double ar = e[4] + e[3] * lr, ai = e[5] + e[3] * li;     // R_{q-1}
+      e[6] = lnr;
+      e[7] = lni;
+      e[4] = ar;
+      e[5] = ai;
+      lnr = lr;
+      lni = li;
+    }
+  }
+  __syncthreads();
+  // pass 3
+  const double xar = my[4], xai = my[5];
+  double xnr = my[6], xni = my[7];
 #pragma unroll
-    for (int r = C - 1; r >= 0; r--) {
-      const int jj = c * C + r;
-      cpx<double> x = y[r];
-      if (jj < N - 1) x = cmk(x.x - cpv[r] * xnext.x, x.y - cpv[r] * xnext.y);
-      xnext = x;
-      s[(size_t)jj * stride] = cmk((real)x.x, (real)x.y);
-    }
+  for (int r = M - 1; r >= 0; r--) {
+    const double xr = (yr[r] - sp[r] * xar) - cp[r] * xnr;
+    const double xi = (yi[r] - sp[r] * xai) - cp[r] * xni;
+    if (valid) s[(size_t)r * stride] = cmk((real)xr, (real)xi);
+    xnr = xr;
+    xni = xi;
   }
 }
 
@@ -902,24 +977,25 @@ hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t
 hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
   const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
+  const dim3 gridp((unsigned)((nm + kBlock / 8 - 1) / (kBlock / 8)));
   switch (g.nym) {
-    case 64: hipLaunchKernelGGL((k_tri_chunked<double, 4, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
-    case 128: hipLaunchKernelGGL((k_tri_chunked<double, 8, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
-    case 192: hipLaunchKernelGGL((k_tri_chunked<double, 12, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 64: hipLaunchKernelGGL((k_tri_part<double, 8>), gridp, dim3(kBlock), 0, s, g, S); break;
+    case 128: hipLaunchKernelGGL((k_tri_part<double, 16>), gridp, dim3(kBlock), 0, s, g, S); break;
+    case 192: hipLaunchKernelGGL((k_tri_part<double, 24>), gridp, dim3(kBlock), 0, s, g, S); break;
     default: hipLaunchKernelGGL(k_tri64, grid, dim3(kBlock), 0, s, g, S, CP); break;
   }
   return hipGetLastError();
 }
 
-bool tri32_supported(const DevGrid& g) { return g.nym == 64 || g.nym == 128 || g.nym == 192; }
+bool tri_part_supported(const DevGrid& g) { return g.nym == 64 || g.nym == 128 || g.nym == 192; }
 
-hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, double* CP, hipStream_t s) {
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
-  const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
+  const dim3 grid((unsigned)((nm + kBlock / 8 - 1) / (kBlock / 8)));
   switch (g.nym) {
-    case 64: hipLaunchKernelGGL((k_tri_chunked<float, 4, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
-    case 128: hipLaunchKernelGGL((k_tri_chunked<float, 8, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
-    case 192: hipLaunchKernelGGL((k_tri_chunked<float, 12, 16>), grid, dim3(kBlock), 0, s, g, S, CP); break;
+    case 64: hipLaunchKernelGGL((k_tri_part<float, 8>), grid, dim3(kBlock), 0, s, g, S); break;
+    case 128: hipLaunchKernelGGL((k_tri_part<float, 16>), grid, dim3(kBlock), 0, s, g, S); break;
+    case 192: hipLaunchKernelGGL((k_tri_part<float, 24>), grid, dim3(kBlock), 0, s, g, S); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -51,7 +51,10 @@ __host__ __device__ __forceinline__ size_t ridx(int i, int j, int k, int nxg, in
 hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const double* V,
                         const double* W, const double* rhs, cpx<double>* S, hipStream_t s);
 hipError_t launch_fft_z(const DevGrid& g, int sign, cpx<double>* S, hipStream_t s);
-// Tridiagonal solve along y for every (kx,kz); CP is nym*nxh*nzp scratch.
+// Tridiagonal solve along y for every (kx,kz).  nym = 64, 128, 192 use the
+// partitioned register solver; other sizes the plain Thomas sweep with CP,
+// nym*nxh*nzp doubles of scratch (may be null when tri_part_supported).
+bool tri_part_supported(const DevGrid& g);
 hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s);
 // Inverse x transform.  mode 0: out = solution rhs_p (Poisson drop-in).
 // mode 1: out = P_interim = solution/dt (projection).  rk1: also write P.
@@ -64,8 +67,7 @@ hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double
 hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
                           double* b, const float* r, cpx<float>* S, hipStream_t s);
 hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t s);
-bool tri32_supported(const DevGrid& g);
-hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, double* CP, hipStream_t s);
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, hipStream_t s);  // tri_part_supported sizes
 // mode 0: out = phi, 1: out = phi/dt, 3: out = phi_in + phi,
 // 4: out = (phi_in + phi)/dt; rk1 writes P from the same potential.
 hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
