@@ -147,6 +147,8 @@ def ib_section(nsteps=200, learned=True):
         slip_step = ch.check_slip()
         div_step = ch.check_divergence()
         umax = float(np.max(np.abs(ch.get_fields()["U"])))
+    starts = {name: ib_rk3_run(nx, ny, nz, nu, dt, U, V, W, nsteps, mode)
+              for name, mode in (("previous_force", 0), ("history", 4))}
     learned = learned_section(nx, ny, nz, dt) if learned else None
     return {"workload": "two counter-rotating cylinders r=0.2 at (2.0,0.7),(4.0,1.3), 96x128 "
                         "markers each, 128x128x128 cells uniform y, dt 0.01",
@@ -158,7 +160,41 @@ def ib_section(nsteps=200, learned=True):
             "iterations_per_step_zero_guess": {"first": iters[:3], "last": iters[-3:],
                                                "mean": float(np.mean(iters)),
                                                "per_solve_mean": float(np.mean(iters)) / 3},
-            "after_steps": {"max_slip": slip_step, "max_div": div_step, "max_abs_U": umax}}
+            "after_steps": {"max_slip": slip_step, "max_div": div_step, "max_abs_U": umax},
+            "rk3_other_starts": starts}
+
+
+def ib_rk3_run(nx, ny, nz, nu, dt, U, V, W, nsteps, guess_mode):
+    """The same `nsteps` RK3 steps with another initial guess of the Schur
+    solve: BiCGSTAB iterations per step, time per step, final slip and
+    divergence."""
+    from channel3d_parallel_amd import Channel, ib_geometry
+
+    with Channel(nx, ny, nz, grid_type=0, nu=nu) as ch:
+        ch.setup_IB(ib_geometry.two_cylinders(ch.grid(11)[9]))
+        ch.set_fields(U, V, W)
+        ch.apply_boundary_conditions()
+        ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=1)
+        ch.compute_non_IB_projection(dt, 2)
+        ch.apply_boundary_conditions()
+        ch.compute_IB_projection(dt, 2)
+        ch.set_pressure_gradient(3.0 * nu, 0.0, 0.0)
+        ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=guess_mode)
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            _, before, _ = ch.ib_stats()
+            ch.compute_time_step_RK3(dt)
+            _, after, _ = ch.ib_stats()
+            iters.append(after - before)
+        step_ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        print(f"[bench] IB guess mode {guess_mode}: {np.mean(iters):.1f} iterations per step",
+              file=sys.stderr, flush=True)
+        return {"guess_mode": guess_mode, "rk3_step_ms": step_ms,
+                "iterations_per_step": {"first": iters[:3], "last": iters[-3:],
+                                        "mean": float(np.mean(iters)),
+                                        "per_solve_mean": float(np.mean(iters)) / 3},
+                "after_steps": {"max_slip": ch.check_slip(), "max_div": ch.check_divergence()}}
 
 
 def learned_section(nx, ny, nz, dt):
@@ -176,14 +212,15 @@ def learned_section(nx, ny, nz, dt):
         t0 = time.perf_counter()
         hist, _, _ = lg.train(ch, X, T, epochs=40, batch=256, lr=1e-3)
         train_s = time.perf_counter() - t0
-        rec = lg.ab_compare(ch, dt, steps=2)
+        rec = lg.ab_compare(ch, dt, steps=4)
     return {"pairs": int(X.shape[0]), "generation_s": gen_s, "training_s": train_s,
             "loss_first_last": [float(hist[0]), float(hist[-1])],
             "held_out_speeds": [s1, s2],
             "iterations_zero_guess": rec["zero"], "iterations_previous_force": rec["previous"],
-            "iterations_learned": rec["learned"],
+            "iterations_learned": rec["learned"], "iterations_history_start": rec["history"],
             "max_rel_force_diff_learned_vs_zero": max(rec["max_force_diff_learned_vs_zero"]),
-            "max_rel_force_diff_previous_vs_zero": max(rec["max_force_diff_previous_vs_zero"])}
+            "max_rel_force_diff_previous_vs_zero": max(rec["max_force_diff_previous_vs_zero"]),
+            "max_rel_force_diff_history_vs_zero": max(rec["max_force_diff_history_vs_zero"])}
 
 
 def read_traffic(kernel, workload):

@@ -53,6 +53,16 @@ struct IBState {
   double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr, *mpart = nullptr;
   double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
   double* fprev = nullptr;  // force before the last solve (learned-guess pairs)
+  // history-projection start (guess_mode 4), allocated on first use: ring of
+  // the last kHist converged forces and their right-hand sides, and the
+  // orthonormal rhs basis hQ with forces hX (S hX = hQ) built from it
+  double* hF[c3d::ib::kHist] = {nullptr};
+  double* hB[c3d::ib::kHist] = {nullptr};
+  double* hQ[c3d::ib::kHist] = {nullptr};
+  double* hX[c3d::ib::kHist] = {nullptr};
+  double *hpart = nullptr, *hcoef = nullptr;
+  int hcount = 0, hhead = 0;
+  double hdt = 0.0;  // dt of the stored pairs (the Schur operator depends on dt)
   double last_dt = 1.0;     // dt of the last IB projection
   double tol = 1e-10;
   int maxit = 1000, guess = 0;
@@ -940,6 +950,105 @@ static int ib_bicgstab_launch(c3d_ctx* c, double* x, const double* b, double tol
   return 0;
 }
 
+// ---- history-projection start (guess_mode 4) ----
+// The Schur operator depends only on dt, nu and the static geometry, so every
+// converged solve (f_i, b_i) of one dt satisfies S f_i = b_i to the solver
+// tolerance.  The start for a new right-hand side b is the force combination
+// whose rhs combination is closest to b in the 2-norm.
+static int ib_hist_reserve(c3d_ctx* c) {
+  IBState* ib = c->ib;
+  if (ib->hF[0]) return 0;
+  const size_t n4 = 4 * (size_t)ib->m.nb;
+  constexpr int H = c3d::ib::kHist;
+  int rc = 0;
+  for (int j = 0; j < H && !rc; j++)
+    if ((rc = dalloc(c, &ib->hF[j], n4)) || (rc = dalloc(c, &ib->hB[j], n4)) ||
+        (rc = dalloc(c, &ib->hQ[j], n4)) || (rc = dalloc(c, &ib->hX[j], n4)))
+      break;
+  if (!rc) rc = dalloc(c, &ib->hpart, (size_t)(H + 1) * c3d::ib::kDotBlocks);
+  if (!rc) rc = dalloc(c, &ib->hcoef, (size_t)4 * (H + 1));
+  if (rc) ib->hF[0] = nullptr;
+  return rc;
+}
+
+static void ib_hist_check_dt(IBState* ib, double dt) {
+  if (ib->hdt != dt) {
+    ib->hcount = 0;
+    ib->hhead = 0;
+    ib->hdt = dt;
+  }
+}
+
+// Appends the converged pair (fb, rhs_ib) of a solve with step dt.
+static int ib_hist_push(c3d_ctx* c, double dt) {
+  IBState* ib = c->ib;
+  const size_t bytes = 4 * (size_t)ib->m.nb * 8;
+  ib_hist_check_dt(ib, dt);
+  HIPCHK(c, hipMemcpyAsync(ib->hF[ib->hhead], ib->fb, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ib->hB[ib->hhead], ib->rhs_ib, bytes, hipMemcpyDeviceToDevice, c->stream));
+  ib->hhead = (ib->hhead + 1) % c3d::ib::kHist;
+  if (ib->hcount < c3d::ib::kHist) ib->hcount++;
+  return 0;
+}
+
+// fb = sum_j <Q_j, rhs_ib> X_j over the basis of the stored pairs (oldest
+// first, Gram-Schmidt with a second pass); fb is left as it is (the previous
+// force) while no pair of this dt is stored.
+static int ib_hist_start(c3d_ctx* c, double dt) {
+  IBState* ib = c->ib;
+  namespace B = c3d::ib;
+  constexpr int H = B::kHist;
+  ib_hist_check_dt(ib, dt);
+  const int cnt = ib->hcount;
+  if (cnt == 0) return 0;
+  const size_t n4 = 4 * (size_t)ib->m.nb;
+  const size_t bytes = n4 * 8;
+  double* ca = ib->hcoef;                // first-pass coefficients; [i] = raw norm^2
+  double* cb = ib->hcoef + (H + 1);      // second-pass coefficients
+  double* cn = ib->hcoef + 2 * (H + 1);  // squared norm after orthogonalisation
+  double* cf = ib->hcoef + 3 * (H + 1);  // coefficients of the start
+  for (int i = 0; i < cnt; i++) {
+    const int src = (ib->hhead - cnt + i + H) % H;
+    HIPCHK(c, hipMemcpyAsync(ib->hQ[i], ib->hB[src], bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ib->hX[i], ib->hF[src], bytes, hipMemcpyDeviceToDevice, c->stream));
+    B::VecSet A{}, X{};
+    for (int j = 0; j < i; j++) {
+      A.v[j] = ib->hQ[j];
+      X.v[j] = ib->hX[j];
+    }
+    A.v[i] = ib->hQ[i];
+    A.k = i + 1;  // projections on Q_0..Q_{i-1} and the raw norm
+    HIPCHK(c, B::launch_mdot(A, ib->hQ[i], n4, ib->hpart, c->stream));
+    HIPCHK(c, B::launch_mreduce(ib->hpart, i + 1, ca, c->stream));
+    if (i == 0) {
+      HIPCHK(c, B::launch_hscale(ib->hQ[0], ib->hX[0], ca, ca, n4, c->stream));
+      continue;
+    }
+    A.k = i;
+    X.k = i;
+    HIPCHK(c, B::launch_combine(A, X, ca, -1.0, false, ib->hQ[i], ib->hX[i], n4, c->stream));
+    HIPCHK(c, B::launch_mdot(A, ib->hQ[i], n4, ib->hpart, c->stream));
+    HIPCHK(c, B::launch_mreduce(ib->hpart, i, cb, c->stream));
+    HIPCHK(c, B::launch_combine(A, X, cb, -1.0, false, ib->hQ[i], ib->hX[i], n4, c->stream));
+    B::VecSet Nq{};
+    Nq.v[0] = ib->hQ[i];
+    Nq.k = 1;
+    HIPCHK(c, B::launch_mdot(Nq, ib->hQ[i], n4, ib->hpart, c->stream));
+    HIPCHK(c, B::launch_mreduce(ib->hpart, 1, cn, c->stream));
+    HIPCHK(c, B::launch_hscale(ib->hQ[i], ib->hX[i], cn, ca + i, n4, c->stream));
+  }
+  B::VecSet A{}, X{};
+  for (int j = 0; j < cnt; j++) {
+    A.v[j] = ib->hQ[j];
+    X.v[j] = ib->hX[j];
+  }
+  A.k = X.k = cnt;
+  HIPCHK(c, B::launch_mdot(A, ib->rhs_ib, n4, ib->hpart, c->stream));
+  HIPCHK(c, B::launch_mreduce(ib->hpart, cnt, cf, c->stream));
+  HIPCHK(c, B::launch_combine(A, X, cf, 1.0, true, nullptr, ib->fb, n4, c->stream));
+  return 0;
+}
+
 // compute_IB_projection (src/projection.f90:48-96).
 static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
   IBState* ib = c->ib;
@@ -959,10 +1068,16 @@ static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
     if (rc0) return rc0;
     r_ready = true;
   }
+  if (ib->guess == 4) {  // least-squares start over earlier solves of this dt
+    int rc0 = ib_hist_reserve(c);
+    if (!rc0) rc0 = ib_hist_start(c, dt);
+    if (rc0) return rc0;
+  }
   int rc = ib_bicgstab_launch(c, ib->fb, ib->rhs_ib, ib->tol, ib->maxit, dt, rk_step, nullptr,
                               nullptr, r_ready);                                         // :60
   if (rc && rc != C3D_ENOCONV) return rc;
   const int conv = rc;
+  if (!conv && ib->guess == 4 && (rc = ib_hist_push(c, dt))) return rc;
   if ((rc = ib_spread_force(c, ib->fb, dt))) return rc;                                  // :66-73
   if ((rc = ib_poisson_reg(c, dt, rk_step))) return rc;                                  // :76-80
   HIPCHK(c, c3d::ib::launch_ib_correct(c->g, ib->Uint, ib->Vint, ib->Wint, ib->Ureg, ib->Vreg,
@@ -1282,7 +1397,7 @@ int c3d_ib_bicgstab(c3d_ctx* c, double* x, const double* b, double tol, int maxi
 int c3d_ib_set_solver(c3d_ctx* c, double tol, int maxit, int guess_mode) {
   int rc = need_ib(c);
   if (rc) return rc;
-  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 3) return C3D_EINVAL;
+  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 4) return C3D_EINVAL;
   c->ib->tol = tol;
   c->ib->maxit = maxit;
   c->ib->guess = guess_mode;

@@ -619,6 +619,65 @@ __global__ void k_line_step(double* __restrict__ x, const double* __restrict__ d
   r[t] = r[t] - a * Sd[t];
 }
 
+// ---------------------------------------------------------------------------
+// History-projection start: block partials of several dot products against
+// one vector, their fixed-order sums, linear combinations and the scaling of
+// a Gram-Schmidt column.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_mdot(VecSet A, const double* __restrict__ q, size_t n,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[kHist + 1][kBlock / 64];
+  double s[kHist + 1];
+#pragma unroll
+  for (int j = 0; j <= kHist; j++) s[j] = 0.0;
+  for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)kDotBlocks * kBlock) {
+    const double qt = q[t];
+#pragma unroll
+    for (int j = 0; j <= kHist; j++)
+      if (j < A.k) s[j] += A.v[j][t] * qt;
+  }
+#pragma unroll
+  for (int j = 0; j <= kHist; j++) {
+    if (j < A.k) {
+      const double r = block_sum(s[j], sh[j]);
+      if (threadIdx.x == 0) part[j * kDotBlocks + blockIdx.x] = r;
+    }
+  }
+}
+
+__global__ void k_mreduce(const double* __restrict__ part, int k, double* __restrict__ coef) {
+  const int j = threadIdx.x;
+  if (j < k) coef[j] = sum_part(part, j);
+}
+
+// qa = (set ? 0 : qa) + sgn * sum_j coef[j] A_j ; xb likewise with B_j
+__global__ void k_combine(VecSet A, VecSet B, const double* __restrict__ coef, double sgn, int set,
+                          double* __restrict__ qa, double* __restrict__ xb, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double a = 0.0, b = 0.0;
+  for (int j = 0; j < A.k; j++) {
+    const double cj = sgn * coef[j];
+    if (qa) a += cj * A.v[j][t];
+    if (xb) b += cj * B.v[j][t];
+  }
+  if (qa) qa[t] = set ? a : qa[t] + a;
+  if (xb) xb[t] = set ? b : xb[t] + b;
+}
+
+// q, x /= ||q|| when ||q||^2 > kHistDrop^2 * raw2 (raw2: the column's squared
+// norm before orthogonalisation), else q = x = 0 (a dependent column).
+__global__ void k_hscale(double* __restrict__ q, double* __restrict__ x,
+                         const double* __restrict__ nrm2, const double* __restrict__ raw2, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double v = *nrm2;
+  const bool keep = v > 0.0 && v > (kHistDrop * kHistDrop) * (*raw2);
+  const double inv = keep ? 1.0 / sqrt(v) : 0.0;
+  q[t] = q[t] * inv;
+  x[t] = x[t] * inv;
+}
+
 // check_slip, src/monitor.f90:223-231
 __global__ __launch_bounds__(kBlock) void k_max_slip(DevGrid g, Markers m, const double* __restrict__ U,
                                                      const double* __restrict__ V,
@@ -773,6 +832,32 @@ hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p,
 hipError_t launch_line_step(double* x, const double* d, double* r, const double* Sd,
                             const double* sc, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_line_step, blocks_for(n), dim3(kBlock), 0, s, x, d, r, Sd, sc, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_mdot(const VecSet& A, const double* q, size_t n, double* part, hipStream_t s) {
+  if (A.k < 1 || A.k > kHist + 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mdot, dim3(kDotBlocks), dim3(kBlock), 0, s, A, q, n, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_mreduce(const double* part, int k, double* coef, hipStream_t s) {
+  if (k < 1 || k > kHist + 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mreduce, dim3(1), dim3(64), 0, s, part, k, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(const VecSet& A, const VecSet& B, const double* coef, double sgn, bool set,
+                          double* qa, double* xb, size_t n, hipStream_t s) {
+  if (A.k < 0 || A.k > kHist + 1 || (xb && B.k != A.k)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_combine, blocks_for(n), dim3(kBlock), 0, s, A, B, coef, sgn, set ? 1 : 0, qa,
+                     xb, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_hscale(double* q, double* x, const double* nrm2, const double* raw2, size_t n,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_hscale, blocks_for(n), dim3(kBlock), 0, s, q, x, nrm2, raw2, n);
   return hipGetLastError();
 }
 

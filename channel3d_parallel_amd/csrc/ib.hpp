@@ -125,6 +125,27 @@ hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p,
 // x += a d; r -= a Sd with a = sc[6]
 hipError_t launch_line_step(double* x, const double* d, double* r, const double* Sd,
                             const double* sc, size_t n, hipStream_t s);
+// History-projection start for successive right-hand sides of one Schur
+// operator: up to kHist earlier (force, rhs) pairs are orthonormalised in rhs
+// space (Gram-Schmidt, two passes) with the same combinations applied to the
+// forces, and the start is sum_j <Q_j, b> X_j.
+constexpr int kHist = 8;
+constexpr double kHistDrop = 1e-12;  // relative norm below which a column is dependent
+struct VecSet {
+  const double* v[kHist + 1];
+  int k;
+};
+// part[j*kDotBlocks + b] = block b's partial of <A.v[j], q>, j < A.k (<= kHist+1)
+hipError_t launch_mdot(const VecSet& A, const double* q, size_t n, double* part, hipStream_t s);
+// coef[j] = fixed-order sum of the partials of slot j, j < k
+hipError_t launch_mreduce(const double* part, int k, double* coef, hipStream_t s);
+// qa = (set ? 0 : qa) + sgn*sum_j coef[j] A.v[j];  xb likewise with B.v[j]
+// (either output may be null; B.k must equal A.k when xb is given)
+hipError_t launch_combine(const VecSet& A, const VecSet& B, const double* coef, double sgn, bool set,
+                          double* qa, double* xb, size_t n, hipStream_t s);
+// q, x /= sqrt(*nrm2), or q = x = 0 when *nrm2 <= kHistDrop^2 * *raw2
+hipError_t launch_hscale(double* q, double* x, const double* nrm2, const double* raw2, size_t n,
+                         hipStream_t s);
 // max |E u - E1nH*dudn - ub| as double bits into *red (zeroed by the caller).
 hipError_t launch_max_slip(const DevGrid& g, const Markers& m, const double* U, const double* V,
                            const double* W, const double* E1nH, const double* fb,

@@ -79,9 +79,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   constexpr int NPL = (N + kBlock - 1) / kBlock;  // x positions per lane
   constexpr int RB = TK < 4 ? TK : 4;             // rows per load batch
   const double ry = g.rdyf[j];
+  double wprev[NPL];  // W(k-1) of the next row: the W row of the row before
+  if constexpr (FROM_UVW) {
+    const double* W0 = W + fidx(1, j, k0 + 1, nxg, nyg);
+#pragma unroll
+    for (int p = 0; p < NPL; p++) {
+      const int n = tid + p * kBlock;
+      wprev[p] = (N % kBlock == 0 || n < N) ? W0[n + 1] : 0.0;
+    }
+  }
 #pragma unroll
   for (int t0 = 0; t0 < TK; t0 += RB) {
-    double v[RB][NPL][6];
+    double v[RB][NPL][FROM_UVW ? 5 : 1];
 #pragma unroll
     for (int r = 0; r < RB; r++) {
       const int k = k0 + t0 + r + 2;
@@ -90,7 +99,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
         const double* Vr = V + fidx(1, j, k, nxg, ny);
         const double* Wr = W + fidx(1, j, k, nxg, nyg);
         const double* Vm = Vr - nxg;
-        const double* Wm = Wr - (size_t)nxg * nyg;
 #pragma unroll
         for (int p = 0; p < NPL; p++) {
           const int n = tid + p * kBlock;
@@ -101,7 +109,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
             v[r][p][2] = Vr[n + 1];
             v[r][p][3] = Vm[n + 1];
             v[r][p][4] = Wr[n + 1];
-            v[r][p][5] = Wm[n + 1];
           }
         }
       } else if constexpr (SRC == kSrcRhs32) {
@@ -134,8 +141,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
               // divergence, src/differential_operators.f90:64-72, with the
               // face-spacing reciprocals tabulated
               const double rz = g.rdzf[k0 + t0 + r + h + 2];
+              const double wm = (r + h == 0) ? wprev[p] : v[r + h - 1][p][4];
               d[h] = (v[r + h][p][0] - v[r + h][p][1]) * rx +
-                     (v[r + h][p][2] - v[r + h][p][3]) * ry + (v[r + h][p][4] - v[r + h][p][5]) * rz;
+                     (v[r + h][p][2] - v[r + h][p][3]) * ry + (v[r + h][p][4] - wm) * rz;
             } else {
               d[h] = v[r + h][p][0];
             }
@@ -143,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
           }
           buf[((t0 + r) >> 1) * LP + lp(n)] = cmk((real)d[0], (real)d[1]);
         }
+        if constexpr (FROM_UVW) wprev[p] = v[RB - 1][p][4];
       }
     }
   }

@@ -100,19 +100,23 @@ def train(ch, X, T, epochs=40, batch=256, lr=1e-3, seed=1):
     return hist, isc, osc
 
 
-def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000):
+def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000, learned=True):
     """At every IB projection of `steps` RK3 steps, solve from the zero guess,
-    from the previous force (the reference's warm start) and from the
-    learned guess, all from the same state; continue with the zero-guess
-    solution.  Returns per-solve iteration counts and force differences."""
-    rec = {"zero": [], "previous": [], "learned": [], "max_force_diff_learned_vs_zero": [],
-           "max_force_diff_previous_vs_zero": []}
+    from the previous force (the reference's warm start), from the learned
+    guess (when `learned`) and from the least-squares history start
+    (guess_mode 4, fed by its own solves of the same states), all from the
+    same state; continue with the zero-guess solution.  Returns per-solve
+    iteration counts and force differences."""
+    names = [("previous", 0)] + ([("learned", 3)] if learned else []) + [("history", 4), ("zero", 1)]
+    rec = {name: [] for name, _ in names}
+    for name, _ in names[:-1]:
+        rec[f"max_force_diff_{name}_vs_zero"] = []
 
     def variants(c, st):
         snap = c.get_fields()
         fb0 = c.get_IB_force()
         out = {}
-        for name, mode in (("previous", 0), ("learned", 3), ("zero", 1)):
+        for name, mode in names:
             c.set_fields(snap["U"], snap["V"], snap["W"], P=snap["P"])
             c.set_IB_force(fb0)
             c.set_IB_solver(tol=tol, maxit=maxit, guess_mode=mode)
@@ -120,8 +124,9 @@ def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000):
             rec[name].append(it)
             out[name] = c.get_IB_force()
         scale = max(np.max(np.abs(out["zero"])), 1e-300)
-        rec["max_force_diff_learned_vs_zero"].append(float(np.max(np.abs(out["learned"] - out["zero"])) / scale))
-        rec["max_force_diff_previous_vs_zero"].append(float(np.max(np.abs(out["previous"] - out["zero"])) / scale))
+        for name, _ in names[:-1]:
+            rec[f"max_force_diff_{name}_vs_zero"].append(
+                float(np.max(np.abs(out[name] - out["zero"])) / scale))
         return True  # the zero-guess solve ran last: the state continues from it
 
     for _ in range(steps):

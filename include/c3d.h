@@ -219,7 +219,11 @@ int c3d_ib_bicgstab(c3d_ctx *ctx, double *x, const double *b, double tol, int ma
  * cg_max_iter, and the initial guess: guess_mode 0 = the previous force
  * fb (the reference's warm start), 1 = zero, 2 = the force last set by
  * c3d_ib_set_force for the next solve only, then 0, 3 = the previous force
- * plus the learned increment (c3d_mlp_*). */
+ * plus the learned increment (c3d_mlp_*), 4 = the least-squares combination
+ * of the last 8 converged forces of the same dt, fitted through their
+ * right-hand sides (the Schur operator depends on dt only; the previous
+ * force while none is stored).  Mode 4 is an addition with no counterpart in
+ * the reference; BiCGSTAB starts from the true residual in every mode. */
 int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
 /* Resident IB force fb = (dudn_jump[3nb], p_jump[nb]) [4nb]. */
 int c3d_ib_set_force(c3d_ctx *ctx, const double *fb);

@@ -20,7 +20,7 @@ Bars (stated per check):
 import numpy as np
 import pytest
 
-from channel3d_parallel_amd import Channel, fields, ib_geometry
+from channel3d_parallel_amd import Channel, fields, ib_geometry, learned_guess
 from oracle import orc
 
 pytestmark = pytest.mark.gpu
@@ -223,3 +223,42 @@ def test_config4_projection_golden():
     for k, name in enumerate(("dudn_x", "dudn_y", "dudn_z", "p_jump")):
         got = np.linalg.norm(fb[k * nb:(k + 1) * nb])
         assert abs(got - ref["fb_" + name + "_l2"]) <= 1e-7 * max(ref["fb_" + name + "_l2"], 1e-3 * scale), name
+
+
+def test_history_start():
+    """guess_mode 4, the least-squares start over earlier solves of the same
+    dt: re-solving an unchanged system starts at its converged force (one
+    iteration, the minimum: the solve loop, as the reference's, tests the
+    residual after an iteration), and over RK3 steps its forces equal the
+    zero-guess forces within the solver tolerance and it needs no more
+    iterations in total than the previous-force start."""
+    print("\n[case] history start, 34x33x34, 24x16 markers per cylinder", flush=True)
+    nx, ny, nz, nth, nzb, dt, tol = 34, 33, 34, 24, 16, 0.004, 1e-10
+    s1, s2 = learned_guess.speeds(99, 0)
+    ch, m = learned_guess.make_channel(nx, ny, nz, nth, nzb, s1, s2)
+    repeat = {}
+
+    def twice(c, st):
+        snap = c.get_fields()
+        f0 = c.get_IB_force()
+        c.set_IB_solver(tol=tol, maxit=2000, guess_mode=4)
+        repeat["first"], _ = c.compute_IB_projection(dt, st)
+        f1 = c.get_IB_force()
+        c.set_fields(snap["U"], snap["V"], snap["W"], P=snap["P"])
+        c.set_IB_force(f0)
+        repeat["second"], repeat["err"] = c.compute_IB_projection(dt, st)
+        repeat["diff"] = float(np.max(np.abs(c.get_IB_force() - f1)) / np.max(np.abs(f1)))
+        return True
+
+    with ch:
+        learned_guess.stage(ch, 1, dt, before_ib=twice)
+        rec = learned_guess.ab_compare(ch, dt, steps=3, learned=False)
+    print(f"  repeat: {repeat}")
+    print(f"  iterations zero {rec['zero']} previous {rec['previous']} history {rec['history']}")
+    print(f"  max |force diff| / max |force| vs zero: history "
+          f"{max(rec['max_force_diff_history_vs_zero']):.2e} previous "
+          f"{max(rec['max_force_diff_previous_vs_zero']):.2e}")
+    assert repeat["first"] > 5 and repeat["second"] == 1 and repeat["err"] < tol ** 2
+    assert repeat["diff"] < 1e-11
+    assert max(rec["max_force_diff_history_vs_zero"]) < 1e-6
+    assert sum(rec["history"]) <= sum(rec["previous"])
