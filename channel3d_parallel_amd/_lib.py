@@ -71,6 +71,7 @@ SIGNATURES = [
     ("c3d_ib_bicgstab", C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_int,
                                   _ip, _dp]),
     ("c3d_ib_set_solver", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int]),
+    ("c3d_ib_set_graphs", C.c_int, [C.c_void_p, C.c_int]),
     ("c3d_ib_set_force", C.c_int, [C.c_void_p, _dp]),
     ("c3d_ib_get_force", C.c_int, [C.c_void_p, _dp]),
     ("c3d_compute_ib_projection", C.c_int, [C.c_void_p, C.c_double, C.c_int, _ip, _dp]),

@@ -245,6 +245,11 @@ class Channel:
     def set_IB_solver(self, tol=1e-10, maxit=1000, guess_mode=0):
         self._chk(self.L.c3d_ib_set_solver(self.h, tol, maxit, guess_mode))
 
+    def set_IB_graphs(self, on=True):
+        """Replay each BiCGSTAB iteration from a captured HIP graph (default)
+        or launch it kernel by kernel; the results are identical."""
+        self._chk(self.L.c3d_ib_set_graphs(self.h, 1 if on else 0))
+
     def set_IB_force(self, fb):
         self._chk(self.L.c3d_ib_set_force(self.h, ptr(np.ascontiguousarray(fb, dtype=np.float64))))
 

@@ -144,11 +144,12 @@ int c3d_add_to_u(c3d_ctx *ctx, double d);
  * which also updates dPdx. */
 int c3d_time_step_rk3(c3d_ctx *ctx, double dt, int x_mass_cte, double q0);
 
-/* Timing of `iters` back-to-back resident projections with HIP events on
- * the context stream.  total_ms: average ms per projection;
- * kernel_ms[0..nk): average ms per kernel stage (order from
- * c3d_stage_name), nk <= c3d_stage_count().  No reference counterpart:
- * measurement hook for the hot path. */
+/* Timing of resident projections with HIP events on the context stream:
+ * a pass of `iters` back-to-back projections with an event between kernel
+ * stages gives kernel_ms[0..nk), the average ms per stage (order from
+ * c3d_stage_name, nk <= c3d_stage_count()); a second pass of `iters`
+ * projections between two events only gives total_ms, the average ms per
+ * projection.  No reference counterpart: measurement hook for the hot path. */
 int c3d_time_projection(c3d_ctx *ctx, int iters, double dt, double *total_ms, double *kernel_ms,
                         int nk);
 int c3d_stage_count(void);
@@ -225,6 +226,12 @@ int c3d_ib_bicgstab(c3d_ctx *ctx, double *x, const double *b, double tol, int ma
  * force while none is stored).  Mode 4 is an addition with no counterpart in
  * the reference; BiCGSTAB starts from the true residual in every mode. */
 int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
+/* on != 0 (the default): each BiCGSTAB iteration of the IB solves is
+ * replayed from a HIP graph captured once per (solution vector, dt, rk_step
+ * == 1); 0: launched kernel by kernel.  Both run the same kernels in the same
+ * order and give identical results.  No reference counterpart (launch
+ * mechanics of src/projection.f90:160-182). */
+int c3d_ib_set_graphs(c3d_ctx *ctx, int on);
 /* Resident IB force fb = (dudn_jump[3nb], p_jump[nb]) [4nb]. */
 int c3d_ib_set_force(c3d_ctx *ctx, const double *fb);
 int c3d_ib_get_force(c3d_ctx *ctx, double *fb);
