@@ -913,9 +913,7 @@ static int ib_spread_force(c3d_ctx* c, const double* f, double dt) {
   IBState* ib = c->ib;
   const int nb = ib->m.nb;
   HIPCHK(c, c3d::ib::launch_aux(ib->m, f, dt, c->desc.nu, ib->aux, c->stream));
-  double* F[3] = {ib->Ureg, ib->Vreg, ib->Wreg};
-  for (int q = 0; q < 3; q++)
-    HIPCHK(c, c3d::ib::launch_spread(ib->m, ib->sp[q], q, ib->aux + (size_t)q * nb, F[q], c->stream));
+  HIPCHK(c, c3d::ib::launch_spread3(ib->m, ib->sp, ib->aux, ib->Ureg, ib->Vreg, ib->Wreg, c->stream));
   HIPCHK(c, c3d::launch_apply_bc(c->g, ib->Ureg, ib->Vreg, ib->Wreg, c->stream));
   return 0;
 }
@@ -927,10 +925,9 @@ static int ib_schur_launch(c3d_ctx* c, const double* f, double* Sf, double dt, i
   int rc = ib_spread_force(c, f, dt);
   if (!rc) rc = ib_poisson_reg(c, dt, rk_step);
   if (rc) return rc;
-  HIPCHK(c, c3d::ib::launch_interp_c(c->g, ib->m, ib->phi, -1.0, Sf + 3 * nb, c->stream));
+  HIPCHK(c, c3d::ib::launch_schur_rows(c->g, ib->m, ib->phi, dt, ib->Ureg, ib->Vreg, ib->Wreg,
+                                       ib->E1nH, f, Sf, c->stream));
   HIPCHK(c, c3d::ib::launch_remove_mean(ib->m, Sf + 3 * nb, ib->E1nHc, f + 3 * nb, ib->mpart, c->stream));
-  HIPCHK(c, c3d::ib::launch_schur_u(c->g, ib->m, ib->phi, dt, ib->Ureg, ib->Vreg, ib->Wreg,
-                                    ib->E1nH, f, Sf, c->stream));
   return 0;
 }
 
@@ -943,17 +940,14 @@ static int ib_bicg_iteration(c3d_ctx* c, double* x, double dt, int rk_step) {
   double *r = ib->vec[1], *rhat = ib->vec[2], *p = ib->vec[3], *v = ib->vec[4], *h = ib->vec[5],
          *sv = ib->vec[6], *t = ib->vec[7];
   int rc = 0;
-  HIPCHK(c, B::launch_dot(rhat, r, nullptr, nullptr, n, ib->part, 0, c->stream));
-  HIPCHK(c, B::launch_bicg_scalar(1, ib->sc, ib->part, nullptr, c->stream));
-  HIPCHK(c, B::launch_bicg_p(r, v, ib->sc, p, n, c->stream));
+  // partial slot 0 holds rhat.r on entry (the initial dot, then k_bicg_xr)
+  HIPCHK(c, B::launch_bicg_p(r, v, ib->sc, ib->part, p, n, c->stream));
   if ((rc = ib_schur_launch(c, p, v, dt, rk_step))) return rc;
   HIPCHK(c, B::launch_dot(rhat, v, nullptr, nullptr, n, ib->part, 1, c->stream));
-  HIPCHK(c, B::launch_bicg_scalar(2, ib->sc, ib->part, nullptr, c->stream));
-  HIPCHK(c, B::launch_bicg_hs(x, p, r, v, ib->sc, h, sv, n, c->stream));
+  HIPCHK(c, B::launch_bicg_hs(x, p, r, v, ib->sc, ib->part, h, sv, n, c->stream));
   if ((rc = ib_schur_launch(c, sv, t, dt, rk_step))) return rc;
   HIPCHK(c, B::launch_dot(t, sv, t, t, n, ib->part, 2, c->stream));
-  HIPCHK(c, B::launch_bicg_scalar(3, ib->sc, ib->part, nullptr, c->stream));
-  HIPCHK(c, B::launch_bicg_xr(h, sv, t, ib->sc, x, r, ib->part, n, c->stream));
+  HIPCHK(c, B::launch_bicg_xr(h, sv, t, rhat, ib->sc, ib->part, x, r, n, c->stream));
   HIPCHK(c, B::launch_bicg_scalar(4, ib->sc, ib->part, nullptr, c->stream));
   HIPCHK(c, hipMemcpyAsync(ib->herr, ib->sc + 5, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return 0;
@@ -1007,6 +1001,7 @@ static int ib_bicgstab_launch(c3d_ctx* c, double* x, const double* b, double tol
   if (!r_ready && (rc = ib_schur_launch(c, x, r, dt, rk_step))) return rc;
   HIPCHK(c, B::launch_bicg_init(r_ready ? nullptr : b, r, rhat, p, v, n, c->stream));
   HIPCHK(c, B::launch_bicg_scalar(0, ib->sc, ib->part, nullptr, c->stream));
+  HIPCHK(c, B::launch_dot(rhat, r, nullptr, nullptr, n, ib->part, 0, c->stream));
   hipGraphExec_t exec = nullptr;
   if (ib->use_graphs && maxit > 0 && (rc = ib_iteration_graph(c, x, dt, rk_step, &exec))) return rc;
   double error = 1.0;

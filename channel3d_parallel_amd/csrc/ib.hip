@@ -258,12 +258,10 @@ __global__ __launch_bounds__(kBlock) void k_interp_uvw(DevGrid g, Markers m, con
     for (int c = 0; c < 3; c++) out[c * m.nb + l] = r[c];
 }
 
-// E1n of a centre field in rhs_p layout (:503-526).
-__global__ __launch_bounds__(kBlock) void k_interp_c(DevGrid g, Markers m, const double* __restrict__ P,
-                                                     double sign, double* __restrict__ out) {
-  const int l = wave_marker(m.nb);
-  const int lane = threadIdx.x & 63;
-  if (l >= m.nb) return;
+// E1n of a centre field in rhs_p layout (:503-526), marker l on one wave.
+__device__ __forceinline__ void interp_c_marker(const DevGrid& g, const Markers& m, int l, int lane,
+                                                const double* __restrict__ P, double sign,
+                                                double* __restrict__ out) {
   const LaneStencil ls = load_lane_stencil<true>(m, l, lane, 3, 1);
   double acc = 0.0;
 #pragma unroll
@@ -276,20 +274,25 @@ __global__ __launch_bounds__(kBlock) void k_interp_c(DevGrid g, Markers m, const
   if (lane == 0) out[l] = sign * acc;
 }
 
+__global__ __launch_bounds__(kBlock) void k_interp_c(DevGrid g, Markers m, const double* __restrict__ P,
+                                                     double sign, double* __restrict__ out) {
+  const int l = wave_marker(m.nb);
+  if (l >= m.nb) return;
+  interp_c_marker(g, m, l, threadIdx.x & 63, P, sign, out);
+}
+
 // Schur velocity rows (src/projection.f90:131-142) with the gradient of
 // dt*phi evaluated at the stencil nodes; every node lies in the gradient's
 // index ranges (checked at set-up), where apply_boundary_conditions does not
 // write.
-__global__ __launch_bounds__(kBlock) void k_schur_u(DevGrid g, Markers m, const double* __restrict__ phi,
-                                                    double dt, const double* __restrict__ Ureg,
-                                                    const double* __restrict__ Vreg,
-                                                    const double* __restrict__ Wreg,
-                                                    const double* __restrict__ E1nH,
-                                                    const double* __restrict__ dudn,
-                                                    double* __restrict__ Sf) {
-  const int l = wave_marker(m.nb);
-  const int lane = threadIdx.x & 63;
-  if (l >= m.nb) return;
+__device__ __forceinline__ void schur_u_marker(const DevGrid& g, const Markers& m, int l, int lane,
+                                               const double* __restrict__ phi, double dt,
+                                               const double* __restrict__ Ureg,
+                                               const double* __restrict__ Vreg,
+                                               const double* __restrict__ Wreg,
+                                               const double* __restrict__ E1nH,
+                                               const double* __restrict__ dudn,
+                                               double* __restrict__ Sf) {
   const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
   double r[3];
   gather3<false>(m, l, lane, [&](int c, const Pt& q) {
@@ -310,6 +313,26 @@ __global__ __launch_bounds__(kBlock) void k_schur_u(DevGrid g, Markers m, const 
   if (lane == 0)
     for (int c = 0; c < 3; c++)
       Sf[c * m.nb + l] = r[c] - E1nH[c * m.nb + l] * dudn[c * m.nb + l];
+}
+
+// Both row groups of the Schur product in one launch: block row 0 the
+// velocity rows (schur_u_marker), row 1 the pressure row -E1n(phi) into
+// Sf[3nb..4nb) (as k_interp_c with sign -1).
+__global__ __launch_bounds__(kBlock) void k_schur_rows(DevGrid g, Markers m, const double* __restrict__ phi,
+                                                       double dt, const double* __restrict__ Ureg,
+                                                       const double* __restrict__ Vreg,
+                                                       const double* __restrict__ Wreg,
+                                                       const double* __restrict__ E1nH,
+                                                       const double* __restrict__ dudn,
+                                                       double* __restrict__ Sf) {
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int l = (int)bx * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  if (l >= m.nb) return;
+  if (by == 0)
+    schur_u_marker(g, m, l, threadIdx.x & 63, phi, dt, Ureg, Vreg, Wreg, E1nH, dudn, Sf);
+  else
+    interp_c_marker(g, m, l, threadIdx.x & 63, phi, -1.0, Sf + 3 * (size_t)m.nb);
 }
 
 // -E u + ub (src/projection.f90:51-52).
@@ -356,10 +379,8 @@ __global__ void k_spread_coeffs(int nb, double A, double factor, const double* _
 
 // R, src/immersed_boundary_operators.f90:544-603: one thread per support node,
 // contributions in ascending marker order.
-__global__ __launch_bounds__(kBlock) void k_spread(Spread sp, const double* __restrict__ f,
-                                                   double* __restrict__ F) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= sp.nnodes) return;
+__device__ __forceinline__ void spread_node(const Spread& sp, int t, const double* __restrict__ f,
+                                            double* __restrict__ F) {
   double acc = 0.0;
   const int n = sp.cnt[t];
   // loads of several entries in flight; the sum stays in entry order
@@ -369,6 +390,26 @@ __global__ __launch_bounds__(kBlock) void k_spread(Spread sp, const double* __re
     acc = acc + sp.wfs[e] * f[sp.ml[e]];
   }
   F[sp.node[t]] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_spread(Spread sp, const double* __restrict__ f,
+                                                   double* __restrict__ F) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= sp.nnodes) return;
+  spread_node(sp, t, f, F);
+}
+
+// The three components in one launch, block row c = component c, reading
+// f_c = f + c*nb.
+__global__ __launch_bounds__(kBlock) void k_spread3(Spread s0, Spread s1, Spread s2,
+                                                    const double* __restrict__ f, int nb,
+                                                    double* __restrict__ F0, double* __restrict__ F1,
+                                                    double* __restrict__ F2) {
+  const int c = blockIdx.y;
+  const Spread sp = c == 0 ? s0 : (c == 1 ? s1 : s2);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= sp.nnodes) return;
+  spread_node(sp, t, f + (size_t)c * nb, c == 0 ? F0 : (c == 1 ? F1 : F2));
 }
 
 // remove_mean_per_body for the two-group slab layout
@@ -521,7 +562,10 @@ __device__ __forceinline__ double sum_part(const double* part, int slot) {
   return r;
 }
 
-// Scalar slots: 0 rho_o, 1 alpha, 2 om, 3 rho_n, 4 bta, 5 err.
+// Scalar slots: 0 rho_o, 1 alpha, 2 om, 3 rho_n, 4 bta, 5 err, 6 line-search
+// step.  The vector kernels below compute their scalar from the dot partials
+// themselves (thread 0 of every block, the same fixed-order sum) and block 0
+// stores it; no block of a kernel reads a slot that the same kernel writes.
 __global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ part,
                               double* host_err) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -530,19 +574,6 @@ __global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ p
       sc[0] = 1.0;
       sc[1] = 1.0;
       sc[2] = 1.0;
-      break;
-    case 1: {  // rho_n = rhat.r ; bta = (rho_n/rho_o)*(alpha/om) ; rho_o = rho_n
-      const double rho_n = sum_part(part, 0);
-      sc[4] = (rho_n / sc[0]) * (sc[1] / sc[2]);
-      sc[3] = rho_n;
-      sc[0] = rho_n;
-      break;
-    }
-    case 2:  // alpha = rho_n / rhat.nu
-      sc[1] = sc[3] / sum_part(part, 1);
-      break;
-    case 3:  // om = tv.sv / tv.tv
-      sc[2] = sum_part(part, 2) / sum_part(part, 3);
       break;
     case 4: {
       const double e = sum_part(part, 4);
@@ -558,42 +589,86 @@ __global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ p
   }
 }
 
+// rho_n = rhat.r (partial slot 0); bta = (rho_n/rho_o)*(alpha/om);
+// p = r + bta*(p - om*v).  Writes sc[3] = rho_n, sc[4] = bta.
 __global__ void k_bicg_p(const double* __restrict__ r, const double* __restrict__ v,
-                         const double* __restrict__ sc, double* __restrict__ p, size_t n) {
+                         double* __restrict__ sc, const double* __restrict__ part,
+                         double* __restrict__ p, size_t n) {
+  __shared__ double bsh;
+  if (threadIdx.x == 0) {
+    const double rho_n = sum_part(part, 0);
+    const double bta = (rho_n / sc[0]) * (sc[1] / sc[2]);
+    bsh = bta;
+    if (blockIdx.x == 0) {
+      sc[3] = rho_n;
+      sc[4] = bta;
+    }
+  }
+  __syncthreads();
+  const double bta = bsh, om = sc[2];
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double bta = sc[4], om = sc[2];
   p[t] = r[t] + bta * (p[t] - om * v[t]);
 }
 
+// alpha = rho_n / rhat.v (partial slot 1); h = x + alpha*p; s = r - alpha*v.
+// Writes sc[1] = alpha and sc[0] = rho_n (rho_o of the next iteration).
 __global__ void k_bicg_hs(const double* __restrict__ x, const double* __restrict__ p,
                           const double* __restrict__ r, const double* __restrict__ v,
-                          const double* __restrict__ sc, double* __restrict__ h,
-                          double* __restrict__ sv, size_t n) {
+                          double* __restrict__ sc, const double* __restrict__ part,
+                          double* __restrict__ h, double* __restrict__ sv, size_t n) {
+  __shared__ double ash;
+  if (threadIdx.x == 0) {
+    const double rho_n = sc[3];
+    const double alpha = rho_n / sum_part(part, 1);
+    ash = alpha;
+    if (blockIdx.x == 0) {
+      sc[1] = alpha;
+      sc[0] = rho_n;
+    }
+  }
+  __syncthreads();
+  const double alpha = ash;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double alpha = sc[1];
   h[t] = x[t] + alpha * p[t];
   sv[t] = r[t] - alpha * v[t];
 }
 
+// om = t.s / t.t (partial slots 2, 3); x = h + om*s; r = s - om*t; partials
+// of r.r into slot 4 and of rhat.r into slot 0 (the next iteration's rho_n).
+// Writes sc[2] = om.
 __global__ __launch_bounds__(kBlock) void k_bicg_xr(const double* __restrict__ h,
                                                     const double* __restrict__ sv,
                                                     const double* __restrict__ tv,
-                                                    const double* __restrict__ sc,
+                                                    const double* __restrict__ rhat,
+                                                    double* __restrict__ sc,
+                                                    double* __restrict__ part,
                                                     double* __restrict__ x, double* __restrict__ r,
-                                                    double* __restrict__ part, size_t n) {
-  __shared__ double sh[kBlock / 64];
-  const double om = sc[2];
-  double s = 0.0;
+                                                    size_t n) {
+  __shared__ double sh[2][kBlock / 64];
+  __shared__ double osh;
+  if (threadIdx.x == 0) {
+    const double om = sum_part(part, 2) / sum_part(part, 3);
+    osh = om;
+    if (blockIdx.x == 0) sc[2] = om;
+  }
+  __syncthreads();
+  const double om = osh;
+  double s4 = 0.0, s0 = 0.0;
   for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (size_t)kDotBlocks * kBlock) {
     x[t] = h[t] + om * sv[t];
     const double rr = sv[t] - om * tv[t];
     r[t] = rr;
-    s += rr * rr;
+    s4 += rr * rr;
+    s0 += rhat[t] * rr;
   }
-  const double tot = block_sum(s, sh);
-  if (threadIdx.x == 0) part[4 * kDotBlocks + blockIdx.x] = tot;
+  const double t4 = block_sum(s4, sh[0]);
+  const double t0 = block_sum(s0, sh[1]);
+  if (threadIdx.x == 0) {
+    part[4 * kDotBlocks + blockIdx.x] = t4;
+    part[0 * kDotBlocks + blockIdx.x] = t0;
+  }
 }
 
 __global__ void k_bicg_init(const double* __restrict__ b, double* __restrict__ r,
@@ -728,11 +803,13 @@ hipError_t launch_interp_c(const DevGrid& g, const Markers& m, const double* P, 
   return hipGetLastError();
 }
 
-hipError_t launch_schur_u(const DevGrid& g, const Markers& m, const double* phi, double dt,
-                          const double* Ureg, const double* Vreg, const double* Wreg,
-                          const double* E1nH, const double* dudn, double* Sf, hipStream_t s) {
-  hipLaunchKernelGGL(k_schur_u, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, phi, dt, Ureg, Vreg,
-                     Wreg, E1nH, dudn, Sf);
+hipError_t launch_schur_rows(const DevGrid& g, const Markers& m, const double* phi, double dt,
+                             const double* Ureg, const double* Vreg, const double* Wreg,
+                             const double* E1nH, const double* dudn, double* Sf, hipStream_t s) {
+  dim3 grid = wave_blocks(m.nb);
+  grid.y = 2;
+  hipLaunchKernelGGL(k_schur_rows, grid, dim3(kBlock), 0, s, g, m, phi, dt, Ureg, Vreg, Wreg, E1nH,
+                     dudn, Sf);
   return hipGetLastError();
 }
 
@@ -745,6 +822,17 @@ hipError_t launch_rhs_u(const DevGrid& g, const Markers& m, const double* U, con
 hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, double* aux,
                       hipStream_t s) {
   hipLaunchKernelGGL(k_aux, blocks_for((size_t)3 * m.nb), dim3(kBlock), 0, s, m, f, dt, nu, aux);
+  return hipGetLastError();
+}
+
+hipError_t launch_spread3(const Markers& m, const Spread sp[3], const double* f, double* F0,
+                          double* F1, double* F2, hipStream_t s) {
+  int mx = 0;
+  for (int c = 0; c < 3; c++) mx = sp[c].nnodes > mx ? sp[c].nnodes : mx;
+  if (mx == 0) return hipSuccess;
+  dim3 grid = blocks_for((size_t)mx);
+  grid.y = 3;
+  hipLaunchKernelGGL(k_spread3, grid, dim3(kBlock), 0, s, sp[0], sp[1], sp[2], f, m.nb, F0, F1, F2);
   return hipGetLastError();
 }
 
@@ -805,21 +893,23 @@ hipError_t launch_bicg_scalar(int step, double* sc, const double* part, double* 
   return hipGetLastError();
 }
 
-hipError_t launch_bicg_p(const double* r, const double* v, const double* sc, double* p, size_t n,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_bicg_p, blocks_for(n), dim3(kBlock), 0, s, r, v, sc, p, n);
+hipError_t launch_bicg_p(const double* r, const double* v, double* sc, const double* part, double* p,
+                         size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_p, blocks_for(n), dim3(kBlock), 0, s, r, v, sc, part, p, n);
   return hipGetLastError();
 }
 
 hipError_t launch_bicg_hs(const double* x, const double* p, const double* r, const double* v,
-                          const double* sc, double* h, double* sv, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bicg_hs, blocks_for(n), dim3(kBlock), 0, s, x, p, r, v, sc, h, sv, n);
+                          double* sc, const double* part, double* h, double* sv, size_t n,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_hs, blocks_for(n), dim3(kBlock), 0, s, x, p, r, v, sc, part, h, sv, n);
   return hipGetLastError();
 }
 
-hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* sc,
-                          double* x, double* r, double* part, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bicg_xr, dim3(kDotBlocks), dim3(kBlock), 0, s, h, sv, t, sc, x, r, part, n);
+hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* rhat,
+                          double* sc, double* part, double* x, double* r, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3(kDotBlocks), dim3(kBlock), 0, s, h, sv, t, rhat, sc, part, x, r,
+                     n);
   return hipGetLastError();
 }
 

@@ -62,11 +62,13 @@ hipError_t launch_interp_uvw(const DevGrid& g, const Markers& m, const double* U
 // E1n of a centre field in rhs_p layout -> out[nb], times sign.
 hipError_t launch_interp_c(const DevGrid& g, const Markers& m, const double* P, double sign,
                            double* out, hipStream_t s);
-// Schur velocity rows: Sf[c*nb+l] = E_c(G(dt*phi) - Reg_c)[l] - E1nH[c*nb+l]*dudn[c*nb+l],
-// G the gradient of the rhs_p-shaped phi evaluated at the stencil nodes.
-hipError_t launch_schur_u(const DevGrid& g, const Markers& m, const double* phi, double dt,
-                          const double* Ureg, const double* Vreg, const double* Wreg,
-                          const double* E1nH, const double* dudn, double* Sf, hipStream_t s);
+// Schur rows: velocity rows Sf[c*nb+l] = E_c(G(dt*phi) - Reg_c)[l] -
+// E1nH[c*nb+l]*dudn[c*nb+l], G the gradient of the rhs_p-shaped phi evaluated
+// at the stencil nodes, and the pressure row Sf[3nb+l] = -E1n(phi)[l], in
+// one launch.
+hipError_t launch_schur_rows(const DevGrid& g, const Markers& m, const double* phi, double dt,
+                             const double* Ureg, const double* Vreg, const double* Wreg,
+                             const double* E1nH, const double* dudn, double* Sf, hipStream_t s);
 // Velocity-rhs rows: out[c*nb+l] = -E_c(U)[l] + ub[c*nb+l].
 hipError_t launch_rhs_u(const DevGrid& g, const Markers& m, const double* U, const double* V,
                         const double* W, double* out, hipStream_t s);
@@ -77,6 +79,9 @@ hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, d
 // nodes only (other nodes keep the zeros they were created with).
 hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const double* f,
                          double* F, hipStream_t s);
+// The three components at once: F_c from f + c*nb with map sp[c].
+hipError_t launch_spread3(const Markers& m, const Spread sp[3], const double* f, double* F0,
+                          double* F1, double* F2, hipStream_t s);
 // Fills sp.ml and sp.wfs of component comp from the stencils (once per body).
 hipError_t launch_spread_coeffs(const Markers& m, const Spread& sp, int comp, hipStream_t s);
 // f[nb] -= per-body mean; then f -= coef*pj when coef != nullptr.  part
@@ -105,20 +110,23 @@ constexpr int kDotBlocks = 128;
 // the partial arrays); two dots at once when a2 != nullptr (slot + 1).
 hipError_t launch_dot(const double* a, const double* b, const double* a2, const double* b2,
                       size_t n, double* part, int slot, hipStream_t s);
-// Scalar steps: 0 initialise, 1 rho_n/bta, 2 alpha, 3 om, 4 err (also to
-// *host_err when not null), 5 minimal-residual step sc[6] = <r,Sd>/<Sd,Sd>
-// from partial slots 2 and 3.
+// Scalar steps: 0 initialise, 4 err (also to *host_err when not null), 5
+// minimal-residual step sc[6] = <r,Sd>/<Sd,Sd> from partial slots 2 and 3.
 hipError_t launch_bicg_scalar(int step, double* sc, const double* part, double* host_err,
                               hipStream_t s);
+// rho_n = rhat.r (slot 0), bta = (rho_n/rho_o)*(alpha/om) -> sc[3], sc[4];
 // p = r + bta*(p - om*v)
-hipError_t launch_bicg_p(const double* r, const double* v, const double* sc, double* p, size_t n,
-                         hipStream_t s);
+hipError_t launch_bicg_p(const double* r, const double* v, double* sc, const double* part, double* p,
+                         size_t n, hipStream_t s);
+// alpha = rho_n / rhat.v (slot 1) -> sc[1], rho_o = rho_n -> sc[0];
 // h = x + alpha*p;  s = r - alpha*v
 hipError_t launch_bicg_hs(const double* x, const double* p, const double* r, const double* v,
-                          const double* sc, double* h, double* sv, size_t n, hipStream_t s);
-// x = h + om*s;  r = s - om*t; partials of r.r into part slot 4
-hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* sc,
-                          double* x, double* r, double* part, size_t n, hipStream_t s);
+                          double* sc, const double* part, double* h, double* sv, size_t n,
+                          hipStream_t s);
+// om = t.s / t.t (slots 2, 3) -> sc[2]; x = h + om*s;  r = s - om*t;
+// partials of r.r into slot 4 and of rhat.r into slot 0
+hipError_t launch_bicg_xr(const double* h, const double* sv, const double* t, const double* rhat,
+                          double* sc, double* part, double* x, double* r, size_t n, hipStream_t s);
 // r = b - r (b == null: r unchanged); rhat = r; p = v = 0
 hipError_t launch_bicg_init(const double* b, double* r, double* rhat, double* p, double* v,
                             size_t n, hipStream_t s);
