@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1186,6 +1187,10 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
       return fail(c, C3D_EINVAL, "marker %d outside the periodic box", l);
   IBState* ib = new IBState();
   c->ib = ib;
+  {  // C3D_IB_GRAPHS=0: launch each BiCGSTAB iteration kernel by kernel
+    const char* e = getenv("C3D_IB_GRAPHS");
+    ib->use_graphs = !(e && atoi(e) == 0);
+  }
   auto bail = [&](int rc) {
     c->ib = nullptr;
     delete ib;  // device memory stays in c->allocs until c3d_destroy

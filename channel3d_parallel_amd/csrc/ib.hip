@@ -315,9 +315,10 @@ __device__ __forceinline__ void schur_u_marker(const DevGrid& g, const Markers& 
       Sf[c * m.nb + l] = r[c] - E1nH[c * m.nb + l] * dudn[c * m.nb + l];
 }
 
-// Both row groups of the Schur product in one launch: block row 0 the
-// velocity rows (schur_u_marker), row 1 the pressure row -E1n(phi) into
-// Sf[3nb..4nb) (as k_interp_c with sign -1).
+// Both row groups of the Schur product in one launch, one wave per marker:
+// the velocity rows (schur_u_marker), then the pressure row -E1n(phi) into
+// Sf[3nb..4nb) (as k_interp_c with sign -1), whose nodes the velocity rows
+// have just read.
 __global__ __launch_bounds__(kBlock) void k_schur_rows(DevGrid g, Markers m, const double* __restrict__ phi,
                                                        double dt, const double* __restrict__ Ureg,
                                                        const double* __restrict__ Vreg,
@@ -325,14 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_schur_rows(DevGrid g, Markers m, con
                                                        const double* __restrict__ E1nH,
                                                        const double* __restrict__ dudn,
                                                        double* __restrict__ Sf) {
-  unsigned bx, by, bz;
-  xcd_tile(bx, by, bz);
-  const int l = (int)bx * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  const int l = wave_marker(m.nb);
   if (l >= m.nb) return;
-  if (by == 0)
-    schur_u_marker(g, m, l, threadIdx.x & 63, phi, dt, Ureg, Vreg, Wreg, E1nH, dudn, Sf);
-  else
-    interp_c_marker(g, m, l, threadIdx.x & 63, phi, -1.0, Sf + 3 * (size_t)m.nb);
+  const int lane = threadIdx.x & 63;
+  schur_u_marker(g, m, l, lane, phi, dt, Ureg, Vreg, Wreg, E1nH, dudn, Sf);
+  interp_c_marker(g, m, l, lane, phi, -1.0, Sf + 3 * (size_t)m.nb);
 }
 
 // -E u + ub (src/projection.f90:51-52).
@@ -806,10 +804,8 @@ hipError_t launch_interp_c(const DevGrid& g, const Markers& m, const double* P, 
 hipError_t launch_schur_rows(const DevGrid& g, const Markers& m, const double* phi, double dt,
                              const double* Ureg, const double* Vreg, const double* Wreg,
                              const double* E1nH, const double* dudn, double* Sf, hipStream_t s) {
-  dim3 grid = wave_blocks(m.nb);
-  grid.y = 2;
-  hipLaunchKernelGGL(k_schur_rows, grid, dim3(kBlock), 0, s, g, m, phi, dt, Ureg, Vreg, Wreg, E1nH,
-                     dudn, Sf);
+  hipLaunchKernelGGL(k_schur_rows, wave_blocks(m.nb), dim3(kBlock), 0, s, g, m, phi, dt, Ureg, Vreg,
+                     Wreg, E1nH, dudn, Sf);
   return hipGetLastError();
 }
 

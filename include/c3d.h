@@ -229,8 +229,9 @@ int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
 /* on != 0 (the default): each BiCGSTAB iteration of the IB solves is
  * replayed from a HIP graph captured once per (solution vector, dt, rk_step
  * == 1); 0: launched kernel by kernel.  Both run the same kernels in the same
- * order and give identical results.  No reference counterpart (launch
- * mechanics of src/projection.f90:160-182). */
+ * order and give identical results.  The default at c3d_ib_setup is on
+ * unless the environment sets C3D_IB_GRAPHS=0.  No reference counterpart
+ * (launch mechanics of src/projection.f90:160-182). */
 int c3d_ib_set_graphs(c3d_ctx *ctx, int on);
 /* Resident IB force fb = (dudn_jump[3nb], p_jump[nb]) [4nb]. */
 int c3d_ib_set_force(c3d_ctx *ctx, const double *fb);
