@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 
 from channel3d_parallel_amd import Channel, fields, ib_geometry, learned_guess
+from channel3d_parallel_amd.solver import C3DError
 from oracle import orc
 
 pytestmark = pytest.mark.gpu
@@ -290,3 +291,56 @@ def test_iteration_graph_replays_match_eager():
     for k in ("U", "V", "W", "P"):
         assert np.array_equal(res[False][1][k], res[True][1][k]), k
     assert np.array_equal(res[False][2], res[True][2])
+
+
+def test_history_start_is_least_squares():
+    """The guess-mode-4 start x0 is the least-squares combination of the last
+    8 converged forces: its residual ||b - S x0|| matches numpy's lstsq
+    optimum over the same pairs (S f_i from the library's own Schur product),
+    up to the solver tolerance carried by the stored pairs, and is far below
+    the previous force's residual.  maxit = 0 leaves the start in fb."""
+    print("\n[case] history start vs numpy lstsq, 34x33x34, 24x16 markers per cylinder", flush=True)
+    nx, ny, nz, nth, nzb, dt, tol = 34, 33, 34, 24, 16, 0.004, 1e-10
+    s1, s2 = learned_guess.speeds(99, 0)
+    ch, m = learned_guess.make_channel(nx, ny, nz, nth, nzb, s1, s2, dt_guess_mode=4)
+    forces, out = [], {}
+
+    def record(c, st):
+        c.compute_IB_projection(dt, st)
+        forces.append(c.get_IB_force())
+        return True
+
+    def probe(c, st):
+        snap = c.get_fields()
+        out["fprev"] = c.get_IB_force()
+        c.set_IB_solver(tol=tol, maxit=0, guess_mode=4)
+        with pytest.raises(C3DError) as e:
+            c.compute_IB_projection(dt, st)
+        assert e.value.code == -4  # C3D_ENOCONV: no iteration allowed
+        out["x0"] = c.get_IB_force()
+        c.set_fields(snap["U"], snap["V"], snap["W"], P=snap["P"])
+        c.set_IB_force(out["fprev"])
+        c.set_IB_solver(tol=1e-12, maxit=4000, guess_mode=1)
+        c.compute_IB_projection(dt, st)
+        out["fstar"] = c.get_IB_force()
+        return True
+
+    with ch:
+        for _ in range(3):
+            for st in (1, 2, 3):
+                learned_guess.stage(ch, st, dt, before_ib=record)
+        learned_guess.stage(ch, 1, dt, before_ib=probe)
+        b = ch.schur(out["fstar"], dt, 2)
+        X = np.stack(forces[-8:], 1)
+        B = np.stack([ch.schur(f, dt, 2) for f in forces[-8:]], 1)
+        r_dev = np.linalg.norm(b - ch.schur(out["x0"], dt, 2))
+        r_prev = np.linalg.norm(b - ch.schur(out["fprev"], dt, 2))
+    c, *_ = np.linalg.lstsq(B, b, rcond=None)
+    r_ls = np.linalg.norm(b - B @ c)
+    nb = np.linalg.norm(b)
+    print(f"  ||b|| {nb:.3e}; relative residuals: device start {r_dev / nb:.3e}, "
+          f"numpy lstsq {r_ls / nb:.3e}, previous force {r_prev / nb:.3e}; "
+          f"max |x0 - X c| / max |X c| {np.max(np.abs(out['x0'] - X @ c)) / np.max(np.abs(X @ c)):.2e}")
+    assert r_dev <= 1.01 * r_ls + 8 * tol
+    assert np.max(np.abs(out["x0"] - X @ c)) <= 1e-6 * np.max(np.abs(X @ c))
+    assert r_dev < 1e-3 * r_prev
