@@ -303,6 +303,7 @@ const char* kStageNames[] = {"fwd_x_div", "fft_z_fwd", "tri_y", "fft_z_inv", "in
 constexpr int kNumStages = 7;
 constexpr int kMaxRef = 8;          // refinement sweeps per mixed projection
 constexpr int kRedRefine = 4;       // first reduction slot of the sweep residual maxima
+constexpr int kResSlots = 64;       // max slots per sweep (workgroup b uses slot b % 64)
 
 }  // namespace
 
@@ -386,7 +387,7 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
       (rc = dalloc(c, &c->Pint, c->nr)) || (rc = dalloc(c, &c->rhs, c->nr)) ||
       (rc = dalloc(c, &c->S, c->nspec)) ||
       (!c3d::tri_part_supported(c->g) && (rc = dalloc(c, &c->CP, c->nspec))) ||
-      (rc = dalloc(c, &c->red, kRedRefine + kMaxRef)))
+      (rc = dalloc(c, &c->red, kRedRefine + (size_t)kMaxRef * kResSlots)))
     return cleanup(rc);
   if (d.precision == C3D_PREC_MIXED &&
       ((rc = dalloc(c, &c->S32, c->nspec)) || (rc = dalloc(c, &c->b64, c->nr)) ||
@@ -529,7 +530,7 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   hipStream_t st = c->stream;
   if (ev) HIPCHK(c, hipEventRecord(ev[0], st));
   if (mixed) {
-    HIPCHK(c, hipMemsetAsync(c->red + kRedRefine, 0, kMaxRef * sizeof(unsigned long long), st));
+    HIPCHK(c, hipMemsetAsync(c->red + kRedRefine, 0, (size_t)kMaxRef * kResSlots * sizeof(unsigned long long), st));
     HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, c->b64, nullptr, c->S32, st));
   } else {
     HIPCHK(c, c3d::launch_fwd_x(g, true, c->U, c->V, c->W, nullptr, c->S, st));
@@ -551,7 +552,7 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   for (int s = 1; mixed && s <= c->nref; s++) {
     const bool last = s == c->nref;
     HIPCHK(c, c3d::launch_residual(g, c->b64, c->phi64, c->gauge, c->r32,
-                                   c->red + kRedRefine + s - 1, st));
+                                   c->red + kRedRefine + (size_t)(s - 1) * kResSlots, st));
     HIPCHK(c, c3d::launch_fwd_x32(g, nullptr, nullptr, nullptr, nullptr, c->r32, c->S32, st));
     HIPCHK(c, c3d::launch_fft_z32(g, -1, c->S32, st));
     HIPCHK(c, c3d::launch_tri32(g, c->S32, st));
@@ -831,12 +832,16 @@ int c3d_refinement_residuals(c3d_ctx* c, double* rmax, int n) {
   if (c->desc.precision != C3D_PREC_MIXED)
     return fail(c, C3D_EINVAL, "refinement sweeps apply to C3D_PREC_MIXED contexts");
   const int m = n < c->nref ? n : c->nref;
-  unsigned long long bits[kMaxRef] = {0};
-  HIPCHK(c, hipMemcpyAsync(bits, c->red + kRedRefine, sizeof(bits), hipMemcpyDeviceToHost,
-                           c->stream));
+  std::vector<unsigned long long> bits((size_t)kMaxRef * kResSlots, 0ull);
+  HIPCHK(c, hipMemcpyAsync(bits.data(), c->red + kRedRefine, bits.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
   int rc = sync(c);
   if (rc) return rc;
-  std::memcpy(rmax, bits, m * sizeof(double));
+  for (int s = 0; s < m; s++) {  // non-negative doubles order as their bit patterns
+    unsigned long long mx = 0;
+    for (int q = 0; q < kResSlots; q++) mx = std::max(mx, bits[(size_t)s * kResSlots + q]);
+    std::memcpy(&rmax[s], &mx, sizeof(double));
+  }
   return m;
 }
 

@@ -291,8 +291,11 @@ __global__ __launch_bounds__(kBlock) void k_tri64(DevGrid g, cpx<double>* __rest
 // ---------------------------------------------------------------------------
 // Same system for NYM = 8*M rows (64, 128, 192), partitioned: each mode's rows are split into 8
 // parts of M rows, one lane per part, so every row is read once and written
-// once and the whole part stays in registers.  A wave holds 8 modes (lane =
-// part*8 + mode) so that each row access is a 128-byte run of 8 modes.
+// once and the whole part stays in registers.  A workgroup holds 32 modes;
+// each wave holds 16 of them for 4 of the parts (lane = part*16 + mode), so
+// that each row access is a run of 16 consecutive modes (256 bytes in FP64,
+// 128 in FP32), and the two waves of a mode group exchange interface rows
+// through LDS.
 //   pass 1  forward elimination inside the part with the unknown x_{a-1}
 //           above it carried as a spike column sp:
 //             x_i = y_i - sp_i x_{a-1} - cp_i x_{i+1}
@@ -311,7 +314,9 @@ template <typename real, int M>
 __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __restrict__ S) {
   constexpr int P = 8;            // parts per mode
   constexpr int N = P * M;        // rows
-  constexpr int MW = 64 / P;      // modes per wave
+  constexpr int MW = 16;          // modes per wave
+  constexpr int PW = 64 / MW;     // parts per wave
+  static_assert(PW * 2 == P, "two waves cover the parts of a mode group");
   constexpr int MB = kBlock / P;  // modes per workgroup
   constexpr int CPT = M + 1;      // padded pitch of the row tables
   constexpr int RP = 8 * P + 1;   // padded pitch of a mode's interface rows
@@ -325,8 +330,9 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
     tu[q * CPT + r] = (e == N - 1) ? 0.0 : g.dyy_u[e];
   }
   const int lane = tid & 63;
-  const int p = lane / MW;
-  const int mb = (tid >> 6) * MW + lane % MW;
+  const int wv = tid >> 6;
+  const int p = (wv & 1) * PW + lane / MW;
+  const int mb = (wv >> 1) * MW + lane % MW;
   const int nzp = g.nzp;
   const size_t stride = (size_t)g.nxh * nzp;
   const size_t m = (size_t)blockIdx.x * MB + mb;
@@ -620,8 +626,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
   if (i <= nx - 1 && j <= nyg - 1) {
     const int im = (i == 2) ? nx - 1 : i - 1;  // periodic image of cell 1
     const double rx = g.rdxf[i], ry = g.rdyf[j];
-    const double hxm = g.dxg[(i == 2) ? nx - 1 : i - 1], hx = g.dxg[i];
-    const double hym = g.dyg[j - 1], hy = g.dyg[j];
+    // the residual is stored in FP32: tabulated reciprocals of the spacings
+    // instead of divisions (within an ulp of FP64, far below FP32)
+    const double rhxm = g.rdxg[(i == 2) ? nx - 1 : i - 1], rhx = g.rdxg[i];
+    const double rhym = g.rdyg[j - 1], rhy = g.rdyg[j];
     const double gj = (j == 2) ? *gauge : 0.0;  // (0,0)-mode row of the spectral operator
     const int km0 = (kb == 2) ? nz - 1 : kb - 1;
     double pm = phi[ridx(i, j, km0, nxg, nyg)];
@@ -632,12 +640,12 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
       if (k > nzp + 1) break;
       const int km = (k == 2) ? nz - 1 : k - 1;
       const double pp = phi[ridx(i, j, k + 1, nxg, nyg)];
-      const double gxp = (phi[ridx(i + 1, j, k, nxg, nyg)] - p0) / hx;
-      const double gxm = (p0 - phi[ridx(im, j, k, nxg, nyg)]) / hxm;
-      const double gyp = (j <= ny - 1) ? (phi[ridx(i, j + 1, k, nxg, nyg)] - p0) / hy : 0.0;
-      const double gym = (j >= 3) ? (p0 - phi[ridx(i, j - 1, k, nxg, nyg)]) / hym : 0.0;
-      const double gzp = (pp - p0) / g.dzg[k];
-      const double gzm = (p0 - pm) / g.dzg[km];
+      const double gxp = (phi[ridx(i + 1, j, k, nxg, nyg)] - p0) * rhx;
+      const double gxm = (p0 - phi[ridx(im, j, k, nxg, nyg)]) * rhxm;
+      const double gyp = (j <= ny - 1) ? (phi[ridx(i, j + 1, k, nxg, nyg)] - p0) * rhy : 0.0;
+      const double gym = (j >= 3) ? (p0 - phi[ridx(i, j - 1, k, nxg, nyg)]) * rhym : 0.0;
+      const double gzp = (pp - p0) * g.rdzg[k];
+      const double gzm = (p0 - pm) * g.rdzg[km];
       const double lap = (gxp - gxm) * rx + (gyp - gym) * ry + (gzp - gzm) * g.rdzf[k];
       const double res = (b[ridx(i, j, k, nxg, nyg)] - lap) - gj;
       r[ridx(i, j, k, nxg, nyg)] = (float)res;
@@ -646,7 +654,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
       p0 = pp;
     }
   }
-  // block max, one atomic per block
+  // block max, one atomic per block into one of 64 slots (linear workgroup
+  // index % 64) so that the ~1e4 workgroups do not serialise on one
+  // address; the host takes the max of the slots
   __shared__ double sm[kBlock / 64];
   for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = amax;
@@ -654,7 +664,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __
   if (threadIdx.x == 0 && rmax) {
     double m = sm[0];
     for (int w = 1; w < kBlock / 64; w++) m = fmax(m, sm[w]);
-    atomicMax(rmax, (unsigned long long)__double_as_longlong(m));
+    const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    atomicMax(rmax + (lin & 63), (unsigned long long)__double_as_longlong(m));
   }
 }
 

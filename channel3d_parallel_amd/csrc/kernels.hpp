@@ -74,7 +74,8 @@ hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const
                           double* out, double* P, double dt, bool rk1, hipStream_t s);
 // r = b - M phi (FP64, stored FP32) on the unique cells, M = D G plus the
 // spectral solver's (0,0)-mode gauge row; gauge is nzp + 1 doubles of scratch;
-// max |r| as double bits into *rmax (may be null; zeroed by the caller).
+// max |r| as double bits into rmax[0..64): workgroup w maxes into slot w % 64
+// (may be null; zeroed by the caller; the max over the slots is max |r|).
 hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
                            float* r, unsigned long long* rmax, hipStream_t s);
 // Ghost copies of P (src/poisson.f90:93-98), in the reference order.
