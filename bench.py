@@ -57,9 +57,21 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def mixed_section(nx, ny, nz, U, V, W, dt, steps, ref, fp64_ms):
+def wall_stress(ch, um):
+    """nu*dU/dy at the bottom wall minus that at the top wall, divided by Ly:
+    the argument of utau's square root in compute_statistics
+    (src/statistics.f90:114-127), from the plane-mean profile um."""
+    yg = ch.grid(4)
+    Ly = ch.grid(11)[6]
+    dub = (um[1] - um[0]) / (yg[1] - yg[0])
+    dut = (um[-1] - um[-2]) / (yg[-1] - yg[-2])
+    return ch.nu * (dub - dut) / Ly
+
+
+def mixed_section(nx, ny, nz, U, V, W, dt, steps, ref, fp64_ms, ws_ref):
     """Mixed precision (FP32 spectra + FP64 refinement) against the FP64 GPU
-    result of the same projection: speed ratio and error per sweep count."""
+    result of the same projection: speed ratio and error per sweep count
+    (velocity, P_interim, max|div|, wall stress)."""
     from channel3d_parallel_amd import Channel
     from channel3d_parallel_amd._lib import PREC_MIXED
 
@@ -73,6 +85,8 @@ def mixed_section(nx, ny, nz, U, V, W, dt, steps, ref, fp64_ms):
             rr = ch.refinement_residuals()
             ch.apply_boundary_conditions()
             div = ch.check_divergence()
+            _, _, um, _ = ch.wall_shear()
+            tau = wall_stress(ch, um)
             f = ch.get_fields()
             err_u = max(rel(f["U"], ref["U"]), rel(f["V"], ref["V"]), rel(f["W"], ref["W"]))
             err_p = rel(f["P_interim"], ref["P_interim"])
@@ -83,6 +97,7 @@ def mixed_section(nx, ny, nz, U, V, W, dt, steps, ref, fp64_ms):
             res[str(nref)] = {"solves_per_s": 1e3 / ms, "ms": ms, "speed_vs_fp64": fp64_ms / ms,
                               "post_projection_max_div": div, "rel_err_velocity_vs_fp64": err_u,
                               "rel_err_P_interim_vs_fp64": err_p,
+                              "rel_err_wall_stress_vs_fp64": abs(tau - ws_ref) / abs(ws_ref),
                               "sweep_residual_max": [float(x) for x in rr], "stages_ms": stages}
     return res
 
@@ -312,6 +327,8 @@ def main():
         ch.compute_non_IB_projection(dt)
         ch.apply_boundary_conditions()
         max_div = ch.check_divergence()
+        utau, _, um, _ = ch.wall_shear()
+        ws_ref = wall_stress(ch, um)
         ref = None if a.no_extras else ch.get_fields()
         copy_bytes = 1 << 30
         copy_ms = ch.copy_bandwidth(copy_bytes, iters=10)
@@ -350,6 +367,7 @@ def main():
         "ns_per_point": tot_ms * 1e6 / ncell,
         "input_max_div": div_in,
         "post_projection_max_div": max_div,
+        "post_projection_wall_stress": ws_ref,
         "copy_GBps": 2 * copy_bytes / (copy_ms * 1e-3) / 1e9,
         "whole_solve_GBps": total_bytes / (tot_ms * 1e-3) / 1e9,
         "whole_solve_hbm_frac": total_bytes / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -360,7 +378,7 @@ def main():
                      "traffic": read_traffic(dom, a.workload)},
     }
     if not a.no_extras:
-        out["mixed"] = mixed_section(nx, ny, nz, U, V, W, dt, a.steps, ref, tot_ms)
+        out["mixed"] = mixed_section(nx, ny, nz, U, V, W, dt, a.steps, ref, tot_ms, ws_ref)
         del ref
         print("[bench] RK3 steps", file=sys.stderr, flush=True)
         out["rk3"] = rk3_section(nx, ny, nz, U, V, W, 5)

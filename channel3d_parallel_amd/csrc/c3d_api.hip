@@ -103,6 +103,7 @@ struct c3d_ctx {
   cpx<double>* S = nullptr;
   double* CP = nullptr;
   unsigned long long* red = nullptr;
+  double* psum = nullptr;  // 2*nyg plane sums of the wall-shear statistics
   size_t nu = 0, nv = 0, nw = 0, np = 0, nr = 0, nspec = 0;
   // RK3 state (allocated on first use): step-start copies, stage
   // right-hand sides F[stage][component], reduction scratch
@@ -774,6 +775,39 @@ int c3d_mean_mass_flow_u(c3d_ctx* c, double* q) {
   int rc = ensure_rk(c);
   if (!rc) rc = mass_flow(c, q);
   return rc;
+}
+
+int c3d_wall_shear(c3d_ctx* c, double* utau, double* wtau, double* Umean, double* Wmean) {
+  if (!c || !utau || !wtau) return C3D_EINVAL;
+  const int n = c->nyg;
+  if (!c->psum) {
+    int rc = dalloc(c, &c->psum, 2 * (size_t)n);
+    if (rc) return rc;
+  }
+  HIPCHK(c, c3d::launch_plane_sums(c->g, c->U, c->W, c->psum, c->stream));
+  std::vector<double> s(2 * (size_t)n);
+  HIPCHK(c, hipMemcpyAsync(s.data(), c->psum, s.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  // means over (nxg-3)*(nzg-3) resp. (nxg-3)*(nz-2) cells (src/statistics.f90:103-105)
+  std::vector<double> um(n), wm(n);
+  for (int j = 0; j < n; j++) {
+    um[j] = s[j] / (double)((c->nxg - 3) * (c->nzg - 3));
+    wm[j] = s[n + j] / (double)((c->nxg - 3) * (c->nz - 2));
+  }
+  // wall gradients and friction velocities (:114-127); yg(j) is c->yg[j-1]
+  const std::vector<double>& yg = c->yg;
+  const double dub = (um[1] - um[0]) / (yg[1] - yg[0]);
+  const double dut = (um[n - 1] - um[n - 2]) / (yg[n - 1] - yg[n - 2]);
+  const double dwb = (wm[1] - wm[0]) / (yg[1] - yg[0]);
+  const double dwt = (wm[n - 1] - wm[n - 2]) / (yg[n - 1] - yg[n - 2]);
+  const double nu = c->desc.nu, uv = 0.0, vw = 0.0;  // wall Reynolds stresses are zero
+  *utau = std::sqrt((uv - uv - nu * dut + nu * dub) / c->Ly);
+  *wtau = std::sqrt((vw - vw - nu * dwt + nu * dwb) / c->Ly);
+  if (Umean) std::memcpy(Umean, um.data(), (size_t)n * sizeof(double));
+  if (Wmean) std::memcpy(Wmean, wm.data(), (size_t)n * sizeof(double));
+  return C3D_OK;
 }
 
 int c3d_add_to_u(c3d_ctx* c, double d) {

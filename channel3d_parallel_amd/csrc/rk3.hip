@@ -325,6 +325,44 @@ __global__ __launch_bounds__(kB) void k_sum_pairs(const double* __restrict__ par
   }
 }
 
+// Plane sums of compute_statistics on one domain (src/statistics.f90:41-73,
+// last-rank branch): one workgroup per row j = 1..nyg sums the x-centred U
+// (interpolate_x, src/interpolation.f90:21-36) over i = 2..nxg-2,
+// k = 2..nzg-2 into sums[j-1] and W over i = 2..nxg-2, k = 2..nz-1 into
+// sums[nyg+j-1]; fixed-order tree sums.
+__global__ __launch_bounds__(kB) void k_plane_sums(DevGrid g, const double* __restrict__ U,
+                                                   const double* __restrict__ W,
+                                                   double* __restrict__ sums) {
+  __shared__ double su[kB], sw[kB];
+  const int j = blockIdx.x + 1;
+  const int nx = g.nx, nxg = g.nxg, nyg = g.nyg;
+  const int ni = nxg - 3;
+  const int nku = g.nzg - 3, nkw = g.nz - 2;
+  double a = 0.0, b = 0.0;
+  for (int t = threadIdx.x; t < ni * nku; t += kB) {
+    const int i = t % ni + 2, k = t / ni + 2;
+    a += 0.5 * (U[fidx(i - 1, j, k, nx, nyg)] + U[fidx(i, j, k, nx, nyg)]);
+  }
+  for (int t = threadIdx.x; t < ni * nkw; t += kB) {
+    const int i = t % ni + 2, k = t / ni + 2;
+    b += W[fidx(i, j, k, nxg, nyg)];
+  }
+  su[threadIdx.x] = a;
+  sw[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      su[threadIdx.x] += su[threadIdx.x + w];
+      sw[threadIdx.x] += sw[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[j - 1] = su[0];
+    sums[nyg + j - 1] = sw[0];
+  }
+}
+
 inline dim3 gpts(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
 
 }  // namespace
@@ -379,6 +417,12 @@ hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, con
 }
 
 int massflow_rows(const DevGrid& g) { return (g.nyg - 3) * (g.nzg - 2); }
+
+hipError_t launch_plane_sums(const DevGrid& g, const double* U, const double* W, double* sums,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_plane_sums, dim3(g.nyg), dim3(kB), 0, s, g, U, W, sums);
+  return hipGetLastError();
+}
 
 hipError_t launch_massflow(const DevGrid& g, const double* U, const double* H, double* part,
                            double* out2, hipStream_t s) {

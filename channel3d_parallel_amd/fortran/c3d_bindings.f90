@@ -96,6 +96,14 @@ Module c3d_bindings
        Real(c_double), Intent(Out) :: max_div
      End Function c3d_check_divergence
 
+     Integer(c_int) Function c3d_wall_shear(ctx, utau, wtau, Umean, Wmean) &
+          Bind(C, name="c3d_wall_shear")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: utau, wtau
+       Real(c_double) :: Umean(*), Wmean(*)
+     End Function c3d_wall_shear
+
      Integer(c_int) Function c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz) &
           Bind(C, name="c3d_set_pressure_gradient")
        Import :: c_int, c_ptr, c_double

@@ -6,6 +6,7 @@
 !   differential_operators  :: divergence, gradient (src/differential_operators.f90) !
 !   boundary_conditions     :: apply_boundary_conditions (src/boundary_conditions.f90:20) !
 !   monitor                 :: check_divergence, check_slip (src/monitor.f90:210-231) !
+!   statistics              :: compute_wall_shear (src/statistics.f90:41-131)   !
 !   projection              :: compute_IB_projection, schur, bicgstab          !
 !                              (src/projection.f90:48-186)                     !
 ! The reference keeps its state in module `global`; here c3d_global holds    !
@@ -237,3 +238,23 @@ Contains
     Call c3d_check(c3d_check_slip(ctx, max_slip), 'check_slip')
   End Subroutine check_slip
 End Module monitor
+
+!------------------------------------------------------------------------------!
+Module statistics
+  Use c3d_global
+  Implicit None
+Contains
+  ! Wall shear of compute_statistics, src/statistics.f90:41-131 (resident
+  ! velocity): friction velocities utau, wtau and, optionally, the plane-mean
+  ! profiles Umean(1:nyg), Wmean(1:nyg).
+  Subroutine compute_wall_shear(utau, wtau, Umean, Wmean)
+    Real(Int64), Intent(Out) :: utau, wtau
+    Real(Int64), Intent(Out), Optional, Target :: Umean(:), Wmean(:)
+    Real(Int64), Allocatable, Target :: um(:), wm(:)
+    Allocate(um(nyg), wm(nyg))
+    Call c3d_check(c3d_wall_shear(ctx, utau, wtau, um, wm), 'compute_wall_shear')
+    If (Present(Umean)) Umean = um
+    If (Present(Wmean)) Wmean = wm
+    Deallocate(um, wm)
+  End Subroutine compute_wall_shear
+End Module statistics

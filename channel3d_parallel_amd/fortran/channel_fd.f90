@@ -16,10 +16,11 @@ Program channel_fd
   Use mass_flow
   Use time_integration
   Use monitor
+  Use statistics
   Implicit None
 
   Integer(Int32) :: nxa, nya, nza, gtype, nsteps, nmonitor, istep, iu
-  Real(Int64)    :: alpha, nu_in, max_div, q
+  Real(Int64)    :: alpha, nu_in, max_div, q, utau, wtau
   Character(256) :: arg, fin, fout
   Real(Int64), Allocatable :: U(:,:,:), V(:,:,:), W(:,:,:), P(:,:,:)
 
@@ -68,9 +69,10 @@ Program channel_fd
   End Do
 
   Call check_divergence(max_div)
+  Call compute_wall_shear(utau, wtau)
   Call c3d_download(U, V, W, P)
   Open(newunit=iu, file=fout, access='stream', form='unformatted', status='replace')
-  Write(iu) U, V, W, P, dPdx, t, max_div
+  Write(iu) U, V, W, P, dPdx, t, max_div, utau
   Close(iu)
   Call c3d_finalize
 End Program channel_fd

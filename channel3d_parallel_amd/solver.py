@@ -52,6 +52,7 @@ class Channel:
         if rc != 0:
             raise C3DError(rc, f"c3d_create({nx},{ny},{nz}) failed")
         self.h = h
+        self.nu = nu
         self.nx, self.ny, self.nz = nx, ny, nz
         self.nxg, self.nyg, self.nzg = nx + 1, ny + 1, nz + 1
         self.nxp, self.nzp, self.nym = nx - 2, nz - 2, ny - 1
@@ -133,6 +134,14 @@ class Channel:
         v = C.c_double()
         self._chk(self.L.c3d_check_divergence(self.h, C.byref(v)))
         return v.value
+
+    def wall_shear(self):
+        """(utau, wtau, Umean[nyg], Wmean[nyg]) of compute_statistics
+        (src/statistics.f90:41-131) from the resident fields."""
+        ut, wt = C.c_double(), C.c_double()
+        um, wm = np.zeros(self.ny + 1), np.zeros(self.ny + 1)
+        self._chk(self.L.c3d_wall_shear(self.h, C.byref(ut), C.byref(wt), ptr(um), ptr(wm)))
+        return ut.value, wt.value, um, wm
 
     # RK3 time integration (src/time_integration.f90, src/equations.f90) -------
     def set_pressure_gradient(self, dPdx=0.0, dPdy=0.0, dPdz=0.0):

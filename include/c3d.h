@@ -104,6 +104,14 @@ int c3d_compute_non_ib_projection(c3d_ctx *ctx, double dt, int rk_step);
  * (src/monitor.f90:210-221). */
 int c3d_check_divergence(c3d_ctx *ctx, double *max_div);
 
+/* Wall shear of compute_statistics on the resident U and W
+ * (src/statistics.f90:41-131, one domain): plane means of the x-centred U
+ * and of W over the unique cells, wall gradients from rows 1, 2 and nyg-1,
+ * nyg, zero wall Reynolds stresses, utau = sqrt((nu*dUdy_b - nu*dUdy_t)/Ly)
+ * and wtau likewise (NaN when the argument is negative, as the reference's
+ * **0.5d0).  Umean, Wmean [nyg] receive the plane means when not NULL. */
+int c3d_wall_shear(c3d_ctx *ctx, double *utau, double *wtau, double *Umean, double *Wmean);
+
 /* ---- RK3 time integration ------------------------------------------------ */
 
 /* Mean pressure gradients dPdx, dPdy, dPdz used by the right-hand sides

@@ -94,6 +94,10 @@ double orc_compute_dt(const orc_grid *g, const double *U, const double *V, const
                       double cfl, double nu);
 /* H-masked bulk velocity <U> (src/mass_flow.f90:24-53); Hu may be NULL. */
 double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu);
+/* Wall shear of compute_statistics (src/statistics.f90:41-131): utau,
+ * wtau, and the plane means Umean, Wmean [nyg] (may be NULL). */
+void orc_wall_shear(const orc_grid *g, const double *U, const double *W, double nu,
+                    double *utau, double *wtau, double *Umean, double *Wmean);
 /* One compute_time_step_RK3 without a body (src/time_integration.f90:86-193);
  * q0 = NAN disables the constant-flow correction. */
 void orc_time_step_rk3(const orc_grid *g, double *U, double *V, double *W, double *P,

@@ -240,6 +240,46 @@ double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu
   return s / nrm;
 }
 
+/* Wall shear of compute_statistics on one domain (src/statistics.f90:41-131):
+ * plane means over the unique cells of the x-centred U (interpolate_x,
+ * src/interpolation.f90:21-36: 0.5*(U(i-1) + U(i)) at centre i) over
+ * i = 2..nxg-2, k = 2..nzg-2, and of W over i = 2..nxg-2, k = 2..nz-1
+ * (the last-rank branch, :64-73), divided by (nxg-3)*(nzg-3) resp.
+ * (nxg-3)*(nz-2) (:103-105); wall gradients from rows 1, 2 and nyg-1, nyg
+ * (:114-117); the wall Reynolds stresses are zero (:120-123);
+ * utau = ((UV_t - UV_b - nu*dUdy_t + nu*dUdy_b)/Ly)**0.5 (:126-127).
+ * Sums run in Fortran's array order (i fastest).  Umean, Wmean [nyg] may
+ * be NULL. */
+void orc_wall_shear(const orc_grid *g, const double *U, const double *W, double nu,
+                    double *utau, double *wtau, double *Umean, double *Wmean) {
+  G q;
+  gload(g, &q);
+  double *um = malloc((size_t)q.nyg * sizeof(double));
+  double *wm = malloc((size_t)q.nyg * sizeof(double));
+  for (int j = 1; j <= q.nyg; j++) {
+    double su = 0.0, sw = 0.0;
+    for (int k = 2; k <= q.nzg - 2; k++)
+      for (int i = 2; i <= q.nxg - 2; i++) su += 0.5 * (Uf(i - 1, j, k) + Uf(i, j, k));
+    for (int k = 2; k <= q.nz - 1; k++)
+      for (int i = 2; i <= q.nxg - 2; i++) sw += Wf(i, j, k);
+    um[j - 1] = su / (double)((q.nxg - 3) * (q.nzg - 3));
+    wm[j - 1] = sw / (double)((q.nxg - 3) * (q.nz - 2));
+  }
+  const int n = q.nyg;
+  const double dub = (um[1] - um[0]) / (YG(2) - YG(1));
+  const double dut = (um[n - 1] - um[n - 2]) / (YG(n) - YG(n - 1));
+  const double dwb = (wm[1] - wm[0]) / (YG(2) - YG(1));
+  const double dwt = (wm[n - 1] - wm[n - 2]) / (YG(n) - YG(n - 1));
+  const double uvt = 0.0, uvb = 0.0, vwt = 0.0, vwb = 0.0;
+  *utau = sqrt((uvt - uvb - nu * dut + nu * dub) / q.Ly);
+  *wtau = sqrt((vwt - vwb - nu * dwt + nu * dwb) / q.Ly);
+  if (Umean) memcpy(Umean, um, (size_t)n * sizeof(double));
+  if (Wmean) memcpy(Wmean, wm, (size_t)n * sizeof(double));
+  free(um);
+  free(wm);
+  gfree(&q);
+}
+
 /* compute_time_step_RK3 without a body: src/time_integration.f90:86-193.
  * Per stage: right-hand sides, stage update, boundary conditions,
  * projection, boundary conditions; after stage 3 the constant x-mass-flow

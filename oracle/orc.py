@@ -59,6 +59,8 @@ def lib():
         L.orc_compute_dt.restype = C.c_double
         L.orc_mean_mass_flow_u.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_mean_mass_flow_u.restype = C.c_double
+        L.orc_wall_shear.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_wall_shear.restype = None
         L.orc_time_step_rk3.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         C.c_double, _dp, C.c_double, C.c_int]
         _lib = L
@@ -209,6 +211,16 @@ class Grid:
 
     def mean_mass_flow_u(self, U, Hu=None) -> float:
         return lib().orc_mean_mass_flow_u(self.h, _p(U), _p(Hu))
+
+    def wall_shear(self, U, W, nu):
+        """(utau, wtau, Umean[nyg], Wmean[nyg]) of compute_statistics
+        (src/statistics.f90:41-131)."""
+        ut, wt = C.c_double(), C.c_double()
+        n = self.nyg
+        um, wm = np.zeros(n), np.zeros(n)
+        lib().orc_wall_shear(self.h, _p(U), _p(W), nu, C.cast(C.pointer(ut), _dp),
+                             C.cast(C.pointer(wt), _dp), _p(um), _p(wm))
+        return ut.value, wt.value, um, wm
 
     def time_step_rk3(self, U, V, W, P, dt, nu, dPdx, q0=float("nan"), nthreads=0):
         """One compute_time_step_RK3 in place; returns (new dPdx, P_interim)."""

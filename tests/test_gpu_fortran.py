@@ -41,7 +41,7 @@ def test_fortran_rk3_time_loop(tmp_path):
         n = int(np.prod(s))
         outs.append(raw[off:off + n].reshape(s, order="F"))
         off += n
-    dpdx_gpu, t_gpu, div_gpu = raw[off:off + 3]
+    dpdx_gpu, t_gpu, div_gpu, utau_gpu = raw[off:off + 4]
     g.apply_bc(U, V, W)
     q0 = g.mean_mass_flow_u(U)
     P = g.zeros("p")
@@ -57,6 +57,9 @@ def test_fortran_rk3_time_loop(tmp_path):
     assert abs(dpdx_gpu - dpdx) <= 1e-9 * abs(dpdx)
     assert abs(t_gpu - t) <= 1e-14 * t
     assert div_gpu <= 1e-10
+    utau, _, _, _ = g.wall_shear(U, W, nu)  # statistics::compute_wall_shear
+    print(f"  utau gpu {utau_gpu:.15e} oracle {utau:.15e}")
+    assert abs(utau_gpu - utau) <= 1e-10 * utau
 
 
 @pytest.mark.parametrize("nx,ny,nz,gt,rk", [(34, 65, 34, 1, 1), (66, 33, 34, 0, 2)])

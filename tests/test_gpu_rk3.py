@@ -124,3 +124,25 @@ def test_channel_config1_time_steps(nsteps, tol):
     assert abs(dpdx - dpdx_r) <= 1e-11
     assert div <= 1e-10
     assert abs(q - q0) <= 1e-12
+
+
+@pytest.mark.parametrize("nx,ny,nz,gt", [(34, 65, 34, 1), (194, 129, 194, 1), (130, 129, 130, 0)])
+def test_wall_shear_vs_oracle(nx, ny, nz, gt):
+    """Wall shear of compute_statistics (src/statistics.f90:41-131) on the
+    resident fields: utau and the plane-mean profiles equal the oracle's up
+    to summation order (1e-13 relative); wtau likewise or NaN in both (a
+    negative argument of the square root, as in the reference)."""
+    print(f"\n[case] wall shear {nx}x{ny}x{nz} gt={gt}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=gt)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.1, seed=4, grid_type=gt)
+    g.apply_bc(U, V, W)
+    with Channel(nx, ny, nz, grid_type=gt, nu=NU) as ch:
+        ch.set_fields(U, V, W)
+        ut, wt, um, wm = ch.wall_shear()
+    uo, wo, umo, wmo = g.wall_shear(U, W, NU)
+    print(f"  utau gpu {ut:.15e} oracle {uo:.15e}; wtau gpu {wt:.6e} oracle {wo:.6e}")
+    assert np.isfinite(uo) and abs(ut - uo) <= 1e-13 * uo
+    scale = np.max(np.abs(umo))
+    assert np.max(np.abs(um - umo)) <= 1e-13 * scale
+    assert np.max(np.abs(wm - wmo)) <= 1e-13 * scale
+    assert (np.isnan(wt) and np.isnan(wo)) or abs(wt - wo) <= 1e-12 * abs(wo)

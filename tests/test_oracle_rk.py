@@ -133,3 +133,37 @@ def test_time_step_rk3_keeps_flow_rate_and_divergence():
         dpdx, _ = g.time_step_rk3(U, V, W, P, dt, NU, dpdx, q0)
     assert abs(g.mean_mass_flow_u(U) - q0) < 1e-12
     assert g.max_abs_div(U, V, W) < 1e-10
+
+
+@pytest.mark.parametrize("gt", [0, 1])
+def test_wall_shear_closed_form(gt):
+    """compute_statistics' wall shear (src/statistics.f90:41-131) pinned by a
+    case solved by hand: U = a*y*(2 - y) and W = b*y*(2 - y) at the centres
+    with the no-slip ghosts U(:,1,:) = -U(:,2,:) (mirror ghost centre
+    yg(1) = -yg(2)) give the wall gradients a*(2 - yg(2)) at the bottom and
+    -a*yg(nyg-1) at the top, so utau**2 = nu*a*((2 - yg(2)) + yg(nyg-1))/Ly.
+    A periodic x-dependence of U (a cosine over whole periods) leaves the
+    x-centred plane means unchanged."""
+    g = orc.Grid(34, 65, 34, grid_type=gt)
+    yg = g.get(4)
+    xf = g.get(0)
+    nu, a, b = 1.0 / 2800.0, 1.5, 0.25
+    prof = yg * (2.0 - yg)
+    U = np.zeros(g.shape_u, order="F")
+    W = np.zeros(g.shape_w, order="F")
+    V = np.zeros(g.shape_v, order="F")
+    U[:, :, :] = (a * prof)[None, :, None]
+    W[:, :, :] = (b * prof)[None, :, None]
+    g.apply_bc(U, V, W)
+    ut, wt, um, wm = g.wall_shear(U, W, nu)
+    n = g.nyg
+    exp_u = np.sqrt(nu * a * ((2.0 - yg[1]) + yg[n - 2]) / 2.0)
+    exp_w = np.sqrt(nu * b * ((2.0 - yg[1]) + yg[n - 2]) / 2.0)
+    assert abs(ut - exp_u) <= 1e-14 * exp_u and abs(wt - exp_w) <= 1e-14 * exp_w
+    assert np.allclose(um[1:-1], a * prof[1:-1], rtol=1e-13, atol=0)  # summation rounding
+    assert um[0] == -um[1] and um[-1] == -um[-2]
+    # a whole-period cosine in x on the faces averages out of the centred means
+    U2 = U + 0.3 * np.cos(2 * np.pi * (xf - xf[0]) / (xf[-2] - xf[0]))[:, None, None]
+    g.apply_bc(U2, V, W)
+    ut2, _, um2, _ = g.wall_shear(U2, W, nu)
+    assert np.allclose(um2, um, rtol=0, atol=1e-13) and abs(ut2 - ut) <= 1e-13 * ut
