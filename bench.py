@@ -248,13 +248,14 @@ def read_traffic(kernel, workload):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(nx, ny, nz, U, V, W, dt, nsolves, threads):
+def cpu_baseline(nx, ny, nz, U, V, W, dt, nsolves, threads, warm=True):
     """The oracle (C FP64 restatement, OpenMP) timed on the host cores."""
     from oracle import orc
 
     g = orc.Grid(nx, ny, nz)
     u, v, w = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
-    g.projection(u, v, w, dt=dt, nthreads=threads)  # warm-up
+    if warm:
+        g.projection(u, v, w, dt=dt, nthreads=threads)
     ts = []
     for _ in range(nsolves):
         u, v, w = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
@@ -394,6 +395,12 @@ def main():
                                "sample": f"{a.cpu_solves} full {nx - 2}x{ny}x{nz - 2} projection "
                                          "solves (median), C FP64 restatement with OpenMP; "
                                          "not the Fortran/FFTW binary"}
+        print("[bench] CPU baseline: oracle, 1 thread, 2 solves", file=sys.stderr, flush=True)
+        t1 = cpu_baseline(nx, ny, nz, U, V, W, dt, 2, 1, warm=False)
+        out["cpu_baseline_1_thread"] = {"value": 1.0 / t1, "unit": "solves/s", "cores": 1,
+                                        "kind": "port", "sample": "2 solves (median), same input"}
+        out["host_cores"] = {"available_to_process": len(os.sched_getaffinity(0)),
+                             "os_cpu_count": os.cpu_count()}
     print(json.dumps(out))
 
 
