@@ -103,7 +103,7 @@ struct c3d_ctx {
   cpx<double>* S = nullptr;
   double* CP = nullptr;
   unsigned long long* red = nullptr;
-  double* psum = nullptr;  // 2*nyg plane sums of the wall-shear statistics
+  double* psum = nullptr;  // 7*nyg plane sums of compute_statistics
   size_t nu = 0, nv = 0, nw = 0, np = 0, nr = 0, nspec = 0;
   // RK3 state (allocated on first use): step-start copies, stage
   // right-hand sides F[stage][component], reduction scratch
@@ -777,26 +777,46 @@ int c3d_mean_mass_flow_u(c3d_ctx* c, double* q) {
   return rc;
 }
 
-int c3d_wall_shear(c3d_ctx* c, double* utau, double* wtau, double* Umean, double* Wmean) {
-  if (!c || !utau || !wtau) return C3D_EINVAL;
+// Plane means of compute_statistics (src/statistics.f90:41-111, one domain):
+// means[q*nyg + j-1], q = 0 Umean, 1 Vmean, 2 Wmean, 3 U2mean, 4 V2mean,
+// 5 W2mean, 6 UVmean; U, V, U2, V2, UV over (nxg-3)*(nzg-3) cells, W, W2 over
+// (nxg-3)*(nz-2) (:103-111).
+static int plane_means(c3d_ctx* c, std::vector<double>& m) {
   const int n = c->nyg;
   if (!c->psum) {
-    int rc = dalloc(c, &c->psum, 2 * (size_t)n);
+    int rc = dalloc(c, &c->psum, 7 * (size_t)n);
     if (rc) return rc;
   }
-  HIPCHK(c, c3d::launch_plane_sums(c->g, c->U, c->W, c->psum, c->stream));
-  std::vector<double> s(2 * (size_t)n);
-  HIPCHK(c, hipMemcpyAsync(s.data(), c->psum, s.size() * sizeof(double), hipMemcpyDeviceToHost,
+  HIPCHK(c, c3d::launch_plane_sums(c->g, c->U, c->V, c->W, c->psum, c->stream));
+  m.assign(7 * (size_t)n, 0.0);
+  HIPCHK(c, hipMemcpyAsync(m.data(), c->psum, m.size() * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   int rc = sync(c);
   if (rc) return rc;
-  // means over (nxg-3)*(nzg-3) resp. (nxg-3)*(nz-2) cells (src/statistics.f90:103-105)
-  std::vector<double> um(n), wm(n);
-  for (int j = 0; j < n; j++) {
-    um[j] = s[j] / (double)((c->nxg - 3) * (c->nzg - 3));
-    wm[j] = s[n + j] / (double)((c->nxg - 3) * (c->nz - 2));
-  }
-  // wall gradients and friction velocities (:114-127); yg(j) is c->yg[j-1]
+  const double nuv = (double)((c->nxg - 3) * (c->nzg - 3)), nw = (double)((c->nxg - 3) * (c->nz - 2));
+  for (int q = 0; q < 7; q++)
+    for (int j = 0; j < n; j++) m[(size_t)q * n + j] /= (q == 2 || q == 5) ? nw : nuv;
+  return 0;
+}
+
+int c3d_plane_statistics(c3d_ctx* c, double* means) {
+  if (!c || !means) return C3D_EINVAL;
+  std::vector<double> m;
+  int rc = plane_means(c, m);
+  if (rc) return rc;
+  std::memcpy(means, m.data(), m.size() * sizeof(double));
+  return C3D_OK;
+}
+
+int c3d_wall_shear(c3d_ctx* c, double* utau, double* wtau, double* Umean, double* Wmean) {
+  if (!c || !utau || !wtau) return C3D_EINVAL;
+  const int n = c->nyg;
+  std::vector<double> m;
+  int rc = plane_means(c, m);
+  if (rc) return rc;
+  const double* um = m.data();
+  const double* wm = m.data() + 2 * (size_t)n;
+  // wall gradients and friction velocities (src/statistics.f90:114-127); yg(j) is c->yg[j-1]
   const std::vector<double>& yg = c->yg;
   const double dub = (um[1] - um[0]) / (yg[1] - yg[0]);
   const double dut = (um[n - 1] - um[n - 2]) / (yg[n - 1] - yg[n - 2]);
@@ -805,8 +825,8 @@ int c3d_wall_shear(c3d_ctx* c, double* utau, double* wtau, double* Umean, double
   const double nu = c->desc.nu, uv = 0.0, vw = 0.0;  // wall Reynolds stresses are zero
   *utau = std::sqrt((uv - uv - nu * dut + nu * dub) / c->Ly);
   *wtau = std::sqrt((vw - vw - nu * dwt + nu * dwb) / c->Ly);
-  if (Umean) std::memcpy(Umean, um.data(), (size_t)n * sizeof(double));
-  if (Wmean) std::memcpy(Wmean, wm.data(), (size_t)n * sizeof(double));
+  if (Umean) std::memcpy(Umean, um, (size_t)n * sizeof(double));
+  if (Wmean) std::memcpy(Wmean, wm, (size_t)n * sizeof(double));
   return C3D_OK;
 }
 

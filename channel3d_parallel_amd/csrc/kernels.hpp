@@ -114,10 +114,11 @@ hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, con
 int massflow_rows(const DevGrid& g);
 hipError_t launch_massflow(const DevGrid& g, const double* U, const double* H, double* part,
                            double* out2, hipStream_t s);
-// Plane sums of the x-centred U and of W over the unique cells of every row
-// j = 1..nyg (compute_statistics): sums[0..nyg) and sums[nyg..2nyg).
-hipError_t launch_plane_sums(const DevGrid& g, const double* U, const double* W, double* sums,
-                             hipStream_t s);
+// Plane sums of compute_statistics over the unique cells of every row
+// j = 1..nyg: sums[q*nyg + j-1] for q = 0 x-centred U, 1 y-centred V (wall
+// rows mirrored), 2 W, 3 U^2, 4 V^2, 5 W^2, 6 UV (7*nyg doubles).
+hipError_t launch_plane_sums(const DevGrid& g, const double* U, const double* V, const double* W,
+                             double* sums, hipStream_t s);
 
 // Streaming copy of nbytes (multiple of 16) for bandwidth calibration.
 hipError_t launch_copy(const void* src, void* dst, size_t nbytes, hipStream_t s);

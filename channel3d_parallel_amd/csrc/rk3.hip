@@ -326,41 +326,53 @@ __global__ __launch_bounds__(kB) void k_sum_pairs(const double* __restrict__ par
 }
 
 // Plane sums of compute_statistics on one domain (src/statistics.f90:41-73,
-// last-rank branch): one workgroup per row j = 1..nyg sums the x-centred U
-// (interpolate_x, src/interpolation.f90:21-36) over i = 2..nxg-2,
-// k = 2..nzg-2 into sums[j-1] and W over i = 2..nxg-2, k = 2..nz-1 into
-// sums[nyg+j-1]; fixed-order tree sums.
+// last-rank branch): one workgroup per row j = 1..nyg.  term_1 is the
+// x-centred U (interpolate_x, src/interpolation.f90:21-36), term_2 the
+// y-centred V (interpolate_y, :44-58) with term_2(:,1,:) = -term_2(:,2,:)
+// and term_2(:,nyg,:) = -term_2(:,nyg-1,:) (src/statistics.f90:47-49).
+// sums[q*nyg + j-1] for q = 0 U, 1 V, 2 W, 3 U^2, 4 V^2, 5 W^2, 6 UV over
+// i = 2..nxg-2 and k = 2..nzg-2 (W, W^2: k = 2..nz-1); fixed-order sums.
+constexpr int kStats = 7;
+
 __global__ __launch_bounds__(kB) void k_plane_sums(DevGrid g, const double* __restrict__ U,
+                                                   const double* __restrict__ V,
                                                    const double* __restrict__ W,
                                                    double* __restrict__ sums) {
-  __shared__ double su[kB], sw[kB];
+  __shared__ double sh[kStats][kB];
   const int j = blockIdx.x + 1;
-  const int nx = g.nx, nxg = g.nxg, nyg = g.nyg;
+  const int nx = g.nx, ny = g.ny, nxg = g.nxg, nyg = g.nyg;
   const int ni = nxg - 3;
   const int nku = g.nzg - 3, nkw = g.nz - 2;
-  double a = 0.0, b = 0.0;
+  // term_2 at row j from V rows jv - 1, jv with sign sg (the wall rows mirror)
+  const int jv = (j == 1) ? 2 : ((j == nyg) ? nyg - 1 : j);
+  const double sg = (j == 1 || j == nyg) ? -1.0 : 1.0;
+  double a[kStats] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int t = threadIdx.x; t < ni * nku; t += kB) {
     const int i = t % ni + 2, k = t / ni + 2;
-    a += 0.5 * (U[fidx(i - 1, j, k, nx, nyg)] + U[fidx(i, j, k, nx, nyg)]);
+    const double t1 = 0.5 * (U[fidx(i - 1, j, k, nx, nyg)] + U[fidx(i, j, k, nx, nyg)]);
+    const double t2 = sg * (0.5 * (V[fidx(i, jv - 1, k, nxg, ny)] + V[fidx(i, jv, k, nxg, ny)]));
+    a[0] += t1;
+    a[1] += t2;
+    a[3] += t1 * t1;
+    a[4] += t2 * t2;
+    a[6] += t1 * t2;
   }
   for (int t = threadIdx.x; t < ni * nkw; t += kB) {
     const int i = t % ni + 2, k = t / ni + 2;
-    b += W[fidx(i, j, k, nxg, nyg)];
+    const double w = W[fidx(i, j, k, nxg, nyg)];
+    a[2] += w;
+    a[5] += w * w;
   }
-  su[threadIdx.x] = a;
-  sw[threadIdx.x] = b;
+#pragma unroll
+  for (int q = 0; q < kStats; q++) sh[q][threadIdx.x] = a[q];
   __syncthreads();
   for (int w = kB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      su[threadIdx.x] += su[threadIdx.x + w];
-      sw[threadIdx.x] += sw[threadIdx.x + w];
-    }
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < kStats; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    sums[j - 1] = su[0];
-    sums[nyg + j - 1] = sw[0];
-  }
+  if (threadIdx.x < kStats) sums[threadIdx.x * (size_t)nyg + j - 1] = sh[threadIdx.x][0];
 }
 
 inline dim3 gpts(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
@@ -418,9 +430,9 @@ hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, con
 
 int massflow_rows(const DevGrid& g) { return (g.nyg - 3) * (g.nzg - 2); }
 
-hipError_t launch_plane_sums(const DevGrid& g, const double* U, const double* W, double* sums,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_plane_sums, dim3(g.nyg), dim3(kB), 0, s, g, U, W, sums);
+hipError_t launch_plane_sums(const DevGrid& g, const double* U, const double* V, const double* W,
+                             double* sums, hipStream_t s) {
+  hipLaunchKernelGGL(k_plane_sums, dim3(g.nyg), dim3(kB), 0, s, g, U, V, W, sums);
   return hipGetLastError();
 }
 

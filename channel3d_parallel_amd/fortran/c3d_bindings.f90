@@ -104,6 +104,13 @@ Module c3d_bindings
        Real(c_double) :: Umean(*), Wmean(*)
      End Function c3d_wall_shear
 
+     Integer(c_int) Function c3d_plane_statistics(ctx, means) &
+          Bind(C, name="c3d_plane_statistics")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: means(*)
+     End Function c3d_plane_statistics
+
      Integer(c_int) Function c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz) &
           Bind(C, name="c3d_set_pressure_gradient")
        Import :: c_int, c_ptr, c_double

@@ -6,7 +6,8 @@
 !   differential_operators  :: divergence, gradient (src/differential_operators.f90) !
 !   boundary_conditions     :: apply_boundary_conditions (src/boundary_conditions.f90:20) !
 !   monitor                 :: check_divergence, check_slip (src/monitor.f90:210-231) !
-!   statistics              :: compute_wall_shear (src/statistics.f90:41-131)   !
+!   statistics              :: compute_wall_shear, compute_plane_statistics     !
+!                              (src/statistics.f90:41-131)                     !
 !   projection              :: compute_IB_projection, schur, bicgstab          !
 !                              (src/projection.f90:48-186)                     !
 ! The reference keeps its state in module `global`; here c3d_global holds    !
@@ -257,4 +258,22 @@ Contains
     If (Present(Wmean)) Wmean = wm
     Deallocate(um, wm)
   End Subroutine compute_wall_shear
+
+  ! Plane-mean profiles of compute_statistics, src/statistics.f90:41-111
+  ! (resident velocity); each argument has nyg entries.
+  Subroutine compute_plane_statistics(Umean, Vmean, Wmean, U2mean, V2mean, W2mean, UVmean)
+    Real(Int64), Intent(Out) :: Umean(:), Vmean(:), Wmean(:), U2mean(:), V2mean(:), W2mean(:), &
+                                UVmean(:)
+    Real(Int64), Allocatable :: m(:)
+    Allocate(m(7 * nyg))
+    Call c3d_check(c3d_plane_statistics(ctx, m), 'compute_plane_statistics')
+    Umean  = m(        1:    nyg)
+    Vmean  = m(  nyg + 1:2 * nyg)
+    Wmean  = m(2*nyg + 1:3 * nyg)
+    U2mean = m(3*nyg + 1:4 * nyg)
+    V2mean = m(4*nyg + 1:5 * nyg)
+    W2mean = m(5*nyg + 1:6 * nyg)
+    UVmean = m(6*nyg + 1:7 * nyg)
+    Deallocate(m)
+  End Subroutine compute_plane_statistics
 End Module statistics

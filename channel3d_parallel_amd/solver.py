@@ -135,6 +135,16 @@ class Channel:
         self._chk(self.L.c3d_check_divergence(self.h, C.byref(v)))
         return v.value
 
+    STAT_NAMES = ("Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean")
+
+    def plane_statistics(self):
+        """Plane-mean profiles of compute_statistics (src/statistics.f90:41-111)
+        from the resident fields, as a dict of [nyg] arrays."""
+        n = self.ny + 1
+        out = np.zeros(7 * n)
+        self._chk(self.L.c3d_plane_statistics(self.h, ptr(out)))
+        return {k: out[q * n:(q + 1) * n].copy() for q, k in enumerate(self.STAT_NAMES)}
+
     def wall_shear(self):
         """(utau, wtau, Umean[nyg], Wmean[nyg]) of compute_statistics
         (src/statistics.f90:41-131) from the resident fields."""

@@ -111,6 +111,12 @@ int c3d_check_divergence(c3d_ctx *ctx, double *max_div);
  * and wtau likewise (NaN when the argument is negative, as the reference's
  * **0.5d0).  Umean, Wmean [nyg] receive the plane means when not NULL. */
 int c3d_wall_shear(c3d_ctx *ctx, double *utau, double *wtau, double *Umean, double *Wmean);
+/* Plane-mean profiles of compute_statistics on the resident fields
+ * (src/statistics.f90:41-111, one domain): means[q*nyg + j-1] for q = 0
+ * Umean, 1 Vmean, 2 Wmean, 3 U2mean, 4 V2mean, 5 W2mean, 6 UVmean, from the
+ * x-centred U and the y-centred V (wall rows mirrored) over the unique
+ * cells; 7*nyg doubles. */
+int c3d_plane_statistics(c3d_ctx *ctx, double *means);
 
 /* ---- RK3 time integration ------------------------------------------------ */
 

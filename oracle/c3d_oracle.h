@@ -94,6 +94,10 @@ double orc_compute_dt(const orc_grid *g, const double *U, const double *V, const
                       double cfl, double nu);
 /* H-masked bulk velocity <U> (src/mass_flow.f90:24-53); Hu may be NULL. */
 double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu);
+/* Plane means of compute_statistics (src/statistics.f90:41-111):
+ * out[q*nyg + j-1], q = 0 U, 1 V, 2 W, 3 U^2, 4 V^2, 5 W^2, 6 UV. */
+void orc_plane_statistics(const orc_grid *g, const double *U, const double *V, const double *W,
+                          double *out);
 /* Wall shear of compute_statistics (src/statistics.f90:41-131): utau,
  * wtau, and the plane means Umean, Wmean [nyg] (may be NULL). */
 void orc_wall_shear(const orc_grid *g, const double *U, const double *W, double nu,

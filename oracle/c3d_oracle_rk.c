@@ -240,6 +240,44 @@ double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu
   return s / nrm;
 }
 
+/* Plane means of compute_statistics on one domain (src/statistics.f90:41-111,
+ * last-rank branch :64-73): term_1 = interpolate_x(U) (0.5*(U(i-1) + U(i))
+ * at centre i, src/interpolation.f90:21-36), term_2 = interpolate_y(V)
+ * (0.5*(V(j-1) + V(j)) at centre j, :44-58) with term_2(:,1,:) =
+ * -term_2(:,2,:) and term_2(:,nyg,:) = -term_2(:,nyg-1,:) (:47-49); sums
+ * over i = 2..nxg-2, k = 2..nzg-2 (W: k = 2..nz-1) in Fortran's array order,
+ * divided as in :103-111.  out[q*nyg + j-1], q = 0 Umean, 1 Vmean, 2 Wmean,
+ * 3 U2mean, 4 V2mean, 5 W2mean, 6 UVmean. */
+void orc_plane_statistics(const orc_grid *g, const double *U, const double *V, const double *W,
+                          double *out) {
+  G q;
+  gload(g, &q);
+  const int n = q.nyg;
+  const double nuv = (double)((q.nxg - 3) * (q.nzg - 3)), nw = (double)((q.nxg - 3) * (q.nz - 2));
+  for (int j = 1; j <= n; j++) {
+    const int jv = (j == 1) ? 2 : (j == n ? n - 1 : j);
+    const double sg = (j == 1 || j == n) ? -1.0 : 1.0;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 2; k <= q.nzg - 2; k++)
+      for (int i = 2; i <= q.nxg - 2; i++) {
+        const double t1 = 0.5 * (Uf(i - 1, j, k) + Uf(i, j, k));
+        const double t2 = sg * (0.5 * (Vf(i, jv - 1, k) + Vf(i, jv, k)));
+        s[0] += t1;
+        s[1] += t2;
+        s[3] += t1 * t1;
+        s[4] += t2 * t2;
+        s[6] += t1 * t2;
+      }
+    for (int k = 2; k <= q.nz - 1; k++)
+      for (int i = 2; i <= q.nxg - 2; i++) {
+        s[2] += Wf(i, j, k);
+        s[5] += Wf(i, j, k) * Wf(i, j, k);
+      }
+    for (int c = 0; c < 7; c++) out[(size_t)c * n + j - 1] = s[c] / ((c == 2 || c == 5) ? nw : nuv);
+  }
+  gfree(&q);
+}
+
 /* Wall shear of compute_statistics on one domain (src/statistics.f90:41-131):
  * plane means over the unique cells of the x-centred U (interpolate_x,
  * src/interpolation.f90:21-36: 0.5*(U(i-1) + U(i)) at centre i) over

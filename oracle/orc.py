@@ -61,6 +61,8 @@ def lib():
         L.orc_mean_mass_flow_u.restype = C.c_double
         L.orc_wall_shear.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_wall_shear.restype = None
+        L.orc_plane_statistics.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_plane_statistics.restype = None
         L.orc_time_step_rk3.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         C.c_double, _dp, C.c_double, C.c_int]
         _lib = L
@@ -211,6 +213,15 @@ class Grid:
 
     def mean_mass_flow_u(self, U, Hu=None) -> float:
         return lib().orc_mean_mass_flow_u(self.h, _p(U), _p(Hu))
+
+    STAT_NAMES = ("Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean")
+
+    def plane_statistics(self, U, V, W):
+        """Plane-mean profiles of compute_statistics (src/statistics.f90:41-111)
+        as a dict of [nyg] arrays."""
+        out = np.zeros(7 * self.nyg)
+        lib().orc_plane_statistics(self.h, _p(U), _p(V), _p(W), _p(out))
+        return {k: out[q * self.nyg:(q + 1) * self.nyg].copy() for q, k in enumerate(self.STAT_NAMES)}
 
     def wall_shear(self, U, W, nu):
         """(utau, wtau, Umean[nyg], Wmean[nyg]) of compute_statistics

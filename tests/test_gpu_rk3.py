@@ -146,3 +146,29 @@ def test_wall_shear_vs_oracle(nx, ny, nz, gt):
     assert np.max(np.abs(um - umo)) <= 1e-13 * scale
     assert np.max(np.abs(wm - wmo)) <= 1e-13 * scale
     assert (np.isnan(wt) and np.isnan(wo)) or abs(wt - wo) <= 1e-12 * abs(wo)
+
+
+@pytest.mark.parametrize("nx,ny,nz,gt", [(34, 65, 34, 1), (194, 129, 194, 1), (130, 129, 130, 0)])
+def test_plane_statistics_vs_oracle(nx, ny, nz, gt):
+    """Plane-mean profiles of compute_statistics (src/statistics.f90:41-111)
+    on the resident fields equal the oracle's up to summation order: 1e-13
+    of the component's magnitude (the largest entry of its mean square, or
+    its square root for first moments; the plane mean of V of a
+    divergence-free field is zero up to round-off, so its own size is no
+    scale)."""
+    print(f"\n[case] plane statistics {nx}x{ny}x{nz} gt={gt}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=gt)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.1, seed=6, grid_type=gt)
+    g.apply_bc(U, V, W)
+    with Channel(nx, ny, nz, grid_type=gt, nu=NU) as ch:
+        ch.set_fields(U, V, W)
+        st = ch.plane_statistics()
+    ref = g.plane_statistics(U, V, W)
+    sq = {c: np.max(ref[f"{c}2mean"]) for c in "UVW"}
+    scale = {"Umean": np.sqrt(sq["U"]), "Vmean": np.sqrt(sq["V"]), "Wmean": np.sqrt(sq["W"]),
+             "U2mean": sq["U"], "V2mean": sq["V"], "W2mean": sq["W"],
+             "UVmean": np.sqrt(sq["U"] * sq["V"])}
+    for k, r in ref.items():
+        err = np.max(np.abs(st[k] - r)) / scale[k]
+        print(f"  {k}: max err / scale {err:.2e}")
+        assert err <= 1e-13, k

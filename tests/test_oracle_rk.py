@@ -167,3 +167,45 @@ def test_wall_shear_closed_form(gt):
     g.apply_bc(U2, V, W)
     ut2, _, um2, _ = g.wall_shear(U2, W, nu)
     assert np.allclose(um2, um, rtol=0, atol=1e-13) and abs(ut2 - ut) <= 1e-13 * ut
+
+
+def test_plane_statistics_closed_form():
+    """compute_statistics' plane means (src/statistics.f90:41-111) on a case
+    solved by hand: y-only profiles give the profiles back (term_2 the
+    average of the two V faces around a centre, mirrored at the walls), and
+    a whole-period x-cosine of amplitude c added to U leaves Umean and
+    UVmean unchanged and raises U2mean by c**2*cos(pi/nxp)**2/2 (the
+    centred average of cos over faces is cos at the centre times
+    cos(dtheta/2))."""
+    g = orc.Grid(34, 65, 34, grid_type=1)
+    yg, yf, xf = g.get(4), g.get(1), g.get(0)
+    f = 1.5 * yg * (2.0 - yg)
+    v = 0.2 * np.sin(np.pi * yf / 2.0)
+    h = 0.1 * np.cos(yg)
+    U = np.zeros(g.shape_u, order="F")
+    V = np.zeros(g.shape_v, order="F")
+    W = np.zeros(g.shape_w, order="F")
+    U[:] = f[None, :, None]
+    V[:] = v[None, :, None]
+    W[:] = h[None, :, None]
+    g.apply_bc(U, V, W)
+    st = g.plane_statistics(U, V, W)
+    fb = U[2, :, 2]          # profiles after the no-slip ghosts
+    vb = V[2, :, 2]          # V with the wall faces set to zero
+    hb = W[2, :, 2]
+    t2 = np.zeros(g.nyg)
+    t2[1:-1] = 0.5 * (vb[:-1] + vb[1:])
+    t2[0], t2[-1] = -t2[1], -t2[-2]
+    tol = dict(rtol=1e-13, atol=1e-15)
+    assert np.allclose(st["Umean"], fb, **tol) and np.allclose(st["U2mean"], fb ** 2, **tol)
+    assert np.allclose(st["Vmean"], t2, **tol) and np.allclose(st["V2mean"], t2 ** 2, **tol)
+    assert np.allclose(st["UVmean"], fb * t2, **tol)
+    assert np.allclose(st["Wmean"], hb, **tol) and np.allclose(st["W2mean"], hb ** 2, **tol)
+    c = 0.3
+    U2 = U + c * np.cos(2 * np.pi * (xf - xf[0]) / (xf[-2] - xf[0]))[:, None, None]
+    g.apply_bc(U2, V, W)
+    st2 = g.plane_statistics(U2, V, W)
+    add = c ** 2 * np.cos(np.pi / g.nxp) ** 2 / 2.0
+    assert np.allclose(st2["Umean"], st["Umean"], rtol=0, atol=1e-13)
+    assert np.allclose(st2["UVmean"], st["UVmean"], rtol=0, atol=1e-13)
+    assert np.allclose(st2["U2mean"] - st["U2mean"], add, rtol=0, atol=1e-13)
