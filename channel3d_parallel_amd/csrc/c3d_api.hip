@@ -104,6 +104,7 @@ struct c3d_ctx {
   double* CP = nullptr;
   unsigned long long* red = nullptr;
   double* psum = nullptr;  // 7*nyg plane sums of compute_statistics
+  double* mon = nullptr;   // output_monitor partials and results
   size_t nu = 0, nv = 0, nw = 0, np = 0, nr = 0, nspec = 0;
   // RK3 state (allocated on first use): step-start copies, stage
   // right-hand sides F[stage][component], reduction scratch
@@ -797,6 +798,31 @@ static int plane_means(c3d_ctx* c, std::vector<double>& m) {
   for (int q = 0; q < 7; q++)
     for (int j = 0; j < n; j++) m[(size_t)q * n + j] /= (q == 2 || q == 5) ? nw : nuv;
   return 0;
+}
+
+int c3d_monitor(c3d_ctx* c, double* out) {
+  if (!c || !out) return C3D_EINVAL;
+  if (!c->mon) {
+    int rc = dalloc(c, &c->mon, (size_t)c3d::monitor_part_size() + 8);
+    if (rc) return rc;
+  }
+  double* res = c->mon + c3d::monitor_part_size();
+  HIPCHK(c, c3d::launch_monitor(c->g, c->U, c->V, c->W, c->mon, res, c->stream));
+  double r[7];
+  HIPCHK(c, hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  // src/monitor.f90:27-39, 57-59: maxima, and sums over nxm*nym*nzm with
+  // nxm = nx - 1 etc. (src/initialization.f90:79-81), as written
+  const double vol = (double)(c->nx - 1) * (double)(c->ny - 1) * (double)(c->nz - 1);
+  out[0] = r[3];
+  out[1] = r[4];
+  out[2] = r[5];
+  out[3] = r[0] / vol;
+  out[4] = r[1] / vol;
+  out[5] = r[2] / vol;
+  out[6] = r[6];
+  return C3D_OK;
 }
 
 int c3d_plane_statistics(c3d_ctx* c, double* means) {

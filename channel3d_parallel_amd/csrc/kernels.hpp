@@ -114,6 +114,12 @@ hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, con
 int massflow_rows(const DevGrid& g);
 hipError_t launch_massflow(const DevGrid& g, const double* U, const double* H, double* part,
                            double* out2, hipStream_t s);
+// output_monitor field summaries: out7 = sums of U, V, W over their
+// interiors, maxima of the same, and the count of non-finite entries; part
+// holds monitor_part_size() doubles of scratch.
+int monitor_part_size();
+hipError_t launch_monitor(const DevGrid& g, const double* U, const double* V, const double* W,
+                          double* part, double* out7, hipStream_t s);
 // Plane sums of compute_statistics over the unique cells of every row
 // j = 1..nyg: sums[q*nyg + j-1] for q = 0 x-centred U, 1 y-centred V (wall
 // rows mirrored), 2 W, 3 U^2, 4 V^2, 5 W^2, 6 UV (7*nyg doubles).

@@ -375,6 +375,69 @@ __global__ __launch_bounds__(kB) void k_plane_sums(DevGrid g, const double* __re
   if (threadIdx.x < kStats) sums[threadIdx.x * (size_t)nyg + j - 1] = sh[threadIdx.x][0];
 }
 
+// output_monitor's field summaries (src/monitor.f90:27-47): per workgroup
+// the sums and maxima of U(2:nx-1,2:nyg-1,2:nzg-1), V(2:nxg-1,2:ny-1,2:nzg-1)
+// and W(2:nxg-1,2:nyg-1,2:nz-1), and the count of non-finite entries of the
+// three (a NaN guard), into part[b*kMon + q]; k_monitor_final reduces the
+// kMonBlocks partials in a fixed order.
+constexpr int kMon = 7;  // sum U, V, W; max U, V, W; non-finite count
+constexpr int kMonBlocks = 256;
+
+__device__ __forceinline__ void mon_field(const double* __restrict__ F, int n1, int n2, int i1, int j1,
+                                          int k1, double& sum, double& mx, double& bad) {
+  // interior (2..i1, 2..j1, 2..k1) of an (n1, n2, *) array, i fastest
+  const size_t ni = i1 - 1, nj = j1 - 1, n = ni * nj * (size_t)(k1 - 1);
+  for (size_t t = (size_t)blockIdx.x * kB + threadIdx.x; t < n; t += (size_t)kMonBlocks * kB) {
+    const int i = (int)(t % ni) + 2, j = (int)((t / ni) % nj) + 2, k = (int)(t / (ni * nj)) + 2;
+    const double v = F[fidx(i, j, k, n1, n2)];
+    sum += v;
+    mx = fmax(mx, v);
+    bad += isfinite(v) ? 0.0 : 1.0;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_monitor_part(DevGrid g, const double* __restrict__ U,
+                                                     const double* __restrict__ V,
+                                                     const double* __restrict__ W,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kMon][kB];
+  double a[kMon] = {0.0, 0.0, 0.0, -INFINITY, -INFINITY, -INFINITY, 0.0};
+  mon_field(U, g.nx, g.nyg, g.nx - 1, g.nyg - 1, g.nzg - 1, a[0], a[3], a[6]);
+  mon_field(V, g.nxg, g.ny, g.nxg - 1, g.ny - 1, g.nzg - 1, a[1], a[4], a[6]);
+  mon_field(W, g.nxg, g.nyg, g.nxg - 1, g.nyg - 1, g.nz - 1, a[2], a[5], a[6]);
+#pragma unroll
+  for (int q = 0; q < kMon; q++) sh[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < kMon; q++)
+        sh[q][threadIdx.x] = (q >= 3 && q < 6) ? fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + w])
+                                                : sh[q][threadIdx.x] + sh[q][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kMon) part[(size_t)blockIdx.x * kMon + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kB) void k_monitor_final(const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[kMon][kB];
+#pragma unroll
+  for (int q = 0; q < kMon; q++) sh[q][threadIdx.x] = part[(size_t)threadIdx.x * kMon + q];
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < kMon; q++)
+        sh[q][threadIdx.x] = (q >= 3 && q < 6) ? fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + w])
+                                                : sh[q][threadIdx.x] + sh[q][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kMon) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
 inline dim3 gpts(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
 
 }  // namespace
@@ -429,6 +492,16 @@ hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, con
 }
 
 int massflow_rows(const DevGrid& g) { return (g.nyg - 3) * (g.nzg - 2); }
+
+int monitor_part_size() { return kMonBlocks * kMon; }
+
+hipError_t launch_monitor(const DevGrid& g, const double* U, const double* V, const double* W,
+                          double* part, double* out7, hipStream_t s) {
+  static_assert(kMonBlocks == kB, "the final reduction holds one partial per thread");
+  hipLaunchKernelGGL(k_monitor_part, dim3(kMonBlocks), dim3(kB), 0, s, g, U, V, W, part);
+  hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(kB), 0, s, (const double*)part, out7);
+  return hipGetLastError();
+}
 
 hipError_t launch_plane_sums(const DevGrid& g, const double* U, const double* V, const double* W,
                              double* sums, hipStream_t s) {

@@ -111,6 +111,12 @@ Module c3d_bindings
        Real(c_double) :: means(*)
      End Function c3d_plane_statistics
 
+     Integer(c_int) Function c3d_monitor(ctx, out) Bind(C, name="c3d_monitor")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: out(7)
+     End Function c3d_monitor
+
      Integer(c_int) Function c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz) &
           Bind(C, name="c3d_set_pressure_gradient")
        Import :: c_int, c_ptr, c_double

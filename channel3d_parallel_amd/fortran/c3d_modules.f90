@@ -5,7 +5,8 @@
 !   projection              :: compute_non_IB_projection (src/projection.f90:22)!
 !   differential_operators  :: divergence, gradient (src/differential_operators.f90) !
 !   boundary_conditions     :: apply_boundary_conditions (src/boundary_conditions.f90:20) !
-!   monitor                 :: check_divergence, check_slip (src/monitor.f90:210-231) !
+!   monitor                 :: check_divergence, check_slip, monitor_fields     !
+!                              (src/monitor.f90:27-59, 210-231)                 !
 !   statistics              :: compute_wall_shear, compute_plane_statistics     !
 !                              (src/statistics.f90:41-131)                     !
 !   projection              :: compute_IB_projection, schur, bicgstab          !
@@ -238,6 +239,19 @@ Contains
     Real(Int64), Intent(Out) :: max_slip
     Call c3d_check(c3d_check_slip(ctx, max_slip), 'check_slip')
   End Subroutine check_slip
+
+  ! Field summaries of output_monitor, src/monitor.f90:27-39, 57-59 (resident
+  ! velocity): maxima and "Mean U/V/W" as the reference prints them, and the
+  ! number of non-finite entries (a NaN guard for the time loop).
+  Subroutine monitor_fields(maxU, maxV, maxW, meanU, meanV, meanW, nonfinite)
+    Real(Int64), Intent(Out) :: maxU, maxV, maxW, meanU, meanV, meanW
+    Integer(Int64), Intent(Out) :: nonfinite
+    Real(Int64) :: r(7)
+    Call c3d_check(c3d_monitor(ctx, r), 'monitor_fields')
+    maxU = r(1); maxV = r(2); maxW = r(3)
+    meanU = r(4); meanV = r(5); meanW = r(6)
+    nonfinite = Int(r(7), Int64)
+  End Subroutine monitor_fields
 End Module monitor
 
 !------------------------------------------------------------------------------!

@@ -135,6 +135,15 @@ class Channel:
         self._chk(self.L.c3d_check_divergence(self.h, C.byref(v)))
         return v.value
 
+    MONITOR_NAMES = ("maxU", "maxV", "maxW", "meanU", "meanV", "meanW", "nonfinite")
+
+    def monitor(self):
+        """output_monitor's field summaries (src/monitor.f90:27-39, 57-59) of the
+        resident fields, plus the count of non-finite entries."""
+        out = np.zeros(7)
+        self._chk(self.L.c3d_monitor(self.h, ptr(out)))
+        return dict(zip(self.MONITOR_NAMES, out.tolist()))
+
     STAT_NAMES = ("Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean")
 
     def plane_statistics(self):

@@ -111,6 +111,14 @@ int c3d_check_divergence(c3d_ctx *ctx, double *max_div);
  * and wtau likewise (NaN when the argument is negative, as the reference's
  * **0.5d0).  Umean, Wmean [nyg] receive the plane means when not NULL. */
 int c3d_wall_shear(c3d_ctx *ctx, double *utau, double *wtau, double *Umean, double *Wmean);
+/* Field summaries of output_monitor on the resident fields
+ * (src/monitor.f90:27-39, 57-59): out[0..2] = maxval of U, V, W over their
+ * interiors (U(2:nx-1,2:nyg-1,2:nzg-1), V(2:nxg-1,2:ny-1,2:nzg-1),
+ * W(2:nxg-1,2:nyg-1,2:nz-1)); out[3..5] = their sums divided by
+ * (nx-1)*(ny-1)*(nz-1), as the reference prints "Mean U/V/W"; out[6] = the
+ * number of non-finite entries in those interiors (a NaN guard for a host
+ * loop; the reference has none). */
+int c3d_monitor(c3d_ctx *ctx, double *out);
 /* Plane-mean profiles of compute_statistics on the resident fields
  * (src/statistics.f90:41-111, one domain): means[q*nyg + j-1] for q = 0
  * Umean, 1 Vmean, 2 Wmean, 3 U2mean, 4 V2mean, 5 W2mean, 6 UVmean, from the

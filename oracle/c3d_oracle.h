@@ -94,6 +94,10 @@ double orc_compute_dt(const orc_grid *g, const double *U, const double *V, const
                       double cfl, double nu);
 /* H-masked bulk velocity <U> (src/mass_flow.f90:24-53); Hu may be NULL. */
 double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu);
+/* output_monitor field summaries (src/monitor.f90:27-39, 57-59): max U, V,
+ * W; sums / (nx-1)(ny-1)(nz-1); count of non-finite entries. */
+void orc_monitor(const orc_grid *g, const double *U, const double *V, const double *W,
+                 double *out);
 /* Plane means of compute_statistics (src/statistics.f90:41-111):
  * out[q*nyg + j-1], q = 0 U, 1 V, 2 W, 3 U^2, 4 V^2, 5 W^2, 6 UV. */
 void orc_plane_statistics(const orc_grid *g, const double *U, const double *V, const double *W,

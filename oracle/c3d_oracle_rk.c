@@ -240,6 +240,43 @@ double orc_mean_mass_flow_u(const orc_grid *g, const double *U, const double *Hu
   return s / nrm;
 }
 
+/* output_monitor's field summaries (src/monitor.f90:27-39, 57-59) on one
+ * domain: out[0..2] = Maxval of U(2:nx-1,2:nyg-1,2:nzg-1),
+ * V(2:nxg-1,2:ny-1,2:nzg-1), W(2:nxg-1,2:nyg-1,2:nz-1); out[3..5] = their
+ * Sums divided by nxm*nym*nzm (nxm = nx-1 etc., src/initialization.f90:79-81);
+ * out[6] = count of non-finite entries in the three interiors. */
+void orc_monitor(const orc_grid *g, const double *U, const double *V, const double *W,
+                 double *out) {
+  G q;
+  gload(g, &q);
+  double s[3] = {0, 0, 0}, m[3] = {-INFINITY, -INFINITY, -INFINITY}, bad = 0;
+  for (int k = 2; k <= q.nzg - 1; k++)
+    for (int j = 2; j <= q.nyg - 1; j++)
+      for (int i = 2; i <= q.nx - 1; i++) {
+        const double v = Uf(i, j, k);
+        s[0] += v; m[0] = fmax(m[0], v); bad += isfinite(v) ? 0 : 1;
+      }
+  for (int k = 2; k <= q.nzg - 1; k++)
+    for (int j = 2; j <= q.ny - 1; j++)
+      for (int i = 2; i <= q.nxg - 1; i++) {
+        const double v = Vf(i, j, k);
+        s[1] += v; m[1] = fmax(m[1], v); bad += isfinite(v) ? 0 : 1;
+      }
+  for (int k = 2; k <= q.nz - 1; k++)
+    for (int j = 2; j <= q.nyg - 1; j++)
+      for (int i = 2; i <= q.nxg - 1; i++) {
+        const double v = Wf(i, j, k);
+        s[2] += v; m[2] = fmax(m[2], v); bad += isfinite(v) ? 0 : 1;
+      }
+  const double vol = (double)(q.nx - 1) * (double)(q.ny - 1) * (double)(q.nz - 1);
+  for (int c = 0; c < 3; c++) {
+    out[c] = m[c];
+    out[3 + c] = s[c] / vol;
+  }
+  out[6] = bad;
+  gfree(&q);
+}
+
 /* Plane means of compute_statistics on one domain (src/statistics.f90:41-111,
  * last-rank branch :64-73): term_1 = interpolate_x(U) (0.5*(U(i-1) + U(i))
  * at centre i, src/interpolation.f90:21-36), term_2 = interpolate_y(V)

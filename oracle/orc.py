@@ -63,6 +63,8 @@ def lib():
         L.orc_wall_shear.restype = None
         L.orc_plane_statistics.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_plane_statistics.restype = None
+        L.orc_monitor.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_monitor.restype = None
         L.orc_time_step_rk3.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         C.c_double, _dp, C.c_double, C.c_int]
         _lib = L
@@ -213,6 +215,14 @@ class Grid:
 
     def mean_mass_flow_u(self, U, Hu=None) -> float:
         return lib().orc_mean_mass_flow_u(self.h, _p(U), _p(Hu))
+
+    MONITOR_NAMES = ("maxU", "maxV", "maxW", "meanU", "meanV", "meanW", "nonfinite")
+
+    def monitor(self, U, V, W):
+        """output_monitor's field summaries (src/monitor.f90:27-39, 57-59)."""
+        out = np.zeros(7)
+        lib().orc_monitor(self.h, _p(U), _p(V), _p(W), _p(out))
+        return dict(zip(self.MONITOR_NAMES, out.tolist()))
 
     STAT_NAMES = ("Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean")
 

@@ -172,3 +172,31 @@ def test_plane_statistics_vs_oracle(nx, ny, nz, gt):
         err = np.max(np.abs(st[k] - r)) / scale[k]
         print(f"  {k}: max err / scale {err:.2e}")
         assert err <= 1e-13, k
+
+
+@pytest.mark.parametrize("nx,ny,nz,gt", [(34, 65, 34, 1), (194, 129, 194, 1)])
+def test_monitor_vs_oracle(nx, ny, nz, gt):
+    """output_monitor's field summaries (src/monitor.f90:27-39, 57-59) of the
+    resident fields: maxima exactly, means to 1e-13 relative (summation
+    order), and the non-finite count, also after planting NaN/Inf values in
+    the interiors (and one in a ghost cell, which is not counted)."""
+    print(f"\n[case] monitor {nx}x{ny}x{nz} gt={gt}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=gt)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.1, seed=8, grid_type=gt)
+    g.apply_bc(U, V, W)
+    with Channel(nx, ny, nz, grid_type=gt, nu=NU) as ch:
+        ch.set_fields(U, V, W)
+        m = ch.monitor()
+        U2, W2 = U.copy(order="F"), W.copy(order="F")
+        U2[4, 3, 2] = np.nan
+        W2[2, 2, 3] = -np.inf
+        W2[0, 2, 3] = np.nan      # ghost column i = 1
+        ch.set_fields(U2, V, W2)
+        m_bad = ch.monitor()
+    r = g.monitor(U, V, W)
+    print(f"  gpu {m}\n  oracle {r}")
+    for k in ("maxU", "maxV", "maxW", "nonfinite"):
+        assert m[k] == r[k], k
+    for k in ("meanU", "meanV", "meanW"):
+        assert abs(m[k] - r[k]) <= 1e-13 * max(abs(r["meanU"]), 1e-300), k
+    assert m_bad["nonfinite"] == 2.0 == g.monitor(U2, V, W2)["nonfinite"]

@@ -209,3 +209,29 @@ def test_plane_statistics_closed_form():
     assert np.allclose(st2["Umean"], st["Umean"], rtol=0, atol=1e-13)
     assert np.allclose(st2["UVmean"], st["UVmean"], rtol=0, atol=1e-13)
     assert np.allclose(st2["U2mean"] - st["U2mean"], add, rtol=0, atol=1e-13)
+
+
+def test_monitor_closed_form():
+    """output_monitor's field summaries (src/monitor.f90:27-39, 57-59):
+    constant fields give "Mean U" = c*(nx-2)/(nx-1) (the sum over the
+    (nx-2)(ny-1)(nz-1) interior U faces divided by (nx-1)(ny-1)(nz-1), as
+    written), a spike at an interior point is the maximum while one in a
+    ghost cell is not, and NaN/Inf entries in the interiors are counted."""
+    nx, ny, nz = 34, 65, 34
+    g = orc.Grid(nx, ny, nz, grid_type=1)
+    U = np.full(g.shape_u, 2.0, order="F")
+    V = np.full(g.shape_v, -1.0, order="F")
+    W = np.full(g.shape_w, 0.5, order="F")
+    m = g.monitor(U, V, W)
+    assert m["meanU"] == 2.0 * (nx - 2) * (ny - 1) * (nz - 1) / ((nx - 1) * (ny - 1) * (nz - 1))
+    assert m["meanV"] == -1.0 * (nx - 1) * (ny - 2) * (nz - 1) / ((nx - 1) * (ny - 1) * (nz - 1))
+    assert m["meanW"] == 0.5 * (nx - 1) * (ny - 1) * (nz - 2) / ((nx - 1) * (ny - 1) * (nz - 1))
+    assert (m["maxU"], m["maxV"], m["maxW"], m["nonfinite"]) == (2.0, -1.0, 0.5, 0.0)
+    U[0, 5, 5] = 9.0              # ghost face i = 1: outside the interior
+    V[3, 2, 4] = 7.0              # interior V face (i=4, j=3, k=5)
+    m = g.monitor(U, V, W)
+    assert m["maxU"] == 2.0 and m["maxV"] == 7.0
+    U[5, 5, 5] = np.nan
+    W[2, 2, 2] = np.inf
+    W[0, 0, 0] = np.nan           # ghost: not counted
+    assert g.monitor(U, V, W)["nonfinite"] == 2.0
