@@ -769,70 +769,89 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
 // ---------------------------------------------------------------------------
 // Boundary conditions, src/boundary_conditions.f90:20-252, one domain.
 // ---------------------------------------------------------------------------
-__global__ void k_bc_x(DevGrid g, double* __restrict__ U, double* __restrict__ V,
-                       double* __restrict__ W) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
-  if (j > nyg) return;
-  {  // U: x faces
-    const double a = U[fidx(nx - 1, j, k, nx, nyg)], b = U[fidx(2, j, k, nx, nyg)];
-    U[fidx(1, j, k, nx, nyg)] = a;
-    U[fidx(nx, j, k, nx, nyg)] = b;
-  }
-  if (j <= ny) {  // V: x centres
-    const double a = V[fidx(nxg - 2, j, k, nxg, ny)], b = V[fidx(2, j, k, nxg, ny)],
-                 c = V[fidx(3, j, k, nxg, ny)];
-    V[fidx(1, j, k, nxg, ny)] = a;
-    V[fidx(nxg - 1, j, k, nxg, ny)] = b;
-    V[fidx(nxg, j, k, nxg, ny)] = c;
-  }
-  if (k <= nz) {  // W: x centres
-    const double a = W[fidx(nxg - 2, j, k, nxg, nyg)], b = W[fidx(2, j, k, nxg, nyg)],
-                 c = W[fidx(3, j, k, nxg, nyg)];
-    W[fidx(1, j, k, nxg, nyg)] = a;
-    W[fidx(nxg - 1, j, k, nxg, nyg)] = b;
-    W[fidx(nxg, j, k, nxg, nyg)] = c;
-  }
+// All three boundary-condition passes in one launch.  Applied in the
+// reference's order (periodic x, periodic z, walls in y), every boundary
+// value ends as a copy of an interior value: the periodic image in x and z
+// (x faces 1 -> nx-1, nx -> 2; x centres 1 -> nxg-2, nxg-1 -> 2, nxg -> 3;
+// z alike) and, in the y ghost rows of U and W, the negated value of the
+// first interior row; V's wall rows are 0.  Each thread writes the boundary
+// cells of one (j,k), (i,j) or (i,k) line from those interior values only,
+// so no read depends on another write and cells shared by two lines get the
+// same value: the result equals the three passes run in order.
+__device__ __forceinline__ int img_face(int i, int n) { return i == 1 ? n - 1 : (i == n ? 2 : i); }
+__device__ __forceinline__ int img_ctr(int i, int ng) {
+  return i == 1 ? ng - 2 : (i == ng - 1 ? 2 : (i == ng ? 3 : i));
 }
-__global__ void k_bc_z(DevGrid g, double* __restrict__ U, double* __restrict__ V,
-                       double* __restrict__ W) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+struct BcMap {
+  int nx, ny, nz, nxg, nyg, nzg;
+  __device__ double u(const double* U, int i, int j, int k) const {  // U(nx, nyg, nzg)
+    const int jj = j == 1 ? 2 : (j == nyg ? nyg - 1 : j);
+    const double v = U[fidx(img_face(i, nx), jj, img_ctr(k, nzg), nx, nyg)];
+    return jj == j ? v : -v;
+  }
+  __device__ double v(const double* V, int i, int j, int k) const {  // V(nxg, ny, nzg)
+    if (j == 1 || j == ny) return 0.0;
+    return V[fidx(img_ctr(i, nxg), j, img_ctr(k, nzg), nxg, ny)];
+  }
+  __device__ double w(const double* W, int i, int j, int k) const {  // W(nxg, nyg, nz)
+    const int jj = j == 1 ? 2 : (j == nyg ? nyg - 1 : j);
+    const double v = W[fidx(img_ctr(i, nxg), jj, img_face(k, nz), nxg, nyg)];
+    return jj == j ? v : -v;
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void k_bc_all(DevGrid g, double* __restrict__ U,
+                                                   double* __restrict__ V, double* __restrict__ W) {
+  const BcMap m{g.nx, g.ny, g.nz, g.nxg, g.nyg, g.nzg};
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
-  if (i > nxg) return;
-  if (i <= nx) {  // U: z centres
-    const double a = U[fidx(i, j, nzg - 2, nx, nyg)], b = U[fidx(i, j, 2, nx, nyg)],
-                 c = U[fidx(i, j, 3, nx, nyg)];
-    U[fidx(i, j, 1, nx, nyg)] = a;
-    U[fidx(i, j, nzg - 1, nx, nyg)] = b;
-    U[fidx(i, j, nzg, nx, nyg)] = c;
+  const long long nX = (long long)nyg * nzg, nZ = (long long)nxg * nyg, nY = (long long)nxg * nzg;
+  long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t < nX) {  // x lines: (j, k)
+    const int j = (int)(t % nyg) + 1, k = (int)(t / nyg) + 1;
+    U[fidx(1, j, k, nx, nyg)] = m.u(U, 1, j, k);
+    U[fidx(nx, j, k, nx, nyg)] = m.u(U, nx, j, k);
+    if (j <= ny) {
+      V[fidx(1, j, k, nxg, ny)] = m.v(V, 1, j, k);
+      V[fidx(nxg - 1, j, k, nxg, ny)] = m.v(V, nxg - 1, j, k);
+      V[fidx(nxg, j, k, nxg, ny)] = m.v(V, nxg, j, k);
+    }
+    if (k <= nz) {
+      W[fidx(1, j, k, nxg, nyg)] = m.w(W, 1, j, k);
+      W[fidx(nxg - 1, j, k, nxg, nyg)] = m.w(W, nxg - 1, j, k);
+      W[fidx(nxg, j, k, nxg, nyg)] = m.w(W, nxg, j, k);
+    }
+    return;
   }
-  if (j <= ny) {  // V: z centres
-    const double a = V[fidx(i, j, nzg - 2, nxg, ny)], b = V[fidx(i, j, 2, nxg, ny)],
-                 c = V[fidx(i, j, 3, nxg, ny)];
-    V[fidx(i, j, 1, nxg, ny)] = a;
-    V[fidx(i, j, nzg - 1, nxg, ny)] = b;
-    V[fidx(i, j, nzg, nxg, ny)] = c;
+  t -= nX;
+  if (t < nZ) {  // z lines: (i, j)
+    const int i = (int)(t % nxg) + 1, j = (int)(t / nxg) + 1;
+    if (i <= nx) {
+      U[fidx(i, j, 1, nx, nyg)] = m.u(U, i, j, 1);
+      U[fidx(i, j, nzg - 1, nx, nyg)] = m.u(U, i, j, nzg - 1);
+      U[fidx(i, j, nzg, nx, nyg)] = m.u(U, i, j, nzg);
+    }
+    if (j <= ny) {
+      V[fidx(i, j, 1, nxg, ny)] = m.v(V, i, j, 1);
+      V[fidx(i, j, nzg - 1, nxg, ny)] = m.v(V, i, j, nzg - 1);
+      V[fidx(i, j, nzg, nxg, ny)] = m.v(V, i, j, nzg);
+    }
+    W[fidx(i, j, 1, nxg, nyg)] = m.w(W, i, j, 1);
+    W[fidx(i, j, nz, nxg, nyg)] = m.w(W, i, j, nz);
+    return;
   }
-  {  // W: z faces
-    const double a = W[fidx(i, j, nz - 1, nxg, nyg)], b = W[fidx(i, j, 2, nxg, nyg)];
-    W[fidx(i, j, 1, nxg, nyg)] = a;
-    W[fidx(i, j, nz, nxg, nyg)] = b;
-  }
-}
-__global__ void k_bc_y(DevGrid g, double* __restrict__ U, double* __restrict__ V,
-                       double* __restrict__ W) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
-  if (i > nxg) return;
-  if (i <= nx) {  // U: y centres
-    U[fidx(i, 1, k, nx, nyg)] = -U[fidx(i, 2, k, nx, nyg)];
-    U[fidx(i, nyg, k, nx, nyg)] = -U[fidx(i, nyg - 1, k, nx, nyg)];
-  }
-  V[fidx(i, 1, k, nxg, ny)] = 0.0;  // V: y faces
-  V[fidx(i, ny, k, nxg, ny)] = 0.0;
-  if (k <= nz) {  // W: y centres
-    W[fidx(i, 1, k, nxg, nyg)] = -W[fidx(i, 2, k, nxg, nyg)];
-    W[fidx(i, nyg, k, nxg, nyg)] = -W[fidx(i, nyg - 1, k, nxg, nyg)];
+  t -= nZ;
+  if (t < nY) {  // y lines: (i, k)
+    const int i = (int)(t % nxg) + 1, k = (int)(t / nxg) + 1;
+    if (i <= nx) {
+      U[fidx(i, 1, k, nx, nyg)] = m.u(U, i, 1, k);
+      U[fidx(i, nyg, k, nx, nyg)] = m.u(U, i, nyg, k);
+    }
+    V[fidx(i, 1, k, nxg, ny)] = 0.0;
+    V[fidx(i, ny, k, nxg, ny)] = 0.0;
+    if (k <= nz) {
+      W[fidx(i, 1, k, nxg, nyg)] = m.w(W, i, 1, k);
+      W[fidx(i, nyg, k, nxg, nyg)] = m.w(W, i, nyg, k);
+    }
   }
 }
 
@@ -1108,11 +1127,12 @@ hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W
 }
 
 hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s) {
-  hipLaunchKernelGGL(k_bc_x, dim3((g.nyg + 63) / 64, g.nzg), dim3(64), 0, s, g, U, V, W);
-  hipLaunchKernelGGL(k_bc_z, dim3((g.nxg + 63) / 64, g.nyg), dim3(64), 0, s, g, U, V, W);
-  hipLaunchKernelGGL(k_bc_y, dim3((g.nxg + 63) / 64, g.nzg), dim3(64), 0, s, g, U, V, W);
+  const long long n = (long long)g.nyg * g.nzg + (long long)g.nxg * g.nyg + (long long)g.nxg * g.nzg;
+  hipLaunchKernelGGL(k_bc_all, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, g, U,
+                     V, W);
   return hipGetLastError();
 }
+
 
 hipError_t launch_divergence(const DevGrid& g, const double* U, const double* V, const double* W,
                              double* div, hipStream_t s) {

@@ -48,8 +48,11 @@ def test_grid_bitwise(nx, ny, nz, gt):
             assert np.array_equal(ch.grid(what), g.get(what)), what
 
 
-def test_boundary_conditions_bitwise():
-    nx, ny, nz = 34, 33, 18
+@pytest.mark.parametrize("nx,ny,nz", [(34, 33, 18), (18, 5, 34), (66, 17, 34), (130, 129, 130)])
+def test_boundary_conditions_bitwise(nx, ny, nz):
+    """apply_boundary_conditions on every cell (the fields start as noise in
+    the ghost cells too) against the oracle's ordered x, z, y passes."""
+    print(f"\n[case] boundary conditions {nx}x{ny}x{nz}", flush=True)
     g = orc.Grid(nx, ny, nz)
     U, V, W = fields.white_faces(nx, ny, nz, seed=11)
     with Channel(nx, ny, nz) as ch:
