@@ -6,8 +6,8 @@ two-cylinder run with seeded rotation speeds, the features the network sees
 (per marker and 3x3 neighbourhood: the Schur residual of the previous force,
 b - S f_prev, and that force) and the converged force increment.
 The network is trained on the GPU by the library's own kernels
-(c3d_mlp_train).  Plain BiCGSTAB from a zero guess stays the judge: every
-variant is solved to the same absolute tolerance.
+(c3d_mlp_train).  Plain BiCGSTAB from a zero guess stays the reference:
+every variant is solved to the same absolute tolerance.
 """
 from __future__ import annotations
 

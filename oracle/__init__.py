@@ -1,7 +1,7 @@
 """CPU FP64 oracle for the channel3D projection path (test infrastructure).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` use this package, as the checker.  The product package
+``bench.py`` use this package, to test and time against.  The product package
 ``channel3d_parallel_amd`` never imports it.
 
 The C library ``liborc.so`` (built by ``oracle/Makefile``) restates the

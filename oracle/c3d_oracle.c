@@ -3,8 +3,9 @@
  * projection path (grid, BCs, staggered div/grad, FFT-xz + tridiagonal-y
  * Poisson solve, non-IB projection).
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
- * baseline timed beside it.  Never linked into the product library.
+ * TEST INFRASTRUCTURE ONLY: the reference against which the HIP path is
+ * tested, and the CPU baseline timed beside it.  Never linked into the
+ * product library.
  *
  * Every routine cites the reference lines it restates (paths relative to the
  * reference repository root).  Index conventions are documented in

@@ -1,8 +1,9 @@
 /*
  * c3d_oracle.h -- CPU FP64 restatement of the channel3D projection path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
- * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * TEST INFRASTRUCTURE ONLY.  The HIP path is tested against this library; it
+ * is loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg
+ * of bench.py,
  * never by the product library (channel3d_parallel_amd/), which must fail
  * loudly when its HIP build is missing instead of falling back to this code.
  *
