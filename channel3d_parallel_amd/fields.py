@@ -153,7 +153,7 @@ def spectral_faces(nx, ny, nz, seed=3, slope=-11.0 / 6.0):
     ncell = nxp * nym * nzp
     for c in range(3):
         white = normal(seed, ncell, offset=c * ncell).reshape((nxp, nym, nzp), order="F")
-        f = np.fft.irfftn(np.fft.rfftn(white) * amp, s=(nxp, nym, nzp))
+        f = np.fft.irfftn(np.fft.rfftn(white) * amp, s=(nxp, nym, nzp), axes=(0, 1, 2))
         f /= np.sqrt(np.mean(f * f))
         out.append(f)
         del white

@@ -1,4 +1,4 @@
-"""Writes ib_c4_projection.json: summary of one IB projection of the
+"""Writes ib_twocyl130_projection.json: summary of one IB projection of the
 two-cylinder body on the 130x129x130 uniform grid, computed by the oracle.
 
 Case: laminar parabola (U_b = 1, no perturbation), boundary conditions,
@@ -9,7 +9,7 @@ divergence metrics, block norms of the force and norms and weighted sums of
 the projected velocity (weights: SplitMix64 uniforms, seed 77), enough to
 check a GPU run without storing fields.
 
-Run from the repository root: python tests/golden/make_ib_c4.py
+Run from the repository root: python tests/golden/make_ib_twocyl130.py
 """
 import json
 import os
@@ -55,7 +55,7 @@ def main():
                max_slip=ib.max_slip(U, V, W, fb), max_div=g.max_abs_div(U, V, W),
                oracle_seconds=time.time() - t)
     res.update(summarize(U, V, W, fb, m.nb))
-    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "ib_c4_projection.json"),
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "ib_twocyl130_projection.json"),
               "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res, indent=1))

@@ -20,7 +20,7 @@ CHANNEL_FD = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "cha
 
 def test_fortran_rk3_time_loop(tmp_path):
     """The Fortran time loop (compute_dt + compute_time_step_RK3 with
-    constant mass flow, as app/main.f90 runs it) on config 1 for 5 steps."""
+    constant mass flow, as app/main.f90 runs it) on the 32x64x32-cell channel for 5 steps."""
     if not os.path.exists(CHANNEL_FD):
         pytest.fail("Fortran channel_fd not built (make -C channel3d_parallel_amd/fortran)")
     nx, ny, nz, nsteps = 34, 65, 34, 5

@@ -188,15 +188,15 @@ def test_time_step_with_body():
         assert rel(a, b) < 1e-8
 
 
-def test_config4_projection_golden():
+def test_twocyl130_projection_golden():
     """Two cylinders, 96x128 markers each (nb = 24,576), 130x129x130 uniform
     grid: one IB projection from a zero guess at tolerance 1e-10 against the
-    oracle's summary (tests/golden/make_ib_c4.py)."""
+    oracle's summary (tests/golden/make_ib_twocyl130.py)."""
     import json
     import os
-    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ib_c4_projection.json")))
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ib_twocyl130_projection.json")))
     nx, ny, nz = ref["nx"], ref["ny"], ref["nz"]
-    print(f"\n[case] IB config 4 {nx}x{ny}x{nz} nb={ref['nb']}", flush=True)
+    print(f"\n[case] IB two cylinders {nx}x{ny}x{nz} nb={ref['nb']}", flush=True)
     g = orc.Grid(nx, ny, nz, grid_type=0)
     m = ib_geometry.two_cylinders(g.scalars["Lzp"])
     assert m.nb == ref["nb"]

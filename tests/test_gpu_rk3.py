@@ -4,7 +4,7 @@ full time step) against the oracle.
 Tolerances: right-hand sides and stage updates are bitwise (same operations
 in the same order, FMA contraction off); the bulk-flow reduction sums in a
 different order (relative 1e-14); full time steps include the projection
-(relative L2 1e-12 after one step, 1e-10 after 100 steps of config 1).
+(relative L2 1e-12 after one step, 1e-10 after 100 steps of the 32x64x32-cell channel).
 """
 import numpy as np
 import pytest
@@ -90,8 +90,8 @@ def test_compute_dt_and_mass_flow():
 
 
 @pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (100, 1e-10)])
-def test_channel_config1_time_steps(nsteps, tol):
-    """Config 1: 32x64x32 cells (34x65x34 faces), tanh alpha 2, Re_b 2800,
+def test_channel_32x64x32_time_steps(nsteps, tol):
+    """32x64x32 cells (34x65x34 faces), tanh alpha 2, Re_b 2800,
     laminar parabola + 10% divergence-free perturbation, constant flow rate,
     dt from compute_dt at CFL 0.5."""
     nx, ny, nz = 34, 65, 34
