@@ -25,7 +25,7 @@ class c3d_desc(C.Structure):
     _fields_ = [
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("grid_type", C.c_int),
         ("Lxp", C.c_double), ("Lzp", C.c_double), ("Ly", C.c_double), ("alpha", C.c_double),
-        ("nu", C.c_double), ("precision", C.c_int),
+        ("nu", C.c_double), ("precision", C.c_int), ("min_buffer_width", C.c_double),
     ]
 
 

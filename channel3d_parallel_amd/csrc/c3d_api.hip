@@ -95,6 +95,7 @@ struct c3d_ctx {
   // host grid (Fortran index i stored at [i-1])
   std::vector<double> x, y, z, xg, yg, zg, kxx, kzz, dyy_d, dyy_l, dyy_u;
   double dx = 0, dz = 0, dxmin = 0, dymin = 0, dzmin = 0, Lx = 0, Ly = 0, Lz = 0, Lxp = 0, Lzp = 0;
+  int n_uniform = 1;  // uniform buffer points per wall (grid_type 2)
   // device state
   hipStream_t stream = nullptr;
   DevGrid g{};
@@ -210,9 +211,31 @@ std::vector<cpx<real>> twiddles(int n) {
   return t;
 }
 
-// Grid generation, src/input_output.f90:140-206 (types 0, 1) and the
-// derived quantities of src/initialization.f90:166-390.
-void build_grid(c3d_ctx* c) {
+// src/input_output.f90:208-247: n_total points, the middle
+// n_total - 2(n_uniform - 1) of them tanh-clustered on [-1, 1], scaled so
+// that the clustered part and two half buffers span L; the n_uniform - 1
+// points beyond each end continue at the smallest spacing.
+void stretched_with_buffers(std::vector<double>& y, double L, int n_uniform, double alpha) {
+  const int n = (int)y.size(), m = n - 2 * (n_uniform - 1);
+  std::vector<double> s(m);
+  for (int i = 0; i < m; i++) s[i] = (double)i;
+  const double top = s[m - 1];
+  for (int i = 0; i < m; i++) s[i] = 2.0 * s[i] / top - 1.0;
+  if (alpha > 0.0)
+    for (int i = 0; i < m; i++) s[i] = std::tanh(alpha * s[i]) / std::tanh(alpha);
+  const double h_unit = s[1] - s[0];
+  const double h = L * h_unit / (2.0 + (double)(n_uniform - 1) * h_unit);
+  for (int i = 0; i < m; i++) y[n_uniform - 1 + i] = s[i] * h / h_unit;
+  for (int b = 1; b < n_uniform; b++) {
+    y[n - n_uniform + b] = y[n - n_uniform + b - 1] + h;
+    y[n_uniform - 1 - b] = y[n_uniform - b] - h;
+  }
+}
+
+// Grid generation, src/input_output.f90:140-206 and the derived quantities
+// of src/initialization.f90:166-390.  Returns false when the type-2 buffers
+// do not fit in ny points.
+bool build_grid(c3d_ctx* c) {
   const c3d_desc& d = c->desc;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   c->x.resize(nx);
@@ -222,6 +245,18 @@ void build_grid(c3d_ctx* c) {
   for (int i = 0; i < nz; i++) c->z[i] = d.Lzp * (double)i / (double)(nz - 2);
   if (d.grid_type == 0) {
     for (int i = 0; i < ny; i++) c->y[i] = d.Ly * (double)i / (double)(ny - 1);
+  } else if (d.grid_type == 2) {
+    // :180-203: grow the buffers one point at a time until they hold the
+    // body amplitude plus the 2*suppy + 2 = 6 cells a delta stencil needs
+    int nbuf = 1;
+    for (;;) {
+      stretched_with_buffers(c->y, d.Ly, nbuf, d.alpha);
+      const double h = c->y[1] - c->y[0];
+      if ((double)nbuf * h >= d.min_buffer_width + 6 * h) break;
+      if (2 * ++nbuf >= ny) return false;
+    }
+    c->n_uniform = nbuf;
+    for (int i = 0; i < ny; i++) c->y[i] += 1.0;
   } else {
     for (int i = 0; i < ny; i++) c->y[i] = d.Ly * (double)i / (double)(ny - 1) - 1.0;
     if (d.alpha > 0.0)
@@ -272,6 +307,7 @@ void build_grid(c3d_ctx* c) {
       c->dyy_l[jj - 1] = cc;
     }
   }
+  return true;
 }
 
 // Fortran-indexed spacing table: t[i] = a(i) - a(i-1) for i = 2..n (t[0], t[1] = 0).
@@ -319,7 +355,8 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   if (d.nx < 6 || d.nz < 6 || d.ny < 4 || (d.nx % 2) || (d.nz % 2)) return C3D_EINVAL;
   if (!c3d::fft_length_supported(d.nx - 2) || !c3d::fft_length_supported(d.nz - 2))
     return C3D_EINVAL;
-  if (d.grid_type != 0 && d.grid_type != 1) return C3D_EINVAL;
+  if (d.grid_type < 0 || d.grid_type > 2) return C3D_EINVAL;
+  if (d.grid_type == 2 && !(d.min_buffer_width >= 0.0)) return C3D_EINVAL;
   if (d.precision != C3D_PREC_FP64 && d.precision != C3D_PREC_MIXED) return C3D_EINVAL;
   if (d.precision == C3D_PREC_MIXED && d.ny - 1 != 64 && d.ny - 1 != 128 && d.ny - 1 != 192)
     return C3D_EINVAL;  // FP32-storage tridiagonal lengths
@@ -330,7 +367,10 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   c->nx = d.nx; c->ny = d.ny; c->nz = d.nz;
   c->nxg = d.nx + 1; c->nyg = d.ny + 1; c->nzg = d.nz + 1;
   c->nxp = d.nx - 2; c->nzp = d.nz - 2; c->nym = d.ny - 1; c->nxh = c->nxp / 2 + 1;
-  build_grid(c);
+  if (!build_grid(c)) {  // type-2 buffers exceed the grid (src/input_output.f90:199)
+    delete c;
+    return C3D_EINVAL;
+  }
 
   auto cleanup = [&](int rc) {
     c3d_destroy(c);
@@ -440,6 +480,10 @@ int c3d_grid_get(const c3d_ctx* c, int what, double* out, int n) {
     case 10: v = &c->dyy_u; break;
     case 11:
       s = {c->dx, c->dz, c->dxmin, c->dymin, c->dzmin, c->Lx, c->Ly, c->Lz, c->Lxp, c->Lzp};
+      v = &s;
+      break;
+    case 12:
+      s = {(double)c->n_uniform};
       v = &s;
       break;
     default: return C3D_EINVAL;

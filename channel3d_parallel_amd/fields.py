@@ -62,8 +62,36 @@ def white_faces(nx, ny, nz, seed=2):
     return U, V, W
 
 
-def grid_faces(nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1):
-    """Face coordinates of create_grid (src/input_output.f90:140-178)."""
+def _buffered_y(ny, Ly, alpha, min_buffer_width):
+    """grid_type 2 (src/input_output.f90:180-247): tanh-clustered interior
+    with n_uniform - 1 extra points of the smallest spacing beyond each end,
+    n_uniform the first count whose buffer holds min_buffer_width + 6 dymin,
+    shifted by +1."""
+    nbuf = 1
+    while True:
+        m = ny - 2 * (nbuf - 1)
+        s = np.arange(m, dtype=np.float64)
+        s = 2.0 * s / s[-1] - 1.0
+        if alpha > 0:
+            s = np.tanh(alpha * s) / np.tanh(alpha)
+        h_unit = s[1] - s[0]
+        h = Ly * h_unit / (2.0 + (nbuf - 1) * h_unit)
+        y = np.zeros(ny)
+        y[nbuf - 1:nbuf - 1 + m] = s * h / h_unit
+        for b in range(1, nbuf):
+            y[ny - nbuf + b] = y[ny - nbuf + b - 1] + h
+            y[nbuf - 1 - b] = y[nbuf - b] - h
+        h1 = y[1] - y[0]
+        if nbuf * h1 >= min_buffer_width + 6 * h1:
+            return y + 1.0
+        nbuf += 1
+        if 2 * nbuf >= ny:
+            raise ValueError("uniform buffers exceed the grid")
+
+
+def grid_faces(nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1,
+               min_buffer_width=0.0):
+    """Face coordinates of create_grid (src/input_output.f90:140-247)."""
     x = Lxp * np.arange(nx, dtype=np.float64) / (nx - 2)
     z = Lzp * np.arange(nz, dtype=np.float64) / (nz - 2)
     y = Ly * np.arange(ny, dtype=np.float64) / (ny - 1)
@@ -73,13 +101,17 @@ def grid_faces(nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_typ
             y = np.tanh(alpha * y) / np.tanh(alpha)
         y = y - y.min()
         y = y * Ly / y.max()
+    elif grid_type == 2:
+        y = _buffered_y(ny, Ly, alpha, min_buffer_width)
     return x, y, z
 
 
 def channel_ic(nx, ny, nz, Ub=1.0, pert=0.1, seed=1, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0,
-               grid_type=1):
+               grid_type=1, min_buffer_width=0.0):
     """Laminar parabola with bulk velocity Ub plus a divergence-free
-    perturbation of RMS pert*Ub.
+    perturbation of RMS pert*Ub.  On the buffered grid (grid_type 2) the
+    parabola spans the channel 0 < y < Ly and the buffers beyond the walls
+    start at rest, as init_flow sets them (src/input_output.f90:274-280).
 
     The perturbation is the discrete curl of a vector potential A whose
     components live on cell edges (Ax on (x-centre, y-face, z-face), Ay on
@@ -88,7 +120,7 @@ def channel_ic(nx, ny, nz, Ub=1.0, pert=0.1, seed=1, Lxp=2 * np.pi, Lzp=np.pi, L
     the walls.  The curl uses the face spacings that the divergence uses, so
     the discrete divergence of the perturbation telescopes to round-off.
     Boundary conditions are not applied."""
-    x, y, z = grid_faces(nx, ny, nz, Lxp, Lzp, Ly, alpha, grid_type)
+    x, y, z = grid_faces(nx, ny, nz, Lxp, Lzp, Ly, alpha, grid_type, min_buffer_width)
     nxg, nyg, nzg = nx + 1, ny + 1, nz + 1
     nxp, nzp = nx - 2, nz - 2
     hx = np.zeros(nxg + 1)
@@ -130,6 +162,7 @@ def channel_ic(nx, ny, nz, Ub=1.0, pert=0.1, seed=1, Lxp=2 * np.pi, Lzp=np.pi, L
     W = np.zeros((nxg, nyg, nz), order="F")
     ym = 0.5 * (y[:-1] + y[1:])  # centres yg(2..ny)
     lam = 1.5 * Ub * ym * (Ly - ym) / (0.25 * Ly * Ly)  # parabola with bulk Ub
+    lam = np.where((ym > 0.0) & (ym < Ly), lam, 0.0)
     U[1:nxp + 1, 1:ny, 1:nzp + 1] = lam[None, :, None] + s * Up
     V[1:nxp + 1, :, 1:nzp + 1] = s * Vp
     W[1:nxp + 1, 1:ny, 1:nzp + 1] = s * Wp

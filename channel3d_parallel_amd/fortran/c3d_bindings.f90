@@ -17,6 +17,7 @@ Module c3d_bindings
      Integer(c_int) :: nx, ny, nz, grid_type
      Real(c_double) :: Lxp, Lzp, Ly, alpha, nu
      Integer(c_int) :: precision
+     Real(c_double) :: min_buffer_width
   End Type c3d_desc
 
   Interface

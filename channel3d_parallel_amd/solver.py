@@ -44,9 +44,10 @@ class Channel:
     """One GPU-resident channel state (the reference's module-global state)."""
 
     def __init__(self, nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1,
-                 nu=1.0, precision=_lib.PREC_FP64):
+                 nu=1.0, precision=_lib.PREC_FP64, min_buffer_width=0.0):
         self.L = _lib.load()
-        d = _lib.c3d_desc(nx, ny, nz, grid_type, Lxp, Lzp, Ly, alpha, nu, precision)
+        d = _lib.c3d_desc(nx, ny, nz, grid_type, Lxp, Lzp, Ly, alpha, nu, precision,
+                          min_buffer_width)
         h = C.c_void_p()
         rc = self.L.c3d_create(C.byref(d), C.byref(h))
         if rc != 0:

@@ -47,13 +47,16 @@ typedef struct c3d_ctx c3d_ctx;
  * src/input_output.f90:27-40 that the projection path reads. */
 typedef struct c3d_desc {
   int nx, ny, nz;   /* nx_global, ny_global, nz_global (face counts, even nx,nz) */
-  int grid_type;    /* 0 uniform y, 1 tanh-stretched y (src/input_output.f90:156-178) */
+  int grid_type;    /* 0 uniform y, 1 tanh-stretched y, 2 tanh-stretched y with
+                       uniform wall buffers (src/input_output.f90:156-247) */
   double Lxp;       /* periodic length in x (default 2*pi) */
   double Lzp;       /* periodic length in z (default pi) */
   double Ly;        /* Ly_channel (default 2) */
   double alpha;     /* alpha_stretch */
   double nu;        /* kinematic viscosity */
   int precision;    /* C3D_PREC_FP64 or C3D_PREC_MIXED */
+  double min_buffer_width; /* grid_type 2: each buffer holds at least this
+                              plus 6 dymin (src/input_output.f90:195) */
 } c3d_desc;
 
 /* Creates a context on the current HIP device: builds the grid, spacing
@@ -66,7 +69,8 @@ const char *c3d_last_error(const c3d_ctx *ctx);
 
 /* Copies a grid vector to host memory (n = capacity in doubles).
  * what: 0 x 1 y 2 z 3 xg 4 yg 5 zg 6 kxx 7 kzz 8 Dyy diag 9 Dyy lower
- *       10 Dyy upper 11 scalars{dx,dz,dxmin,dymin,dzmin,Lx,Ly,Lz,Lxp,Lzp}.
+ *       10 Dyy upper 11 scalars{dx,dz,dxmin,dymin,dzmin,Lx,Ly,Lz,Lxp,Lzp}
+ *       12 {n_uniform} (buffer points per wall, grid_type 2; else 1).
  * Returns the number of values written (>= 0) or an error code. */
 int c3d_grid_get(const c3d_ctx *ctx, int what, double *out, int n);
 

@@ -90,13 +90,37 @@ static void modified_wavenumbers(int n, double d, double *k) {
     k[i] = 2.0 * (cos(2.0 * pi * (double)(-n + i) / (double)n) - 1.0) / (d * d);
 }
 
+/* create_stretched_grid, src/input_output.f90:208-247: tanh points on
+ * [-1, 1] over the middle n_total - 2(n_uniform-1) points, scaled so that the
+ * stretched part plus two half buffers span L, extended by n_uniform - 1
+ * points of the smallest spacing at each end. */
+static void stretched_with_buffers(double *grid, double L, int n_total, int n_uniform,
+                                   double alpha) {
+  const int nun = n_total - 2 * (n_uniform - 1);
+  double *gu = malloc(sizeof(double) * nun);
+  for (int i = 0; i < nun; i++) gu[i] = (double)i;
+  double mx = gu[0];
+  for (int i = 1; i < nun; i++) if (gu[i] > mx) mx = gu[i];
+  for (int i = 0; i < nun; i++) gu[i] = 2.0 * gu[i] / mx - 1.0;
+  if (alpha > 0.0)
+    for (int i = 0; i < nun; i++) gu[i] = tanh(alpha * gu[i]) / tanh(alpha);
+  const double ds_un = gu[1] - gu[0];
+  const double ds_sc = L * ds_un / (2.0 + (double)(n_uniform - 1) * ds_un);
+  for (int i = 0; i < nun; i++) grid[n_uniform - 1 + i] = gu[i] * ds_sc / ds_un;
+  for (int i = 2; i <= n_uniform; i++) {
+    grid[n_total - n_uniform + i - 1] = grid[n_total - n_uniform + i - 2] + ds_sc;
+    grid[n_uniform - i] = grid[n_uniform - i + 1] - ds_sc;
+  }
+  free(gu);
+}
+
 orc_grid *orc_grid_new(int nx, int ny, int nz, double Lxp, double Lzp,
-                       double Ly, double alpha, int grid_type) {
+                       double Ly, double alpha, int grid_type, double min_buffer_width) {
   /* src/initialization.f90:46-47 (even sizes), one domain */
   if (nx < 6 || ny < 4 || nz < 6 || (nx % 2) || (nz % 2)) return NULL;
-  if (grid_type != 0 && grid_type != 1) return NULL;
+  if (grid_type < 0 || grid_type > 2) return NULL;
   orc_grid *g = (orc_grid *)calloc(1, sizeof(orc_grid));
-  g->nx = nx; g->ny = ny; g->nz = nz;
+  g->nx = nx; g->ny = ny; g->nz = nz; g->n_uniform = 1;
   g->nxg = nx + 1; g->nyg = ny + 1; g->nzg = nz + 1;
   g->nxp = nx - 2; g->nzp = nz - 2; g->nym = ny - 1;
   g->x = calloc(nx, sizeof(double)); g->y = calloc(ny, sizeof(double));
@@ -113,6 +137,17 @@ orc_grid *orc_grid_new(int nx, int ny, int nz, double Lxp, double Lzp,
   for (int i = 0; i < nz; i++) g->z[i] = Lzp * (double)i / (double)(nz - 2);
   if (grid_type == 0) { /* :157-162 */
     for (int i = 0; i < ny; i++) g->y[i] = Ly * (double)i / (double)(ny - 1);
+  } else if (grid_type == 2) { /* :180-203, suppy = 2 */
+    int nu = 1;
+    for (;;) {
+      stretched_with_buffers(g->y, Ly, ny, nu, alpha);
+      const double dy1 = g->y[1] - g->y[0];
+      if ((double)nu * dy1 >= min_buffer_width + (2 * 2 + 2) * dy1) break;
+      nu++;
+      if (2 * nu >= ny) { orc_grid_free(g); return NULL; }
+    }
+    g->n_uniform = nu;
+    for (int i = 0; i < ny; i++) g->y[i] = g->y[i] + 1.0;
   } else { /* :164-178 */
     for (int i = 0; i < ny; i++) g->y[i] = Ly * (double)i / (double)(ny - 1) - 1.0;
     if (alpha > 0.0)
@@ -192,6 +227,7 @@ int orc_grid_get(const orc_grid *g, int what, double *out) {
     memcpy(out, s, sizeof(s));
     return 10;
   }
+  case 12: out[0] = (double)g->n_uniform; return 1;
   default: return -1;
   }
 }

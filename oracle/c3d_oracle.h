@@ -24,16 +24,19 @@ extern "C" {
 
 typedef struct orc_grid orc_grid;
 
-/* Grid and solver constants (src/input_output.f90:140-206,
+/* Grid and solver constants (src/input_output.f90:140-247,
  * src/initialization.f90:166-305,346-390). grid_type 0 = uniform y,
- * 1 = tanh-stretched y. Returns NULL on invalid sizes. */
+ * 1 = tanh-stretched y, 2 = tanh-stretched y with uniform buffers of at
+ * least min_buffer_width + 6 dymin at each wall, shifted by +1 (only read
+ * for grid_type 2). Returns NULL on invalid sizes. */
 orc_grid *orc_grid_new(int nx, int ny, int nz, double Lxp, double Lzp,
-                       double Ly, double alpha, int grid_type);
+                       double Ly, double alpha, int grid_type, double min_buffer_width);
 void orc_grid_free(orc_grid *g);
 /* what: 0 x[nx] 1 y[ny] 2 z[nz] 3 xg[nxg] 4 yg[nyg] 5 zg[nzg]
  *       6 kxx[nxp] 7 kzz[nzp] 8 Dyy diag[nym] 9 Dyy lower[nym-1]
  *       10 Dyy upper[nym-1]
  *       11 scalars[10] = dx dz dxmin dymin dzmin Lx Ly Lz Lxp Lzp
+ *       12 n_uniform[1]
  * Returns the number of doubles written, or -1. */
 int orc_grid_get(const orc_grid *g, int what, double *out);
 
@@ -115,16 +118,28 @@ void orc_time_step_rk3(const orc_grid *g, double *U, double *V, double *W, doubl
 
 /* ---- Immersed boundary (c3d_oracle_ib.c) ----
  * Marker arrays: xb, yb, zb, sb [nb]; normals, ub [3nb] component-major
- * (x block, y block, z block).  Requires a uniform y grid (y_ref_index = 1).
+ * (x block, y block, z block).
  * nzb z slabs of nxb markers; the first nxb1 of each slab are body 1 and the
  * rest body 2 (remove_mean_per_body grouping). */
 typedef struct orc_ib orc_ib;
 /* deltafnc (src/immersed_boundary_operators.f90:270-294). */
 double orc_deltafnc(double x1, double x2, double dr);
-/* setup_IB_operators (src/immersed_boundary_operators.f90:19-256). */
+/* setup_IB_operators (src/immersed_boundary_operators.f90:19-256); yref
+ * [nb] = y_ref_index (NULL: all 1). */
 orc_ib *orc_ib_new(const orc_grid *g, int nb, const double *xb, const double *yb,
                    const double *zb, const double *normals, const double *sb,
-                   const double *ub, int nzb, int nxb, int nxb1);
+                   const double *ub, const int *yref, int nzb, int nxb, int nxb1);
+/* setup_IB_geometry for the moving walls (src/immersed_boundary_geometry.f90:
+ * 176-390): body 1 standing_wave, 2 traveling_wave_x; nb = 2 nxb nzb markers
+ * at time t, slab j holding nxb bottom-wall then nxb top-wall markers.
+ * p1..p3 = body_param_1..3, ramp = body_ramp_up_time (traveling wave only),
+ * Lyc = Ly_channel.  Fills xb, yb, zb, sb [nb], normals, ub [3nb] and
+ * yref [nb] (1 bottom, ny top).  Returns the index (1-based) of the first
+ * marker whose y lies outside [y(1 + suppy), y(ny - suppy)] (the
+ * reference's stop condition), 0 if none, -1 for an invalid body. */
+int orc_wave_geometry(const orc_grid *g, int body, int nxb, int nzb, double p1, double p2,
+                      double p3, double ramp, double t, double Lyc, double *xb, double *yb,
+                      double *zb, double *normals, double *sb, double *ub, int *yref);
 void orc_ib_free(orc_ib *ib);
 /* stencil s (0 u, 1 v, 2 w, 3 centre): 125 entries per marker, marker-major,
  * Fortran indices of the field (centre: of P). */

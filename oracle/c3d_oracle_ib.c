@@ -66,11 +66,12 @@ static double *dup(const double *a, size_t n) {
 
 static long floordiv_d(double a) { return (long)floor(a); }
 
-/* setup_IB_operators: src/immersed_boundary_operators.f90:19-256 (one domain,
- * y_ref_index = 1, i.e. a uniform y grid around the body). */
+/* setup_IB_operators: src/immersed_boundary_operators.f90:19-256 (one domain).
+ * yref[l] is y_ref_index(l), the face whose coordinate anchors the y pivots
+ * (NULL: 1 for every marker). */
 orc_ib *orc_ib_new(const orc_grid *g, int nb, const double *xb, const double *yb,
                    const double *zb, const double *normals, const double *sb,
-                   const double *ub, int nzb, int nxb, int nxb1) {
+                   const double *ub, const int *yref_in, int nzb, int nxb, int nxb1) {
   orc_ib *ib = calloc(1, sizeof(orc_ib));
   ib->g = g;
   ib->nb = nb; ib->nzb = nzb; ib->nxb = nxb; ib->nxb1 = nxb1;
@@ -90,8 +91,8 @@ orc_ib *orc_ib_new(const orc_grid *g, int nb, const double *xb, const double *yb
   const double dx = g->dx, dz = g->dz, dymin = g->dymin, Lxp = g->Lxp, Lzp = g->Lzp;
   const double *x = g->x, *y = g->y, *z = g->z;
   /* xm(i) = (x(i)+x(i+1))/2 = xg(i+1) */
-  const int yref = 1;
   for (int l = 0; l < nb; l++) {
+    const int yref = yref_in ? yref_in[l] : 1;
     /* pivots (:28-42); 0.5*dx uses the real(4) literal 0.5 (exact) */
     int x_piv = (int)floordiv_d(xb[l] / dx) + 1;
     int xm_piv = (int)floordiv_d((xb[l] + 0.5 * dx) / dx);

@@ -9,6 +9,7 @@ struct orc_grid {
   double *kxx, *kzz;
   double *dyy_d, *dyy_l, *dyy_u; /* D(j), DL(j)=Dyy(j+1,j), DU(j)=Dyy(j,j+1) */
   double dx, dz, dxmin, dymin, dzmin, Lx, Ly, Lz, Lxp, Lzp;
+  int n_uniform; /* uniform buffer points per wall (grid_type 2), else 1 */
 };
 
 #endif

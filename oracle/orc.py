@@ -36,7 +36,7 @@ def lib():
         L = C.CDLL(_SO)
         L.orc_grid_new.restype = C.c_void_p
         L.orc_grid_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
-                                   C.c_double, C.c_double, C.c_int]
+                                   C.c_double, C.c_double, C.c_int, C.c_double]
         L.orc_grid_free.argtypes = [C.c_void_p]
         L.orc_grid_get.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_grid_get.restype = C.c_int
@@ -101,17 +101,21 @@ class Grid:
     """Single-domain grid (src/input_output.f90:140-206, src/initialization.f90:166-390).
 
     ``nx, ny, nz`` are face counts as in the reference namelist
-    (``nx_global`` ...).  ``grid_type`` 0 uniform y, 1 tanh-stretched y.
+    (``nx_global`` ...).  ``grid_type`` 0 uniform y, 1 tanh-stretched y,
+    2 tanh-stretched y with uniform buffers of at least
+    ``min_buffer_width`` + 6 dymin at each wall (src/input_output.f90:180-247).
     """
 
-    def __init__(self, nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1):
+    def __init__(self, nx, ny, nz, Lxp=2 * np.pi, Lzp=np.pi, Ly=2.0, alpha=2.0, grid_type=1,
+                 min_buffer_width=0.0):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.nxg, self.nyg, self.nzg = nx + 1, ny + 1, nz + 1
         self.nxp, self.nzp, self.nym = nx - 2, nz - 2, ny - 1
-        self.h = lib().orc_grid_new(nx, ny, nz, Lxp, Lzp, Ly, alpha, grid_type)
+        self.h = lib().orc_grid_new(nx, ny, nz, Lxp, Lzp, Ly, alpha, grid_type, min_buffer_width)
         if not self.h:
             raise ValueError("invalid grid")
         self.Lxp_in, self.Lzp_in, self.Ly_in, self.alpha, self.grid_type = Lxp, Lzp, Ly, alpha, grid_type
+        self.min_buffer_width = min_buffer_width
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -120,10 +124,16 @@ class Grid:
 
     def get(self, what: int) -> np.ndarray:
         sizes = {0: self.nx, 1: self.ny, 2: self.nz, 3: self.nxg, 4: self.nyg, 5: self.nzg,
-                 6: self.nxp, 7: self.nzp, 8: self.nym, 9: self.nym - 1, 10: self.nym - 1, 11: 10}
+                 6: self.nxp, 7: self.nzp, 8: self.nym, 9: self.nym - 1, 10: self.nym - 1, 11: 10,
+                 12: 1}
         out = np.zeros(sizes[what])
         lib().orc_grid_get(self.h, what, _p(out))
         return out
+
+    @property
+    def n_uniform(self):
+        """Uniform buffer points per wall (grid_type 2; 1 otherwise)."""
+        return int(self.get(12)[0])
 
     @property
     def scalars(self):
@@ -261,8 +271,12 @@ def _ib_sigs(L):
     L.orc_deltafnc.argtypes = [C.c_double, C.c_double, C.c_double]
     L.orc_deltafnc.restype = C.c_double
     L.orc_ib_new.restype = C.c_void_p
-    L.orc_ib_new.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+    L.orc_ib_new.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
                              C.c_int, C.c_int, C.c_int]
+    L.orc_wave_geometry.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+    L.orc_wave_geometry.restype = C.c_int
     L.orc_ib_free.argtypes = [C.c_void_p]
     L.orc_ib_get_stencil.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]
     L.orc_ib_regT.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp]
@@ -290,6 +304,34 @@ def deltafnc(x1, x2, dr) -> float:
     return L.orc_deltafnc(x1, x2, dr)
 
 
+WAVE_BODIES = {"standing_wave": 1, "traveling_wave_x": 2}
+
+
+class WaveMarkers:
+    """Markers of a moving wall body at one time (orc_wave_geometry):
+    ``xb, yb, zb, sb, yref`` [nb], ``normals, ub`` [3nb]; slab j holds nxb
+    bottom-wall then nxb top-wall markers (``nxb`` here is 2 x the wall's
+    marker count, ``nxb1`` the bottom wall's), so that the two walls are
+    the two bodies of remove_mean_per_body.  ``stop`` is the 1-based index
+    of the first marker outside [y(3), y(ny-2)] (0 if none)."""
+
+    def __init__(self, grid: Grid, body: str, nxb: int, nzb: int, p1: float, p2: float,
+                 p3: float, t: float, ramp: float = 0.0, Lyc: float = 2.0):
+        L = lib()
+        _ib_sigs(L)
+        nb = 2 * nxb * nzb
+        self.xb, self.yb, self.zb, self.sb = (np.zeros(nb) for _ in range(4))
+        self.normals, self.ub = np.zeros(3 * nb), np.zeros(3 * nb)
+        self.yref = np.zeros(nb, dtype=np.int32)
+        self.stop = L.orc_wave_geometry(grid.h, WAVE_BODIES[body], nxb, nzb, p1, p2, p3, ramp, t,
+                                        Lyc, _p(self.xb), _p(self.yb), _p(self.zb),
+                                        _p(self.normals), _p(self.sb), _p(self.ub),
+                                        self.yref.ctypes.data_as(_ip))
+        if self.stop < 0:
+            raise ValueError("invalid wave body")
+        self.nb, self.nzb, self.nxb, self.nxb1 = nb, nzb, 2 * nxb, nxb
+
+
 class IB:
     """Immersed-boundary operators on a uniform-y oracle grid.
 
@@ -304,7 +346,10 @@ class IB:
         self.nb = int(m.xb.size)
         self._keep = [np.ascontiguousarray(a, dtype=np.float64)
                       for a in (m.xb, m.yb, m.zb, m.normals, m.sb, m.ub)]
+        yref = getattr(m, "yref", None)
+        self._yref = None if yref is None else np.ascontiguousarray(yref, dtype=np.int32)
         self.h = L.orc_ib_new(grid.h, self.nb, *[_p(a) for a in self._keep],
+                              None if self._yref is None else self._yref.ctypes.data_as(_ip),
                               m.nzb, m.nxb, m.nxb1)
         self.heaviside_done = False
 

@@ -28,24 +28,32 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
+MBW = 0.1  # min_buffer_width of the grid_type 2 cases
+
+
 def case(nx, ny, nz, gt=1, seed=2, kind="white"):
     print(f"\n[case] nx={nx} ny={ny} nz={nz} grid_type={gt} seed={seed} kind={kind}", flush=True)
-    g = orc.Grid(nx, ny, nz, grid_type=gt)
+    g = orc.Grid(nx, ny, nz, grid_type=gt, min_buffer_width=MBW)
     if kind == "white":
         U, V, W = fields.white_faces(nx, ny, nz, seed)
     else:
         U, V, W = fields.spectral_faces(nx, ny, nz, seed)
     g.apply_bc(U, V, W)
-    ch = Channel(nx, ny, nz, grid_type=gt)
+    ch = Channel(nx, ny, nz, grid_type=gt, min_buffer_width=MBW)
     return g, ch, U, V, W
 
 
-@pytest.mark.parametrize("nx,ny,nz,gt", [(34, 65, 34, 1), (194, 129, 194, 1), (130, 129, 130, 0)])
-def test_grid_bitwise(nx, ny, nz, gt):
-    g = orc.Grid(nx, ny, nz, grid_type=gt)
-    with Channel(nx, ny, nz, grid_type=gt) as ch:
-        for what in range(12):
+@pytest.mark.parametrize("nx,ny,nz,gt,mbw", [(34, 65, 34, 1, 0.0), (194, 129, 194, 1, 0.0),
+                                              (130, 129, 130, 0, 0.0), (34, 65, 34, 2, 0.1),
+                                              (66, 129, 34, 2, 0.0), (34, 97, 34, 2, 0.25)])
+def test_grid_bitwise(nx, ny, nz, gt, mbw):
+    g = orc.Grid(nx, ny, nz, grid_type=gt, min_buffer_width=mbw)
+    with Channel(nx, ny, nz, grid_type=gt, min_buffer_width=mbw) as ch:
+        for what in range(13):
             assert np.array_equal(ch.grid(what), g.get(what)), what
+        if gt == 2:
+            print(f"\n  n_uniform = {g.n_uniform}")
+            assert g.n_uniform > 1
 
 
 @pytest.mark.parametrize("nx,ny,nz", [(34, 33, 18), (18, 5, 34), (66, 17, 34), (130, 129, 130)])
@@ -77,7 +85,7 @@ def test_divergence_and_gradient_bitwise():
 
 
 @pytest.mark.parametrize("nx,ny,nz,gt", [(18, 5, 18, 1), (34, 65, 34, 1), (66, 17, 34, 0),
-                                         (194, 129, 194, 1)])
+                                         (194, 129, 194, 1), (34, 65, 34, 2)])
 @pytest.mark.parametrize("rk_step", [1, 2])
 def test_poisson_parity(nx, ny, nz, gt, rk_step):
     g, ch, U, V, W = case(nx, ny, nz, gt, seed=3)
@@ -102,7 +110,8 @@ def test_poisson_parity(nx, ny, nz, gt, rk_step):
 
 @pytest.mark.parametrize("nx,ny,nz,gt,kind", [(34, 65, 34, 1, "white"), (194, 129, 194, 1, "white"),
                                               (130, 129, 130, 0, "white"),
-                                              (66, 33, 98, 1, "spectral")])
+                                              (66, 33, 98, 1, "spectral"),
+                                              (66, 97, 34, 2, "white")])
 def test_projection_parity(nx, ny, nz, gt, kind):
     g, ch, U, V, W = case(nx, ny, nz, gt, seed=2, kind=kind)
     dt = 2.5e-3

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 from scipy.linalg import lapack
 
+from channel3d_parallel_amd import fields
 from oracle import orc
 from tests import refimpl
 
@@ -100,6 +101,49 @@ def test_grid_closed_form(nx, ny, nz, gt):
     assert abs(xg[0] + 0.5 * (x[1] - x[0])) < 1e-14  # mirror ghost
     yg = g.get(4)
     assert abs(yg[0] - (2 * y[0] - yg[1])) < 1e-14 and abs(yg[-1] - (2 * y[-1] - yg[-2])) < 1e-14
+
+
+@pytest.mark.parametrize("ny,mbw,alpha", [(65, 0.1, 2.0), (129, 0.0, 2.6), (97, 0.25, 1.5),
+                                           (49, 0.05, 0.0)])
+def test_grid_buffered_closed_form(ny, mbw, alpha):
+    """grid_type 2 (src/input_output.f90:180-247) against its closed form:
+    tanh points 1 + (h/h1) tanh(alpha eta)/tanh(alpha) on the middle
+    m = ny - 2(n_u - 1) points, h1 the first spacing of the tanh points on
+    [-1, 1], h = Ly h1 / (2 + (n_u - 1) h1), straight lines of spacing h
+    beyond; the channel walls y = 0 and y = 2 sit in the middle of the
+    buffers; n_u is the smallest count with n_u h >= mbw + 6 h."""
+    g = orc.Grid(34, ny, 34, alpha=alpha, grid_type=2, min_buffer_width=mbw)
+    nu = g.n_uniform
+    y = g.get(1)
+
+    def spacing(n):
+        m = ny - 2 * (n - 1)
+        t = (lambda e: np.tanh(alpha * e) / np.tanh(alpha)) if alpha > 0 else (lambda e: e)
+        h1 = t(2.0 / (m - 1) - 1.0) - t(-1.0)
+        return m, t, h1, 2.0 * h1 / (2.0 + (n - 1) * h1)
+
+    m, t, h1, h = spacing(nu)
+    eta = 2.0 * np.arange(m) / (m - 1) - 1.0
+    ref = np.empty(ny)
+    ref[nu - 1:nu - 1 + m] = 1.0 + (h / h1) * t(eta)
+    b = np.arange(1, nu)
+    ref[nu - 1 - b] = ref[nu - 1] - b * h
+    ref[ny - nu + b] = ref[ny - nu] + b * h
+    assert np.allclose(y, ref, rtol=0, atol=2e-14)
+    assert abs(y[0] + 0.5 * (nu - 1) * h) < 2e-14 and abs(y[-1] - 2.0 - 0.5 * (nu - 1) * h) < 2e-14
+    assert nu * h >= mbw + 6 * h
+    if nu > 1:
+        _, _, _, hp = spacing(nu - 1)
+        assert (nu - 1) * hp < mbw + 6 * hp
+    # the Python field generator builds the same faces
+    _, yf, _ = fields.grid_faces(34, ny, 34, alpha=alpha, grid_type=2, min_buffer_width=mbw)
+    assert np.allclose(yf, y, rtol=0, atol=1e-14)
+
+
+def test_grid_buffered_too_wide():
+    """Buffers that cannot fit stop the set-up (src/input_output.f90:199)."""
+    with pytest.raises(ValueError):
+        orc.Grid(34, 17, 34, grid_type=2, min_buffer_width=1.0)
 
 
 @pytest.mark.parametrize("ny,dymin", [(65, 4.8674e-3), (129, 2.3606e-3), (193, 1.5579e-3)])
