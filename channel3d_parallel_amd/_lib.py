@@ -64,6 +64,14 @@ SIGNATURES = [
     ("c3d_refinement_residuals", C.c_int, [C.c_void_p, _dp, C.c_int]),
     ("c3d_ib_setup", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
                                C.c_int]),
+    ("c3d_ib_setup_ref", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
+                                   C.c_int, C.c_int, C.c_int]),
+    ("c3d_ib_setup_wave", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double]),
+    ("c3d_ib_move", C.c_int, [C.c_void_p, C.c_double]),
+    ("c3d_ib_get_markers", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    ("c3d_set_time", C.c_int, [C.c_void_p, C.c_double]),
+    ("c3d_get_time", C.c_int, [C.c_void_p, _dp]),
     ("c3d_ib_get_stencil", C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]),
     ("c3d_ib_get_heaviside", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("c3d_ib_interpolate", C.c_int, [C.c_void_p, C.c_int, _dp]),

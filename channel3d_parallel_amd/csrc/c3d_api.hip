@@ -80,6 +80,18 @@ struct IBState {
     for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
     if (herr) (void)hipHostFree(herr);
   }
+  // marker arrays (device) and the grid the stencil set-up reads
+  double *xb = nullptr, *yb = nullptr, *zb = nullptr, *normals = nullptr, *sb = nullptr,
+         *ub = nullptr;
+  int* yref = nullptr;
+  int* flags = nullptr;  // [0] geometry and [1] stencil range flags, [2..4] node counts
+  c3d::ib::SetupGrid sg{};
+  // moving wall body (c3d_ib_setup_wave): re-set-up at every Runge-Kutta stage
+  bool moving = false;
+  c3d::ib::WaveBody wave{};
+  double ramp = 0.0;      // body_ramp_up_time
+  double t_geom = 0.0;    // time of the current marker positions
+  void* map_work = nullptr;  // scratch of the device map construction
   double last_dt = 1.0;     // dt of the last IB projection
   double tol = 1e-10;
   int maxit = 1000, guess = 0;
@@ -96,6 +108,7 @@ struct c3d_ctx {
   std::vector<double> x, y, z, xg, yg, zg, kxx, kzz, dyy_d, dyy_l, dyy_u;
   double dx = 0, dz = 0, dxmin = 0, dymin = 0, dzmin = 0, Lx = 0, Ly = 0, Lz = 0, Lxp = 0, Lzp = 0;
   int n_uniform = 1;  // uniform buffer points per wall (grid_type 2)
+  double t = 0.0;     // simulation time (t of src/time_integration.f90)
   // device state
   hipStream_t stream = nullptr;
   DevGrid g{};
@@ -342,6 +355,9 @@ constexpr int kNumStages = 7;
 constexpr int kMaxRef = 8;          // refinement sweeps per mixed projection
 constexpr int kRedRefine = 4;       // first reduction slot of the sweep residual maxima
 constexpr int kResSlots = 64;       // max slots per sweep (workgroup b uses slot b % 64)
+
+// Moving-body re-set-up at time t (defined with the immersed-boundary set-up).
+int ib_move_to(c3d_ctx* c, double t);
 
 }  // namespace
 
@@ -910,9 +926,13 @@ int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
   if (!c || !(dt > 0.0)) return C3D_EINVAL;
   int rc = ensure_rk(c);
   if (!rc) rc = save_state(c);  // src/time_integration.f90:91-94
+  const double t0 = c->t;
+  static const double rk_t[3] = {8.0 / 15.0, 2.0 / 3.0, 1.0};  // src/initialization.f90:408-410
   for (int st = 1; st <= 3 && !rc; st++) {
     rc = rhs_launch(c, st);                                     // :101-103
     if (!rc) rc = rk_update_launch(c, st, dt);                  // :105-107
+    c->t = t0 + rk_t[st - 1] * dt;                              // :108
+    if (!rc && c->ib && c->ib->moving) rc = ib_move_to(c, c->t);  // :110-115
     if (!rc && c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
       rc = fail(c, C3D_EHIP, "boundary conditions");            // :117
     if (!rc) rc = save_interim(c);
@@ -1300,65 +1320,86 @@ static int need_ib(c3d_ctx* c) {
   return 0;
 }
 
-int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const double* zb,
-                 const double* normals, const double* sb, const double* ub, int nzb, int nxb,
-                 int nxb1) {
-  if (!c || !xb || !yb || !zb || !normals || !sb || !ub) return C3D_EINVAL;
-  if (c->ib) return fail(c, C3D_EINVAL, "an immersed body is already set up on this context");
-  if (c->desc.grid_type != 0)
-    return fail(c, C3D_EINVAL, "immersed bodies need a uniform y grid (grid_type 0)");
-  if (nb <= 0 || nzb <= 0 || nxb <= 0 || (long long)nzb * nxb != nb || nxb1 <= 0 || nxb1 >= nxb)
-    return fail(c, C3D_EINVAL, "marker counts: nb = nzb*nxb and 0 < nxb1 < nxb required");
-  if ((long long)nb * 125 > 0x7fffffff) return fail(c, C3D_EINVAL, "too many markers");
-  for (int l = 0; l < nb; l++)
-    if (!(xb[l] >= 0.0 && xb[l] < c->Lxp && zb[l] >= 0.0 && zb[l] < c->Lzp && yb[l] > 0.0 &&
-          yb[l] < c->Ly))
-      return fail(c, C3D_EINVAL, "marker %d outside the periodic box", l);
+}  // extern "C"
+
+namespace {
+
+// Marker, stencil and resident field storage for nb markers (one body set-up
+// per context).  On failure c->ib keeps the partial state for ib_discard.
+int ib_create(c3d_ctx* c, int nb, int nzb, int nxb, int nxb1) {
   IBState* ib = new IBState();
   c->ib = ib;
   {  // C3D_IB_GRAPHS=0: launch each BiCGSTAB iteration kernel by kernel
     const char* e = getenv("C3D_IB_GRAPHS");
     ib->use_graphs = !(e && atoi(e) == 0);
   }
-  auto bail = [&](int rc) {
-    c->ib = nullptr;
-    delete ib;  // device memory stays in c->allocs until c3d_destroy
-    return rc;
-  };
   namespace B = c3d::ib;
   B::Markers& m = ib->m;
   m.nb = nb; m.nzb = nzb; m.nxb = nxb; m.nxb1 = nxb1;
   m.A = c->dx * c->dymin * c->dz;
   m.factor = 1.0 / (c->dx * c->dymin * c->dz);
-  const size_t n4 = 4 * (size_t)nb;
+  const size_t n4 = 4 * (size_t)nb, n3 = 3 * (size_t)nb;
   int rc = 0;
-  std::vector<double> hv;
-  auto up = [&](const double** dst, const double* src, size_t n) {
-    hv.assign(src, src + n);
-    return upload(c, dst, hv);
-  };
-  if ((rc = up(&m.xb, xb, nb)) || (rc = up(&m.yb, yb, nb)) || (rc = up(&m.zb, zb, nb)) ||
-      (rc = up(&m.normals, normals, 3 * (size_t)nb)) || (rc = up(&m.sb, sb, nb)) ||
-      (rc = up(&m.ub, ub, 3 * (size_t)nb)))
-    return bail(rc);
+  if ((rc = dalloc(c, &ib->xb, nb)) || (rc = dalloc(c, &ib->yb, nb)) ||
+      (rc = dalloc(c, &ib->zb, nb)) || (rc = dalloc(c, &ib->normals, n3)) ||
+      (rc = dalloc(c, &ib->sb, nb)) || (rc = dalloc(c, &ib->ub, n3)) ||
+      (rc = dalloc(c, &ib->yref, nb)) || (rc = dalloc(c, &ib->flags, 8)))
+    return rc;
+  m.xb = ib->xb; m.yb = ib->yb; m.zb = ib->zb; m.normals = ib->normals; m.sb = ib->sb;
+  m.ub = ib->ub; m.yref = ib->yref;
   for (int s = 0; s < 4; s++)
     if ((rc = dalloc(c, &m.st[s].idx, 15 * (size_t)nb)) || (rc = dalloc(c, &m.st[s].phi, 15 * (size_t)nb)) ||
         (rc = dalloc(c, &m.st[s].dd, 15 * (size_t)nb)))
-      return bail(rc);
-  B::SetupGrid sg{};
+      return rc;
+  B::SetupGrid& sg = ib->sg;
   if ((rc = upload(c, &sg.x, c->x)) || (rc = upload(c, &sg.y, c->y)) || (rc = upload(c, &sg.z, c->z)))
-    return bail(rc);
+    return rc;
   sg.dx = c->dx; sg.dz = c->dz; sg.dymin = c->dymin; sg.Lxp = c->Lxp; sg.Lzp = c->Lzp;
-  if (B::launch_setup(c->g, sg, m, c->stream) != hipSuccess)
-    return bail(fail(c, C3D_EHIP, "IB stencil set-up launch failed"));
+  // resident buffers
+  if ((rc = dalloc(c, &ib->Ureg, c->nu)) || (rc = dalloc(c, &ib->Vreg, c->nv)) ||
+      (rc = dalloc(c, &ib->Wreg, c->nw)) || (rc = dalloc(c, &ib->phi, c->nr)) ||
+      (rc = dalloc(c, &ib->Hc_ext, c->nr)) || (rc = dalloc(c, &ib->Hu_int, c->nu)) ||
+      (rc = dalloc(c, &ib->Hv_int, c->nv)) || (rc = dalloc(c, &ib->Hw_int, c->nw)) ||
+      (rc = dalloc(c, &ib->Uint, c->nu)) || (rc = dalloc(c, &ib->Vint, c->nv)) ||
+      (rc = dalloc(c, &ib->Wint, c->nw)) || (rc = dalloc(c, &ib->E1nHc, (size_t)nb)) ||
+      (rc = dalloc(c, &ib->E1nH, n3)) || (rc = dalloc(c, &ib->aux, n3)) ||
+      (rc = dalloc(c, &ib->rhs_ib, n4)) || (rc = dalloc(c, &ib->fb, n4)) ||
+      (rc = dalloc(c, &ib->fin, n4)) || (rc = dalloc(c, &ib->fout, n4)) ||
+      (rc = dalloc(c, &ib->fprev, n4)) ||
+      (rc = dalloc(c, &ib->sc, 8)) || (rc = dalloc(c, &ib->part, 5 * (size_t)B::kDotBlocks)) ||
+      (rc = dalloc(c, &ib->mpart, 128)))
+    return rc;
+  for (int v = 0; v < 8; v++)
+    if ((rc = dalloc(c, &ib->vec[v], n4))) return rc;
+  if (hipMemsetAsync(ib->fb, 0, n4 * 8, c->stream) != hipSuccess ||
+      hipMemsetAsync(ib->phi, 0, c->nr * 8, c->stream) != hipSuccess)
+    return fail(c, C3D_EHIP, "IB buffer clear failed");
+  return 0;
+}
+
+void ib_discard(c3d_ctx* c) {
+  delete c->ib;  // device memory stays in c->allocs until c3d_destroy
+  c->ib = nullptr;
+}
+
+// Stencils (setup_IB_operators, src/immersed_boundary_operators.f90:19-256)
+// and the spreading maps built on the host from the stencil indices.
+int ib_operators_host_map(c3d_ctx* c) {
+  namespace B = c3d::ib;
+  IBState* ib = c->ib;
+  B::Markers& m = ib->m;
+  const int nb = m.nb;
+  int rc = 0;
+  if (B::launch_setup(c->g, ib->sg, m, nullptr, c->stream) != hipSuccess)
+    return fail(c, C3D_EHIP, "IB stencil set-up launch failed");
   std::vector<int> idx[4];
   for (int s = 0; s < 4; s++) {
     idx[s].resize(15 * (size_t)nb);
     if (hipMemcpyAsync(idx[s].data(), m.st[s].idx, idx[s].size() * 4, hipMemcpyDeviceToHost,
                        c->stream) != hipSuccess)
-      return bail(fail(c, C3D_EHIP, "stencil copy-back failed"));
+      return fail(c, C3D_EHIP, "stencil copy-back failed");
   }
-  if ((rc = sync(c))) return bail(rc);
+  if ((rc = sync(c))) return rc;
   // every node must be a canonical interior node inside the gradient's
   // ranges (the Schur kernels evaluate G at the nodes and skip the boundary
   // conditions there): x in [2, nx-1], z in [2, nz-1], y in [2, nyg-1]
@@ -1370,9 +1411,9 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
         const int X = idx[s][(0 * 5 + t) * (size_t)nb + l], Y = idx[s][(1 * 5 + t) * (size_t)nb + l],
                   Z = idx[s][(2 * 5 + t) * (size_t)nb + l];
         if (X < 2 || X > c->nx - 1 || Z < 2 || Z > c->nz - 1)
-          return bail(fail(c, C3D_EINVAL, "stencil %d of marker %d: x/z index out of range", s, l));
+          return fail(c, C3D_EINVAL, "stencil %d of marker %d: x/z index out of range", s, l);
         if (Y < 2 || Y > ylim[s])
-          return bail(fail(c, C3D_EINVAL, "marker %d: stencil reaches a wall ghost row", l));
+          return fail(c, C3D_EINVAL, "marker %d: stencil reaches a wall ghost row", l);
       }
   // node -> entry maps of the three spreading operators (counting sort by
   // node offset; entries within a node stay in ascending marker order)
@@ -1427,32 +1468,55 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
     if ((rc = upload(c, &ib->sp[q].node, jnode)) || (rc = upload(c, &ib->sp[q].cnt, jcnt)) ||
         (rc = upload(c, &ib->sp[q].slot, slot)) || (rc = upload(c, &ib->sp[q].q, qq)) ||
         (rc = dalloc(c, &ib->sp[q].ml, qq.size())) || (rc = dalloc(c, &ib->sp[q].wfs, qq.size())))
-      return bail(rc);
+      return rc;
     if (B::launch_spread_coeffs(m, ib->sp[q], q, c->stream) != hipSuccess)
-      return bail(fail(c, C3D_EHIP, "spreading coefficients"));
+      return fail(c, C3D_EHIP, "spreading coefficients");
   }
-  // resident buffers
-  if ((rc = dalloc(c, &ib->Ureg, c->nu)) || (rc = dalloc(c, &ib->Vreg, c->nv)) ||
-      (rc = dalloc(c, &ib->Wreg, c->nw)) || (rc = dalloc(c, &ib->phi, c->nr)) ||
-      (rc = dalloc(c, &ib->Hc_ext, c->nr)) || (rc = dalloc(c, &ib->Hu_int, c->nu)) ||
-      (rc = dalloc(c, &ib->Hv_int, c->nv)) || (rc = dalloc(c, &ib->Hw_int, c->nw)) ||
-      (rc = dalloc(c, &ib->Uint, c->nu)) || (rc = dalloc(c, &ib->Vint, c->nv)) ||
-      (rc = dalloc(c, &ib->Wint, c->nw)) || (rc = dalloc(c, &ib->E1nHc, (size_t)nb)) ||
-      (rc = dalloc(c, &ib->E1nH, 3 * (size_t)nb)) || (rc = dalloc(c, &ib->aux, 3 * (size_t)nb)) ||
-      (rc = dalloc(c, &ib->rhs_ib, n4)) || (rc = dalloc(c, &ib->fb, n4)) ||
-      (rc = dalloc(c, &ib->fin, n4)) || (rc = dalloc(c, &ib->fout, n4)) ||
-      (rc = dalloc(c, &ib->fprev, n4)) ||
-      (rc = dalloc(c, &ib->sc, 8)) || (rc = dalloc(c, &ib->part, 5 * (size_t)B::kDotBlocks)) ||
-      (rc = dalloc(c, &ib->mpart, 128)))
-    return bail(rc);
-  for (int v = 0; v < 8; v++)
-    if ((rc = dalloc(c, &ib->vec[v], n4))) return bail(rc);
+  return 0;
+}
+
+// Stencils and spreading maps built on the device (the moving-body path):
+// no stencil index leaves the device; the node counts and two range flags
+// come back in one 32-byte read.
+int ib_operators_device_map(c3d_ctx* c) {
+  namespace B = c3d::ib;
+  IBState* ib = c->ib;
+  B::Markers& m = ib->m;
+  int hf[8] = {0};
+  HIPCHK(c, hipMemsetAsync(ib->flags + 1, 0, 4, c->stream));
+  HIPCHK(c, B::launch_setup(c->g, ib->sg, m, ib->flags + 1, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hf, ib->flags, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  if (hf[1])
+    return fail(c, C3D_EINVAL, "marker %d: stencil leaves the interior rows (wall ghost row)",
+                hf[1] - 1);
+  const int n1[3] = {c->nx, c->nxg, c->nxg}, n2[3] = {c->nyg, c->ny, c->nyg};
+  const size_t nf[3] = {c->nu, c->nv, c->nw};
+  for (int q = 0; q < 3; q++) {
+    HIPCHK(c, B::build_spread_map(m, q, n1[q], n2[q], nf[q], ib->map_work, ib->sp[q],
+                                  ib->flags + 2 + q, c->stream));
+    HIPCHK(c, B::launch_spread_coeffs(m, ib->sp[q], q, c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(hf, ib->flags, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = sync(c))) return rc;
+  for (int q = 0; q < 3; q++) ib->sp[q].nnodes = hf[2 + q];
+  return 0;
+}
+
+// compute_heaviside (src/heaviside.f90:20-54) for the current markers.  The
+// spread targets are cleared first: the spreading kernels write only the
+// support nodes, which move with the body.
+int ib_heaviside(c3d_ctx* c) {
+  namespace B = c3d::ib;
+  IBState* ib = c->ib;
+  B::Markers& m = ib->m;
+  const int nb = m.nb;
+  int rc = 0;
   if (hipMemsetAsync(ib->Ureg, 0, c->nu * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(ib->Vreg, 0, c->nv * 8, c->stream) != hipSuccess ||
-      hipMemsetAsync(ib->Wreg, 0, c->nw * 8, c->stream) != hipSuccess ||
-      hipMemsetAsync(ib->fb, 0, n4 * 8, c->stream) != hipSuccess ||
-      hipMemsetAsync(ib->phi, 0, c->nr * 8, c->stream) != hipSuccess)
-    return bail(fail(c, C3D_EHIP, "IB buffer clear failed"));
+      hipMemsetAsync(ib->Wreg, 0, c->nw * 8, c->stream) != hipSuccess)
+    return fail(c, C3D_EHIP, "IB buffer clear failed");
   // compute_heaviside (src/heaviside.f90:20-54); the exterior face fields
   // are built in the U_interim buffers
   double *Hu = ib->Uint, *Hv = ib->Vint, *Hw = ib->Wint;
@@ -1464,8 +1528,8 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
   ok = ok && c3d::launch_divergence(c->g, ib->Ureg, ib->Vreg, ib->Wreg, ib->Hc_ext, c->stream) == hipSuccess;
   ok = ok && B::launch_affine(ib->Hc_ext, -0.0, -1.0, ib->Hc_ext, c->nr, c->stream) == hipSuccess;
   ok = ok && c3d::launch_fwd_x(c->g, false, nullptr, nullptr, nullptr, ib->Hc_ext, c->S, c->stream) == hipSuccess;
-  if (!ok) return bail(fail(c, C3D_EHIP, "Heaviside set-up failed"));
-  if ((rc = spectral_core(c))) return bail(rc);
+  if (!ok) return fail(c, C3D_EHIP, "Heaviside set-up failed");
+  if ((rc = spectral_core(c))) return rc;
   ok = c3d::launch_inv_x(c->g, 0, c->S, ib->Hc_ext, c->P, 1.0, false, c->stream) == hipSuccess;
   ok = ok && B::launch_affine(ib->Hc_ext, 1.0, -1.0, ib->Hc_ext, c->nr, c->stream) == hipSuccess;
   ok = ok && hipMemsetAsync(Hu, 0, c->nu * 8, c->stream) == hipSuccess &&
@@ -1479,9 +1543,183 @@ int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const d
   ok = ok && c3d::launch_apply_bc(c->g, ib->Hu_int, ib->Hv_int, ib->Hw_int, c->stream) == hipSuccess;
   ok = ok && B::launch_interp_c(c->g, m, ib->Hc_ext, 1.0, ib->E1nHc, c->stream) == hipSuccess;
   ok = ok && B::launch_interp_uvw(c->g, m, Hu, Hv, Hw, true, ib->E1nH, c->stream) == hipSuccess;
-  if (!ok) return bail(fail(c, C3D_EHIP, "Heaviside set-up failed"));
-  if ((rc = sync(c))) return bail(rc);
+  if (!ok) return fail(c, C3D_EHIP, "Heaviside set-up failed");
+  if ((rc = sync(c))) return rc;
   c->Hm = ib->Hu_int;
+  return 0;
+}
+
+// setup_IB_geometry of the moving walls at time t
+// (src/immersed_boundary_geometry.f90:176-390) into the marker arrays.
+int ib_wave_geometry(c3d_ctx* c, double t) {
+  namespace B = c3d::ib;
+  IBState* ib = c->ib;
+  const B::WaveBody& w = ib->wave;
+  B::WaveTime wt{};
+  wt.t = t;
+  wt.ct = std::cos(w.p2 * t);
+  wt.st = std::sin(w.p2 * t);
+  wt.amp = w.p1;
+  wt.damp = 0.0;
+  if (w.body == 2 && t < ib->ramp && ib->ramp > 0.0) {  // :274-281
+    const double tau = t / ib->ramp;
+    wt.amp = w.p1 * 0.5 * (1.0 - std::cos(w.pi * tau));
+    wt.damp = w.p1 * 0.5 * (w.pi / ib->ramp) * std::sin(w.pi * tau);
+  }
+  int bad = 0;
+  HIPCHK(c, hipMemsetAsync(ib->flags, 0, 4, c->stream));
+  HIPCHK(c, B::launch_wave_geometry(w, wt, ib->xb, ib->yb, ib->zb, ib->normals, ib->sb, ib->ub,
+                                    ib->yref, ib->flags, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&bad, ib->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  if (bad)  // :199-215, :305-321
+    return fail(c, C3D_EINVAL, "marker %d at t = %g: body points support exceeds grid dimensions",
+                bad - 1, t);
+  return 0;
+}
+
+// Geometry, operators and Heaviside at time t; the cached iteration graphs
+// (launch shapes follow the node counts) and the history start (the Schur
+// operator changed) are dropped.
+int ib_move_to(c3d_ctx* c, double t) {
+  IBState* ib = c->ib;
+  int rc = ib_wave_geometry(c, t);
+  if (!rc) rc = ib_operators_device_map(c);
+  if (!rc) rc = ib_heaviside(c);
+  if (rc) return rc;
+  for (auto& gr : ib->graphs) (void)hipGraphExecDestroy(gr.exec);
+  ib->graphs.clear();
+  ib->hcount = 0;
+  ib->hhead = 0;
+  ib->t_geom = t;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int c3d_ib_setup_ref(c3d_ctx* c, int nb, const double* xb, const double* yb, const double* zb,
+                     const double* normals, const double* sb, const double* ub, const int* yref,
+                     int nzb, int nxb, int nxb1) {
+  if (!c || !xb || !yb || !zb || !normals || !sb || !ub) return C3D_EINVAL;
+  if (c->ib) return fail(c, C3D_EINVAL, "an immersed body is already set up on this context");
+  if (nb <= 0 || nzb <= 0 || nxb <= 0 || (long long)nzb * nxb != nb || nxb1 <= 0 || nxb1 >= nxb)
+    return fail(c, C3D_EINVAL, "marker counts: nb = nzb*nxb and 0 < nxb1 < nxb required");
+  if ((long long)nb * 125 > 0x7fffffff) return fail(c, C3D_EINVAL, "too many markers");
+  for (int l = 0; l < nb; l++) {
+    if (!(xb[l] >= 0.0 && xb[l] < c->Lxp && zb[l] >= 0.0 && zb[l] < c->Lzp && yb[l] > c->y[0] &&
+          yb[l] < c->y[c->ny - 1]))
+      return fail(c, C3D_EINVAL, "marker %d outside the periodic box", l);
+    if (yref && (yref[l] < 1 || yref[l] > c->ny))
+      return fail(c, C3D_EINVAL, "marker %d: y_ref_index outside 1..ny", l);
+  }
+  int rc = ib_create(c, nb, nzb, nxb, nxb1);
+  IBState* ib = c->ib;
+  const size_t n3 = 3 * (size_t)nb;
+  std::vector<int> yr(nb, 1);
+  if (yref) yr.assign(yref, yref + nb);
+  if (!rc &&
+      (hipMemcpyAsync(ib->xb, xb, nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(ib->yb, yb, nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(ib->zb, zb, nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(ib->normals, normals, n3 * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(ib->sb, sb, nb * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(ib->ub, ub, n3 * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       hipMemcpyAsync(ib->yref, yr.data(), nb * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess))
+    rc = fail(c, C3D_EHIP, "marker upload failed");
+  if (!rc) rc = ib_operators_host_map(c);
+  if (!rc) rc = ib_heaviside(c);
+  if (rc) ib_discard(c);
+  return rc;
+}
+
+int c3d_ib_setup(c3d_ctx* c, int nb, const double* xb, const double* yb, const double* zb,
+                 const double* normals, const double* sb, const double* ub, int nzb, int nxb,
+                 int nxb1) {
+  return c3d_ib_setup_ref(c, nb, xb, yb, zb, normals, sb, ub, nullptr, nzb, nxb, nxb1);
+}
+
+int c3d_ib_setup_wave(c3d_ctx* c, int body, int nxb, int nzb, double p1, double p2, double p3,
+                      double ramp, double t) {
+  if (!c) return C3D_EINVAL;
+  if (c->ib) return fail(c, C3D_EINVAL, "an immersed body is already set up on this context");
+  if (body != 1 && body != 2) return fail(c, C3D_EINVAL, "body must be 1 (standing_wave) or 2 (traveling_wave_x)");
+  if (nxb < 3 || nzb < 1) return fail(c, C3D_EINVAL, "nxb >= 3 and nzb >= 1 required");
+  if ((long long)2 * nxb * nzb * 125 >= 0x7fffffff) return fail(c, C3D_EINVAL, "too many markers");
+  if (body == 1 && p1 > c->desc.min_buffer_width)  // src/immersed_boundary_geometry.f90:177
+    return fail(c, C3D_EINVAL, "IB amplitude is bigger than the minimum buffer width");
+  if (body == 2 && !(p2 * p3 != 0.0))
+    return fail(c, C3D_EINVAL, "traveling wave: body_param_2 * body_param_3 must be nonzero");
+  const int nb = 2 * nxb * nzb;
+  int rc = ib_create(c, nb, nzb, 2 * nxb, nxb);
+  IBState* ib = c->ib;
+  namespace B = c3d::ib;
+  if (!rc) {
+    ib->moving = true;
+    ib->ramp = ramp;
+    B::WaveBody& w = ib->wave;
+    w.body = body; w.nxb = nxb; w.nzb = nzb; w.ny = c->ny;
+    w.p1 = p1; w.p2 = p2; w.p3 = p3;
+    w.Lxp = c->Lxp; w.Lzp = c->Lzp; w.Lyc = c->desc.Ly; w.pi = 4.0 * std::atan(1.0);
+    w.ylo = c->y[2];
+    w.yhi = c->y[c->ny - 3];
+    // spreading maps at their capacity: nb*125 entries, as many nodes
+    const size_t ne = (size_t)nb * 125;
+    size_t nfmax = std::max(c->nu, std::max(c->nv, c->nw));
+    for (int q = 0; q < 3 && !rc; q++) {
+      B::Spread& sp = ib->sp[q];
+      int *node = nullptr, *cnt = nullptr, *slot = nullptr, *qq = nullptr;
+      if ((rc = dalloc(c, &node, ne)) || (rc = dalloc(c, &cnt, ne)) ||
+          (rc = dalloc(c, &slot, ne + 1)) || (rc = dalloc(c, &qq, ne)) ||
+          (rc = dalloc(c, &sp.ml, ne)) || (rc = dalloc(c, &sp.wfs, ne)))
+        break;
+      sp.node = node; sp.cnt = cnt; sp.slot = slot; sp.q = qq;
+      sp.nentries = (int)ne;
+    }
+    if (!rc) {
+      unsigned char* work = nullptr;
+      rc = dalloc(c, &work, B::spread_map_work_bytes(nb, nfmax));
+      ib->map_work = work;
+    }
+  }
+  if (!rc) rc = ib_move_to(c, t);
+  if (rc) ib_discard(c);
+  return rc;
+}
+
+int c3d_ib_move(c3d_ctx* c, double t) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!c->ib->moving) return fail(c, C3D_EINVAL, "the immersed body does not move");
+  return ib_move_to(c, t);
+}
+
+int c3d_ib_get_markers(c3d_ctx* c, double* xb, double* yb, double* zb, double* normals,
+                       double* sb, double* ub, int* yref) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  IBState* ib = c->ib;
+  const size_t nb = ib->m.nb;
+  const double* src[6] = {ib->xb, ib->yb, ib->zb, ib->normals, ib->sb, ib->ub};
+  double* dst[6] = {xb, yb, zb, normals, sb, ub};
+  const size_t n[6] = {nb, nb, nb, 3 * nb, nb, 3 * nb};
+  for (int a = 0; a < 6; a++)
+    if (dst[a]) HIPCHK(c, hipMemcpyAsync(dst[a], src[a], n[a] * 8, hipMemcpyDeviceToHost, c->stream));
+  if (yref) HIPCHK(c, hipMemcpyAsync(yref, ib->yref, nb * 4, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_set_time(c3d_ctx* c, double t) {
+  if (!c) return C3D_EINVAL;
+  c->t = t;
+  return C3D_OK;
+}
+
+int c3d_get_time(const c3d_ctx* c, double* t) {
+  if (!c || !t) return C3D_EINVAL;
+  *t = c->t;
   return C3D_OK;
 }
 

@@ -48,7 +48,7 @@ __device__ __forceinline__ int floor_div(int a, int b) {
 // and the per-axis windows of the u (:46-98), v (:101-148), w (:151-194) and
 // centre (:197-244) stencils on one periodic domain, centre z index nzg-1
 // stored as its canonical plane 2.
-__global__ void k_setup(DevGrid g, SetupGrid sg, Markers m) {
+__global__ void k_setup(DevGrid g, SetupGrid sg, Markers m, int* bad) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nb = m.nb;
   if (tid >= 4 * nb) return;
@@ -61,8 +61,10 @@ __global__ void k_setup(DevGrid g, SetupGrid sg, Markers m) {
   const double* z = sg.z;
   const int x_piv = (int)floor(xb / dx) + 1;
   const int xm_piv = (int)floor((xb + 0.5 * dx) / dx);
-  const int y_piv = (int)floor((yb - y[0]) / dymin) + 1;
-  const int ym_piv = (int)floor((yb - (y[0] + 0.5 * dymin)) / dymin) + 1;
+  const int yr = m.yref ? m.yref[l] : 1;
+  const int y_piv = (int)floor((yb - y[yr - 1]) / dymin) + yr;
+  const int ym_piv = (int)floor((yb - (y[yr - 1] + 0.5 * dymin)) / dymin) + yr;
+  const int ylim = (s == 1) ? g.ny - 1 : g.nyg - 1;
   const int z_piv = (int)floor(zb / dz) + 1;
   const int zm_piv = (int)floor((zb + 0.5 * dz) / dz) + 1;
   Stencil st = m.st[s];
@@ -118,6 +120,8 @@ __global__ void k_setup(DevGrid g, SetupGrid sg, Markers m) {
       zgd = 0.5 * (z[kkp - 2] + z[kkp - 1]) + zs * sg.Lzp;
     }
     const double ddx = xgd - xb, ddy = ygd - yb, ddz = zgd - zb;
+    if (bad && (X < 2 || X > nx - 1 || Z < 2 || Z > nz - 1 || Y < 2 || Y > ylim))
+      atomicMax(bad, l + 1);
     st.idx[(0 * 5 + t) * nb + l] = X;
     st.idx[(1 * 5 + t) * nb + l] = Y;
     st.idx[(2 * 5 + t) * nb + l] = Z;
@@ -128,6 +132,80 @@ __global__ void k_setup(DevGrid g, SetupGrid sg, Markers m) {
     st.phi[(1 * 5 + t) * nb + l] = deltafnc(ddy, dymin);
     st.phi[(2 * 5 + t) * nb + l] = deltafnc(ddz, dz);
   }
+}
+
+// Wall height of marker i (1-based, x-periodic) of one wall at the time wt
+// (src/immersed_boundary_geometry.f90:186-191 standing, :295-299 traveling),
+// and x itself.  The standing wave's top wall returns the bottom wall's
+// curve: its areas are copied from the bottom wall (:264).
+__device__ __forceinline__ void wall_point(const WaveBody& w, const WaveTime& wt, int i, bool top,
+                                           double& x, double& y) {
+  const double dxb = w.Lxp / (double)w.nxb;
+  x = ((double)i - 0.5) * dxb;
+  if (w.body == 1) {
+    y = w.p1 * sin(2.0 * w.pi * w.p3 * x / w.Lxp) * wt.ct;
+  } else {
+    const double phase = w.p3 * (x - w.p2 * wt.t);
+    y = wt.amp / (w.p2 * w.p3) * sin(top ? phase + w.pi : phase);
+  }
+}
+
+// One thread per marker: position, velocity, normal, area and y_ref_index of
+// the moving walls; the area sb = dzb (a1/2 + a2/2) from the chords to the
+// x-periodic neighbours on the same wall (:234-262, :346-378).
+__global__ void k_wave_geometry(WaveBody w, WaveTime wt, double* __restrict__ xb,
+                                double* __restrict__ yb, double* __restrict__ zb,
+                                double* __restrict__ normals, double* __restrict__ sb,
+                                double* __restrict__ ub, int* __restrict__ yref, int* bad) {
+  const int nxb = w.nxb, nb = 2 * nxb * w.nzb;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int j = k / (2 * nxb) + 1, r = k - (j - 1) * 2 * nxb;
+  const bool top = r >= nxb;
+  const int i = (top ? r - nxb : r) + 1;
+  const double dzb = w.Lzp / (double)w.nzb;
+  // own point, and the curve the neighbours' chords are measured on
+  const bool curve_top = top && w.body == 2;
+  double x, yc;
+  wall_point(w, wt, i, curve_top, x, yc);
+  const double y = top ? w.Lyc + yc : yc;
+  double s, vy;
+  if (w.body == 1) {
+    const double phase = 2.0 * w.pi * w.p3 * x / w.Lxp;
+    vy = -(w.p1 * w.p2 * sin(phase) * wt.st);
+    s = w.p1 * 2.0 * w.pi * w.p3 / w.Lxp * cos(phase) * wt.ct;
+  } else {
+    const double phase = w.p3 * (x - w.p2 * wt.t);
+    const double ph = top ? phase + w.pi : phase;
+    vy = -wt.amp * cos(ph) + (wt.damp / (w.p2 * w.p3)) * sin(ph);
+    s = wt.amp / w.p2 * cos(ph);
+  }
+  const double t1 = 1.0 / sqrt(1.0 + s * s), t2 = s / sqrt(1.0 + s * s);
+  // chords on the curve (the top wall of a traveling wave on its own curve)
+  double xp, yp, xn, yn;
+  wall_point(w, wt, i == 1 ? nxb : i - 1, curve_top, xp, yp);
+  wall_point(w, wt, i == nxb ? 1 : i + 1, curve_top, xn, yn);
+  const double ys = curve_top ? w.Lyc + yc : yc;
+  if (curve_top) {
+    yp = w.Lyc + yp;
+    yn = w.Lyc + yn;
+  }
+  if (i == 1) xp = xp - w.Lxp;
+  if (i == nxb) xn = xn + w.Lxp;
+  const double a1 = sqrt((x - xp) * (x - xp) + (ys - yp) * (ys - yp));
+  const double a2 = sqrt((x - xn) * (x - xn) + (ys - yn) * (ys - yn));
+  xb[k] = x;
+  yb[k] = y;
+  zb[k] = ((double)j - 0.5) * dzb;
+  sb[k] = dzb * (0.5 * a1 + 0.5 * a2);
+  yref[k] = top ? w.ny : 1;
+  normals[k] = top ? -t2 : t2;
+  normals[nb + k] = top ? t1 : -t1;
+  normals[2 * nb + k] = 0.0;
+  ub[k] = 0.0;
+  ub[nb + k] = vy;
+  ub[2 * nb + k] = 0.0;
+  if (top ? (y > w.yhi) : (y < w.ylo)) atomicMax(bad, k + 1);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -781,8 +859,18 @@ inline dim3 wave_blocks(int nb) { return dim3((unsigned)((nb + kBlock / 64 - 1) 
 
 }  // namespace
 
-hipError_t launch_setup(const DevGrid& g, const SetupGrid& sg, const Markers& m, hipStream_t s) {
-  hipLaunchKernelGGL(k_setup, blocks_for((size_t)4 * m.nb), dim3(kBlock), 0, s, g, sg, m);
+hipError_t launch_setup(const DevGrid& g, const SetupGrid& sg, const Markers& m, int* bad,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_setup, blocks_for((size_t)4 * m.nb), dim3(kBlock), 0, s, g, sg, m, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_wave_geometry(const WaveBody& w, const WaveTime& wt, double* xb, double* yb,
+                                double* zb, double* normals, double* sb, double* ub, int* yref,
+                                int* bad, hipStream_t s) {
+  const size_t nb = 2 * (size_t)w.nxb * w.nzb;
+  hipLaunchKernelGGL(k_wave_geometry, blocks_for(nb), dim3(kBlock), 0, s, w, wt, xb, yb, zb,
+                     normals, sb, ub, yref, bad);
   return hipGetLastError();
 }
 

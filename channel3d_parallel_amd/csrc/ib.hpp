@@ -28,6 +28,7 @@ struct Markers {
   Stencil st[4];  // 0 u, 1 v, 2 w, 3 centre
   double A;       // dx*dymin*dz
   double factor;  // 1/(dx*dymin*dz)
+  const int* yref;  // y_ref_index per marker, the face anchoring the y pivots (nullptr: 1)
 };
 
 // Node -> entry map of the spreading operator of one component in jagged
@@ -54,7 +55,32 @@ struct SetupGrid {
   double dx, dz, dymin, Lxp, Lzp;
 };
 
-hipError_t launch_setup(const DevGrid& g, const SetupGrid& sg, const Markers& m, hipStream_t s);
+// bad (may be nullptr): set to the largest 1-based index of a marker with a
+// stencil node off the canonical interior (x in [2, nx-1], z in [2, nz-1],
+// y in [2, nyg-1], v: [2, ny-1]); left unchanged when there is none.
+hipError_t launch_setup(const DevGrid& g, const SetupGrid& sg, const Markers& m, int* bad,
+                        hipStream_t s);
+
+// Moving wall bodies of setup_IB_geometry (src/immersed_boundary_geometry.f90:
+// 176-390).  Markers k = i + 2 nxb (j-1) (bottom wall) and k + nxb (top wall),
+// i = 1..nxb, j = 1..nzb.  Scalars that depend on t alone are evaluated on the
+// host: ct, st = cos, sin(p2 t) (standing wave), amp and damp = amp' (the
+// traveling wave's ramp).
+struct WaveBody {
+  int body;  // 1 standing_wave, 2 traveling_wave_x
+  int nxb, nzb, ny;
+  double p1, p2, p3;
+  double Lxp, Lzp, Lyc, pi;
+  double ylo, yhi;  // y(1 + suppy), y(ny - suppy): the reference's stop bounds
+};
+struct WaveTime {
+  double t, ct, st, amp, damp;
+};
+// xb, yb, zb, sb, yref [nb]; normals, ub [3nb] (z blocks zero).  bad: largest
+// 1-based index of a marker outside [ylo, yhi] (unchanged if none).
+hipError_t launch_wave_geometry(const WaveBody& w, const WaveTime& wt, double* xb, double* yb,
+                                double* zb, double* normals, double* sb, double* ub, int* yref,
+                                int* bad, hipStream_t s);
 
 // E (with_dxn false) or E1n (true) of three staggered fields -> out[3nb].
 hipError_t launch_interp_uvw(const DevGrid& g, const Markers& m, const double* U, const double* V,
@@ -84,6 +110,17 @@ hipError_t launch_spread3(const Markers& m, const Spread sp[3], const double* f,
                           double* F1, double* F2, hipStream_t s);
 // Fills sp.ml and sp.wfs of component comp from the stencils (once per body).
 hipError_t launch_spread_coeffs(const Markers& m, const Spread& sp, int comp, hipStream_t s);
+
+// Device construction of a Spread map from the stencil indices (ibmap.hip),
+// identical to the host construction: entries sorted by node offset (stable,
+// so ascending l*125 + point within a node), ranks by descending count
+// (stable in the offset).  The Spread arrays node, cnt, q, ml, wfs hold
+// nb*125 entries and slot nb*125 + 1; work is the scratch of
+// spread_map_work_bytes(nb, nf).  The node count is written to *nnodes
+// (device int); the caller copies it into sp.nnodes before spreading.
+size_t spread_map_work_bytes(int nb, size_t nf);
+hipError_t build_spread_map(const Markers& m, int comp, int n1, int n2, size_t nf, void* work,
+                            Spread& sp, int* nnodes, hipStream_t s);
 // f[nb] -= per-body mean; then f -= coef*pj when coef != nullptr.  part
 // holds 128 doubles of scratch.
 hipError_t launch_remove_mean(const Markers& m, double* f, const double* coef, const double* pj,

@@ -210,12 +210,50 @@ class Channel:
     # src/projection.f90:48-186) ------------------------------------------------
     def setup_IB(self, m):
         """setup_IB_operators + compute_heaviside for a marker set ``m``
-        (attributes xb, yb, zb, normals, sb, ub, nzb, nxb, nxb1)."""
+        (attributes xb, yb, zb, normals, sb, ub, nzb, nxb, nxb1 and, if
+        present, yref = y_ref_index per marker)."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64)
                 for a in (m.xb, m.yb, m.zb, m.normals, m.sb, m.ub)]
         self.nb = int(arrs[0].size)
-        self._chk(self.L.c3d_ib_setup(self.h, self.nb, *[ptr(a) for a in arrs],
-                                      int(m.nzb), int(m.nxb), int(m.nxb1)))
+        yref = getattr(m, "yref", None)
+        yr = None if yref is None else np.ascontiguousarray(yref, dtype=np.int32)
+        self._chk(self.L.c3d_ib_setup_ref(self.h, self.nb, *[ptr(a) for a in arrs],
+                                          None if yr is None else iptr(yr),
+                                          int(m.nzb), int(m.nxb), int(m.nxb1)))
+
+    WAVE_BODIES = {"standing_wave": 1, "traveling_wave_x": 2}
+
+    def setup_IB_wave(self, body, nxb, nzb, p1, p2, p3, ramp=0.0, t=None):
+        """Moving wall body (standing_wave or traveling_wave_x) built on the
+        device at time t (default: the context's time)."""
+        t = self.time if t is None else t
+        self._chk(self.L.c3d_ib_setup_wave(self.h, self.WAVE_BODIES[body], nxb, nzb, p1, p2, p3,
+                                           ramp, t))
+        self.nb = 2 * nxb * nzb
+
+    def move_IB(self, t):
+        self._chk(self.L.c3d_ib_move(self.h, t))
+
+    def get_IB_markers(self):
+        """Marker arrays of the current body: xb, yb, zb, sb, yref [nb];
+        normals, ub [3nb]."""
+        nb = self.nb
+        xb, yb, zb, sb = (np.zeros(nb) for _ in range(4))
+        nrm, ub = np.zeros(3 * nb), np.zeros(3 * nb)
+        yref = np.zeros(nb, dtype=np.int32)
+        self._chk(self.L.c3d_ib_get_markers(self.h, ptr(xb), ptr(yb), ptr(zb), ptr(nrm), ptr(sb),
+                                            ptr(ub), iptr(yref)))
+        return dict(xb=xb, yb=yb, zb=zb, normals=nrm, sb=sb, ub=ub, yref=yref)
+
+    @property
+    def time(self):
+        t = np.zeros(1)
+        self._chk(self.L.c3d_get_time(self.h, ptr(t)))
+        return float(t[0])
+
+    @time.setter
+    def time(self, t):
+        self._chk(self.L.c3d_set_time(self.h, float(t)))
 
     def ib_stencil(self, s):
         n = self.nb * 125

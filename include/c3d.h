@@ -200,9 +200,8 @@ const char *c3d_stage_name(int i);
  * normals, ub [3nb] component-major (x block, y block, z block).  Markers
  * are grouped in nzb z slabs of nxb; the first nxb1 of a slab belong to
  * body 1, the rest to body 2 (the remove_mean_per_body grouping of
- * src/immersed_boundary_geometry.f90:417-441).  Static bodies on a uniform
- * y grid (grid_type 0, y_ref_index = 1), whose 5x5x5 stencils stay clear of
- * the wall ghost rows. */
+ * src/immersed_boundary_geometry.f90:417-441).  Every 5x5x5 stencil must
+ * stay clear of the wall ghost rows. */
 
 /* setup_IB_operators (src/immersed_boundary_operators.f90:19-256) followed
  * by compute_heaviside (src/heaviside.f90:20-54): builds the four delta
@@ -211,6 +210,40 @@ const char *c3d_stage_name(int i);
 int c3d_ib_setup(c3d_ctx *ctx, int nb, const double *xb, const double *yb, const double *zb,
                  const double *normals, const double *sb, const double *ub, int nzb, int nxb,
                  int nxb1);
+/* c3d_ib_setup with the per-marker y_ref_index (src/global.f90:247; the
+ * y pivots of src/immersed_boundary_operators.f90:35-37 count from face
+ * yref[l]); yref NULL means 1 for every marker.  Any grid type: the body's
+ * stencils must stay on interior rows (y pivots measured in dymin, so a
+ * body belongs in a region of uniform spacing). */
+int c3d_ib_setup_ref(c3d_ctx *ctx, int nb, const double *xb, const double *yb, const double *zb,
+                     const double *normals, const double *sb, const double *ub, const int *yref,
+                     int nzb, int nxb, int nxb1);
+
+/* Moving wall bodies: setup_IB_geometry for body 1 = standing_wave
+ * (src/immersed_boundary_geometry.f90:176-268) or 2 = traveling_wave_x
+ * (:270-385) with nxb markers along x per wall and nzb along z
+ * (nb = 2 nxb nzb, the bottom wall y_ref_index 1, the top wall ny), the
+ * walls at y = 0 and y = Ly of the descriptor; p1..p3 = body_param_1..3,
+ * ramp = body_ramp_up_time.  The markers, the stencils, the spreading maps
+ * and the Heaviside fields are all built on the device at time t
+ * (setup_IB_geometry + setup_IB_operators + compute_heaviside).  Errors: the
+ * standing wave's amplitude above desc.min_buffer_width (:177), a marker
+ * outside [y(3), y(ny-2)] (:199-215), a stencil reaching a wall ghost row. */
+int c3d_ib_setup_wave(c3d_ctx *ctx, int body, int nxb, int nzb, double p1, double p2, double p3,
+                      double ramp, double t);
+/* Moves the wall body to time t: the per-stage re-set-up of
+ * src/time_integration.f90:110-115 (c3d_time_step_rk3 calls it at every
+ * stage time t0 + rk_t(stage) dt).  Drops cached iteration graphs and the
+ * history start, whose Schur operator no longer applies. */
+int c3d_ib_move(c3d_ctx *ctx, double t);
+/* Marker arrays as the device holds them (any pointer may be NULL). */
+int c3d_ib_get_markers(c3d_ctx *ctx, double *xb, double *yb, double *zb, double *normals,
+                       double *sb, double *ub, int *yref);
+/* Simulation time t (src/initialization.f90:39; advanced by dt per
+ * c3d_time_step_rk3, src/time_integration.f90:91,108,137,169). */
+int c3d_set_time(c3d_ctx *ctx, double t);
+int c3d_get_time(const c3d_ctx *ctx, double *t);
+
 /* Stencil s (0 u, 1 v, 2 w, 3 centre) as the reference stores it
  * (src/immersed_boundary_operators.f90:46-244): for marker l and point
  * p = i + 5*(j + 5*k), entry [l*125 + p] of the Fortran node indices, the
