@@ -212,6 +212,65 @@ def ib_rk3_run(nx, ny, nz, nu, dt, U, V, W, nsteps, guess_mode):
                 "after_steps": {"max_slip": ch.check_slip(), "max_div": ch.check_divergence()}}
 
 
+def moving_section(nmoves=20, nsteps=3, cpu=True):
+    """Moving walls (standing wave, amplitude 0.03, omega 2, one wavelength
+    in x) on the buffered grid 130x129x66 faces (grid_type 2, alpha 2.6,
+    min_buffer_width 0.1), 128 x 64 markers per wall (nb = 16,384): the
+    per-stage re-set-up (geometry, stencils, spreading maps, Heaviside
+    Poisson solve) on the device, RK3 steps with it, and the same set-up
+    by the CPU oracle (what the reference does on the host every stage)."""
+    from channel3d_parallel_amd import Channel, fields
+    nx, ny, nz, nxb, nzb = 130, 129, 66, 128, 64
+    body, p1, p2, p3 = "standing_wave", 0.03, 2.0, 1.0
+    nu, dt = 1.0 / 2800.0, 0.005
+    kw = dict(alpha=2.6, grid_type=2, min_buffer_width=0.1)
+    out = {"workload": f"{body} A={p1} omega={p2} k=1, {nxb}x{nzb} markers per wall (nb = "
+                       f"{2 * nxb * nzb}), {nx}x{ny}x{nz} faces, buffered tanh y (grid_type 2)"}
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.0, **kw)
+    with Channel(nx, ny, nz, nu=nu, **kw) as ch:
+        ch.setup_IB_wave(body, nxb, nzb, p1, p2, p3, 0.0, 0.0)
+        for i in range(3):  # warm
+            ch.move_IB(0.01 * i)
+        ts = []
+        for i in range(nmoves):
+            t0 = time.perf_counter()
+            ch.move_IB(0.1 + 0.05 * i)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out["move_ms"] = {"median": float(np.median(ts)), "min": float(np.min(ts))}
+        ch.set_fields(U, V, W)
+        ch.apply_boundary_conditions()
+        ch.set_pressure_gradient(3.0 * nu, 0.0, 0.0)
+        ch.set_IB_solver(tol=1e-10, maxit=2000, guess_mode=0)  # previous force (reference)
+        ch.time = 0.0
+        ch.compute_time_step_RK3(dt, x_mass_cte=0)  # warm (graphs, first solves)
+        _, acc0, _ = ch.ib_stats()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            ch.compute_time_step_RK3(dt, x_mass_cte=0)
+        step_ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        _, acc1, err = ch.ib_stats()
+        out["rk3_step_ms"] = step_ms
+        out["iterations_per_solve"] = (acc1 - acc0) / (3 * nsteps)
+        out["max_div"] = ch.check_divergence()
+        out["max_slip"] = ch.check_slip()
+    if cpu:
+        from oracle import orc
+        g = orc.Grid(nx, ny, nz, **kw)
+        tc = []
+        for i in range(2):
+            t0 = time.perf_counter()
+            m = orc.WaveMarkers(g, body, nxb, nzb, p1, p2, p3, 0.1 + 0.05 * i)
+            ib = orc.IB(g, m)
+            ib.heaviside()
+            tc.append((time.perf_counter() - t0) * 1e3)
+            del ib
+        out["cpu_move_ms"] = {"value": float(np.median(tc)), "kind": "port",
+                              "sample": "2 re-set-ups (median): oracle geometry, stencils, "
+                                        "Heaviside Poisson solve, as the reference's host does "
+                                        "every stage"}
+    return out
+
+
 def learned_section(nx, ny, nz, dt):
     """Learned Schur guess on the two-cylinder case: 20k training pairs from
     two runs with seeded rotation speeds in [0.3, 0.7] U_b, GPU training,
@@ -275,7 +334,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, RK3 and IB sections")
-    ap.add_argument("--only", choices=["ib", "mixed", "rk3"],
+    ap.add_argument("--only", choices=["ib", "mixed", "rk3", "moving"],
                     help="profiling runs: only the IB case (no learned guess), only the "
                          "mixed-precision projection with 2 refinement sweeps, or only RK3 steps")
     a = ap.parse_args()
@@ -287,6 +346,9 @@ def main():
         return
     if a.only == "ib":
         print(json.dumps({"ib": ib_section(nsteps=3, learned=False)}))
+        return
+    if a.only == "moving":
+        print(json.dumps({"moving_walls": moving_section(nmoves=5, nsteps=2, cpu=False)}))
         return
     if a.only == "mixed":
         from channel3d_parallel_amd import Channel, fields
@@ -385,6 +447,8 @@ def main():
         out["rk3"] = rk3_section(nx, ny, nz, U, V, W, 5)
         print("[bench] IB two-cylinder case", file=sys.stderr, flush=True)
         out["ib"] = ib_section()
+        print("[bench] moving walls", file=sys.stderr, flush=True)
+        out["moving_walls"] = moving_section(cpu=not a.no_cpu)
     if not a.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         print(f"[bench] CPU baseline: oracle, {threads} threads, {a.cpu_solves} solves",

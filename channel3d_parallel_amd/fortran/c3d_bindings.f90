@@ -171,6 +171,49 @@ Module c3d_bindings
        Real(c_double), Intent(In) :: xb(*), yb(*), zb(*), normals(*), sb(*), ub(*)
      End Function c3d_ib_setup
 
+     Integer(c_int) Function c3d_ib_setup_ref(ctx, nb, xb, yb, zb, normals, sb, ub, yref, nzb, &
+          nxb, nxb1) Bind(C, name="c3d_ib_setup_ref")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: nb, nzb, nxb, nxb1
+       Real(c_double), Intent(In) :: xb(*), yb(*), zb(*), normals(*), sb(*), ub(*)
+       Integer(c_int), Intent(In) :: yref(*)
+     End Function c3d_ib_setup_ref
+
+     Integer(c_int) Function c3d_ib_setup_wave(ctx, body, nxb, nzb, p1, p2, p3, ramp, t) &
+          Bind(C, name="c3d_ib_setup_wave")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: body, nxb, nzb
+       Real(c_double), Value :: p1, p2, p3, ramp, t
+     End Function c3d_ib_setup_wave
+
+     Integer(c_int) Function c3d_ib_move(ctx, t) Bind(C, name="c3d_ib_move")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: t
+     End Function c3d_ib_move
+
+     Integer(c_int) Function c3d_ib_get_markers(ctx, xb, yb, zb, normals, sb, ub, yref) &
+          Bind(C, name="c3d_ib_get_markers")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: xb(*), yb(*), zb(*), normals(*), sb(*), ub(*)
+       Integer(c_int), Intent(Out) :: yref(*)
+     End Function c3d_ib_get_markers
+
+     Integer(c_int) Function c3d_set_time(ctx, t) Bind(C, name="c3d_set_time")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: t
+     End Function c3d_set_time
+
+     Integer(c_int) Function c3d_get_time(ctx, t) Bind(C, name="c3d_get_time")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: t
+     End Function c3d_get_time
+
      Integer(c_int) Function c3d_ib_set_solver(ctx, tol, maxit, guess_mode) &
           Bind(C, name="c3d_ib_set_solver")
        Import :: c_int, c_ptr, c_double

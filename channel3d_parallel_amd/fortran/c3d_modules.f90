@@ -29,6 +29,7 @@ Module c3d_global
   ! immersed boundary: a body is set up (body_type /= 'none'), BiCGSTAB
   ! settings (src/global.f90, namelist &params) and iteration counters
   Logical        :: ib_active = .False.
+  Logical        :: moving_body = .False.  ! wall body re-set-up every stage
   Real(Int64)    :: cg_tol = 1d-10
   Integer(Int32) :: cg_max_iter = 1000
   Integer(Int32) :: cg_guess = 0   ! 0 previous force (reference), 1 zero

@@ -58,6 +58,46 @@ Contains
 End Module mass_flow
 
 !------------------------------------------------------------------------------!
+Module immersed_boundary_geometry
+  Use c3d_global
+  Use c3d_flow
+  Implicit None
+Contains
+  ! setup_IB_geometry + setup_IB_operators + compute_heaviside for the moving
+  ! walls (src/immersed_boundary_geometry.f90:176-390): body_type
+  ! 'standing_wave' or 'traveling_wave_x', nxb x nzb markers per wall, at
+  ! the current time t; everything is built on the device.
+  Subroutine setup_IB_geometry(body_type, nxb, nzb, body_param_1, body_param_2, body_param_3, &
+                               body_ramp_up_time)
+    Character(*), Intent(In)   :: body_type
+    Integer(Int32), Intent(In) :: nxb, nzb
+    Real(Int64), Intent(In)    :: body_param_1, body_param_2, body_param_3, body_ramp_up_time
+    Integer(c_int) :: body
+    Select Case (Trim(body_type))
+    Case ('standing_wave')
+       body = 1
+    Case ('traveling_wave_x')
+       body = 2
+    Case Default
+       Write(error_unit,*) 'setup_IB_geometry: unsupported moving body ', Trim(body_type)
+       Error Stop 1
+    End Select
+    Call c3d_check(c3d_ib_setup_wave(ctx, body, Int(nxb, c_int), Int(nzb, c_int), body_param_1, &
+                                     body_param_2, body_param_3, body_ramp_up_time, t), &
+                   'setup_IB_geometry')
+    Call c3d_check(c3d_ib_set_solver(ctx, cg_tol, Int(cg_max_iter, c_int), Int(cg_guess, c_int)), &
+                   'ib_set_solver')
+    ib_active = .True.
+    moving_body = .True.
+  End Subroutine setup_IB_geometry
+
+  ! The per-stage update of src/time_integration.f90:110-115 at time t.
+  Subroutine move_IB_geometry
+    Call c3d_check(c3d_ib_move(ctx, t), 'setup_IB_geometry (move)')
+  End Subroutine move_IB_geometry
+End Module immersed_boundary_geometry
+
+!------------------------------------------------------------------------------!
 Module time_integration
   Use c3d_global
   Use c3d_flow
@@ -65,9 +105,10 @@ Module time_integration
   Use mass_flow
   Use boundary_conditions
   Use projection
+  Use immersed_boundary_geometry
   Implicit None
 Contains
-  ! compute_time_step_RK3, src/time_integration.f90:86-213 (static body)
+  ! compute_time_step_RK3, src/time_integration.f90:86-213
   Subroutine compute_time_step_RK3
     Real(Int64) :: to
     Integer(Int32) :: stage
@@ -78,6 +119,7 @@ Contains
        Call compute_rhs(stage)                                         ! F_stage
        Call c3d_check(c3d_rk3_update(ctx, Int(stage, c_int), dt), 'rk3_update')
        t = to + rk_t(stage)*dt
+       If (moving_body) Call move_IB_geometry                          ! :110-115
        Call apply_boundary_conditions_resident
        Call compute_non_IB_projection
        If (ib_active) Then                                             ! :119-122

@@ -145,3 +145,60 @@ def test_fortran_cylinders_matches_c_abi(tmp_path):
     assert np.array_equal(fb_f, fb)
     assert int(iters_f) == accum
     assert div_f < 1e-12 and slip_f < 1e-9
+
+
+WAVES_FD = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "waves_fd")
+
+
+@pytest.mark.parametrize("body,p1,p2,p3,ramp", [("standing_wave", 0.04, 2.0, 1.0, 0.0),
+                                                 ("traveling_wave_x", 0.03, 1.0, 1.0, 0.5)])
+def test_fortran_waves_matches_c_abi(tmp_path, body, p1, p2, p3, ramp):
+    """The Fortran time loop with moving walls (setup_IB_geometry, then the
+    per-stage move inside compute_time_step_RK3) gives bitwise the fields,
+    force, marker heights and time of the same steps driven through the
+    C ABI's compute_time_step_RK3."""
+    from channel3d_parallel_amd import Channel
+    if not os.path.exists(WAVES_FD):
+        pytest.fail("Fortran waves_fd not built (make -C channel3d_parallel_amd/fortran)")
+    nx, ny, nz, alpha, mbw, nsteps = 34, 65, 18, 2.6, 0.1, 3
+    nxb, nzb = 32, 8
+    nu, dt, dpdx, t0 = 1.0 / 2800.0, 0.01, 3.0 / 2800.0, 0.2
+    print(f"\n[case] fortran waves_fd {body} {nx}x{ny}x{nz} steps={nsteps}", flush=True)
+    kw = dict(alpha=alpha, grid_type=2, min_buffer_width=mbw)
+    g = orc.Grid(nx, ny, nz, **kw)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.05, seed=3, **kw)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
+    r = subprocess.run([WAVES_FD, str(nx), str(ny), str(nz), repr(alpha), repr(mbw), repr(nu),
+                        repr(dt), repr(dpdx), str(nsteps), body, str(nxb), str(nzb), repr(p1),
+                        repr(p2), repr(p3), repr(ramp), repr(t0), str(fin), str(fout)],
+                       check=True, timeout=120, capture_output=True, text=True)
+    print(r.stdout)
+    nb = 2 * nxb * nzb
+    raw = np.fromfile(fout)
+    outs, off = [], 0
+    for s in (g.shape_u, g.shape_v, g.shape_w):
+        n = int(np.prod(s))
+        outs.append(raw[off:off + n].reshape(s, order="F"))
+        off += n
+    fb_f, yb_f = raw[off:off + 4 * nb], raw[off + 4 * nb:off + 5 * nb]
+    t_f, div_f, slip_f, iters_f = raw[off + 5 * nb:off + 5 * nb + 4]
+    g.apply_bc(U, V, W)
+    with Channel(nx, ny, nz, nu=nu, **kw) as ch:
+        ch.set_fields(U, V, W)
+        ch.time = t0
+        ch.setup_IB_wave(body, nxb, nzb, p1, p2, p3, ramp)
+        ch.set_pressure_gradient(dpdx, 0.0, 0.0)
+        ch.set_IB_solver(tol=1e-10, maxit=1000, guess_mode=0)
+        for _ in range(nsteps):
+            ch.compute_time_step_RK3(dt, x_mass_cte=0)
+        ref = ch.get_fields()
+        fb = ch.get_IB_force()
+        yb = ch.get_IB_markers()["yb"]
+        _, accum, _ = ch.ib_stats()
+        t = ch.time
+    for name, a, b in (("U", outs[0], ref["U"]), ("V", outs[1], ref["V"]), ("W", outs[2], ref["W"])):
+        assert np.array_equal(a, b), name
+    assert np.array_equal(fb_f, fb) and np.array_equal(yb_f, yb)
+    assert t_f == t and int(iters_f) == accum
+    assert div_f < 1e-12 and slip_f < 1e-9
