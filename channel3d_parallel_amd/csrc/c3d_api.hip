@@ -1605,8 +1605,8 @@ int c3d_ib_setup_ref(c3d_ctx* c, int nb, const double* xb, const double* yb, con
                      int nzb, int nxb, int nxb1) {
   if (!c || !xb || !yb || !zb || !normals || !sb || !ub) return C3D_EINVAL;
   if (c->ib) return fail(c, C3D_EINVAL, "an immersed body is already set up on this context");
-  if (nb <= 0 || nzb <= 0 || nxb <= 0 || (long long)nzb * nxb != nb || nxb1 <= 0 || nxb1 >= nxb)
-    return fail(c, C3D_EINVAL, "marker counts: nb = nzb*nxb and 0 < nxb1 < nxb required");
+  if (nb <= 0 || nzb <= 0 || nxb <= 0 || (long long)nzb * nxb != nb || nxb1 <= 0 || nxb1 > nxb)
+    return fail(c, C3D_EINVAL, "marker counts: nb = nzb*nxb and 0 < nxb1 <= nxb required");
   if ((long long)nb * 125 > 0x7fffffff) return fail(c, C3D_EINVAL, "too many markers");
   for (int l = 0; l < nb; l++) {
     if (!(xb[l] >= 0.0 && xb[l] < c->Lxp && zb[l] >= 0.0 && zb[l] < c->Lzp && yb[l] > c->y[0] &&

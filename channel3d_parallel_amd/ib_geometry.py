@@ -1,4 +1,4 @@
-"""Lagrangian marker sets for z-spanning cylinders.
+"""Lagrangian marker sets for z-spanning cylinders and the centre wall.
 
 Arrays follow the reference's conventions (src/immersed_boundary_geometry.f90):
 ``xb, yb, zb, sb`` have ``nb`` entries; ``normals`` and ``ub`` are
@@ -32,10 +32,30 @@ class Markers:
     nzb: int
     nxb: int
     nxb1: int
+    yref: np.ndarray | None = None  # y_ref_index per marker (None: 1)
 
     @property
     def nb(self) -> int:
         return int(self.xb.size)
+
+
+def center_wall(Lxp, Lzp, Ly, ny, nxb, nzb):
+    """``center_wall`` (src/immersed_boundary_geometry.f90:80-115): a static
+    planar wall at y = Ly/2 across the whole channel, nxb x nzb markers at
+    the cell-centred positions ((i - 1/2) dxb, Ly/2, (j - 1/2) dzb), area
+    dxb dzb, normal (0, -1, 0), at rest, y pivots counted from face ny / 2
+    (integer division, :87).  One body: nxb1 = nxb, so the mean removal
+    subtracts the plain mean (:410-411).  The reference allows it on the
+    uniform grid only (:81)."""
+    nb = nxb * nzb
+    dxb, dzb = Lxp / nxb, Lzp / nzb
+    i = np.tile(np.arange(1, nxb + 1, dtype=np.float64), nzb)
+    j = np.repeat(np.arange(1, nzb + 1, dtype=np.float64), nxb)
+    normals = np.zeros(3 * nb)
+    normals[nb:2 * nb] = -1.0
+    return Markers(xb=(i - 0.5) * dxb, yb=np.full(nb, 0.5 * Ly), zb=(j - 0.5) * dzb,
+                   normals=normals, sb=np.full(nb, dxb * dzb), ub=np.zeros(3 * nb),
+                   nzb=nzb, nxb=nxb, nxb1=nxb, yref=np.full(nb, ny // 2, dtype=np.int32))
 
 
 def _ring(r, xc, yc, n, dzb, nzb, j, speed, normal_sign):

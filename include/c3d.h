@@ -200,7 +200,8 @@ const char *c3d_stage_name(int i);
  * normals, ub [3nb] component-major (x block, y block, z block).  Markers
  * are grouped in nzb z slabs of nxb; the first nxb1 of a slab belong to
  * body 1, the rest to body 2 (the remove_mean_per_body grouping of
- * src/immersed_boundary_geometry.f90:417-441).  Every 5x5x5 stencil must
+ * src/immersed_boundary_geometry.f90:417-441); nxb1 = nxb is a single body
+ * (center_wall, :410-411).  Every 5x5x5 stencil must
  * stay clear of the wall ghost rows. */
 
 /* setup_IB_operators (src/immersed_boundary_operators.f90:19-256) followed

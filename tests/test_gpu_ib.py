@@ -147,6 +147,44 @@ def test_ib_projection_parity():
         assert rel(a, b) < 1e-8
 
 
+def test_center_wall_projection_parity():
+    """center_wall: a single-body planar wall at y = 1 with y pivots from face
+    ny/2 (src/immersed_boundary_geometry.f90:80-115); IB projection and the
+    one-body mean removal against the oracle."""
+    nx, ny, nz = 34, 33, 34
+    print(f"\n[case] center_wall {nx}x{ny}x{nz}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=0)
+    s = g.scalars
+    m = ib_geometry.center_wall(s["Lxp"], s["Lzp"], 2.0, ny, 32, 32)
+    ib = orc.IB(g, m)
+    ib.heaviside()
+    U, V, W = _laminar(g, nx, ny, nz)
+    dt, tol = 0.005, 1e-10
+    Uo, Vo, Wo = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    out = g.projection(Uo, Vo, Wo, rk_step=2, dt=dt)
+    g.apply_bc(Uo, Vo, Wo)
+    fbo = np.zeros(4 * m.nb)
+    ito, eo = ib.projection(Uo, Vo, Wo, out["U_interim"], out["V_interim"], out["W_interim"],
+                            out["P_interim"], fbo, dt, NU, tol=tol, maxit=1000)
+    f = np.random.default_rng(8).standard_normal(m.nb)
+    with Channel(nx, ny, nz, grid_type=0, nu=NU) as ch:
+        ch.setup_IB(m)
+        assert rel(ch.remove_mean_per_body(f), ib.remove_mean(f.copy())) < 1e-13
+        ch.set_fields(U, V, W)
+        ch.set_IB_solver(tol=tol, maxit=1000, guess_mode=1)
+        ch.compute_non_IB_projection(dt, 2)
+        ch.apply_boundary_conditions()
+        itg, eg = ch.compute_IB_projection(dt, 2)
+        r = ch.get_fields()
+        fbg = ch.get_IB_force()
+        slip, div = ch.check_slip(), ch.check_divergence()
+    print(f"  iterations gpu={itg} oracle={ito}; slip={slip:.2e}; div={div:.2e}")
+    assert eg < tol ** 2 and eo < tol ** 2 and slip < 1e-9 and div < 1e-12
+    assert rel(fbg, fbo) < 1e-6
+    for a, b in ((r["U"], Uo), (r["V"], Vo), (r["W"], Wo)):
+        assert rel(a, b) < 1e-8
+
+
 def test_time_step_with_body():
     """Two RK3 steps with the body against the oracle composed stage by stage
     (src/time_integration.f90:98-186 with the IB projection of :118-122)."""

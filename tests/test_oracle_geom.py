@@ -114,3 +114,24 @@ def test_wave_stencils_partition_of_unity(grid, body, p1, p2, p3, ramp, t):
             assert np.allclose(w.sum(axis=1), 1.0, rtol=0, atol=1e-12), s
         ycoord = (yf if s == 1 else yg)[yi.reshape(m.nb, 125) - 1]
         assert np.all(np.abs(ycoord - m.yb[:, None]) < 3.01 * grid.scalars["dymin"])
+
+
+def test_center_wall_markers_and_single_body_mean():
+    """center_wall (src/immersed_boundary_geometry.f90:80-115): markers on
+    the plane y = 1 at the cell-centred x, z positions, pivots from face
+    ny/2, weights that add up to one on the uniform grid, and the one-body
+    mean removal f - mean(f) (:410-411)."""
+    from channel3d_parallel_amd import ib_geometry
+    g = orc.Grid(34, 33, 34, grid_type=0)
+    s = g.scalars
+    m = ib_geometry.center_wall(s["Lxp"], s["Lzp"], 2.0, 33, 32, 16)
+    assert m.nb == 512 and m.nxb1 == m.nxb and np.all(m.yref == 16)
+    assert np.all(m.yb == 1.0) and np.allclose(m.sb, s["Lxp"] / 32 * s["Lzp"] / 16, rtol=1e-15)
+    assert np.all(m.normals[m.nb:2 * m.nb] == -1.0) and np.all(m.ub == 0.0)
+    ib = orc.IB(g, m)
+    for st in range(4):
+        w = ib.stencil(st)[3].reshape(m.nb, 125)
+        assert np.allclose(w.sum(axis=1), 1.0, rtol=0, atol=1e-12), st
+    f = np.random.default_rng(2).standard_normal(m.nb)
+    r = ib.remove_mean(f.copy())
+    assert np.allclose(r, f - f.mean(), rtol=0, atol=1e-15)
