@@ -19,6 +19,7 @@ C3D_OK = 0
 ERRORS = {-1: "C3D_EINVAL", -2: "C3D_EHIP", -3: "C3D_ENOMEM", -4: "C3D_ENOCONV", -5: "C3D_ENAN"}
 PREC_FP64 = 0
 PREC_MIXED = 1
+RESTORE_P, RESTORE_STATE, RESTORE_FORCE, RESTORE_ALL = 1, 2, 4, 7
 
 
 class c3d_desc(C.Structure):
@@ -72,6 +73,8 @@ SIGNATURES = [
     ("c3d_ib_get_markers", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
     ("c3d_set_time", C.c_int, [C.c_void_p, C.c_double]),
     ("c3d_get_time", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_write_snapshot", C.c_int, [C.c_void_p, C.c_char_p, C.c_double, _dp, _dp]),
+    ("c3d_read_snapshot", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _dp]),
     ("c3d_ib_get_stencil", C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]),
     ("c3d_ib_get_heaviside", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("c3d_ib_interpolate", C.c_int, [C.c_void_p, C.c_int, _dp]),

@@ -2261,3 +2261,244 @@ int c3d_ib_last_pair(c3d_ctx* c, float* X, double* dF) {
 }
 
 }  // extern "C"
+
+// ---- Snapshots: the reference's big-endian stream file -----------------
+// src/input_output.f90:498-728 (output_data) and :335-489 (read_input_data),
+// one domain: every field is written whole, in the reference's global shape.
+
+namespace {
+
+struct BigEndianFile {
+  FILE* f = nullptr;
+  bool ok = true;
+  ~BigEndianFile() {
+    if (f) fclose(f);
+  }
+  static uint32_t swap32(uint32_t v) { return __builtin_bswap32(v); }
+  static uint64_t swap64(uint64_t v) { return __builtin_bswap64(v); }
+  void put_i4(int32_t v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u = swap32(u);
+    ok = ok && fwrite(&u, 4, 1, f) == 1;
+  }
+  void put_f8s(const double* a, size_t n) {
+    std::vector<uint64_t> buf(std::min<size_t>(n, 1 << 20));
+    for (size_t o = 0; o < n && ok; o += buf.size()) {
+      const size_t m = std::min(buf.size(), n - o);
+      std::memcpy(buf.data(), a + o, m * 8);
+      for (size_t i = 0; i < m; i++) buf[i] = swap64(buf[i]);
+      ok = fwrite(buf.data(), 8, m, f) == m;
+    }
+  }
+  void put_vec(const double* a, size_t n) {  // Write(1) Shape(a), a
+    put_i4((int32_t)n);
+    put_f8s(a, n);
+  }
+  bool get_i4(int32_t& v) {
+    uint32_t u;
+    if (fread(&u, 4, 1, f) != 1) return ok = false;
+    u = swap32(u);
+    std::memcpy(&v, &u, 4);
+    return true;
+  }
+  bool get_f8s(double* a, size_t n) {
+    if (fread(a, 8, n, f) != n) return ok = false;
+    uint64_t* u = reinterpret_cast<uint64_t*>(a);
+    for (size_t i = 0; i < n; i++) u[i] = swap64(u[i]);
+    return true;
+  }
+  bool at_end() {
+    const int ch = fgetc(f);
+    if (ch == EOF) return true;
+    ungetc(ch, f);
+    return false;
+  }
+};
+
+int download(c3d_ctx* c, const double* dev, size_t n, std::vector<double>& h) {
+  h.resize(n);
+  HIPCHK(c, hipMemcpyAsync(h.data(), dev, n * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+// interior centres xm(i) = xg(i+1), i = 1..n-1 (src/initialization.f90:172-180)
+std::vector<double> centres_of(const std::vector<double>& g) {
+  return std::vector<double>(g.begin() + 1, g.end() - 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int c3d_write_snapshot(c3d_ctx* c, const char* path, double dt, const double* tangents_1,
+                       const double* tangents_2) {
+  if (!c || !path) return C3D_EINVAL;
+  BigEndianFile out;
+  out.f = fopen(path, "wb");
+  if (!out.f) return fail(c, C3D_EINVAL, "cannot open %s for writing", path);
+  // mesh (:520-527)
+  out.put_vec(c->x.data(), c->x.size());
+  out.put_vec(c->y.data(), c->y.size());
+  out.put_vec(c->z.data(), c->z.size());
+  const std::vector<double> xm = centres_of(c->xg), ym = centres_of(c->yg), zm = centres_of(c->zg);
+  out.put_vec(xm.data(), xm.size());
+  out.put_vec(ym.data(), ym.size());
+  out.put_vec(zm.data(), zm.size());
+  // U, V, W, P, then the interior Heaviside masks (:530-698)
+  const int dims[7][3] = {{c->nx, c->nyg, c->nzg},  {c->nxg, c->ny, c->nzg}, {c->nxg, c->nyg, c->nz},
+                          {c->nxg, c->nyg, c->nzg}, {c->nx, c->nyg, c->nzg}, {c->nxg, c->ny, c->nzg},
+                          {c->nxg, c->nyg, c->nz}};
+  IBState* ib = c->ib;
+  const double* src[7] = {c->U, c->V, c->W, c->P, ib ? ib->Hu_int : nullptr,
+                          ib ? ib->Hv_int : nullptr, ib ? ib->Hw_int : nullptr};
+  std::vector<double> h;
+  for (int a = 0; a < 7 && out.ok; a++) {
+    const size_t n = (size_t)dims[a][0] * dims[a][1] * dims[a][2];
+    int rc = 0;
+    if (src[a]) {
+      if ((rc = download(c, src[a], n, h))) return rc;
+    } else {
+      h.assign(n, 0.0);  // no body: the reference's masks are allocated, never set
+    }
+    for (int d = 0; d < 3; d++) out.put_i4(dims[a][d]);
+    out.put_f8s(h.data(), n);
+  }
+  // t, dt, dpdx, nu and the body (:700-724)
+  const double scal[4] = {c->t, dt, c->dPdx, c->desc.nu};
+  out.put_f8s(scal, 4);
+  const size_t nb = ib ? (size_t)ib->m.nb : 0;
+  std::vector<double> mk[6], fb(4 * nb, 0.0);
+  if (ib) {
+    const double* msrc[6] = {ib->xb, ib->yb, ib->zb, ib->normals, ib->sb, ib->ub};
+    const size_t mn[6] = {nb, nb, nb, 3 * nb, nb, 3 * nb};
+    for (int a = 0; a < 6; a++) {
+      int rc = download(c, msrc[a], mn[a], mk[a]);
+      if (rc) return rc;
+    }
+    int rc = download(c, ib->fb, 4 * nb, fb);
+    if (rc) return rc;
+  }
+  for (int a = 0; a < 3; a++) out.put_vec(mk[a].data(), mk[a].size());
+  // nxb, nzb of the namelist: markers per slab (per wall for the wave bodies)
+  const int nxb = !ib ? 0 : (ib->moving ? ib->wave.nxb : ib->m.nxb);
+  out.put_i4(nxb);
+  out.put_i4(ib ? ib->m.nzb : 0);
+  out.put_vec(fb.data(), fb.size());
+  out.put_vec(mk[5].data(), mk[5].size());  // ub
+  out.put_vec(mk[4].data(), mk[4].size());  // sb
+  out.put_vec(mk[3].data(), mk[3].size());  // normals
+  const std::vector<double> zeros(3 * nb, 0.0);
+  out.put_vec(tangents_1 ? tangents_1 : zeros.data(), 3 * nb);
+  out.put_vec(tangents_2 ? tangents_2 : zeros.data(), 3 * nb);
+  if (!out.ok) return fail(c, C3D_EINVAL, "write to %s failed", path);
+  if (fclose(out.f) != 0) {
+    out.f = nullptr;
+    return fail(c, C3D_EINVAL, "closing %s failed", path);
+  }
+  out.f = nullptr;
+  return C3D_OK;
+}
+
+int c3d_read_snapshot(c3d_ctx* c, const char* path, int what, double* dt_out) {
+  if (!c || !path) return C3D_EINVAL;
+  if (what & ~(C3D_RESTORE_P | C3D_RESTORE_STATE | C3D_RESTORE_FORCE))
+    return fail(c, C3D_EINVAL, "unknown restore flags 0x%x", what);
+  BigEndianFile in;
+  in.f = fopen(path, "rb");
+  if (!in.f) return fail(c, C3D_EINVAL, "cannot open %s", path);
+  // mesh: sizes must match (:348-370); the coordinates must be this
+  // context's grid (the library derives every operator from it)
+  const std::vector<double> xm = centres_of(c->xg), ym = centres_of(c->yg), zm = centres_of(c->zg);
+  const std::vector<double>* mesh[6] = {&c->x, &c->y, &c->z, &xm, &ym, &zm};
+  const char* names[6] = {"x", "y", "z", "xm", "ym", "zm"};
+  std::vector<double> h;
+  for (int a = 0; a < 6; a++) {
+    int32_t n = 0;
+    if (!in.get_i4(n)) return fail(c, C3D_EINVAL, "%s: truncated mesh", path);
+    if (n != (int32_t)mesh[a]->size())
+      return fail(c, C3D_EINVAL, "%s: %s has %d points, the context %zu", path, names[a], n,
+                  mesh[a]->size());
+    h.resize(n);
+    if (!in.get_f8s(h.data(), n)) return fail(c, C3D_EINVAL, "%s: truncated mesh", path);
+    const double span = std::fabs(mesh[a]->back() - mesh[a]->front());
+    for (int i = 0; i < n; i++)
+      if (!(std::fabs(h[i] - (*mesh[a])[i]) <= 1e-12 * span))
+        return fail(c, C3D_EINVAL, "%s: %s(%d) = %.17g differs from the context's grid (%.17g)",
+                    path, names[a], i + 1, h[i], (*mesh[a])[i]);
+  }
+  const int dims[4][3] = {{c->nx, c->nyg, c->nzg}, {c->nxg, c->ny, c->nzg}, {c->nxg, c->nyg, c->nz},
+                          {c->nxg, c->nyg, c->nzg}};
+  std::vector<double> fld[4];
+  for (int a = 0; a < 4; a++) {
+    int32_t d[3];
+    if (!in.get_i4(d[0]) || !in.get_i4(d[1]) || !in.get_i4(d[2]))
+      return fail(c, C3D_EINVAL, "%s: truncated field header", path);
+    if (d[0] != dims[a][0] || d[1] != dims[a][1] || d[2] != dims[a][2])
+      return fail(c, C3D_EINVAL, "%s: wrong size in input file (%c)", path, "UVWP"[a]);
+    fld[a].resize((size_t)d[0] * d[1] * d[2]);
+    if (!in.get_f8s(fld[a].data(), fld[a].size()))
+      return fail(c, C3D_EINVAL, "%s: truncated field", path);
+  }
+  double dt = 0.0, t = 0.0, dpdx = 0.0;
+  std::vector<double> fb;
+  const bool more = (what & (C3D_RESTORE_STATE | C3D_RESTORE_FORCE)) || dt_out;
+  if (more) {
+    if (in.at_end()) {
+      if (what & (C3D_RESTORE_STATE | C3D_RESTORE_FORCE))
+        return fail(c, C3D_EINVAL, "%s: no time/body section (fields-only format)", path);
+    } else {
+      for (int a = 0; a < 3; a++) {  // Hu, Hv, Hw interior: recomputed from the body
+        int32_t d[3];
+        if (!in.get_i4(d[0]) || !in.get_i4(d[1]) || !in.get_i4(d[2]))
+          return fail(c, C3D_EINVAL, "%s: truncated mask header", path);
+        if (fseek(in.f, (long)((size_t)d[0] * d[1] * d[2] * 8), SEEK_CUR) != 0)
+          return fail(c, C3D_EINVAL, "%s: truncated mask", path);
+      }
+      double sc[4];
+      if (!in.get_f8s(sc, 4)) return fail(c, C3D_EINVAL, "%s: truncated scalars", path);
+      t = sc[0];
+      dt = sc[1];
+      dpdx = sc[2];
+      int32_t n = 0;
+      for (int a = 0; a < 3; a++) {  // xb, yb, zb
+        if (!in.get_i4(n) || fseek(in.f, (long)n * 8, SEEK_CUR) != 0)
+          return fail(c, C3D_EINVAL, "%s: truncated markers", path);
+      }
+      int32_t nxb = 0, nzb = 0;
+      if (!in.get_i4(nxb) || !in.get_i4(nzb) || !in.get_i4(n))
+        return fail(c, C3D_EINVAL, "%s: truncated body section", path);
+      fb.resize(n);
+      if (!in.get_f8s(fb.data(), n)) return fail(c, C3D_EINVAL, "%s: truncated fb", path);
+    }
+  }
+  bool force = (what & C3D_RESTORE_FORCE) != 0;
+  if (force) {  // a body-free file into a body-free context has no force to restore
+    if (!c->ib && !fb.empty())
+      return fail(c, C3D_EINVAL, "%s holds a body force: set the body up before reading", path);
+    if (c->ib && fb.size() != 4 * (size_t)c->ib->m.nb)
+      return fail(c, C3D_EINVAL, "%s: fb has %zu entries, the body 4*%d", path, fb.size(),
+                  c->ib->m.nb);
+    force = c->ib != nullptr;
+  }
+  // U, V, W always (the reference's restart); the rest on request
+  double* dst[4] = {c->U, c->V, c->W, c->P};
+  for (int a = 0; a < 4; a++) {
+    if (a == 3 && !(what & C3D_RESTORE_P)) break;
+    HIPCHK(c, hipMemcpyAsync(dst[a], fld[a].data(), fld[a].size() * 8, hipMemcpyHostToDevice,
+                             c->stream));
+  }
+  if (force)
+    HIPCHK(c, hipMemcpyAsync(c->ib->fb, fb.data(), fb.size() * 8, hipMemcpyHostToDevice, c->stream));
+  int rc = sync(c);
+  if (rc) return rc;
+  if (what & C3D_RESTORE_STATE) {
+    c->t = t;
+    c->dPdx = dpdx;
+    if (c->ib && c->ib->moving && (rc = ib_move_to(c, t))) return rc;
+  }
+  if (dt_out) *dt_out = dt;
+  return C3D_OK;
+}
+
+}  // extern "C"

@@ -125,6 +125,13 @@ Module c3d_bindings
        Real(c_double), Value :: dPdx, dPdy, dPdz
      End Function c3d_set_pressure_gradient
 
+     Integer(c_int) Function c3d_get_pressure_gradient(ctx, dPdx, dPdy, dPdz) &
+          Bind(C, name="c3d_get_pressure_gradient")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: dPdx, dPdy, dPdz
+     End Function c3d_get_pressure_gradient
+
      Integer(c_int) Function c3d_save_state(ctx) Bind(C, name="c3d_save_state")
        Import :: c_int, c_ptr
        Type(c_ptr), Value :: ctx
@@ -201,6 +208,24 @@ Module c3d_bindings
        Real(c_double), Intent(Out) :: xb(*), yb(*), zb(*), normals(*), sb(*), ub(*)
        Integer(c_int), Intent(Out) :: yref(*)
      End Function c3d_ib_get_markers
+
+     Integer(c_int) Function c3d_write_snapshot(ctx, path, dt, tangents_1, tangents_2) &
+          Bind(C, name="c3d_write_snapshot")
+       Import :: c_int, c_ptr, c_double, c_char
+       Type(c_ptr), Value :: ctx
+       Character(kind=c_char), Intent(In) :: path(*)
+       Real(c_double), Value :: dt
+       Type(c_ptr), Value :: tangents_1, tangents_2
+     End Function c3d_write_snapshot
+
+     Integer(c_int) Function c3d_read_snapshot(ctx, path, what, dt_out) &
+          Bind(C, name="c3d_read_snapshot")
+       Import :: c_int, c_ptr, c_double, c_char
+       Type(c_ptr), Value :: ctx
+       Character(kind=c_char), Intent(In) :: path(*)
+       Integer(c_int), Value :: what
+       Real(c_double), Intent(Out) :: dt_out
+     End Function c3d_read_snapshot
 
      Integer(c_int) Function c3d_set_time(ctx, t) Bind(C, name="c3d_set_time")
        Import :: c_int, c_ptr, c_double

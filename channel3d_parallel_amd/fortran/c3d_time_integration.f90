@@ -58,6 +58,45 @@ Contains
 End Module mass_flow
 
 !------------------------------------------------------------------------------!
+Module input_output
+  Use c3d_global
+  Use c3d_flow
+  Implicit None
+  Integer(c_int), Parameter :: RESTORE_ALL = 7  ! P, t and dPdx, fb (C3D_RESTORE_ALL)
+Contains
+  ! output_data (src/input_output.f90:498-728): the snapshot fname in the
+  ! reference's big-endian format, with the current t and dPdx.
+  Subroutine output_data(fname)
+    Character(*), Intent(In) :: fname
+    Call c3d_check(c3d_set_time(ctx, t), 'set_time')
+    Call c3d_check(c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz), 'set_pressure_gradient')
+    Call c3d_check(c3d_write_snapshot(ctx, Trim(fname)//c_null_char, dt, c_null_ptr, c_null_ptr), &
+                   'output_data')
+  End Subroutine output_data
+
+  ! read_input_data (src/input_output.f90:335-489): U, V, W as the reference
+  ! restores them, plus P, t, dPdx and fb (a body set up first) unless
+  ! reference_only is present and true.
+  Subroutine read_input_data(fname, reference_only)
+    Character(*), Intent(In) :: fname
+    Logical, Intent(In), Optional :: reference_only
+    Integer(c_int) :: what
+    Real(c_double) :: dt_file, dy, dz
+    what = RESTORE_ALL
+    If (.Not. ib_active) what = 3
+    If (Present(reference_only)) Then
+       If (reference_only) what = 0
+    End If
+    Call c3d_check(c3d_read_snapshot(ctx, Trim(fname)//c_null_char, what, dt_file), &
+                   'read_input_data')
+    If (Iand(what, 2) /= 0) Then
+       Call c3d_check(c3d_get_time(ctx, t), 'get_time')
+       Call c3d_check(c3d_get_pressure_gradient(ctx, dPdx, dy, dz), 'get_pressure_gradient')
+    End If
+  End Subroutine read_input_data
+End Module input_output
+
+!------------------------------------------------------------------------------!
 Module immersed_boundary_geometry
   Use c3d_global
   Use c3d_flow

@@ -27,6 +27,7 @@ failure raises :class:`C3DError` with the library's message.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -220,6 +221,22 @@ class Channel:
         self._chk(self.L.c3d_ib_setup_ref(self.h, self.nb, *[ptr(a) for a in arrs],
                                           None if yr is None else iptr(yr),
                                           int(m.nzb), int(m.nxb), int(m.nxb1)))
+
+    def write_snapshot(self, path, dt, tangents_1=None, tangents_2=None):
+        """output_data: the reference's big-endian snapshot of the resident
+        state (fields, Heaviside masks, t, dt, dpdx, nu, body, fb)."""
+        t1 = None if tangents_1 is None else np.ascontiguousarray(tangents_1, dtype=np.float64)
+        t2 = None if tangents_2 is None else np.ascontiguousarray(tangents_2, dtype=np.float64)
+        self._chk(self.L.c3d_write_snapshot(self.h, os.fsencode(path), dt,
+                                            None if t1 is None else ptr(t1),
+                                            None if t2 is None else ptr(t2)))
+
+    def read_snapshot(self, path, what=_lib.RESTORE_ALL):
+        """read_input_data: U, V, W from a snapshot (the reference's restart)
+        plus, per ``what``, P, t and dPdx, and fb.  Returns the recorded dt."""
+        dt = np.zeros(1)
+        self._chk(self.L.c3d_read_snapshot(self.h, os.fsencode(path), what, ptr(dt)))
+        return float(dt[0])
 
     WAVE_BODIES = {"standing_wave": 1, "traveling_wave_x": 2}
 

@@ -245,6 +245,33 @@ int c3d_ib_get_markers(c3d_ctx *ctx, double *xb, double *yb, double *zb, double 
 int c3d_set_time(c3d_ctx *ctx, double t);
 int c3d_get_time(const c3d_ctx *ctx, double *t);
 
+/* ---- Snapshots -------------------------------------------------------
+ * The reference's big-endian stream file (output_data,
+ * src/input_output.f90:498-728; tools/file_io.py:141-212): int32 length +
+ * FP64 x, y, z, xm, ym, zm; int32 dims + U, V, W, P, Hu_interior,
+ * Hv_interior, Hw_interior in the reference's global shapes; FP64 t, dt,
+ * dpdx, nu; xb, yb, zb (int32 length + FP64); int32 nxb, nzb (per wall for
+ * the wave bodies); fb, ub, sb, normals, tangents_1, tangents_2 (int32
+ * length + FP64).  Without a body the masks are zeros and the body arrays
+ * empty.  tangents_1/2 [3nb] come from the host's geometry (NULL: zeros;
+ * the solver never reads them). */
+int c3d_write_snapshot(c3d_ctx *ctx, const char *path, double dt, const double *tangents_1,
+                       const double *tangents_2);
+/* read_input_data (src/input_output.f90:335-489): the mesh sizes must match
+ * and the coordinates equal this context's grid (to 1e-12 of each axis'
+ * span); U, V, W are restored as the reference does.  what adds, beyond
+ * the reference (whose restart loses them): C3D_RESTORE_P the pressure,
+ * C3D_RESTORE_STATE t and dPdx (a moving body is moved to t),
+ * C3D_RESTORE_FORCE fb, the warm start of the next IB solve (the body set
+ * up first, same nb; nothing to do for a body-free file and context).
+ * With all three a restart continues bit for bit.
+ * dt_out (may be NULL) receives the recorded dt. */
+#define C3D_RESTORE_P 1
+#define C3D_RESTORE_STATE 2
+#define C3D_RESTORE_FORCE 4
+#define C3D_RESTORE_ALL 7
+int c3d_read_snapshot(c3d_ctx *ctx, const char *path, int what, double *dt_out);
+
 /* Stencil s (0 u, 1 v, 2 w, 3 centre) as the reference stores it
  * (src/immersed_boundary_operators.f90:46-244): for marker l and point
  * p = i + 5*(j + 5*k), entry [l*125 + p] of the Fortran node indices, the
