@@ -91,23 +91,30 @@ struct Work {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Both radix sorts run over all 32 key bits, so one scratch size serves
-// every component and grid.
-size_t cub_temp_bytes(int n) {
+// Radix passes cover only the bits a key can hold: node offsets below nf,
+// node counts at most 4*nb (a marker lands on one node at most twice per
+// periodic axis, when its 5-point window wraps a period of 4 in x and z).
+int bits_for(size_t v) {
+  int b = 1;
+  while (b < 31 && ((size_t)1 << b) <= v) b++;
+  return b;
+}
+
+size_t cub_temp_bytes(int n, int key_bits, int count_bits) {
   size_t need = 0, b = 0;
   int *i0 = nullptr, *i1 = nullptr, *i2 = nullptr, *i3 = nullptr, *i4 = nullptr;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, i0, i1, i2, i3, n, 0, 32);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, i0, i1, i2, i3, n, 0, key_bits);
   need = std::max(need, b);
   (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, b, i0, i1, i2, i3, n);
   need = std::max(need, b);
   (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, i0, i1, n + 1);
   need = std::max(need, b);
-  (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b, i0, i1, i2, i4, n, 0, 32);
+  (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b, i0, i1, i2, i4, n, 0, count_bits);
   need = std::max(need, b);
   return need;
 }
 
-Work carve(void* base, int nb) {
+Work carve(void* base, int nb, size_t nf) {
   const size_t n = (size_t)nb * 125 + 1, stride = align256(n * sizeof(int));
   char* p = static_cast<char*>(base);
   Work w{};
@@ -115,29 +122,34 @@ Work carve(void* base, int nb) {
                    &w.start, &w.iota, &w.jcnt, &w.order, &w.per};
   for (int a = 0; a < 11; a++) *arr[a] = reinterpret_cast<int*>(p + a * stride);
   w.temp = p + 11 * stride;
-  w.temp_bytes = cub_temp_bytes(nb * 125);
+  w.temp_bytes = cub_temp_bytes(nb * 125, bits_for(nf), bits_for(4 * (size_t)nb));
   return w;
 }
 
 }  // namespace
 
 size_t spread_map_work_bytes(int nb, size_t nf) {
-  (void)nf;  // offsets are below 2^31: 32-bit keys for any field size
   const size_t n = (size_t)nb * 125 + 1;
-  return 11 * align256(n * sizeof(int)) + align256(cub_temp_bytes(nb * 125));
+  size_t temp = 0;  // the largest of the three components' needs (nf: the largest field)
+  for (size_t f = 1; f <= nf; f <<= 1) {
+    temp = std::max(temp, cub_temp_bytes(nb * 125, bits_for(f), bits_for(4 * (size_t)nb)));
+    if (bits_for(f) >= bits_for(nf)) break;
+  }
+  temp = std::max(temp, cub_temp_bytes(nb * 125, bits_for(nf), bits_for(4 * (size_t)nb)));
+  return 11 * align256(n * sizeof(int)) + align256(temp);
 }
 
 hipError_t build_spread_map(const Markers& m, int comp, int n1, int n2, size_t nf, void* work,
                             Spread& sp, int* nnodes, hipStream_t s) {
   const int nb = m.nb, n = nb * 125;
-  Work w = carve(work, nb);
-  (void)nf;
+  Work w = carve(work, nb, nf);
+  const int kb = bits_for(nf), cb = bits_for(4 * (size_t)nb);
   hipError_t e;
   hipLaunchKernelGGL(k_map_keys, map_blocks(n), dim3(kMapBlock), 0, s, m.st[comp], nb, n1, n2,
                      w.key, w.val);
   size_t tb = w.temp_bytes;
   if ((e = hipcub::DeviceRadixSort::SortPairs(w.temp, tb, w.key, w.key_s, w.val, w.val_s, n, 0,
-                                               32, s)) != hipSuccess)
+                                               kb, s)) != hipSuccess)
     return e;
   hipLaunchKernelGGL(k_map_fill, map_blocks(n + 1), dim3(kMapBlock), 0, s, w.counts, 0, n + 1);
   tb = w.temp_bytes;
@@ -150,7 +162,7 @@ hipError_t build_spread_map(const Markers& m, int comp, int n1, int n2, size_t n
   hipLaunchKernelGGL(k_map_iota, map_blocks(n), dim3(kMapBlock), 0, s, w.iota, n);
   tb = w.temp_bytes;
   if ((e = hipcub::DeviceRadixSort::SortPairsDescending(w.temp, tb, w.counts, w.jcnt, w.iota,
-                                                         w.order, n, 0, 32, s)) != hipSuccess)
+                                                         w.order, n, 0, cb, s)) != hipSuccess)
     return e;
   hipLaunchKernelGGL(k_map_fill, map_blocks(n + 1), dim3(kMapBlock), 0, s, w.per, 0, n + 1);
   hipLaunchKernelGGL(k_map_per, map_blocks(n), dim3(kMapBlock), 0, s, w.jcnt, n, w.per);
