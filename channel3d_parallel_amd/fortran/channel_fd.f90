@@ -3,10 +3,12 @@
 ! (compute_dt, compute_time_step_RK3, monitor) with the initial mass flow    !
 ! of app/main.f90:90 and constant x mass flow (x_mass_cte = 1).             !
 !                                                                            !
-! usage: channel_fd nx ny nz grid_type alpha nu cfl dPdx nsteps nmonitor in out
+! usage: channel_fd nx ny nz grid_type alpha nu cfl dPdx nsteps nmonitor in out [snapshot]
 !   in : raw FP64 U, V, W (reference Fortran layout, boundary conditions     !
 !        applied by the driver)                                              !
-!   out: raw FP64 U, V, W, P, then dPdx, t, max|div|                        !
+!   out: raw FP64 U, V, W, P, then dPdx, t, max|div|, utau                  !
+!   snapshot (optional): the final state written by output_data             !
+!        (src/input_output.f90:498-728 format)                               !
 !------------------------------------------------------------------------------!
 Program channel_fd
   Use iso_fortran_env, Only : Int32, Int64, error_unit, output_unit
@@ -17,15 +19,17 @@ Program channel_fd
   Use time_integration
   Use monitor
   Use statistics
+  Use input_output
   Implicit None
 
   Integer(Int32) :: nxa, nya, nza, gtype, nsteps, nmonitor, istep, iu
   Real(Int64)    :: alpha, nu_in, max_div, q, utau, wtau
-  Character(256) :: arg, fin, fout
+  Character(256) :: arg, fin, fout, fsnap
   Real(Int64), Allocatable :: U(:,:,:), V(:,:,:), W(:,:,:), P(:,:,:)
 
-  If (command_argument_count() /= 12) Then
-     Write(error_unit,*) 'usage: channel_fd nx ny nz grid_type alpha nu cfl dPdx nsteps nmonitor in out'
+  If (command_argument_count() /= 12 .And. command_argument_count() /= 13) Then
+     Write(error_unit,*) 'usage: channel_fd nx ny nz grid_type alpha nu cfl dPdx nsteps nmonitor ', &
+          'in out [snapshot]'
      Error Stop 2
   End If
   Call get_command_argument(1, arg);  Read(arg,*) nxa
@@ -40,6 +44,8 @@ Program channel_fd
   Call get_command_argument(10, arg); Read(arg,*) nmonitor
   Call get_command_argument(11, fin)
   Call get_command_argument(12, fout)
+  fsnap = ''
+  If (command_argument_count() == 13) Call get_command_argument(13, fsnap)
 
   Call c3d_initialize(nxa, nya, nza, gtype, 8d0*Atan(1d0), 4d0*Atan(1d0), 2d0, alpha, nu_in)
   x_mass_cte = 1
@@ -74,5 +80,6 @@ Program channel_fd
   Open(newunit=iu, file=fout, access='stream', form='unformatted', status='replace')
   Write(iu) U, V, W, P, dPdx, t, max_div, utau
   Close(iu)
+  If (Len_Trim(fsnap) > 0) Call output_data(fsnap)
   Call c3d_finalize
 End Program channel_fd

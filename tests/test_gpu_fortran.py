@@ -202,3 +202,34 @@ def test_fortran_waves_matches_c_abi(tmp_path, body, p1, p2, p3, ramp):
     assert np.array_equal(fb_f, fb) and np.array_equal(yb_f, yb)
     assert t_f == t and int(iters_f) == accum
     assert div_f < 1e-12 and slip_f < 1e-9
+
+
+def test_fortran_output_data_snapshot(tmp_path):
+    """input_output::output_data from the Fortran time loop writes the
+    reference's snapshot format: the oracle's reader of that format finds
+    the final fields, t, dt, dPdx and nu of the run, and no body."""
+    from oracle import snapshot
+    if not os.path.exists(CHANNEL_FD):
+        pytest.fail("Fortran channel_fd not built (make -C channel3d_parallel_amd/fortran)")
+    nx, ny, nz, nsteps = 34, 65, 34, 3
+    nu, cfl = 1.0 / 2800.0, 0.5
+    dpdx0 = (180 * nu) ** 2
+    print(f"\n[case] fortran output_data {nx}x{ny}x{nz} steps={nsteps}", flush=True)
+    U, V, W = fields.channel_ic(nx, ny, nz, seed=2)
+    fin, fout, fsnap = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "test.3"
+    np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
+    subprocess.run([CHANNEL_FD, str(nx), str(ny), str(nz), "1", "2.0", repr(nu), repr(cfl),
+                    repr(dpdx0), str(nsteps), "0", str(fin), str(fout), str(fsnap)],
+                   check=True, timeout=120, capture_output=True, text=True)
+    g = orc.Grid(nx, ny, nz)
+    raw = np.fromfile(fout)
+    d = snapshot.read(fsnap)
+    off = 0
+    for name, s in (("U", g.shape_u), ("V", g.shape_v), ("W", g.shape_w), ("P", g.shape_p)):
+        n = int(np.prod(s))
+        assert np.array_equal(d[name], raw[off:off + n].reshape(s, order="F")), name
+        off += n
+    dpdx, t = raw[off:off + 2]
+    assert d["t"] == t and d["dpdx"] == dpdx and d["nu"] == nu and d["dt"] > 0
+    assert np.array_equal(d["y"], g.get(1)) and d["xb"].size == 0 and d["nxb"] == 0
+    assert not d["Hu_interior"].any()
