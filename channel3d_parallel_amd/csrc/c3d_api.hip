@@ -2452,22 +2452,25 @@ int c3d_read_snapshot(c3d_ctx* c, const char* path, int what, double* dt_out) {
         int32_t d[3];
         if (!in.get_i4(d[0]) || !in.get_i4(d[1]) || !in.get_i4(d[2]))
           return fail(c, C3D_EINVAL, "%s: truncated mask header", path);
-        if (fseek(in.f, (long)((size_t)d[0] * d[1] * d[2] * 8), SEEK_CUR) != 0)
-          return fail(c, C3D_EINVAL, "%s: truncated mask", path);
+        if (d[0] < 0 || d[1] < 0 || d[2] < 0 ||
+            fseek(in.f, (long)((size_t)d[0] * d[1] * d[2] * 8), SEEK_CUR) != 0)
+          return fail(c, C3D_EINVAL, "%s: truncated or corrupt mask", path);
       }
       double sc[4];
       if (!in.get_f8s(sc, 4)) return fail(c, C3D_EINVAL, "%s: truncated scalars", path);
       t = sc[0];
       dt = sc[1];
       dpdx = sc[2];
+      // marker counts come from the file: bounded before any seek or allocation
+      const int32_t kMaxLen = 1 << 28;
       int32_t n = 0;
       for (int a = 0; a < 3; a++) {  // xb, yb, zb
-        if (!in.get_i4(n) || fseek(in.f, (long)n * 8, SEEK_CUR) != 0)
-          return fail(c, C3D_EINVAL, "%s: truncated markers", path);
+        if (!in.get_i4(n) || n < 0 || n > kMaxLen || fseek(in.f, (long)n * 8, SEEK_CUR) != 0)
+          return fail(c, C3D_EINVAL, "%s: truncated or corrupt markers", path);
       }
       int32_t nxb = 0, nzb = 0;
-      if (!in.get_i4(nxb) || !in.get_i4(nzb) || !in.get_i4(n))
-        return fail(c, C3D_EINVAL, "%s: truncated body section", path);
+      if (!in.get_i4(nxb) || !in.get_i4(nzb) || !in.get_i4(n) || n < 0 || n > kMaxLen)
+        return fail(c, C3D_EINVAL, "%s: truncated or corrupt body section", path);
       fb.resize(n);
       if (!in.get_f8s(fb.data(), n)) return fail(c, C3D_EINVAL, "%s: truncated fb", path);
     }

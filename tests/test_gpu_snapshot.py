@@ -191,3 +191,31 @@ def test_restart_continues_bitwise_with_body(tmp_path, body):
     for k in ("U", "V", "W", "P"):
         assert np.array_equal(out[k], ref[k]), k
     assert np.array_equal(fb, fb_ref)
+
+
+def test_reader_rejects_truncated_and_corrupt_files(tmp_path):
+    """Truncated fields and corrupt marker counts are errors, not reads past
+    the data."""
+    import struct
+    nx, ny, nz = 34, 33, 34
+    print(f"\n[case] snapshot corrupt files {nx}x{ny}x{nz}", flush=True)
+    p = tmp_path / "snap"
+    with Channel(nx, ny, nz) as ch:
+        ch.write_snapshot(p, 0.01)
+    raw = p.read_bytes()
+    g = orc.Grid(nx, ny, nz)
+    mesh = sum(4 + 8 * n for n in (nx, ny, nz, nx - 1, ny - 1, nz - 1))
+    cut = tmp_path / "cut"
+    cut.write_bytes(raw[:mesh + 12 + 1000])  # inside U
+    fields_end = mesh + sum(12 + 8 * int(np.prod(s)) for s in
+                            (g.shape_u, g.shape_v, g.shape_w, g.shape_p, g.shape_u, g.shape_v,
+                             g.shape_w))
+    bad = bytearray(raw)
+    bad[fields_end + 32:fields_end + 36] = struct.pack(">i", -5)  # xb length
+    corrupt = tmp_path / "corrupt"
+    corrupt.write_bytes(bytes(bad))
+    with Channel(nx, ny, nz) as ch:
+        for f in (cut, corrupt):
+            with pytest.raises(C3DError):
+                ch.read_snapshot(f)
+        ch.read_snapshot(p)  # the intact file still reads
