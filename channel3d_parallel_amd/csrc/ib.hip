@@ -667,13 +667,17 @@ __global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ p
 
 // rho_n = rhat.r (partial slot 0); bta = (rho_n/rho_o)*(alpha/om);
 // p = r + bta*(p - om*v).  Writes sc[3] = rho_n, sc[4] = bta.
+// Breakdown guards here and in k_bicg_hs / k_bicg_xr: an exactly zero
+// denominator gives a zero step instead of the reference's 0/0
+// (src/projection.f90:165-172); a zero right-hand side then converges to
+// x = 0 in one iteration.  Nonzero denominators are untouched.
 __global__ void k_bicg_p(const double* __restrict__ r, const double* __restrict__ v,
                          double* __restrict__ sc, const double* __restrict__ part,
                          double* __restrict__ p, size_t n) {
   __shared__ double bsh;
   if (threadIdx.x == 0) {
     const double rho_n = sum_part(part, 0);
-    const double bta = (rho_n / sc[0]) * (sc[1] / sc[2]);
+    const double bta = (sc[0] != 0.0 && sc[2] != 0.0) ? (rho_n / sc[0]) * (sc[1] / sc[2]) : 0.0;
     bsh = bta;
     if (blockIdx.x == 0) {
       sc[3] = rho_n;
@@ -696,7 +700,8 @@ __global__ void k_bicg_hs(const double* __restrict__ x, const double* __restrict
   __shared__ double ash;
   if (threadIdx.x == 0) {
     const double rho_n = sc[3];
-    const double alpha = rho_n / sum_part(part, 1);
+    const double den = sum_part(part, 1);
+    const double alpha = den != 0.0 ? rho_n / den : 0.0;
     ash = alpha;
     if (blockIdx.x == 0) {
       sc[1] = alpha;
@@ -725,7 +730,8 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(const double* __restrict__ h
   __shared__ double sh[2][kBlock / 64];
   __shared__ double osh;
   if (threadIdx.x == 0) {
-    const double om = sum_part(part, 2) / sum_part(part, 3);
+    const double tt = sum_part(part, 3);
+    const double om = tt != 0.0 ? sum_part(part, 2) / tt : 0.0;
     osh = om;
     if (blockIdx.x == 0) sc[2] = om;
   }

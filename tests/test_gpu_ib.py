@@ -185,6 +185,34 @@ def test_center_wall_projection_parity():
         assert rel(a, b) < 1e-8
 
 
+def test_zero_rhs_is_guarded():
+    """Bodies at rest in fluid at rest: the IB right-hand side is exactly
+    zero.  The reference's BiCGSTAB divides 0/0 there (src/projection.f90:
+    170); here the solve converges in one iteration to x = 0 and the fields
+    stay zero."""
+    nx, ny, nz = 34, 33, 34
+    print(f"\n[case] zero IB right-hand side {nx}x{ny}x{nz}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=0)
+    m = ib_geometry.two_cylinders(g.scalars["Lzp"], nzb=16, nth=24, speed1=0.0, speed2=0.0)
+    with Channel(nx, ny, nz, grid_type=0, nu=NU) as ch:
+        ch.setup_IB(m)
+        for graphs in (True, False):
+            ch.set_IB_graphs(graphs)
+            x, it, err = ch.bicgstab(np.zeros(4 * m.nb), tol=1e-10, maxit=10, dt=0.005)
+            assert it == 1 and err == 0.0 and np.all(x == 0.0)
+        z = [g.zeros(c) for c in "uvw"]
+        ch.set_fields(*z, P=g.zeros("p"))
+        ch.set_IB_solver(tol=1e-10, maxit=10, guess_mode=1)
+        ch.compute_non_IB_projection(0.005, 2)
+        ch.apply_boundary_conditions()
+        it, err = ch.compute_IB_projection(0.005, 2)
+        r = ch.get_fields()
+        fb = ch.get_IB_force()
+    assert it == 1 and err == 0.0 and np.all(fb == 0.0)
+    for k in ("U", "V", "W"):
+        assert np.all(r[k] == 0.0), k
+
+
 def test_time_step_with_body():
     """Two RK3 steps with the body against the oracle composed stage by stage
     (src/time_integration.f90:98-186 with the IB projection of :118-122)."""
