@@ -166,9 +166,12 @@ template <typename T>
 int dalloc(c3d_ctx* c, T** p, size_t n) {
   void* q = nullptr;
   hipError_t e = hipMalloc(&q, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
-  if (e != hipSuccess)
+  if (e != hipSuccess) {
+    // clear the runtime's last-error slot, which the kernel launchers read
+    (void)hipGetLastError();
     return fail(c, C3D_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T),
                 hipGetErrorString(e));
+  }
   c->allocs.push_back(q);
   *p = static_cast<T*>(q);
   return 0;

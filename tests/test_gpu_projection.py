@@ -179,3 +179,19 @@ def test_projection_idempotent_and_repeatable():
     assert rel(b["U"], a["U"]) <= 1e-13 and rel(b["V"], a["V"]) <= 1e-13
     for k in ("U", "V", "W", "P_interim"):
         assert np.array_equal(a[k], c[k]), k
+
+
+def test_allocation_failure_is_reported():
+    """A grid whose U field alone exceeds the GPU's memory (386 x 250,001 x
+    387 doubles, 299 GB) fails in c3d_create with C3D_ENOMEM after freeing
+    what it allocated; the device stays usable."""
+    from channel3d_parallel_amd.solver import C3DError
+    print("\n[case] allocation failure 386x250001x386", flush=True)
+    with pytest.raises(C3DError) as e:
+        Channel(386, 250001, 386)
+    assert e.value.code == -3  # C3D_ENOMEM
+    g, ch, U, V, W = case(34, 33, 34, seed=13)
+    with ch:
+        ch.set_fields(U, V, W)
+        ch.apply_boundary_conditions()
+        assert np.array_equal(ch.get_fields()["U"], U)
