@@ -212,7 +212,7 @@ def ib_rk3_run(nx, ny, nz, nu, dt, U, V, W, nsteps, guess_mode):
                 "after_steps": {"max_slip": ch.check_slip(), "max_div": ch.check_divergence()}}
 
 
-def moving_section(nmoves=20, nsteps=3, cpu=True):
+def moving_section(nmoves=20, nsteps=3, cpu=True, long_steps=100):
     """Moving walls (standing wave, amplitude 0.03, omega 2, one wavelength
     in x) on the buffered grid 130x129x66 faces (grid_type 2, alpha 2.6,
     min_buffer_width 0.1), 128 x 64 markers per wall (nb = 16,384): the
@@ -253,6 +253,15 @@ def moving_section(nmoves=20, nsteps=3, cpu=True):
         out["iterations_per_solve"] = (acc1 - acc0) / (3 * nsteps)
         out["max_div"] = ch.check_divergence()
         out["max_slip"] = ch.check_slip()
+        if long_steps:
+            for _ in range(long_steps):
+                ch.compute_time_step_RK3(dt, x_mass_cte=0)
+            _, acc2, _ = ch.ib_stats()
+            mon = ch.monitor()
+            out["long_run"] = {"steps": long_steps, "t": ch.time,
+                               "iterations_per_solve": (acc2 - acc1) / (3 * long_steps),
+                               "max_div": ch.check_divergence(), "max_slip": ch.check_slip(),
+                               "max_U": mon["maxU"], "nonfinite": mon["nonfinite"]}
     if cpu:
         from oracle import orc
         g = orc.Grid(nx, ny, nz, **kw)
@@ -348,7 +357,8 @@ def main():
         print(json.dumps({"ib": ib_section(nsteps=3, learned=False)}))
         return
     if a.only == "moving":
-        print(json.dumps({"moving_walls": moving_section(nmoves=5, nsteps=2, cpu=False)}))
+        print(json.dumps({"moving_walls": moving_section(nmoves=5, nsteps=2, cpu=False,
+                                                         long_steps=0)}))
         return
     if a.only == "mixed":
         from channel3d_parallel_amd import Channel, fields
