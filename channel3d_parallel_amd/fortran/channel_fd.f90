@@ -5,7 +5,8 @@
 !                                                                            !
 ! usage: channel_fd nx ny nz grid_type alpha nu cfl dPdx nsteps nmonitor in out [snapshot]
 !   in : raw FP64 U, V, W (reference Fortran layout, boundary conditions     !
-!        applied by the driver)                                              !
+!        applied by the driver), or snapshot:<file> to restart from a       !
+!        snapshot with read_input_data (U, V, W, P, t, dPdx restored)        !
 !   out: raw FP64 U, V, W, P, then dPdx, t, max|div|, utau                  !
 !   snapshot (optional): the final state written by output_data             !
 !        (src/input_output.f90:498-728 format)                               !
@@ -51,12 +52,17 @@ Program channel_fd
   x_mass_cte = 1
 
   Allocate(U(nx, nyg, nzg), V(nxg, ny, nzg), W(nxg, nyg, nz), P(nxg, nyg, nzg))
-  Open(newunit=iu, file=fin, access='stream', form='unformatted', status='old')
-  Read(iu) U, V, W
-  Close(iu)
-  P = 0d0
-  Call c3d_upload(U, V, W, P)
-  Call apply_boundary_conditions_resident
+  If (fin(1:9) == 'snapshot:') Then
+     ! restart (init_type = 0): the flow rate to hold is the snapshot's
+     Call read_input_data(Trim(fin(10:)))
+  Else
+     Open(newunit=iu, file=fin, access='stream', form='unformatted', status='old')
+     Read(iu) U, V, W
+     Close(iu)
+     P = 0d0
+     Call c3d_upload(U, V, W, P)
+     Call apply_boundary_conditions_resident
+  End If
 
   ! initial mass flow (app/main.f90:90)
   Call compute_mean_mass_flow_U(Qflow_x_0)

@@ -233,3 +233,38 @@ def test_fortran_output_data_snapshot(tmp_path):
     assert d["t"] == t and d["dpdx"] == dpdx and d["nu"] == nu and d["dt"] > 0
     assert np.array_equal(d["y"], g.get(1)) and d["xb"].size == 0 and d["nxb"] == 0
     assert not d["Hu_interior"].any()
+
+
+def test_fortran_restart_from_snapshot(tmp_path):
+    """read_input_data in the Fortran host: 3 steps + snapshot, then a new
+    run restarted from it for 2 steps, against 5 uninterrupted steps.  The
+    restarted run recomputes the flow rate to hold from the restored field
+    (app/main.f90:90, as the reference does), so it agrees to round-off;
+    the time, restored from the file, is exact."""
+    if not os.path.exists(CHANNEL_FD):
+        pytest.fail("Fortran channel_fd not built (make -C channel3d_parallel_amd/fortran)")
+    nx, ny, nz = 34, 65, 34
+    nu, cfl = 1.0 / 2800.0, 0.5
+    dpdx0 = (180 * nu) ** 2
+    print(f"\n[case] fortran restart {nx}x{ny}x{nz}", flush=True)
+    U, V, W = fields.channel_ic(nx, ny, nz, seed=6)
+    fin = tmp_path / "in.bin"
+    np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
+
+    def run(src, nsteps, out, snap=None):
+        args = [CHANNEL_FD, str(nx), str(ny), str(nz), "1", "2.0", repr(nu), repr(cfl),
+                repr(dpdx0), str(nsteps), "0", src, str(out)]
+        subprocess.run(args + ([str(snap)] if snap else []), check=True, timeout=120,
+                       capture_output=True, text=True)
+        return np.fromfile(out)
+
+    full = run(str(fin), 5, tmp_path / "a.bin")
+    run(str(fin), 3, tmp_path / "b.bin", tmp_path / "snap.3")
+    rest = run("snapshot:" + str(tmp_path / "snap.3"), 2, tmp_path / "c.bin")
+    g = orc.Grid(nx, ny, nz)
+    n = sum(int(np.prod(s)) for s in (g.shape_u, g.shape_v, g.shape_w, g.shape_p))
+    a, c = full[:n], rest[:n]
+    assert np.linalg.norm(a - c) <= 1e-13 * np.linalg.norm(a)
+    dpdx_a, t_a = full[n:n + 2]
+    dpdx_c, t_c = rest[n:n + 2]
+    assert t_c == t_a and abs(dpdx_c - dpdx_a) <= 1e-9 * abs(dpdx_a)
