@@ -1181,9 +1181,9 @@ static int ib_bicgstab_launch(c3d_ctx* c, double* x, const double* b, double tol
 }
 
 // ---- history-projection start (guess_mode 4) ----
-// The Schur operator depends only on dt, nu and the static geometry, so every
-// converged solve (f_i, b_i) of one dt satisfies S f_i = b_i to the solver
-// tolerance.  The start for a new right-hand side b is the force combination
+// The Schur operator depends only on dt, nu and the geometry, so for a static
+// body every converged solve (f_i, b_i) of one dt satisfies S f_i = b_i to the
+// solver tolerance (ib_move_to clears the pairs when the body moves).  The start for a new right-hand side b is the force combination
 // whose rhs combination is closest to b in the 2-norm.
 static int ib_hist_reserve(c3d_ctx* c) {
   IBState* ib = c->ib;

@@ -309,8 +309,9 @@ int c3d_ib_bicgstab(c3d_ctx *ctx, double *x, const double *b, double tol, int ma
  * c3d_ib_set_force for the next solve only, then 0, 3 = the previous force
  * plus the learned increment (c3d_mlp_*), 4 = the least-squares combination
  * of the last 8 converged forces of the same dt, fitted through their
- * right-hand sides (the Schur operator depends on dt only; the previous
- * force while none is stored).  Mode 4 is an addition with no counterpart in
+ * right-hand sides (for a static body the Schur operator depends on dt only;
+ * a moving body clears the stored forces at every move; the previous force
+ * while none is stored).  Mode 4 is an addition with no counterpart in
  * the reference; BiCGSTAB starts from the true residual in every mode. */
 int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
 /* on != 0 (the default): each BiCGSTAB iteration of the IB solves is
