@@ -459,8 +459,9 @@ __device__ __forceinline__ void spread_node(const Spread& sp, int t, const doubl
                                             double* __restrict__ F) {
   double acc = 0.0;
   const int n = sp.cnt[t];
-  // loads of several entries in flight; the sum stays in entry order
-#pragma unroll 4
+  // loads of several entries in flight (8 measured 1% faster than 4 per
+  // BiCGSTAB iteration); the sum stays in entry order
+#pragma unroll 8
   for (int i = 0; i < n; i++) {
     const int e = sp.slot[i] + t;
     acc = acc + sp.wfs[e] * f[sp.ml[e]];
