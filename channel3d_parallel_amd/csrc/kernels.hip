@@ -1134,7 +1134,7 @@ static int grad_kc() {
   static const int kc = [] {
     const char* e = getenv("C3D_GRAD_KC");
     const int v = e ? atoi(e) : 0;
-    return (v == 2 || v == 4 || v == 8 || v == 16) ? v : kGradKCDefault;
+    return (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ? v : kGradKCDefault;
   }();
   return kc;
 }
@@ -1144,6 +1144,7 @@ hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W
   const int kc = grad_kc();
   const dim3 grid = grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kc - 1) / kc);
   switch (kc) {
+    case 1: hipLaunchKernelGGL(k_grad_correct<1>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
     case 4: hipLaunchKernelGGL(k_grad_correct<4>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
     case 8: hipLaunchKernelGGL(k_grad_correct<8>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
     case 16: hipLaunchKernelGGL(k_grad_correct<16>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
