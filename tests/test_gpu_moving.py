@@ -244,3 +244,24 @@ def test_wave_setup_errors():
     with channel() as ch:
         with pytest.raises(C3DError):
             ch.setup_IB_wave("traveling_wave_x", 16, 4, 0.1, 1.0, 1.0)  # 0.1/(1*1) > |y(3)|
+
+
+def test_device_map_dense_markers():
+    """Markers four per cell in x and two per cell in z (64 x 32 per wall on
+    a 16 x 16 cell plane): about 200 entries per support node, a wide count
+    range for the device sort; operators bitwise equal to the host-built
+    set-up of the same markers."""
+    nx, ny, nz = 18, 65, 18
+    print(f"\n[case] dense wall markers {nx}x{ny}x{nz}", flush=True)
+    kw = dict(alpha=2.6, grid_type=2, min_buffer_width=0.1, nu=NU)
+    nxb, nzb = 64, 32
+    nb = 2 * nxb * nzb
+    f = np.random.default_rng(9).standard_normal(4 * nb)
+    with Channel(nx, ny, nz, **kw) as ch:
+        ch.setup_IB_wave("standing_wave", nxb, nzb, 0.04, 2.0, 1.0, 0.0, 0.3)
+        m = Markers(ch.get_IB_markers(), nzb, nxb)
+        a = operators(ch, nb, f)
+    with Channel(nx, ny, nz, **kw) as ch:
+        ch.setup_IB(m)
+        b = operators(ch, nb, f)
+    assert_same(a, b)
