@@ -318,7 +318,8 @@ int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
  * replayed from a HIP graph captured once per (solution vector, dt, rk_step
  * == 1); 0: launched kernel by kernel.  Both run the same kernels in the same
  * order and give identical results.  The default at c3d_ib_setup is on
- * unless the environment sets C3D_IB_GRAPHS=0.  No reference counterpart
+ * unless the environment sets C3D_IB_GRAPHS=0; c3d_ib_setup_wave starts
+ * with it off (each move would force a recapture).  No reference counterpart
  * (launch mechanics of src/projection.f90:160-182). */
 int c3d_ib_set_graphs(c3d_ctx *ctx, int on);
 /* Resident IB force fb = (dudn_jump[3nb], p_jump[nb]) [4nb]. */
