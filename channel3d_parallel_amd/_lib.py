@@ -41,6 +41,8 @@ SIGNATURES = [
     ("c3d_apply_boundary_conditions", C.c_int, [C.c_void_p]),
     ("c3d_divergence", C.c_int, [C.c_void_p, _dp]),
     ("c3d_gradient", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    ("c3d_apply_boundary_conditions_arrays", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("c3d_divergence_arrays", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     ("c3d_solve_poisson_equation", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double]),
     ("c3d_compute_non_ib_projection", C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     ("c3d_check_divergence", C.c_int, [C.c_void_p, _dp]),
@@ -52,11 +54,14 @@ SIGNATURES = [
     ("c3d_save_state", C.c_int, [C.c_void_p]),
     ("c3d_compute_rhs", C.c_int, [C.c_void_p, C.c_int]),
     ("c3d_get_rhs", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    ("c3d_compute_rhs_arrays", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("c3d_rk3_update", C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     ("c3d_compute_dt", C.c_int, [C.c_void_p, C.c_double, _dp]),
     ("c3d_mean_mass_flow_u", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_mean_mass_flow_u_array", C.c_int, [C.c_void_p, _dp, _dp]),
     ("c3d_add_to_u", C.c_int, [C.c_void_p, C.c_double]),
     ("c3d_time_step_rk3", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double]),
+    ("c3d_set_rhs_exact", C.c_int, [C.c_void_p, C.c_int]),
     ("c3d_time_projection", C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp, _dp, C.c_int]),
     ("c3d_copy_bandwidth", C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _dp]),
     ("c3d_stage_count", C.c_int, []),
@@ -78,6 +83,8 @@ SIGNATURES = [
     ("c3d_ib_get_stencil", C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _dp]),
     ("c3d_ib_get_heaviside", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("c3d_ib_interpolate", C.c_int, [C.c_void_p, C.c_int, _dp]),
+    ("c3d_ib_interpolate_arrays", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]),
+    ("c3d_ib_compute_heaviside", C.c_int, [C.c_void_p]),
     ("c3d_ib_interpolate_c", C.c_int, [C.c_void_p, _dp, _dp]),
     ("c3d_ib_spread", C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     ("c3d_ib_remove_mean", C.c_int, [C.c_void_p, _dp]),

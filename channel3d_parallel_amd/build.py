@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libc3d.so")
 
-SOURCES = ["c3d_api.hip", "kernels.hip", "rk3.hip", "ib.hip", "ibmap.hip", "mlp.hip"]
+SOURCES = ["c3d_api.hip", "kernels.hip", "rk3.hip", "rk3_fast.hip", "ib.hip", "ibmap.hip", "mlp.hip"]
 
 ARCH = os.environ.get("C3D_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

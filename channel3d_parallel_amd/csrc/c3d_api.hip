@@ -35,6 +35,7 @@ struct MLPState {
   float* dH[c3d::mlp::kLayers - 1] = {nullptr};
   std::vector<void*> bufs;  // activation buffers (freed on regrow)
   int* nb9 = nullptr;       // IB marker neighbourhoods
+  double* nrm = nullptr;    // per-solve input normalisation (1/rms of residual, of force)
   bool ready = false;       // weights set by training or c3d_mlp_set_params
 };
 
@@ -114,15 +115,27 @@ struct c3d_ctx {
   DevGrid g{};
   std::vector<void*> allocs;
   double *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr, *Pint = nullptr, *rhs = nullptr;
+  // staging of the host-array procedures (apply_boundary_conditions(U_,V_,W_),
+  // divergence, gradient, regT, compute_rhs_u/v/w, the mass-flow integral):
+  // field-shaped xU, xV, xW and the three right-hand-side interiors in xF,
+  // allocated with the context, so those calls neither allocate nor touch
+  // the resident state
+  double *xU = nullptr, *xV = nullptr, *xW = nullptr, *xF = nullptr;
   cpx<double>* S = nullptr;
   double* CP = nullptr;
   unsigned long long* red = nullptr;
+  unsigned* nanflag = nullptr;  // g.nonfinite: set by the velocity corrections
   double* psum = nullptr;  // 7*nyg plane sums of compute_statistics
   double* mon = nullptr;   // output_monitor partials and results
   size_t nu = 0, nv = 0, nw = 0, np = 0, nr = 0, nspec = 0;
   // RK3 state (allocated on first use): step-start copies, stage
   // right-hand sides F[stage][component], reduction scratch
   double *Uo = nullptr, *Vo = nullptr, *Wo = nullptr;
+  // third set of velocity fields for the fused RK3 stages, which write each
+  // stage's velocity into a set other than their input (the resident U, V,
+  // W pointers then move among U, Uo and Ux)
+  double *Ux = nullptr, *Vx = nullptr, *Wx = nullptr;
+  bool rhs_exact = false;  // c3d_set_rhs_exact: bitwise right-hand sides in the time step
   double* F[3][3] = {{nullptr}};
   double* part = nullptr;  // mass-flow row partials, then 2 result slots
   double dPdx = 0.0, dPdy = 0.0, dPdz = 0.0;
@@ -137,6 +150,12 @@ struct c3d_ctx {
   double *b64 = nullptr, *phi64 = nullptr, *gauge = nullptr;
   float* r32 = nullptr;
   int nref = 2;
+  // modes whose tridiagonal systems are solved with FP64 arithmetic in the
+  // mixed path: |kxx + kzz| < tri_thr (the (0,0) mode and the lowest |k|);
+  // the others use FP32 arithmetic.  tri_low == null: FP64 for every mode.
+  int* tri_low = nullptr;
+  int ntri_low = 0;
+  double tri_thr = 0.0;
   std::string err;
 };
 
@@ -339,6 +358,48 @@ std::vector<double> centre_spacing(const std::vector<double>& a) {
   return t;
 }
 
+// Interior sizes of the right-hand sides F_u (nx-2, nyg-2, nzg-2),
+// F_v (nxg-2, ny-2, nzg-2), F_w (nxg-2, nyg-2, nz-2).
+size_t interior_size(const c3d_ctx* c, int comp) {
+  if (comp == 0) return (size_t)(c->nx - 2) * (c->nyg - 2) * (c->nzg - 2);
+  if (comp == 1) return (size_t)(c->nxg - 2) * (c->ny - 2) * (c->nzg - 2);
+  return (size_t)(c->nxg - 2) * (c->nyg - 2) * (c->nz - 2);
+}
+size_t interior_total(const c3d_ctx* c) {
+  return interior_size(c, 0) + interior_size(c, 1) + interior_size(c, 2);
+}
+
+// Split of the mixed path's tridiagonal solve.  A mode's system Dyy + kxx +
+// kzz has condition number about (max|Dyy| + |k2|)/|k2| with k2 = kxx + kzz;
+// FP32 elimination is accurate to that many FP32 roundings, so the modes with
+// |k2| < kTriLow * max|Dyy(j,j)| (condition above ~1e3, among them the
+// singular (0,0) mode that the gauge row fixes) keep FP64 arithmetic.
+// C3D_TRI32_LOW overrides the fraction; a negative value keeps FP64
+// arithmetic for every mode.
+constexpr double kTriLow = 1e-3;
+
+int mixed_tri_modes(c3d_ctx* c) {
+  static const double frac = [] {
+    const char* e = getenv("C3D_TRI32_LOW");
+    return e ? atof(e) : kTriLow;
+  }();
+  if (!(frac >= 0.0) || !c3d::tri_part_supported(c->g)) return 0;
+  double dmax = 0.0;
+  for (double v : c->dyy_d) dmax = std::max(dmax, std::fabs(v));
+  // at least the smallest normal double, so that the (0,0) mode (k2 = 0) is
+  // always in the FP64 set
+  c->tri_thr = std::max(frac * dmax, 2.2250738585072014e-308);
+  std::vector<int> low;
+  for (int kx = 0; kx < c->nxh; kx++)
+    for (int kz = 0; kz < c->nzp; kz++)
+      if (std::fabs(c->kxx[kx] + c->kzz[kz]) < c->tri_thr) low.push_back(kx * c->nzp + kz);
+  c->ntri_low = (int)low.size();
+  const int* d = nullptr;
+  int rc = upload(c, &d, low);
+  c->tri_low = const_cast<int*>(d);
+  return rc;
+}
+
 int sync(c3d_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -446,6 +507,10 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   if ((rc = dalloc(c, &c->U, c->nu)) || (rc = dalloc(c, &c->V, c->nv)) ||
       (rc = dalloc(c, &c->W, c->nw)) || (rc = dalloc(c, &c->P, c->np)) ||
       (rc = dalloc(c, &c->Pint, c->nr)) || (rc = dalloc(c, &c->rhs, c->nr)) ||
+      (rc = dalloc(c, &c->xU, c->nu)) || (rc = dalloc(c, &c->xV, c->nv)) ||
+      (rc = dalloc(c, &c->xW, c->nw)) || (rc = dalloc(c, &c->xF, interior_total(c))) ||
+      (rc = dalloc(c, &c->part, 2 * (size_t)c3d::massflow_rows(c->g) + 2)) ||
+      (rc = dalloc(c, &c->nanflag, 1)) ||
       (rc = dalloc(c, &c->S, c->nspec)) ||
       (!c3d::tri_part_supported(c->g) && (rc = dalloc(c, &c->CP, c->nspec))) ||
       (rc = dalloc(c, &c->red, kRedRefine + (size_t)kMaxRef * kResSlots)))
@@ -455,7 +520,14 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
        (rc = dalloc(c, &c->phi64, c->nr)) || (rc = dalloc(c, &c->r32, c->nr)) ||
        (rc = dalloc(c, &c->gauge, (size_t)c->nzp + 1))))
     return cleanup(rc);
-  if (hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
+  c->g.nonfinite = c->nanflag;
+  {
+    const char* e = getenv("C3D_RHS_EXACT");
+    c->rhs_exact = e && atoi(e) != 0;
+  }
+  if (d.precision == C3D_PREC_MIXED && (rc = mixed_tri_modes(c))) return cleanup(rc);
+  if (hipMemsetAsync(c->nanflag, 0, sizeof(unsigned), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->V, 0, c->nv * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->W, 0, c->nw * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->P, 0, c->np * 8, c->stream) != hipSuccess ||
@@ -546,28 +618,44 @@ int c3d_divergence(c3d_ctx* c, double* div) {
 
 int c3d_gradient(c3d_ctx* c, const double* p, double* Gx, double* Gy, double* Gz) {
   if (!c || !p || !Gx || !Gy || !Gz) return C3D_EINVAL;
-  double *gx = nullptr, *gy = nullptr, *gz = nullptr;
-  int rc = 0;
-  // scratch for the host-array drop-in (not on the resident hot path)
-  if (hipMalloc(&gx, c->nu * 8) != hipSuccess || hipMalloc(&gy, c->nv * 8) != hipSuccess ||
-      hipMalloc(&gz, c->nw * 8) != hipSuccess) {
-    rc = fail(c, C3D_ENOMEM, "gradient scratch allocation failed");
-  } else if (hipMemcpyAsync(c->rhs, p, c->nr * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-             hipMemsetAsync(gx, 0, c->nu * 8, c->stream) != hipSuccess ||
-             hipMemsetAsync(gy, 0, c->nv * 8, c->stream) != hipSuccess ||
-             hipMemsetAsync(gz, 0, c->nw * 8, c->stream) != hipSuccess ||
-             c3d::launch_gradient(c->g, c->rhs, gx, gy, gz, c->stream) != hipSuccess ||
-             hipMemcpyAsync(Gx, gx, c->nu * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-             hipMemcpyAsync(Gy, gy, c->nv * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-             hipMemcpyAsync(Gz, gz, c->nw * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-             hipStreamSynchronize(c->stream) != hipSuccess) {
-    rc = fail(c, C3D_EHIP, "gradient failed");
-  }
-  (void)hipStreamSynchronize(c->stream);
-  if (gx) (void)hipFree(gx);
-  if (gy) (void)hipFree(gy);
-  if (gz) (void)hipFree(gz);
-  return rc;
+  HIPCHK(c, hipMemcpyAsync(c->rhs, p, c->nr * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->xU, 0, c->nu * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->xV, 0, c->nv * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->xW, 0, c->nw * 8, c->stream));
+  HIPCHK(c, c3d::launch_gradient(c->g, c->rhs, c->xU, c->xV, c->xW, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Gx, c->xU, c->nu * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Gy, c->xV, c->nv * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Gz, c->xW, c->nw * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+// Stages host arrays U_, V_, W_ (any may be NULL) in the context's scratch.
+static int stage_uvw(c3d_ctx* c, const double* U, const double* V, const double* W) {
+  if (U) HIPCHK(c, hipMemcpyAsync(c->xU, U, c->nu * 8, hipMemcpyHostToDevice, c->stream));
+  if (V) HIPCHK(c, hipMemcpyAsync(c->xV, V, c->nv * 8, hipMemcpyHostToDevice, c->stream));
+  if (W) HIPCHK(c, hipMemcpyAsync(c->xW, W, c->nw * 8, hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+int c3d_apply_boundary_conditions_arrays(c3d_ctx* c, double* U, double* V, double* W) {
+  if (!c || !U || !V || !W) return C3D_EINVAL;
+  int rc = stage_uvw(c, U, V, W);
+  if (rc) return rc;
+  HIPCHK(c, c3d::launch_apply_bc(c->g, c->xU, c->xV, c->xW, c->stream));
+  HIPCHK(c, hipMemcpyAsync(U, c->xU, c->nu * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(V, c->xV, c->nv * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(W, c->xW, c->nw * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_divergence_arrays(c3d_ctx* c, const double* U, const double* V, const double* W,
+                          double* div) {
+  if (!c || !U || !V || !W || !div) return C3D_EINVAL;
+  int rc = stage_uvw(c, U, V, W);
+  if (rc) return rc;
+  HIPCHK(c, c3d::launch_divergence(c->g, c->xU, c->xV, c->xW, c->rhs, c->stream));
+  HIPCHK(c, hipMemcpyAsync(div, c->rhs, c->nr * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
 }
 
 int c3d_solve_poisson_equation(c3d_ctx* c, double* rhs, int rk_step, double dt) {
@@ -583,6 +671,17 @@ int c3d_solve_poisson_equation(c3d_ctx* c, double* rhs, int rk_step, double dt) 
   return sync(c);
 }
 
+// Refinement residual of the mixed path: formed inside the forward x
+// transform (default), or as its own FP32 array read by the transform
+// (C3D_MIXED_FUSED=0); both give the same spectrum.
+static bool mixed_fused() {
+  static const bool on = [] {
+    const char* e = getenv("C3D_MIXED_FUSED");
+    return !(e && atoi(e) == 0);
+  }();
+  return on;
+}
+
 // compute_non_IB_projection (src/projection.f90:22-46) on the resident
 // velocity.  FP64: one spectral solve.  Mixed: the FP32-storage solve of the
 // divergence, then nref sweeps of FP64 residual r = b - D G phi and an FP32
@@ -596,14 +695,17 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   if (ev) HIPCHK(c, hipEventRecord(ev[0], st));
   if (mixed) {
     HIPCHK(c, hipMemsetAsync(c->red + kRedRefine, 0, (size_t)kMaxRef * kResSlots * sizeof(unsigned long long), st));
-    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, c->b64, nullptr, c->S32, st));
+    // the FP64 divergence b is kept only for the refinement residuals
+    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, c->nref > 0 ? c->b64 : nullptr, nullptr,
+                                  c->S32, st));
   } else {
     HIPCHK(c, c3d::launch_fwd_x(g, true, c->U, c->V, c->W, nullptr, c->S, st));
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], st));
   HIPCHK(c, mixed ? c3d::launch_fft_z32(g, -1, c->S32, st) : c3d::launch_fft_z(g, -1, c->S, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[2], st));
-  HIPCHK(c, mixed ? c3d::launch_tri32(g, c->S32, st) : c3d::launch_tri(g, c->S, c->CP, st));
+  HIPCHK(c, mixed ? c3d::launch_tri32(g, c->S32, c->tri_low, c->ntri_low, c->tri_thr, st)
+                  : c3d::launch_tri(g, c->S, c->CP, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], st));
   HIPCHK(c, mixed ? c3d::launch_fft_z32(g, +1, c->S32, st) : c3d::launch_fft_z(g, +1, c->S, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[4], st));
@@ -616,11 +718,15 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   if (ev) HIPCHK(c, hipEventRecord(ev[5], st));
   for (int s = 1; mixed && s <= c->nref; s++) {
     const bool last = s == c->nref;
-    HIPCHK(c, c3d::launch_residual(g, c->b64, c->phi64, c->gauge, c->r32,
-                                   c->red + kRedRefine + (size_t)(s - 1) * kResSlots, st));
-    HIPCHK(c, c3d::launch_fwd_x32(g, nullptr, nullptr, nullptr, nullptr, c->r32, c->S32, st));
+    unsigned long long* rslots = c->red + kRedRefine + (size_t)(s - 1) * kResSlots;
+    if (mixed_fused()) {  // residual formed inside the forward x transform
+      HIPCHK(c, c3d::launch_fwd_x_resid32(g, c->b64, c->phi64, c->gauge, c->S32, rslots, st));
+    } else {
+      HIPCHK(c, c3d::launch_residual(g, c->b64, c->phi64, c->gauge, c->r32, rslots, st));
+      HIPCHK(c, c3d::launch_fwd_x32(g, nullptr, nullptr, nullptr, nullptr, c->r32, c->S32, st));
+    }
     HIPCHK(c, c3d::launch_fft_z32(g, -1, c->S32, st));
-    HIPCHK(c, c3d::launch_tri32(g, c->S32, st));
+    HIPCHK(c, c3d::launch_tri32(g, c->S32, c->tri_low, c->ntri_low, c->tri_thr, st));
     HIPCHK(c, c3d::launch_fft_z32(g, +1, c->S32, st));
     if (last)
       HIPCHK(c, c3d::launch_inv_x32(g, 4, c->S32, c->phi64, c->Pint, c->P, dt, rk1, st));
@@ -673,6 +779,7 @@ int c3d_copy_bandwidth(c3d_ctx* c, size_t nbytes, int iters, double* ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
   if (hipMalloc(&a, nbytes) != hipSuccess || hipMalloc(&b, nbytes) != hipSuccess) {
+    (void)hipGetLastError();  // the launchers report hipGetLastError()
     rc = fail(c, C3D_ENOMEM, "copy calibration buffers");
   } else if (hipMemsetAsync(a, 0, nbytes, c->stream) != hipSuccess ||
              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
@@ -707,8 +814,11 @@ static int ensure_rk(c3d_ctx* c) {
   const size_t fw = (size_t)(c->nxg - 2) * (c->nyg - 2) * (c->nz - 2);
   int rc = 0;
   if ((rc = dalloc(c, &c->Uo, c->nu)) || (rc = dalloc(c, &c->Vo, c->nv)) ||
-      (rc = dalloc(c, &c->Wo, c->nw)))
+      (rc = dalloc(c, &c->Wo, c->nw)) || (rc = dalloc(c, &c->Ux, c->nu)) ||
+      (rc = dalloc(c, &c->Vx, c->nv)) || (rc = dalloc(c, &c->Wx, c->nw))) {
+    c->Uo = nullptr;  // allocate again next time
     return rc;
+  }
   for (int s = 0; s < 3; s++)
     if ((rc = dalloc(c, &c->F[s][0], fu)) || (rc = dalloc(c, &c->F[s][1], fv)) ||
         (rc = dalloc(c, &c->F[s][2], fw)))
@@ -718,7 +828,6 @@ static int ensure_rk(c3d_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->F[s][1], 0, fv * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(c->F[s][2], 0, fw * 8, c->stream));
   }
-  if ((rc = dalloc(c, &c->part, 2 * (size_t)c3d::massflow_rows(c->g) + 2))) return rc;
   return 0;
 }
 
@@ -745,9 +854,48 @@ static int rk_update_launch(c3d_ctx* c, int rk_step, double dt) {
   return 0;
 }
 
-static int mass_flow(c3d_ctx* c, double* q) {
+// One fused stage (right-hand side and update in one kernel, to round-off of
+// the reference's operations).  Stage 1 reads U and writes into the Uo set,
+// which then becomes U while the old U (the step's start) becomes Uo; stages
+// 2 and 3 read U, Uo and F1 and write into the Ux set, which then swaps with U.
+static int rk_stage_fast(c3d_ctx* c, int st, double dt) {
+  c3d::RkStage a{};
+  a.U = c->U;
+  a.V = c->V;
+  a.W = c->W;
+  a.Hu = c->Hu;
+  a.Hv = c->Hv;
+  a.Hw = c->Hw;
+  a.nu = c->desc.nu;
+  a.dPdx = c->dPdx;
+  a.dPdy = c->dPdy;
+  a.dPdz = c->dPdz;
+  a.dt = dt;
+  a.Uo = c->Uo;
+  a.Vo = c->Vo;
+  a.Wo = c->Wo;
+  a.F1u = c->F[0][0];
+  a.F1v = c->F[0][1];
+  a.F1w = c->F[0][2];
+  if (st < 3) {
+    a.Fu = c->F[st - 1][0];
+    a.Fv = c->F[st - 1][1];
+    a.Fw = c->F[st - 1][2];
+  }
+  double** dst[3] = {st == 1 ? &c->Uo : &c->Ux, st == 1 ? &c->Vo : &c->Vx, st == 1 ? &c->Wo : &c->Wx};
+  a.Un = *dst[0];
+  a.Vn = *dst[1];
+  a.Wn = *dst[2];
+  HIPCHK(c, c3d::launch_rhs_stage(c->g, st, a, c->stream));
+  std::swap(c->U, *dst[0]);
+  std::swap(c->V, *dst[1]);
+  std::swap(c->W, *dst[2]);
+  return 0;
+}
+
+static int mass_flow(c3d_ctx* c, const double* U, double* q) {
   double* out = c->part + 2 * (size_t)c3d::massflow_rows(c->g);
-  HIPCHK(c, c3d::launch_massflow(c->g, c->U, c->Hm, c->part, out, c->stream));
+  HIPCHK(c, c3d::launch_massflow(c->g, U, c->Hm, c->part, out, c->stream));
   double h[2];
   HIPCHK(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   int rc = sync(c);
@@ -836,9 +984,29 @@ int c3d_compute_dt(c3d_ctx* c, double cfl, double* dt_out) {
 
 int c3d_mean_mass_flow_u(c3d_ctx* c, double* q) {
   if (!c || !q) return C3D_EINVAL;
-  int rc = ensure_rk(c);
-  if (!rc) rc = mass_flow(c, q);
+  return mass_flow(c, c->U, q);
+}
+
+int c3d_mean_mass_flow_u_array(c3d_ctx* c, const double* U, double* q) {
+  if (!c || !U || !q) return C3D_EINVAL;
+  int rc = stage_uvw(c, U, nullptr, nullptr);
+  if (!rc) rc = mass_flow(c, c->xU, q);
   return rc;
+}
+
+int c3d_compute_rhs_arrays(c3d_ctx* c, const double* U, const double* V, const double* W,
+                           double* Fu, double* Fv, double* Fw) {
+  if (!c || !U || !V || !W) return C3D_EINVAL;
+  int rc = stage_uvw(c, U, V, W);
+  if (rc) return rc;
+  double* F[3] = {c->xF, c->xF + interior_size(c, 0), c->xF + interior_size(c, 0) + interior_size(c, 1)};
+  HIPCHK(c, c3d::launch_rhs(c->g, c->xU, c->xV, c->xW, c->Hu, c->Hv, c->Hw, c->desc.nu, c->dPdx,
+                            c->dPdy, c->dPdz, F[0], F[1], F[2], c->stream));
+  double* out[3] = {Fu, Fv, Fw};
+  for (int q = 0; q < 3; q++)
+    if (out[q])
+      HIPCHK(c, hipMemcpyAsync(out[q], F[q], interior_size(c, q) * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
 }
 
 // Plane means of compute_statistics (src/statistics.f90:41-111, one domain):
@@ -928,12 +1096,19 @@ int c3d_add_to_u(c3d_ctx* c, double d) {
 int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
   if (!c || !(dt > 0.0)) return C3D_EINVAL;
   int rc = ensure_rk(c);
-  if (!rc) rc = save_state(c);  // src/time_integration.f90:91-94
+  if (!rc && hipMemsetAsync(c->nanflag, 0, sizeof(unsigned), c->stream) != hipSuccess)
+    rc = fail(c, C3D_EHIP, "non-finite flag reset");
+  const bool exact = c->rhs_exact;
+  if (!rc && exact) rc = save_state(c);  // src/time_integration.f90:91-94 (fused: kept in place)
   const double t0 = c->t;
   static const double rk_t[3] = {8.0 / 15.0, 2.0 / 3.0, 1.0};  // src/initialization.f90:408-410
   for (int st = 1; st <= 3 && !rc; st++) {
-    rc = rhs_launch(c, st);                                     // :101-103
-    if (!rc) rc = rk_update_launch(c, st, dt);                  // :105-107
+    if (exact) {
+      rc = rhs_launch(c, st);                                   // :101-103
+      if (!rc) rc = rk_update_launch(c, st, dt);                // :105-107
+    } else {
+      rc = rk_stage_fast(c, st, dt);                            // :101-107 in one pass
+    }
     c->t = t0 + rk_t[st - 1] * dt;                              // :108
     if (!rc && c->ib && c->ib->moving) rc = ib_move_to(c, c->t);  // :110-115
     if (!rc && c3d::launch_apply_bc(c->g, c->U, c->V, c->W, c->stream) != hipSuccess)
@@ -952,7 +1127,7 @@ int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
   if (!rc && x_mass_cte == 1) {  // :187-193
     const double prev = c->dPdx;
     double ub = 0.0;
-    rc = mass_flow(c, &ub);
+    rc = mass_flow(c, c->U, &ub);
     if (!rc) {
       const double d = q0 - ub;
       if (c3d::launch_add_u(c->g, c->U, d, c->stream) != hipSuccess ||
@@ -961,8 +1136,20 @@ int c3d_time_step_rk3(c3d_ctx* c, double dt, int x_mass_cte, double q0) {
       c->dPdx = 0.5 * (prev + d / dt);
     }
   }
+  // NaN stop of compute_statistics (src/statistics.f90:142-144), checked at
+  // every step: the velocity corrections flag any non-finite value they write
+  unsigned bad = 0;
+  if (!rc && hipMemcpyAsync(&bad, c->nanflag, sizeof(bad), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = fail(c, C3D_EHIP, "non-finite flag read");
   if (!rc) rc = sync(c);
+  if (!rc && bad) rc = fail(c, C3D_ENAN, "non-finite velocity after the time step at t = %.6e", c->t);
   return rc;
+}
+
+int c3d_set_rhs_exact(c3d_ctx* c, int on) {
+  if (!c) return C3D_EINVAL;
+  c->rhs_exact = on != 0;
+  return C3D_OK;
 }
 
 int c3d_set_refinement(c3d_ctx* c, int nref) {
@@ -1085,7 +1272,7 @@ static int ib_schur_launch(c3d_ctx* c, const double* f, double* Sf, double dt, i
 }
 
 // One BiCGSTAB iteration (src/projection.f90:160-182) on the stream, ending
-// with the squared residual norm copied to the pinned word ib->herr.
+// with the squared residual norm stored to the pinned host word ib->herr.
 static int ib_bicg_iteration(c3d_ctx* c, double* x, double dt, int rk_step) {
   IBState* ib = c->ib;
   namespace B = c3d::ib;
@@ -1101,8 +1288,9 @@ static int ib_bicg_iteration(c3d_ctx* c, double* x, double dt, int rk_step) {
   if ((rc = ib_schur_launch(c, sv, t, dt, rk_step))) return rc;
   HIPCHK(c, B::launch_dot(t, sv, t, t, n, ib->part, 2, c->stream));
   HIPCHK(c, B::launch_bicg_xr(h, sv, t, rhat, ib->sc, ib->part, x, r, n, c->stream));
-  HIPCHK(c, B::launch_bicg_scalar(4, ib->sc, ib->part, nullptr, c->stream));
-  HIPCHK(c, hipMemcpyAsync(ib->herr, ib->sc + 5, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // the scalar kernel stores ||r||^2 straight into the pinned host word, so a
+  // captured iteration holds kernel nodes only (no device-to-host copy node)
+  HIPCHK(c, B::launch_bicg_scalar(4, ib->sc, ib->part, ib->herr, c->stream));
   return 0;
 }
 
@@ -1147,8 +1335,12 @@ static int ib_bicgstab_launch(c3d_ctx* c, double* x, const double* b, double tol
   const size_t n = 4 * (size_t)ib->m.nb;
   double *r = ib->vec[1], *rhat = ib->vec[2], *p = ib->vec[3], *v = ib->vec[4];
   int rc = 0;
-  if (!ib->herr && hipHostMalloc((void**)&ib->herr, sizeof(double), hipHostMallocDefault) != hipSuccess) {
+  // coherent (fine-grained) host memory: the kernel's store is visible to the
+  // host once the stream is synchronised
+  if (!ib->herr &&
+      hipHostMalloc((void**)&ib->herr, sizeof(double), hipHostMallocCoherent) != hipSuccess) {
     ib->herr = nullptr;
+    (void)hipGetLastError();
     return fail(c, C3D_ENOMEM, "pinned residual word");
   }
   if (!r_ready && (rc = ib_schur_launch(c, x, r, dt, rk_step))) return rc;
@@ -1786,6 +1978,27 @@ int c3d_ib_interpolate(c3d_ctx* c, int with_dxn, double* out) {
   return sync(c);
 }
 
+int c3d_ib_interpolate_arrays(c3d_ctx* c, int with_dxn, const double* U, const double* V,
+                              const double* W, double* out) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  if (!U || !V || !W || !out) return C3D_EINVAL;
+  IBState* ib = c->ib;
+  if ((rc = stage_uvw(c, U, V, W))) return rc;
+  HIPCHK(c, c3d::ib::launch_interp_uvw(c->g, ib->m, c->xU, c->xV, c->xW, with_dxn != 0, ib->fout,
+                                       c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, ib->fout, 3 * (size_t)ib->m.nb * 8, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int c3d_ib_compute_heaviside(c3d_ctx* c) {
+  int rc = need_ib(c);
+  if (rc) return rc;
+  // the markers, stencils and spreading maps are unchanged, so cached
+  // iteration graphs stay valid
+  return ib_heaviside(c);
+}
+
 int c3d_ib_interpolate_c(c3d_ctx* c, const double* p, double* out) {
   int rc = need_ib(c);
   if (rc) return rc;
@@ -1942,28 +2155,44 @@ static uint64_t splitmix(uint64_t& x) {
   return z ^ (z >> 31);
 }
 
+// Activation buffers for `rows` rows.  The larger set is allocated in full
+// before the old one is released, so a failed allocation leaves the old,
+// smaller set (and cap) intact and no pointer dangling.
 static int mlp_reserve(c3d_ctx* c, int rows) {
   MLPState* m = c->mlp;
   if (rows <= m->cap) return 0;
-  for (void* p : m->bufs) (void)hipFree(p);
-  m->bufs.clear();
-  auto get = [&](float** p, size_t n) -> int {
-    void* q = nullptr;
-    if (hipMalloc(&q, n * sizeof(float)) != hipSuccess)
-      return fail(c, C3D_ENOMEM, "network buffers (%zu floats)", n);
-    m->bufs.push_back(q);
-    *p = static_cast<float*>(q);
-    return 0;
-  };
   using namespace c3d::mlp;
-  int rc = 0;
-  if ((rc = get(&m->X, (size_t)rows * kIn)) || (rc = get(&m->T, (size_t)rows * kOut)) ||
-      (rc = get(&m->dY, (size_t)rows * kOut)))
-    return rc;
-  for (int i = 0; i < kLayers; i++)
-    if ((rc = get(&m->act[i], (size_t)rows * m->L.out[i]))) return rc;
-  for (int i = 0; i < kLayers - 1; i++)
-    if ((rc = get(&m->dH[i], (size_t)rows * m->L.out[i]))) return rc;
+  std::vector<void*> nb;
+  auto get = [&](float** p, size_t n) -> bool {
+    void* q = nullptr;
+    if (hipMalloc(&q, n * sizeof(float)) != hipSuccess) {
+      (void)hipGetLastError();  // the launchers report hipGetLastError()
+      return false;
+    }
+    nb.push_back(q);
+    *p = static_cast<float*>(q);
+    return true;
+  };
+  float *X = nullptr, *T = nullptr, *dY = nullptr;
+  float* act[kLayers] = {nullptr};
+  float* dH[kLayers - 1] = {nullptr};
+  bool ok = get(&X, (size_t)rows * kIn) && get(&T, (size_t)rows * kOut) &&
+            get(&dY, (size_t)rows * kOut);
+  for (int i = 0; i < kLayers && ok; i++) ok = get(&act[i], (size_t)rows * m->L.out[i]);
+  for (int i = 0; i < kLayers - 1 && ok; i++) ok = get(&dH[i], (size_t)rows * m->L.out[i]);
+  if (!ok) {
+    for (void* p : nb) (void)hipFree(p);
+    return fail(c, C3D_ENOMEM, "network buffers for %d rows", rows);
+  }
+  // no kernel may still read the old set (all work of the context is on its stream)
+  (void)hipStreamSynchronize(c->stream);
+  for (void* p : m->bufs) (void)hipFree(p);
+  m->bufs.swap(nb);
+  m->X = X;
+  m->T = T;
+  m->dY = dY;
+  for (int i = 0; i < kLayers; i++) m->act[i] = act[i];
+  for (int i = 0; i < kLayers - 1; i++) m->dH[i] = dH[i];
   m->cap = rows;
   return 0;
 }
@@ -2019,7 +2248,13 @@ static int mlp_guess(c3d_ctx* c, double dt, int rk_step) {
   double *r = ib->vec[1], *d = ib->vec[5], *Sd = ib->vec[6];
   if ((rc = ib_schur_launch(c, ib->fb, r, dt, rk_step))) return rc;  // S fprev
   HIPCHK(c, B::launch_bicg_init(ib->rhs_ib, r, ib->vec[2], ib->vec[3], ib->vec[4], n4, c->stream));
-  HIPCHK(c, N::features(nb, m->nb9, r, ib->fprev, m->iscale, m->X, c->stream));
+  // per-solve normalisation: the residual channels by their rms over the 4nb
+  // rows, the force channels by theirs over dudn_x and dudn_y (2nb)
+  HIPCHK(c, B::launch_dot(r, r, nullptr, nullptr, n4, ib->part, 2, c->stream));
+  HIPCHK(c, B::launch_dot(ib->fprev, ib->fprev, nullptr, nullptr, 2 * (size_t)nb, ib->part, 3,
+                          c->stream));
+  HIPCHK(c, B::launch_norm_scales(ib->part, 2, n4, 3, 2 * (size_t)nb, m->nrm, c->stream));
+  HIPCHK(c, N::features(nb, m->nb9, r, ib->fprev, m->nrm, m->iscale, m->X, c->stream));
   HIPCHK(c, N::forward(m->L, m->P, m->X, nb, m->act, c->stream));
   HIPCHK(c, N::apply(nb, m->act[N::kLayers - 1], m->oscale, nullptr, d, c->stream));
   if ((rc = ib_schur_launch(c, d, Sd, dt, rk_step))) return rc;      // S d
@@ -2042,7 +2277,7 @@ int c3d_mlp_init(c3d_ctx* c, unsigned long long seed) {
     if ((rc = dalloc(c, &m->P, n)) || (rc = dalloc(c, &m->G, n)) || (rc = dalloc(c, &m->M, n)) ||
         (rc = dalloc(c, &m->V, n)) || (rc = dalloc(c, &m->iscale, c3d::mlp::kCh)) ||
         (rc = dalloc(c, &m->oscale, c3d::mlp::kOut)) || (rc = dalloc(c, &m->ones, c3d::mlp::kCh)) ||
-        (rc = dalloc(c, &m->part, c3d::mlp::kLossBlocks)))
+        (rc = dalloc(c, &m->part, c3d::mlp::kLossBlocks)) || (rc = dalloc(c, &m->nrm, 4)))
       return rc;
   }
   MLPState* m = c->mlp;
@@ -2063,6 +2298,8 @@ int c3d_mlp_init(c3d_ctx* c, unsigned long long seed) {
   HIPCHK(c, hipMemcpy(m->iscale, one.data(), c3d::mlp::kCh * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(m->oscale, one.data(), c3d::mlp::kOut * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(m->ones, one.data(), c3d::mlp::kCh * 4, hipMemcpyHostToDevice));
+  const double unit[4] = {1.0, 1.0, 1.0, 1.0};  // nrm[2..3]: no normalisation (training pairs)
+  HIPCHK(c, hipMemcpy(m->nrm, unit, sizeof(unit), hipMemcpyHostToDevice));
   m->t = 0;
   m->ready = false;
   return C3D_OK;
@@ -2255,7 +2492,8 @@ int c3d_ib_last_pair(c3d_ctx* c, float* X, double* dF) {
     if ((rc = ib_schur_launch(c, ib->fprev, r, ib->last_dt, 2))) return rc;
     HIPCHK(c, c3d::ib::launch_bicg_init(ib->rhs_ib, r, ib->vec[2], ib->vec[3], ib->vec[4],
                                         4 * (size_t)nb, c->stream));
-    HIPCHK(c, N::features(nb, c->mlp->nb9, r, ib->fprev, c->mlp->ones, c->mlp->X, c->stream));
+    HIPCHK(c, N::features(nb, c->mlp->nb9, r, ib->fprev, c->mlp->nrm + 2, c->mlp->ones, c->mlp->X,
+                          c->stream));
     HIPCHK(c, hipMemcpyAsync(X, c->mlp->X, (size_t)nb * N::kIn * 4, hipMemcpyDeviceToHost, c->stream));
   }
   if (dF) {

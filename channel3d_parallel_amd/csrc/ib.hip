@@ -563,24 +563,32 @@ __global__ __launch_bounds__(kBlock) void k_ib_correct(DevGrid g, const double* 
     return dt * (Pint[o] - phi[o]);
   };
   const bool core = i >= 2 && i <= nx - 1 && j >= 2 && k >= 2;
+  bool bad = false;
   if (i <= nx) {  // U(nx, nyg, nzg)
     const size_t o = fidx(i, j, k, nx, nyg);
     double G = 0.0;
     if (core && j <= nyg - 1 && k <= nzg - 1) G = (q(i + 1, j, k) - q(i, j, k)) / g.dxg[i];
-    U[o] = Uint[o] - Ureg[o] - G;
+    const double u = Uint[o] - Ureg[o] - G;
+    U[o] = u;
+    bad |= !isfinite(u);
   }
   if (j <= ny) {  // V(nxg, ny, nzg)
     const size_t o = fidx(i, j, k, nxg, ny);
     double G = 0.0;
     if (core && j <= ny - 1 && k <= nzg - 1) G = (q(i, j + 1, k) - q(i, j, k)) / g.dyg[j];
-    V[o] = Vint[o] - Vreg[o] - G;
+    const double v = Vint[o] - Vreg[o] - G;
+    V[o] = v;
+    bad |= !isfinite(v);
   }
   if (k <= nz) {  // W(nxg, nyg, nz)
     const size_t o = fidx(i, j, k, nxg, nyg);
     double G = 0.0;
     if (core && j <= nyg - 1 && k <= nz - 1) G = (q(i, j, k + 1) - q(i, j, k)) / g.dzg[k];
-    W[o] = Wint[o] - Wreg[o] - G;
+    const double w = Wint[o] - Wreg[o] - G;
+    W[o] = w;
+    bad |= !isfinite(w);
   }
+  if (bad && g.nonfinite) atomicOr(g.nonfinite, 1u);
 }
 
 __global__ void k_affine(const double* __restrict__ x, double a, double b, double* __restrict__ y,
@@ -639,6 +647,17 @@ __device__ __forceinline__ double sum_part(const double* part, int slot) {
   return r;
 }
 
+// Per-solve input normalisation of the learned guess: out[0] = 1/rms of the
+// n_r entries whose squares are summed in partial slot sr, out[1] = the
+// same for slot sf (0 for a zero vector).
+__global__ void k_norm_scales(const double* __restrict__ part, int sr, double n_r, int sf, double n_f,
+                              double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double qr = sum_part(part, sr), qf = sum_part(part, sf);
+  out[0] = qr > 0.0 ? 1.0 / sqrt(qr / n_r) : 0.0;
+  out[1] = qf > 0.0 ? 1.0 / sqrt(qf / n_f) : 0.0;
+}
+
 // Scalar slots: 0 rho_o, 1 alpha, 2 om, 3 rho_n, 4 bta, 5 err, 6 line-search
 // step.  The vector kernels below compute their scalar from the dot partials
 // themselves (thread 0 of every block, the same fixed-order sum) and block 0
@@ -655,7 +674,10 @@ __global__ void k_bicg_scalar(int step, double* sc, const double* __restrict__ p
     case 4: {
       const double e = sum_part(part, 4);
       sc[5] = e;
-      if (host_err) *host_err = e;
+      if (host_err) {  // pinned host word read after the stream synchronises
+        *host_err = e;
+        __threadfence_system();
+      }
       break;
     }
     case 5: {  // minimal-residual step along d: a = <r, S d> / <S d, S d>
@@ -975,6 +997,13 @@ hipError_t launch_h_interp(const DevGrid& g, const double* Hc, double* Hu, doubl
 hipError_t launch_dot(const double* a, const double* b, const double* a2, const double* b2,
                       size_t n, double* part, int slot, hipStream_t s) {
   hipLaunchKernelGGL(k_dot, dim3(kDotBlocks), dim3(kBlock), 0, s, a, b, a2, b2, n, part, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm_scales(const double* part, int sr, size_t n_r, int sf, size_t n_f,
+                              double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm_scales, dim3(1), dim3(64), 0, s, part, sr, (double)n_r, sf, (double)n_f,
+                     out);
   return hipGetLastError();
 }
 

@@ -148,6 +148,10 @@ constexpr int kDotBlocks = 128;
 hipError_t launch_dot(const double* a, const double* b, const double* a2, const double* b2,
                       size_t n, double* part, int slot, hipStream_t s);
 // Scalar steps: 0 initialise, 4 err (also to *host_err when not null), 5
+// out[0] = 1/rms over n_r entries of the squares summed in slot sr, out[1]
+// likewise for slot sf (0 where the sum is 0).
+hipError_t launch_norm_scales(const double* part, int sr, size_t n_r, int sf, size_t n_f,
+                              double* out, hipStream_t s);
 // minimal-residual step sc[6] = <r,Sd>/<Sd,Sd> from partial slots 2 and 3.
 hipError_t launch_bicg_scalar(int step, double* sc, const double* part, double* host_err,
                               hipStream_t s);

@@ -50,6 +50,7 @@ constexpr int kSrcUVW = 1;    // divergence of U, V, W computed on the fly
 constexpr int kSrcUVWB = 2;   // same, and the FP64 divergence also stored to bout
 constexpr int kSrcRhs32 = 3;  // rhs_p-shaped FP32 array (refinement residual)
 constexpr int kSrcUVWc = 4;   // as kSrcUVW with cached U and W loads (fields that fit in cache)
+constexpr int kSrcResid = 5;  // FP64 residual b - M phi of the mixed path's refinement, on the fly
 
 template <typename real, int N, int TK, int SRC>
 __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __restrict__ U,
@@ -58,9 +59,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
                                                   const void* __restrict__ rhs_in,
                                                   const cpx<real>* __restrict__ tw,
                                                   cpx<real>* __restrict__ S,
-                                                  double* __restrict__ bout) {
+                                                  double* __restrict__ bout,
+                                                  const double* __restrict__ gauge,
+                                                  unsigned long long* __restrict__ rmax) {
   constexpr bool FROM_UVW = (SRC == kSrcUVW || SRC == kSrcUVWB || SRC == kSrcUVWc);
+  constexpr bool RESID = (SRC == kSrcResid);
   constexpr bool NTL = (SRC != kSrcUVWc);  // non-temporal U and W loads
+  constexpr int NV = FROM_UVW ? 5 : (RESID ? 6 : 1);  // values loaded per point
   constexpr int NP = TK / 2;
   constexpr int NT = kBlock / NP;
   constexpr int NH = N / 2 + 1;
@@ -79,7 +84,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   //    lane owns the x positions n = tid, tid + kBlock, ...  Loads of a
   //    batch of RB rows are all issued before any is used.
   constexpr int NPL = (N + kBlock - 1) / kBlock;  // x positions per lane
-  constexpr int RB = TK < 4 ? TK : 4;             // rows per load batch
+  // rows per load batch (the residual loads 6 values per point: 2 rows)
+  constexpr int RB = TK < 4 ? TK : (RESID ? 2 : 4);
   const double ry = g.rdyf[j];
   double wprev[NPL];  // W(k-1) of the next row: the W row of the row before
   if constexpr (FROM_UVW) {
@@ -90,9 +96,34 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
       wprev[p] = (N % kBlock == 0 || n < N) ? W0[n + 1] : 0.0;
     }
   }
+  // Residual r = b - M phi (the operator of k_residual: D G on the unique
+  // cells, periodic in x and z, no gradient on the wall faces, plus the
+  // (0,0)-mode gauge row at j = 2), with U = phi and rhs_in = b; each lane
+  // walks its x positions down the tile's rows carrying phi(k-1), phi(k).
+  const double* phi = U;
+  const size_t pk = (size_t)(nxg - 2) * (nyg - 2);  // rhs_p plane stride
+  const int rowj = nxg - 2;                          // rhs_p row stride
+  double pm[NPL], p0[NPL];
+  double amax = 0.0;
+  double gj = 0.0, rhy = 0.0, rhym = 0.0;
+  if constexpr (RESID) {
+    gj = (j == 2) ? *gauge : 0.0;
+    rhy = g.rdyg[j];
+    rhym = g.rdyg[j - 1];
+    const int kf = k0 + 2, km0 = (kf == 2) ? g.nz - 1 : kf - 1;
+#pragma unroll
+    for (int p = 0; p < NPL; p++) {
+      const int n = tid + p * kBlock;
+      pm[p] = p0[p] = 0.0;
+      if (N % kBlock == 0 || n < N) {
+        pm[p] = phi[ridx(2, j, km0, nxg, nyg) + n];
+        p0[p] = phi[ridx(2, j, kf, nxg, nyg) + n];
+      }
+    }
+  }
 #pragma unroll
   for (int t0 = 0; t0 < TK; t0 += RB) {
-    double v[RB][NPL][FROM_UVW ? 5 : 1];
+    double v[RB][NPL][NV];
 #pragma unroll
     for (int r = 0; r < RB; r++) {
       const int k = k0 + t0 + r + 2;
@@ -121,6 +152,21 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
             }
             v[r][p][2] = Vr[n + 1];
             v[r][p][3] = Vm[n + 1];
+          }
+        }
+      } else if constexpr (RESID) {
+        const double* Br = static_cast<const double*>(rhs_in) + ridx(2, j, k, nxg, nyg);
+        const double* Pr = phi + ridx(2, j, k, nxg, nyg);
+#pragma unroll
+        for (int p = 0; p < NPL; p++) {
+          const int n = tid + p * kBlock;
+          if (N % kBlock == 0 || n < N) {
+            v[r][p][0] = Br[n];
+            v[r][p][1] = Pr[n + pk];                          // phi(i, j, k+1)
+            v[r][p][2] = Pr[n + 1];                           // phi(i+1, j, k)
+            v[r][p][3] = (n == 0) ? Pr[N - 1] : Pr[n - 1];    // phi(i-1), periodic at i = 2
+            v[r][p][4] = (j <= ny - 1) ? Pr[n + rowj] : 0.0;  // phi(i, j+1, k)
+            v[r][p][5] = (j >= 3) ? Pr[n - rowj] : 0.0;       // phi(i, j-1, k)
           }
         }
       } else if constexpr (SRC == kSrcRhs32) {
@@ -156,6 +202,25 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
               const double wm = (r + h == 0) ? wprev[p] : v[r + h - 1][p][4];
               d[h] = (v[r + h][p][0] - v[r + h][p][1]) * rx +
                      (v[r + h][p][2] - v[r + h][p][3]) * ry + (v[r + h][p][4] - wm) * rz;
+            } else if constexpr (RESID) {
+              // the expressions of k_residual, term by term
+              const int i = n + 2, k = k0 + t0 + r + h + 2;
+              const int km = (k == 2) ? g.nz - 1 : k - 1;
+              const double* q = v[r + h][p];
+              const double c0 = p0[p];
+              const double gxp = (q[2] - c0) * g.rdxg[i];
+              const double gxm = (c0 - q[3]) * g.rdxg[(i == 2) ? nx - 1 : i - 1];
+              const double gyp = (j <= ny - 1) ? (q[4] - c0) * rhy : 0.0;
+              const double gym = (j >= 3) ? (c0 - q[5]) * rhym : 0.0;
+              const double gzp = (q[1] - c0) * g.rdzg[k];
+              const double gzm = (c0 - pm[p]) * g.rdzg[km];
+              const double lap = (gxp - gxm) * rx + (gyp - gym) * ry + (gzp - gzm) * g.rdzf[k];
+              d[h] = (q[0] - lap) - gj;
+              amax = fmax(amax, fabs(d[h]));
+              pm[p] = c0;
+              p0[p] = q[1];
+              // the FFT input is the FP32 residual, as k_residual stores it
+              d[h] = (double)(float)d[h];
             } else {
               d[h] = v[r + h][p][0];
             }
@@ -209,15 +274,28 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
     __builtin_nontemporal_store(v.x, &d->x);
     __builtin_nontemporal_store(v.y, &d->y);
   }
+  if constexpr (RESID) {
+    // max |r| of the workgroup into slot (workgroup % 64), as k_residual
+    __shared__ double sm[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
+    if ((tid & 63) == 0) sm[tid >> 6] = amax;
+    __syncthreads();
+    if (tid == 0 && rmax) {
+      double mx = sm[0];
+      for (int w = 1; w < kBlock / 64; w++) mx = fmax(mx, sm[w]);
+      const unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
+      atomicMax(rmax + (lin & 63), (unsigned long long)__double_as_longlong(mx));
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
 // z transform (C2C) of every column S[j][kx][:], in place.
 // ---------------------------------------------------------------------------
-template <typename real, int N, int SIGN>
+template <typename real, int N, int SIGN, int NT>
 __global__ __launch_bounds__(kBlock) void k_fft_z(cpx<real>* __restrict__ S,
                                                   const cpx<real>* __restrict__ tw, int ncol) {
-  constexpr int NT = 64;
+  // NT threads per column, NC columns per workgroup
   constexpr int NC = kBlock / NT;
   constexpr int LP = lds_row_len(N);
   __shared__ cpx<real> buf[NC * LP];
@@ -322,8 +400,17 @@ __global__ __launch_bounds__(kBlock) void k_tri64(DevGrid g, cpx<double>* __rest
 // Part 0 (l = 0 in its first row) and the last part (u = 0 in its last row)
 // have no outside coupling, so the reduced system has no boundary terms.
 // ---------------------------------------------------------------------------
-template <typename real, int M>
-__global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __restrict__ S) {
+// Mode selection of the partitioned solve (mixed precision runs the solve
+// in two launches: FP32 arithmetic for the well-conditioned modes, FP64 for
+// the (0,0) mode and the lowest |k|, whose systems are nearly singular).
+constexpr int kTriAll = 0;   // every mode
+constexpr int kTriHigh = 1;  // modes with |kxx + kzz| >= lam_thr
+constexpr int kTriList = 2;  // the modes list[0 .. nlist)
+
+template <typename real, typename acc, int M, int SEL>
+__global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __restrict__ S,
+                                                     const int* __restrict__ list, int nlist,
+                                                     double lam_thr) {
   constexpr int P = 8;            // parts per mode
   constexpr int N = P * M;        // rows
   constexpr int MW = 16;          // modes per wave
@@ -332,14 +419,14 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
   constexpr int MB = kBlock / P;  // modes per workgroup
   constexpr int CPT = M + 1;      // padded pitch of the row tables
   constexpr int RP = 8 * P + 1;   // padded pitch of a mode's interface rows
-  __shared__ double td[P * CPT], tl[P * CPT], tu[P * CPT];
-  __shared__ double red[MB * RP];
+  __shared__ acc td[P * CPT], tl[P * CPT], tu[P * CPT];
+  __shared__ acc red[MB * RP];
   const int tid = threadIdx.x;
   for (int e = tid; e < N; e += kBlock) {
     const int q = e / M, r = e % M;
-    td[q * CPT + r] = g.dyy_d[e];
-    tl[q * CPT + r] = (e == 0) ? 0.0 : g.dyy_l[e - 1];
-    tu[q * CPT + r] = (e == N - 1) ? 0.0 : g.dyy_u[e];
+    td[q * CPT + r] = (acc)g.dyy_d[e];
+    tl[q * CPT + r] = (e == 0) ? (acc)0 : (acc)g.dyy_l[e - 1];
+    tu[q * CPT + r] = (e == N - 1) ? (acc)0 : (acc)g.dyy_u[e];
   }
   const int lane = tid & 63;
   const int wv = tid >> 6;
@@ -347,45 +434,54 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
   const int mb = (wv >> 1) * MW + lane % MW;
   const int nzp = g.nzp;
   const size_t stride = (size_t)g.nxh * nzp;
-  const size_t m = (size_t)blockIdx.x * MB + mb;
-  const bool valid = m < stride;
-  const size_t mv = valid ? m : 0;
+  bool valid;
+  size_t mv;
+  if constexpr (SEL == kTriList) {
+    const int e = blockIdx.x * MB + mb;
+    valid = e < nlist;
+    mv = valid ? (size_t)list[e] : 0;
+  } else {
+    const size_t m = (size_t)blockIdx.x * MB + mb;
+    valid = m < stride;
+    mv = valid ? m : 0;
+  }
   const int kx = (int)(mv / nzp), kz = (int)(mv % nzp);
-  const double lamx = g.kxx[kx];
-  const double lamz = g.kzz[kz];
+  if constexpr (SEL == kTriHigh) valid = valid && !(fabs(g.kxx[kx] + g.kzz[kz]) < lam_thr);
+  const acc lamx = (acc)g.kxx[kx];
+  const acc lamz = (acc)g.kzz[kz];
   const bool zero_first = (kx == 0 && kz == 0 && p == 0);
   cpx<real>* s = S + mv + (size_t)(p * M) * stride;
 
-  double yr[M], yi[M], cp[M], sp[M];
+  acc yr[M], yi[M], cp[M], sp[M];
 #pragma unroll
   for (int r = 0; r < M; r++) {
-    yr[r] = 0.0;
-    yi[r] = 0.0;
+    yr[r] = (acc)0;
+    yi[r] = (acc)0;
     if (valid) {
       const cpx<real> t = s[(size_t)r * stride];
-      yr[r] = (double)t.x;
-      yi[r] = (double)t.y;
+      yr[r] = (acc)t.x;
+      yi[r] = (acc)t.y;
     }
   }
   __syncthreads();  // row tables
 
   // pass 1
-  const double* rd = td + p * CPT;
-  const double* rl = tl + p * CPT;
-  const double* ru = tu + p * CPT;
+  const acc* rd = td + p * CPT;
+  const acc* rl = tl + p * CPT;
+  const acc* ru = tu + p * CPT;
 #pragma unroll
   for (int r = 0; r < M; r++) {
-    double d = (rd[r] + lamx) + lamz;
-    const double l = rl[r];
+    acc d = (rd[r] + lamx) + lamz;
+    const acc l = rl[r];
     if (r == 0) {
-      if (zero_first) d = 1.5 * d;
-      const double inv = 1.0 / d;
+      if (zero_first) d = (acc)1.5 * d;
+      const acc inv = (acc)1 / d;
       yr[0] = yr[0] * inv;
       yi[0] = yi[0] * inv;
       sp[0] = l * inv;
       cp[0] = ru[0] * inv;
     } else {
-      const double inv = 1.0 / (d - l * cp[r - 1]);
+      const acc inv = (acc)1 / (d - l * cp[r - 1]);
       yr[r] = (yr[r] - l * yr[r - 1]) * inv;
       yi[r] = (yi[r] - l * yi[r - 1]) * inv;
       sp[r] = -(l * sp[r - 1]) * inv;
@@ -393,9 +489,9 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
     }
   }
   // pass 2
-  double* my = red + mb * RP + p * 8;
+  acc* my = red + mb * RP + p * 8;
   {
-    double gr = yr[M - 1], gi = yi[M - 1], v = sp[M - 1], w = cp[M - 1];
+    acc gr = yr[M - 1], gi = yi[M - 1], v = sp[M - 1], w = cp[M - 1];
     my[4] = gr;
     my[5] = gi;
     my[6] = v;
@@ -417,31 +513,31 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
   // forward sweep leaves lam(2) mu sig_{q-1} rho_{q-1}(2) in it and the
   // back sweep x_{a-1}(2) x_b(2) in positions 4..7
   if (p == 0) {
-    double* rs = red + mb * RP;
-    double rhr = 0.0, rhi = 0.0, sig = 0.0;
+    acc* rs = red + mb * RP;
+    acc rhr = 0, rhi = 0, sig = 0;
     for (int q = 0; q < P; q++) {
-      double* e = rs + q * 8;
-      const double glr = e[0], gli = e[1], vl = e[2], wl = e[3];
-      const double grr = e[4], gri = e[5], vr = e[6], wr = e[7];
-      const double iden = 1.0 / (1.0 + vl * sig);
-      const double lr = (glr - vl * rhr) * iden, li = (gli - vl * rhi) * iden;
-      const double mu = -wl * iden;
+      acc* e = rs + q * 8;
+      const acc glr = e[0], gli = e[1], vl = e[2], wl = e[3];
+      const acc grr = e[4], gri = e[5], vr = e[6], wr = e[7];
+      const acc iden = (acc)1 / ((acc)1 + vl * sig);
+      const acc lr = (glr - vl * rhr) * iden, li = (gli - vl * rhi) * iden;
+      const acc mu = -wl * iden;
       e[0] = lr;
       e[1] = li;
       e[2] = mu;
       e[3] = sig;
       e[4] = rhr;
       e[5] = rhi;
-      const double nr = grr - vr * (rhr + sig * lr), ni = gri - vr * (rhi + sig * li);
+      const acc nr = grr - vr * (rhr + sig * lr), ni = gri - vr * (rhi + sig * li);
       sig = -(vr * sig * mu + wr);
       rhr = nr;
       rhi = ni;
     }
-    double lnr = 0.0, lni = 0.0;  // L_{q+1}
+    acc lnr = 0, lni = 0;  // L_{q+1}
     for (int q = P - 1; q >= 0; q--) {
-      double* e = rs + q * 8;
-      const double lr = e[0] + e[2] * lnr, li = e[1] + e[2] * lni;  // L_q
-      const double ar = e[4] + e[3] * lr, ai = e[5] + e[3] * li;     // R_{q-1}
+      acc* e = rs + q * 8;
+      const acc lr = e[0] + e[2] * lnr, li = e[1] + e[2] * lni;  // L_q
+      const acc ar = e[4] + e[3] * lr, ai = e[5] + e[3] * li;     // R_{q-1}
       e[6] = lnr;
       e[7] = lni;
       e[4] = ar;
@@ -452,12 +548,12 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
   }
   __syncthreads();
   // pass 3
-  const double xar = my[4], xai = my[5];
-  double xnr = my[6], xni = my[7];
+  const acc xar = my[4], xai = my[5];
+  acc xnr = my[6], xni = my[7];
 #pragma unroll
   for (int r = M - 1; r >= 0; r--) {
-    const double xr = (yr[r] - sp[r] * xar) - cp[r] * xnr;
-    const double xi = (yi[r] - sp[r] * xai) - cp[r] * xni;
+    const acc xr = (yr[r] - sp[r] * xar) - cp[r] * xnr;
+    const acc xi = (yi[r] - sp[r] * xai) - cp[r] * xni;
     if (valid) s[(size_t)r * stride] = cmk((real)xr, (real)xi);
     xnr = xr;
     xni = xi;
@@ -757,25 +853,33 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
   const double hx = g.dxg[i];
   const double hy = (j <= ny - 1) ? g.dyg[j] : 1.0;
   double p = dt * Pint[ridx(i, j, kb, nxg, nyg)];
+  bool bad = false;
 #pragma unroll
   for (int kk = 0; kk < KC; kk++) {
     const int k = kb + kk;
     if (k > nzg - 1) break;
     const double px = dt * Pint[ridx(i + 1, j, k, nxg, nyg)];
     const size_t iu = fidx(i, j, k, nx, nyg);
-    U[iu] = U[iu] - (px - p) / hx;
+    const double u = U[iu] - (px - p) / hx;
+    U[iu] = u;
+    bad |= !isfinite(u);
     if (j <= ny - 1) {
       const double py = dt * Pint[ridx(i, j + 1, k, nxg, nyg)];
       const size_t iv = fidx(i, j, k, nxg, ny);
-      V[iv] = V[iv] - (py - p) / hy;
+      const double v = V[iv] - (py - p) / hy;
+      V[iv] = v;
+      bad |= !isfinite(v);
     }
     if (k <= nz - 1) {
       const double pz = dt * Pint[ridx(i, j, k + 1, nxg, nyg)];
       const size_t iw = fidx(i, j, k, nxg, nyg);
-      W[iw] = W[iw] - (pz - p) / g.dzg[k];
+      const double w = W[iw] - (pz - p) / g.dzg[k];
+      W[iw] = w;
+      bad |= !isfinite(w);
       p = pz;
     }
   }
+  if (bad && g.nonfinite) atomicOr(g.nonfinite, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -992,13 +1096,13 @@ hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const 
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
     if (from_uvw && small)
       hipLaunchKernelGGL((k_fwd_x<double, NN, TK, kSrcUVWc>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         (const void*)rhs, g.twx64, S, nullptr);
+                         (const void*)rhs, g.twx64, S, nullptr, nullptr, nullptr);
     else if (from_uvw)
       hipLaunchKernelGGL((k_fwd_x<double, NN, TK, kSrcUVW>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         (const void*)rhs, g.twx64, S, nullptr);
+                         (const void*)rhs, g.twx64, S, nullptr, nullptr, nullptr);
     else
       hipLaunchKernelGGL((k_fwd_x<double, NN, TK, kSrcRhs>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         (const void*)rhs, g.twx64, S, nullptr);
+                         (const void*)rhs, g.twx64, S, nullptr, nullptr, nullptr);
   }));
   return hipGetLastError();
 }
@@ -1008,26 +1112,62 @@ hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, co
   const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
-    if (U)
+    if (U && b)
       hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcUVWB>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         (const void*)nullptr, g.twx32, S, b);
+                         (const void*)nullptr, g.twx32, S, b, nullptr, nullptr);
+    else if (U)
+      hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcUVW>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                         (const void*)nullptr, g.twx32, S, nullptr, nullptr, nullptr);
     else
       hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcRhs32>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         (const void*)r, g.twx32, S, nullptr);
+                         (const void*)r, g.twx32, S, nullptr, nullptr, nullptr);
   }));
   return hipGetLastError();
+}
+
+hipError_t launch_fwd_x_resid32(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                                cpx<float>* S, unsigned long long* rmax, hipStream_t s) {
+  const int tk = tile_rows(g.nxp, g.nzp);
+  dim3 grid(g.nzp / tk, g.nym);
+  // gauge[0] = result, gauge[1 .. nzp] = row sums
+  hipLaunchKernelGGL(k_gauge_rows, dim3(g.nzp), dim3(kBlock), 0, s, g, phi, gauge + 1);
+  hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(kBlock), 0, s, g, (const double*)(gauge + 1), gauge);
+  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
+    hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
+                       nullptr, nullptr, (const void*)b, g.twx32, S, nullptr,
+                       (const double*)gauge, rmax);
+  }));
+  return hipGetLastError();
+}
+
+// Threads per column of the z transforms: 64 (4 columns per workgroup).
+// C3D_ZNT32=32 gives the FP32 transforms 32 threads and 8 columns per
+// workgroup instead (tuning knob).
+static int fft_z_nt32() {
+  static const int nt = [] {
+    const char* e = getenv("C3D_ZNT32");
+    const int v = e ? atoi(e) : 0;
+    return (v == 32 || v == 64) ? v : 64;
+  }();
+  return nt;
 }
 
 template <typename real>
 static hipError_t fft_z_any(const DevGrid& g, int sign, cpx<real>* S, const cpx<real>* tw,
                             hipStream_t s) {
   const int ncol = g.nym * g.nxh;
-  dim3 grid((ncol + 3) / 4);
+  const int nt = (sizeof(real) == 4) ? fft_z_nt32() : 64;
+  const int nc = kBlock / nt;
+  dim3 grid((ncol + nc - 1) / nc);
   C3D_DISPATCH_N(g.nzp, {
-    if (sign < 0)
-      hipLaunchKernelGGL((k_fft_z<real, NN, -1>), grid, dim3(kBlock), 0, s, S, tw, ncol);
+    if (nt == 32 && sign < 0)
+      hipLaunchKernelGGL((k_fft_z<real, NN, -1, 32>), grid, dim3(kBlock), 0, s, S, tw, ncol);
+    else if (nt == 32)
+      hipLaunchKernelGGL((k_fft_z<real, NN, +1, 32>), grid, dim3(kBlock), 0, s, S, tw, ncol);
+    else if (sign < 0)
+      hipLaunchKernelGGL((k_fft_z<real, NN, -1, 64>), grid, dim3(kBlock), 0, s, S, tw, ncol);
     else
-      hipLaunchKernelGGL((k_fft_z<real, NN, +1>), grid, dim3(kBlock), 0, s, S, tw, ncol);
+      hipLaunchKernelGGL((k_fft_z<real, NN, +1, 64>), grid, dim3(kBlock), 0, s, S, tw, ncol);
   });
   return hipGetLastError();
 }
@@ -1039,30 +1179,47 @@ hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t
   return fft_z_any<float>(g, sign, S, g.twz32, s);
 }
 
+#define C3D_DISPATCH_M(nym, BODY)                    \
+  switch (nym) {                                     \
+    case 64: { constexpr int MM = 8; BODY; } break;    \
+    case 128: { constexpr int MM = 16; BODY; } break;  \
+    case 192: { constexpr int MM = 24; BODY; } break;  \
+    default: return hipErrorInvalidValue;            \
+  }
+
 hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
   const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
   const dim3 gridp((unsigned)((nm + kBlock / 8 - 1) / (kBlock / 8)));
-  switch (g.nym) {
-    case 64: hipLaunchKernelGGL((k_tri_part<double, 8>), gridp, dim3(kBlock), 0, s, g, S); break;
-    case 128: hipLaunchKernelGGL((k_tri_part<double, 16>), gridp, dim3(kBlock), 0, s, g, S); break;
-    case 192: hipLaunchKernelGGL((k_tri_part<double, 24>), gridp, dim3(kBlock), 0, s, g, S); break;
-    default: hipLaunchKernelGGL(k_tri64, grid, dim3(kBlock), 0, s, g, S, CP); break;
+  if (!tri_part_supported(g)) {
+    hipLaunchKernelGGL(k_tri64, grid, dim3(kBlock), 0, s, g, S, CP);
+    return hipGetLastError();
   }
+  C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<double, double, MM, kTriAll>), gridp,
+                                           dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0));
   return hipGetLastError();
 }
 
 bool tri_part_supported(const DevGrid& g) { return g.nym == 64 || g.nym == 128 || g.nym == 192; }
 
-hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, hipStream_t s) {
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlow, double lam_thr,
+                        hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
-  const dim3 grid((unsigned)((nm + kBlock / 8 - 1) / (kBlock / 8)));
-  switch (g.nym) {
-    case 64: hipLaunchKernelGGL((k_tri_part<float, 8>), grid, dim3(kBlock), 0, s, g, S); break;
-    case 128: hipLaunchKernelGGL((k_tri_part<float, 16>), grid, dim3(kBlock), 0, s, g, S); break;
-    case 192: hipLaunchKernelGGL((k_tri_part<float, 24>), grid, dim3(kBlock), 0, s, g, S); break;
-    default: return hipErrorInvalidValue;
+  constexpr int MB = kBlock / 8;
+  const dim3 grid((unsigned)((nm + MB - 1) / MB));
+  if (!low) {  // FP64 arithmetic on every mode
+    C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll>), grid,
+                                             dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0));
+    return hipGetLastError();
   }
+  if (nlow <= 0) return hipErrorInvalidValue;  // the (0,0) mode is always a low mode
+  const dim3 gridl((unsigned)((nlow + MB - 1) / MB));
+  C3D_DISPATCH_M(g.nym, {
+    hipLaunchKernelGGL((k_tri_part<float, float, MM, kTriHigh>), grid, dim3(kBlock), 0, s, g, S,
+                       nullptr, 0, lam_thr);
+    hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriList>), gridl, dim3(kBlock), 0, s, g, S,
+                       low, nlow, 0.0);
+  });
   return hipGetLastError();
 }
 

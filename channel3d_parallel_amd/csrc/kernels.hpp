@@ -39,6 +39,9 @@ struct DevGrid {
   const cpx<double>* twz64;
   const cpx<float>* twx32;
   const cpx<float>* twz32;
+  // flag word: the velocity-correction kernels (k_grad_correct,
+  // k_ib_correct) set it to 1 when a corrected velocity is not finite
+  unsigned* nonfinite;
 };
 
 // rhs_p offset of Fortran (i,j,k) with bounds (2:nxg-1, 2:nyg-1, 2:nzg).
@@ -66,8 +69,19 @@ hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double
 // residual r -> FP32 spectrum.
 hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
                           double* b, const float* r, cpx<float>* S, hipStream_t s);
+// The refinement residual r = b - M phi of launch_residual, formed on the fly
+// and rounded to FP32 as the transform's input (same values as
+// launch_residual followed by launch_fwd_x32 of r): the FP32 spectrum of r,
+// max |r| into rmax[0..64) as launch_residual, gauge as its scratch.
+hipError_t launch_fwd_x_resid32(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                                cpx<float>* S, unsigned long long* rmax, hipStream_t s);
 hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t s);
-hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, hipStream_t s);  // tri_part_supported sizes
+// Tridiagonal solve of the FP32 spectrum (tri_part_supported sizes).  low ==
+// null: FP64 arithmetic on every mode.  Else FP32 arithmetic on the modes
+// with |kxx + kzz| >= lam_thr and FP64 arithmetic on the nlow modes of
+// low[] (the kx*nzp + kz indices of exactly the other modes).
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlow, double lam_thr,
+                        hipStream_t s);
 // mode 0: out = phi, 1: out = phi/dt, 3: out = phi_in + phi,
 // 4: out = (phi_in + phi)/dt; rk1 writes P from the same potential.
 hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
@@ -105,6 +119,21 @@ hipError_t launch_rk_update(const DevGrid& g, int which, int stage, double dt, c
                             const double* F2, const double* F3, const double* Xo, double* X,
                             hipStream_t s);
 hipError_t launch_add_u(const DevGrid& g, double* U, double d, hipStream_t s);
+// Fused right-hand side and stage update (rk3_fast.hip), to round-off of the
+// reference's operations: reads the stage input U, V, W, writes the updated
+// interiors Un, Vn, Wn (a different set of fields) and, for stages 1 and 2,
+// the right-hand sides Fu, Fv, Fw; stages 2 and 3 read the step-start
+// fields Uo, Vo, Wo and F1.  H masks may be null.
+struct RkStage {
+  const double *U, *V, *W;
+  const double *Hu, *Hv, *Hw;
+  double nu, dPdx, dPdy, dPdz, dt;
+  const double *Uo, *Vo, *Wo;
+  const double *F1u, *F1v, *F1w;
+  double *Fu, *Fv, *Fw;
+  double *Un, *Vn, *Wn;
+};
+hipError_t launch_rhs_stage(const DevGrid& g, int stage, const RkStage& a, hipStream_t s);
 // compute_dt maxima of dx/|u|, dy/|v|, dz/|w| as double bits into red3[0..2]
 // (red3 zeroed by the caller).
 hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, const double* W,

@@ -126,17 +126,17 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // the residual b - S fprev of the warm start and the previous force --
 // divided by the channel scale.
 __global__ void k_features(int nb, const int* __restrict__ nb9, const double* __restrict__ res,
-                           const double* __restrict__ fprev, const float* __restrict__ iscale,
-                           float* __restrict__ X) {
+                           const double* __restrict__ fprev, const double* __restrict__ nrm,
+                           const float* __restrict__ iscale, float* __restrict__ X) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nb * kIn) return;
   const int l = t / kIn, f = t % kIn, s = f / kCh, c = f % kCh;
   const int o = nb9[9 * l + s];
   double v;
   if (c < 4)
-    v = res[(size_t)c * nb + o];
+    v = res[(size_t)c * nb + o] * nrm[0];
   else
-    v = fprev[(size_t)(c - 4) * nb + o];
+    v = fprev[(size_t)(c - 4) * nb + o] * nrm[1];
   X[t] = (float)v / iscale[c];
 }
 
@@ -230,10 +230,10 @@ hipError_t adam(float* P, const float* G, float* M, float* V, int n, float lr_t,
 }
 
 hipError_t features(int nb, const int* nb9, const double* res, const double* fprev,
-                    const float* iscale, float* X, hipStream_t s) {
+                    const double* nrm, const float* iscale, float* X, hipStream_t s) {
   const size_t n = (size_t)nb * kIn;
   hipLaunchKernelGGL(k_features, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nb,
-                     nb9, res, fprev, iscale, X);
+                     nb9, res, fprev, nrm, iscale, X);
   return hipGetLastError();
 }
 

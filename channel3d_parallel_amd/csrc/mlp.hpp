@@ -36,9 +36,11 @@ hipError_t adam(float* P, const float* G, float* M, float* V, int n, float lr_t,
                 float eps, hipStream_t s);
 // IB feature rows X (nb x 54) from the warm start's Schur residual
 // res = b - S fprev (4nb) and the previous force; nb9 holds 9 neighbour
-// indices per marker.
+// indices per marker.  The residual channels are multiplied by nrm[0] and
+// the force channels by nrm[1] (device scalars: the solve's own 1/rms of
+// each), then divided by iscale per channel.
 hipError_t features(int nb, const int* nb9, const double* res, const double* fprev,
-                    const float* iscale, float* X, hipStream_t s);
+                    const double* nrm, const float* iscale, float* X, hipStream_t s);
 // fb = fprev + Y * oscale per output channel (fprev == null: fb = Y * oscale).
 hipError_t apply(int nb, const float* Y, const float* oscale, const double* fprev, double* fb,
                  hipStream_t s);

@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 
 #pragma clang fp contract(off)
+#include "rk3_point.hpp"
 
 namespace c3d {
 
@@ -22,133 +23,6 @@ constexpr int kB = 256;
 #define V_(i, j, k) V[fidx(i, j, k, g.nxg, g.ny)]
 #define W_(i, j, k) W[fidx(i, j, k, g.nxg, g.nyg)]
 
-// compute_rhs_u, src/equations.f90:24-153; F bounds (2:nx-1, 2:nyg-1, 2:nzg-1)
-// Field accessors of the point functions: straight from global memory, or
-// from the LDS tiles of k_rhs (same values, so the same results).
-struct GlobalAcc {
-  const double *Up, *Vp, *Wp;
-  int nx, nxg, ny, nyg;
-  __device__ __forceinline__ double U(int i, int j, int k) const { return Up[fidx(i, j, k, nx, nyg)]; }
-  __device__ __forceinline__ double V(int i, int j, int k) const { return Vp[fidx(i, j, k, nxg, ny)]; }
-  __device__ __forceinline__ double W(int i, int j, int k) const { return Wp[fidx(i, j, k, nxg, nyg)]; }
-};
-
-// LDS tile geometry of k_rhs: 64 x 4 points per workgroup plus a one-cell
-// halo in x and y, three z planes (k-1, k, k+1) kept as a ring.
-constexpr int kTX = 64 + 2, kTY = 4 + 2, kTP = kTX * kTY;
-
-struct TileAcc {
-  const double* s;  // [field][slot][kTY][kTX]
-  int i0, j0, k0;   // Fortran indices of tile element (1, 1) and of the ring's reference plane
-  __device__ __forceinline__ int slot(int k) const { return ((k - k0) % 3 + 3) % 3; }
-  __device__ __forceinline__ double at(int f, int i, int j, int k) const {
-    return s[(f * 3 + slot(k)) * kTP + (j - j0 + 1) * kTX + (i - i0 + 1)];
-  }
-  __device__ __forceinline__ double U(int i, int j, int k) const { return at(0, i, j, k); }
-  __device__ __forceinline__ double V(int i, int j, int k) const { return at(1, i, j, k); }
-  __device__ __forceinline__ double W(int i, int j, int k) const { return at(2, i, j, k); }
-};
-
-template <typename Acc>
-__device__ __forceinline__ double rhs_u_at(const DevGrid& g, const Acc& A,
-                                         const double* __restrict__ H, double nu,
-                                         double dPdx, int i, int j, int k) {
-  const double uc1 = 0.5 * (A.U(i - 1, j, k) + A.U(i, j, k));
-  const double uc2 = 0.5 * (A.U(i, j, k) + A.U(i + 1, j, k));
-  const double a = -(uc2 * uc2 - uc1 * uc1) / g.dxg[i];
-  const double uv1 = (0.5 * (A.U(i, j, k) + A.U(i, j + 1, k))) * (0.5 * (A.V(i, j, k) + A.V(i + 1, j, k)));
-  const double uv0 =
-      (0.5 * (A.U(i, j - 1, k) + A.U(i, j, k))) * (0.5 * (A.V(i, j - 1, k) + A.V(i + 1, j - 1, k)));
-  const double b = -(uv1 - uv0) / g.dyf[j];
-  const double uw1 = (0.5 * (A.U(i, j, k) + A.U(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i + 1, j, k)));
-  const double uw0 =
-      (0.5 * (A.U(i, j, k - 1) + A.U(i, j, k))) * (0.5 * (A.W(i, j, k - 1) + A.W(i + 1, j, k - 1)));
-  const double c = -(uw1 - uw0) / g.dzf[k];
-  const double t1 = (A.U(i, j + 1, k) - A.U(i, j, k)) / g.dyg[j];
-  const double t1m = (A.U(i, j, k) - A.U(i, j - 1, k)) / g.dyg[j - 1];
-  // 1.0/dx_3 and nu*1.0/dx_2 are index-only factors, tabulated (same
-  // correctly rounded quotients as evaluating them here)
-  const double visc = g.rdxg[i] * (g.nudxf[i + 1] * (A.U(i + 1, j, k) - A.U(i, j, k)) -
-                                   g.nudxf[i] * (A.U(i, j, k) - A.U(i - 1, j, k))) +
-                      g.rdyf[j] * (nu * t1 - nu * t1m) +
-                      g.rdzf[k] * (g.nudzg[k] * (A.U(i, j, k + 1) - A.U(i, j, k)) -
-                                   g.nudzg[k - 1] * (A.U(i, j, k) - A.U(i, j, k - 1)));
-  const double h = H ? H[fidx(i, j, k, g.nx, g.nyg)] : 1.0;
-  double r = a;
-  r = r + b;
-  r = r + c;
-  r = r + visc;
-  r = r + dPdx * h;
-  return r;
-}
-
-// compute_rhs_v, src/equations.f90:160-288; F bounds (2:nxg-1, 2:ny-1, 2:nzg-1)
-template <typename Acc>
-__device__ __forceinline__ double rhs_v_at(const DevGrid& g, const Acc& A,
-                                         const double* __restrict__ H, double nu,
-                                         double dPdy, int i, int j, int k) {
-  const double vc1 = 0.5 * (A.V(i, j - 1, k) + A.V(i, j, k));
-  const double vc2 = 0.5 * (A.V(i, j, k) + A.V(i, j + 1, k));
-  const double a = -(vc2 * vc2 - vc1 * vc1) / g.dyg[j];
-  const double uv1 = (0.5 * (A.U(i, j, k) + A.U(i, j + 1, k))) * (0.5 * (A.V(i, j, k) + A.V(i + 1, j, k)));
-  const double uv0 =
-      (0.5 * (A.U(i - 1, j, k) + A.U(i - 1, j + 1, k))) * (0.5 * (A.V(i - 1, j, k) + A.V(i, j, k)));
-  const double b = -(uv1 - uv0) / g.dxf[i];
-  const double vw1 = (0.5 * (A.V(i, j, k) + A.V(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i, j + 1, k)));
-  const double vw0 =
-      (0.5 * (A.V(i, j, k - 1) + A.V(i, j, k))) * (0.5 * (A.W(i, j, k - 1) + A.W(i, j + 1, k - 1)));
-  const double c = -(vw1 - vw0) / g.dzf[k];
-  const double visc = g.rdxf[i] * (g.nudxg[i] * (A.V(i + 1, j, k) - A.V(i, j, k)) -
-                                   g.nudxg[i - 1] * (A.V(i, j, k) - A.V(i - 1, j, k))) +
-                      g.rdyg[j] * (g.nudyf[j + 1] * (A.V(i, j + 1, k) - A.V(i, j, k)) -
-                                   g.nudyf[j] * (A.V(i, j, k) - A.V(i, j - 1, k))) +
-                      g.rdzf[k] * (g.nudzg[k] * (A.V(i, j, k + 1) - A.V(i, j, k)) -
-                                   g.nudzg[k - 1] * (A.V(i, j, k) - A.V(i, j, k - 1)));
-  const double h = H ? H[fidx(i, j, k, g.nxg, g.ny)] : 1.0;
-  double r = a;
-  r = r + b;
-  r = r + c;
-  r = r + visc;
-  r = r + dPdy * h;
-  return r;
-}
-
-// compute_rhs_w, src/equations.f90:295-423; F bounds (2:nxg-1, 2:nyg-1, 2:nz-1)
-template <typename Acc>
-__device__ __forceinline__ double rhs_w_at(const DevGrid& g, const Acc& A,
-                                         const double* __restrict__ H, double nu,
-                                         double dPdz, int i, int j, int k) {
-  const double wc1 = 0.5 * (A.W(i, j, k - 1) + A.W(i, j, k));
-  const double wc2 = 0.5 * (A.W(i, j, k) + A.W(i, j, k + 1));
-  const double a = -(wc2 * wc2 - wc1 * wc1) / g.dzg[k];
-  const double uw1 = (0.5 * (A.U(i, j, k) + A.U(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i + 1, j, k)));
-  const double uw0 =
-      (0.5 * (A.U(i - 1, j, k) + A.U(i - 1, j, k + 1))) * (0.5 * (A.W(i - 1, j, k) + A.W(i, j, k)));
-  const double b = -(uw1 - uw0) / g.dxf[i];
-  const double vw1 = (0.5 * (A.V(i, j, k) + A.V(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i, j + 1, k)));
-  const double vw0 =
-      (0.5 * (A.V(i, j - 1, k) + A.V(i, j - 1, k + 1))) * (0.5 * (A.W(i, j - 1, k) + A.W(i, j, k)));
-  const double c = -(vw1 - vw0) / g.dyf[j];
-  const double t1 = (A.W(i, j + 1, k) - A.W(i, j, k)) / g.dyg[j];
-  const double t1m = (A.W(i, j, k) - A.W(i, j - 1, k)) / g.dyg[j - 1];
-  const double visc = g.rdxf[i] * (g.nudxg[i] * (A.W(i + 1, j, k) - A.W(i, j, k)) -
-                                   g.nudxg[i - 1] * (A.W(i, j, k) - A.W(i - 1, j, k))) +
-                      g.rdyf[j] * (nu * t1 - nu * t1m) +
-                      g.rdzg[k] * (g.nudzf[k + 1] * (A.W(i, j, k + 1) - A.W(i, j, k)) -
-                                   g.nudzf[k] * (A.W(i, j, k) - A.W(i, j, k - 1)));
-  const double h = H ? H[fidx(i, j, k, g.nxg, g.nyg)] : 1.0;
-  double r = a;
-  r = r + b;
-  r = r + c;
-  r = r + visc;
-  r = r + dPdz * h;
-  return r;
-}
-
-// compute_rhs_u/v/w at one (i, j, k) each, evaluated together so that the
-// neighbour loads they share are issued once.  Ranges: F_u (2:nx-1,
-// 2:nyg-1, 2:nzg-1), F_v (2:nxg-1, 2:ny-1, 2:nzg-1), F_w (2:nxg-1, 2:nyg-1,
-// 2:nz-1); the grid covers their union.
 // compute_rhs_u/v/w evaluated together.  Ranges: F_u (2:nx-1, 2:nyg-1,
 // 2:nzg-1), F_v (2:nxg-1, 2:ny-1, 2:nzg-1), F_w (2:nxg-1, 2:nyg-1, 2:nz-1);
 // the grid covers their union.  Each workgroup owns a 64 x 4 column of
@@ -173,21 +47,7 @@ __global__ __launch_bounds__(kB) void k_rhs(DevGrid g, const double* __restrict_
   const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   TileAcc A{tile, i0, j0, kb};
-  // stage plane k of all three fields into its ring slot
-  auto stage = [&](int k) {
-    const int sl = A.slot(k);
-    for (int e = threadIdx.x; e < kTP; e += kB) {
-      const int ii = i0 - 1 + e % kTX, jj = j0 - 1 + e / kTX;
-      const bool okx = ii >= 1, oky = jj >= 1;
-      double u = 0.0, v = 0.0, w = 0.0;
-      if (okx && oky && ii <= nx && jj <= nyg && k >= 1 && k <= nzg) u = U[fidx(ii, jj, k, nx, nyg)];
-      if (okx && oky && ii <= nxg && jj <= ny && k >= 1 && k <= nzg) v = V[fidx(ii, jj, k, nxg, ny)];
-      if (okx && oky && ii <= nxg && jj <= nyg && k >= 1 && k <= nz) w = W[fidx(ii, jj, k, nxg, nyg)];
-      tile[(0 * 3 + sl) * kTP + e] = u;
-      tile[(1 * 3 + sl) * kTP + e] = v;
-      tile[(2 * 3 + sl) * kTP + e] = w;
-    }
-  };
+  auto stage = [&](int k) { stage_tile_plane(g, U, V, W, tile, A, k, kB); };
   stage(kb - 1);
   stage(kb);
   for (int kk = 0; kk < KC; kk++) {
@@ -198,13 +58,13 @@ __global__ __launch_bounds__(kB) void k_rhs(DevGrid g, const double* __restrict_
     if (i <= nxg - 1 && j <= nyg - 1) {
       if (i <= nx - 1)
         Fu[(size_t)(i - 2) + (size_t)(nx - 2) * ((size_t)(j - 2) + (size_t)(nyg - 2) * (size_t)(k - 2))] =
-            rhs_u_at(g, A, Hu, nu, dPdx, i, j, k);
+            rhs_u_at<false>(g, A, Hu, nu, dPdx, i, j, k);
       if (j <= ny - 1)
         Fv[(size_t)(i - 2) + (size_t)(nxg - 2) * ((size_t)(j - 2) + (size_t)(ny - 2) * (size_t)(k - 2))] =
-            rhs_v_at(g, A, Hv, nu, dPdy, i, j, k);
+            rhs_v_at<false>(g, A, Hv, nu, dPdy, i, j, k);
       if (k <= nz - 1)
         Fw[(size_t)(i - 2) + (size_t)(nxg - 2) * ((size_t)(j - 2) + (size_t)(nyg - 2) * (size_t)(k - 2))] =
-            rhs_w_at(g, A, Hw, nu, dPdz, i, j, k);
+            rhs_w_at<false>(g, A, Hw, nu, dPdz, i, j, k);
     }
     __syncthreads();  // plane k-1's slot is refilled next
   }

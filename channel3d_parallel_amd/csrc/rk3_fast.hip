@@ -1,0 +1,100 @@
+// Fused right-hand side and stage update of the RK3 time step
+// (src/equations.f90:24-423, src/time_integration.f90:101-107, 130-136,
+// 159-168) for gfx950, FP64, to round-off tolerance of the reference's
+// operations: the spacing quotients are products with tabulated reciprocals
+// and FMA contraction is on (rk3.hip keeps the bitwise form).
+//
+// One kernel per stage reads the stage's velocity through the LDS tile of
+// k_rhs, evaluates F_s at each point and writes, in the same pass,
+// X_s = X_0 + dt sum_l a_sl F_l into a second set of fields (the stage's
+// input is still read by the neighbouring workgroups' halos, so the update
+// cannot be in place).  F_s is stored for the later stages that use it.
+#include "kernels.hpp"
+#include "rk3_point.hpp"
+
+namespace c3d {
+
+namespace {
+
+constexpr int kBf = 256;
+
+// Wray coefficients as the reference writes them (src/time_integration.f90:
+// 105-107, 134-136, 163-168): stage 2 uses F1, F2; stage 3 uses F1 and F3
+// (the F2 coefficient of stage 3 is zero).
+constexpr double kC11 = 8.0 / 15.0, kC21 = 1.0 / 4.0, kC22 = 5.0 / 12.0, kC31 = 1.0 / 4.0,
+                 kC33 = 3.0 / 4.0;
+
+template <int KC, int STAGE>
+__global__ __launch_bounds__(kBf) void k_rhs_stage(DevGrid g, RkStage a) {
+  __shared__ double tile[3 * 3 * kTP];
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int i0 = bx * 64 + 2, j0 = by * 4 + 2, kb = bz * KC + 2;
+  const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
+  TileAcc A{tile, i0, j0, kb};
+  auto stage = [&](int k) { stage_tile_plane(g, a.U, a.V, a.W, tile, A, k, kBf); };
+  stage(kb - 1);
+  stage(kb);
+  const double dt = a.dt;
+  for (int kk = 0; kk < KC; kk++) {
+    const int k = kb + kk;
+    if (k > nzg - 1) break;  // uniform across the workgroup
+    stage(k + 1);
+    __syncthreads();
+    if (i <= nxg - 1 && j <= nyg - 1) {
+      if (i <= nx - 1) {  // U(2:nx-1, 2:nyg-1, 2:nzg-1)
+        const size_t f = (size_t)(i - 2) + (size_t)(nx - 2) * ((size_t)(j - 2) + (size_t)(nyg - 2) * (size_t)(k - 2));
+        const size_t o = fidx(i, j, k, nx, nyg);
+        const double F = rhs_u_at<true>(g, A, a.Hu, a.nu, a.dPdx, i, j, k);
+        double x;
+        if constexpr (STAGE == 1) x = A.U(i, j, k) + (dt * kC11) * F;
+        else if constexpr (STAGE == 2) x = a.Uo[o] + dt * (kC21 * a.F1u[f] + kC22 * F);
+        else x = a.Uo[o] + dt * (kC31 * a.F1u[f] + kC33 * F);
+        if constexpr (STAGE < 3) a.Fu[f] = F;
+        a.Un[o] = x;
+      }
+      if (j <= ny - 1) {  // V(2:nxg-1, 2:ny-1, 2:nzg-1)
+        const size_t f = (size_t)(i - 2) + (size_t)(nxg - 2) * ((size_t)(j - 2) + (size_t)(ny - 2) * (size_t)(k - 2));
+        const size_t o = fidx(i, j, k, nxg, ny);
+        const double F = rhs_v_at<true>(g, A, a.Hv, a.nu, a.dPdy, i, j, k);
+        double x;
+        if constexpr (STAGE == 1) x = A.V(i, j, k) + (dt * kC11) * F;
+        else if constexpr (STAGE == 2) x = a.Vo[o] + dt * (kC21 * a.F1v[f] + kC22 * F);
+        else x = a.Vo[o] + dt * (kC31 * a.F1v[f] + kC33 * F);
+        if constexpr (STAGE < 3) a.Fv[f] = F;
+        a.Vn[o] = x;
+      }
+      if (k <= nz - 1) {  // W(2:nxg-1, 2:nyg-1, 2:nz-1)
+        const size_t f = (size_t)(i - 2) + (size_t)(nxg - 2) * ((size_t)(j - 2) + (size_t)(nyg - 2) * (size_t)(k - 2));
+        const size_t o = fidx(i, j, k, nxg, nyg);
+        const double F = rhs_w_at<true>(g, A, a.Hw, a.nu, a.dPdz, i, j, k);
+        double x;
+        if constexpr (STAGE == 1) x = A.W(i, j, k) + (dt * kC11) * F;
+        else if constexpr (STAGE == 2) x = a.Wo[o] + dt * (kC21 * a.F1w[f] + kC22 * F);
+        else x = a.Wo[o] + dt * (kC31 * a.F1w[f] + kC33 * F);
+        if constexpr (STAGE < 3) a.Fw[f] = F;
+        a.Wn[o] = x;
+      }
+    }
+    __syncthreads();  // plane k-1's slot is refilled next
+  }
+}
+
+inline dim3 gpts_f(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
+
+}  // namespace
+
+hipError_t launch_rhs_stage(const DevGrid& g, int stage, const RkStage& a, hipStream_t s) {
+  constexpr int KC = 8;
+  const dim3 grid = gpts_f(g.nxg - 2, g.nyg - 2, (g.nzg - 2 + KC - 1) / KC);
+  switch (stage) {
+    case 1: hipLaunchKernelGGL((k_rhs_stage<KC, 1>), grid, dim3(kBf), 0, s, g, a); break;
+    case 2: hipLaunchKernelGGL((k_rhs_stage<KC, 2>), grid, dim3(kBf), 0, s, g, a); break;
+    case 3: hipLaunchKernelGGL((k_rhs_stage<KC, 3>), grid, dim3(kBf), 0, s, g, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace c3d

@@ -312,6 +312,73 @@ Module c3d_bindings
        Integer(c_int), Value :: flag
      End Function c3d_ib_set_exterior_pressure_gradient
 
+     ! host-array forms (device scratch of the context; resident state untouched)
+     Integer(c_int) Function c3d_apply_boundary_conditions_arrays(ctx, U, V, W) &
+          Bind(C, name="c3d_apply_boundary_conditions_arrays")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: U(*), V(*), W(*)
+     End Function c3d_apply_boundary_conditions_arrays
+
+     Integer(c_int) Function c3d_divergence_arrays(ctx, U, V, W, div) &
+          Bind(C, name="c3d_divergence_arrays")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: U(*), V(*), W(*)
+       Real(c_double) :: div(*)
+     End Function c3d_divergence_arrays
+
+     Integer(c_int) Function c3d_compute_rhs_arrays(ctx, U, V, W, Fu, Fv, Fw) &
+          Bind(C, name="c3d_compute_rhs_arrays")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: U(*), V(*), W(*)
+       Type(c_ptr), Value :: Fu, Fv, Fw
+     End Function c3d_compute_rhs_arrays
+
+     Integer(c_int) Function c3d_mean_mass_flow_u_array(ctx, U, q) &
+          Bind(C, name="c3d_mean_mass_flow_u_array")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: U(*)
+       Real(c_double), Intent(Out) :: q
+     End Function c3d_mean_mass_flow_u_array
+
+     Integer(c_int) Function c3d_ib_interpolate_arrays(ctx, with_dxn, U, V, W, out) &
+          Bind(C, name="c3d_ib_interpolate_arrays")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: with_dxn
+       Real(c_double), Intent(In) :: U(*), V(*), W(*)
+       Real(c_double) :: out(*)
+     End Function c3d_ib_interpolate_arrays
+
+     Integer(c_int) Function c3d_ib_interpolate_c(ctx, p, out) Bind(C, name="c3d_ib_interpolate_c")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: p(*)
+       Real(c_double) :: out(*)
+     End Function c3d_ib_interpolate_c
+
+     Integer(c_int) Function c3d_ib_spread(ctx, comp, fvec, field) Bind(C, name="c3d_ib_spread")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: comp
+       Real(c_double), Intent(In) :: fvec(*)
+       Real(c_double) :: field(*)
+     End Function c3d_ib_spread
+
+     Integer(c_int) Function c3d_ib_remove_mean(ctx, f) Bind(C, name="c3d_ib_remove_mean")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double) :: f(*)
+     End Function c3d_ib_remove_mean
+
+     Integer(c_int) Function c3d_ib_compute_heaviside(ctx) Bind(C, name="c3d_ib_compute_heaviside")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+     End Function c3d_ib_compute_heaviside
+
   End Interface
 
 Contains

@@ -5,6 +5,9 @@
 !   projection              :: compute_non_IB_projection (src/projection.f90:22)!
 !   differential_operators  :: divergence, gradient (src/differential_operators.f90) !
 !   boundary_conditions     :: apply_boundary_conditions (src/boundary_conditions.f90:20) !
+!   immersed_boundary_operators :: regT, regT_1n, regTc_1n, regu, regv, regw   !
+!                              (src/immersed_boundary_operators.f90:296-400)   !
+!   heaviside               :: compute_heaviside (src/heaviside.f90:20)        !
 !   monitor                 :: check_divergence, check_slip, monitor_fields     !
 !                              (src/monitor.f90:27-59, 210-231)                 !
 !   statistics              :: compute_wall_shear, compute_plane_statistics     !
@@ -105,14 +108,14 @@ Module boundary_conditions
   Use c3d_global
   Implicit None
 Contains
-  ! apply_boundary_conditions(U_,V_,W_), src/boundary_conditions.f90:20-55:
-  ! uploads the arrays, applies the conditions on the GPU, copies them back.
-  ! The arrays become the resident velocity.
+  ! apply_boundary_conditions(U_,V_,W_), src/boundary_conditions.f90:20-55,
+  ! on the caller's arrays (the reference applies it to U, V, W, U_reg and
+  ! the Heaviside fields alike): staged in the context's device scratch, the
+  ! resident velocity is not touched.
   Subroutine apply_boundary_conditions(U_, V_, W_)
     Real(Int64), Contiguous, Intent(InOut) :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
-    Call c3d_upload(U_, V_, W_)
-    Call c3d_check(c3d_apply_boundary_conditions(ctx), 'apply_boundary_conditions')
-    Call c3d_download(U_, V_, W_)
+    Call c3d_check(c3d_apply_boundary_conditions_arrays(ctx, U_, V_, W_), &
+                   'apply_boundary_conditions')
   End Subroutine apply_boundary_conditions
 
   ! Same on the resident velocity, no host copies.
@@ -126,14 +129,19 @@ Module differential_operators
   Use c3d_global
   Implicit None
 Contains
-  ! divergence(div_, U_, V_, W_), src/differential_operators.f90:54-74.
-  ! The velocity arrays become the resident velocity.
+  ! divergence(div_, U_, V_, W_), src/differential_operators.f90:54-74, of
+  ! the caller's arrays (device scratch; the resident velocity is untouched).
   Subroutine divergence(div_, U_, V_, W_)
     Real(Int64), Contiguous, Intent(Out) :: div_(2:, 2:, 2:)
     Real(Int64), Contiguous, Intent(In)  :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
-    Call c3d_upload(U_, V_, W_)
-    Call c3d_check(c3d_divergence(ctx, div_), 'divergence')
+    Call c3d_check(c3d_divergence_arrays(ctx, U_, V_, W_, div_), 'divergence')
   End Subroutine divergence
+
+  ! divergence of the resident velocity (no host copies of the velocity).
+  Subroutine divergence_resident(div_)
+    Real(Int64), Contiguous, Intent(Out) :: div_(2:, 2:, 2:)
+    Call c3d_check(c3d_divergence(ctx, div_), 'divergence')
+  End Subroutine divergence_resident
 
   ! gradient(Gpx_, Gpy_, Gpz_, p_), src/differential_operators.f90:16-52.
   Subroutine gradient(Gpx_, Gpy_, Gpz_, p_)
@@ -225,7 +233,62 @@ Contains
                    'ib_set_solver')
     ib_active = .True.
   End Subroutine setup_IB_operators
+
+  ! regT(Eu_, U_, V_, W_), src/immersed_boundary_operators.f90:296-318:
+  ! E of the caller's arrays, Eu_(3*nb) component-major.
+  Subroutine regT(Eu_, U_, V_, W_)
+    Real(Int64), Contiguous, Intent(In)  :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
+    Real(Int64), Contiguous, Intent(Out) :: Eu_(:)
+    Call c3d_check(c3d_ib_interpolate_arrays(ctx, 0_c_int, U_, V_, W_, Eu_), 'regT')
+  End Subroutine regT
+
+  ! regT_1n(E1nu_, U_, V_, W_), :320-342 (the normal-derivative weights).
+  Subroutine regT_1n(E1nu_, U_, V_, W_)
+    Real(Int64), Contiguous, Intent(In)  :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
+    Real(Int64), Contiguous, Intent(Out) :: E1nu_(:)
+    Call c3d_check(c3d_ib_interpolate_arrays(ctx, 1_c_int, U_, V_, W_, E1nu_), 'regT_1n')
+  End Subroutine regT_1n
+
+  ! regTc_1n(E1np_, P_), :344-357, of a centre field with rhs_p bounds.
+  Subroutine regTc_1n(E1np_, P_)
+    Real(Int64), Contiguous, Intent(In)  :: P_(2:, 2:, 2:)
+    Real(Int64), Contiguous, Intent(Out) :: E1np_(:)
+    Call c3d_check(c3d_ib_interpolate_c(ctx, P_, E1np_), 'regTc_1n')
+  End Subroutine regTc_1n
+
+  ! regu / regv / regw(F_, f_), :359-400: R f_ into the whole field F_
+  ! (zero off the support, as the reference zeroes it first).
+  Subroutine regu(U_, f_)
+    Real(Int64), Contiguous, Intent(In)  :: f_(:)
+    Real(Int64), Contiguous, Intent(Out) :: U_(:,:,:)
+    Call c3d_check(c3d_ib_spread(ctx, 0_c_int, f_, U_), 'regu')
+  End Subroutine regu
+
+  Subroutine regv(V_, f_)
+    Real(Int64), Contiguous, Intent(In)  :: f_(:)
+    Real(Int64), Contiguous, Intent(Out) :: V_(:,:,:)
+    Call c3d_check(c3d_ib_spread(ctx, 1_c_int, f_, V_), 'regv')
+  End Subroutine regv
+
+  Subroutine regw(W_, f_)
+    Real(Int64), Contiguous, Intent(In)  :: f_(:)
+    Real(Int64), Contiguous, Intent(Out) :: W_(:,:,:)
+    Call c3d_check(c3d_ib_spread(ctx, 2_c_int, f_, W_), 'regw')
+  End Subroutine regw
 End Module immersed_boundary_operators
+
+!------------------------------------------------------------------------------!
+Module heaviside
+  Use c3d_global
+  Implicit None
+Contains
+  ! compute_heaviside, src/heaviside.f90:20-54, for the current markers: the
+  ! interior masks, Hc_exterior and E1nHc_exterior / E1nH_exterior stay on
+  ! the device (setup_IB_operators has already run it once).
+  Subroutine compute_heaviside
+    Call c3d_check(c3d_ib_compute_heaviside(ctx), 'compute_heaviside')
+  End Subroutine compute_heaviside
+End Module heaviside
 
 !------------------------------------------------------------------------------!
 Module monitor

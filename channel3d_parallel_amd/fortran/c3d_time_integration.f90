@@ -2,6 +2,7 @@
 ! RK3 host loop over the gfx950 library, keeping the reference's module and   !
 ! procedure names and call order:                                            !
 !   equations        :: compute_rhs  (src/equations.f90, u, v and w at once) !
+!                       compute_rhs_u/v/w on host arrays                     !
 !   mass_flow        :: compute_mean_mass_flow_U,                            !
 !                       compute_dPx_for_constant_mass_flow (src/mass_flow.f90)!
 !   time_integration :: compute_time_step_RK3, compute_dt                    !
@@ -34,6 +35,34 @@ Contains
     Call c3d_check(c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz), 'set_pressure_gradient')
     Call c3d_check(c3d_compute_rhs(ctx, Int(slot, c_int)), 'compute_rhs')
   End Subroutine compute_rhs
+
+  ! compute_rhs_u(U_, V_, W_, rhs_u), src/equations.f90:24-153, of the
+  ! caller's arrays; rhs_u has the interior shape (nx-2, nyg-2, nzg-2).
+  Subroutine compute_rhs_u(U_, V_, W_, rhs_u)
+    Real(Int64), Contiguous, Intent(In)          :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
+    Real(Int64), Contiguous, Intent(Out), Target :: rhs_u(2:, 2:, 2:)
+    Call c3d_check(c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz), 'set_pressure_gradient')
+    Call c3d_check(c3d_compute_rhs_arrays(ctx, U_, V_, W_, c_loc(rhs_u), c_null_ptr, c_null_ptr), &
+                   'compute_rhs_u')
+  End Subroutine compute_rhs_u
+
+  ! compute_rhs_v(U_, V_, W_, rhs_v), :160-288; rhs_v (nxg-2, ny-2, nzg-2).
+  Subroutine compute_rhs_v(U_, V_, W_, rhs_v)
+    Real(Int64), Contiguous, Intent(In)          :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
+    Real(Int64), Contiguous, Intent(Out), Target :: rhs_v(2:, 2:, 2:)
+    Call c3d_check(c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz), 'set_pressure_gradient')
+    Call c3d_check(c3d_compute_rhs_arrays(ctx, U_, V_, W_, c_null_ptr, c_loc(rhs_v), c_null_ptr), &
+                   'compute_rhs_v')
+  End Subroutine compute_rhs_v
+
+  ! compute_rhs_w(U_, V_, W_, rhs_w), :295-423; rhs_w (nxg-2, nyg-2, nz-2).
+  Subroutine compute_rhs_w(U_, V_, W_, rhs_w)
+    Real(Int64), Contiguous, Intent(In)          :: U_(:,:,:), V_(:,:,:), W_(:,:,:)
+    Real(Int64), Contiguous, Intent(Out), Target :: rhs_w(2:, 2:, 2:)
+    Call c3d_check(c3d_set_pressure_gradient(ctx, dPdx, dPdy, dPdz), 'set_pressure_gradient')
+    Call c3d_check(c3d_compute_rhs_arrays(ctx, U_, V_, W_, c_null_ptr, c_null_ptr, c_loc(rhs_w)), &
+                   'compute_rhs_w')
+  End Subroutine compute_rhs_w
 End Module equations
 
 !------------------------------------------------------------------------------!
@@ -41,20 +70,43 @@ Module mass_flow
   Use c3d_global
   Use c3d_flow
   Implicit None
+  ! Each procedure has the reference's form on a host array U(nx,nyg,nzg)
+  ! and a resident form without the array argument.
+  Interface compute_mean_mass_flow_U
+     Module Procedure mean_mass_flow_U_array, mean_mass_flow_U_resident
+  End Interface compute_mean_mass_flow_U
+  Interface compute_dPx_for_constant_mass_flow
+     Module Procedure dPx_constant_mass_flow_array, dPx_constant_mass_flow_resident
+  End Interface compute_dPx_for_constant_mass_flow
 Contains
   ! compute_mean_mass_flow_U(U, Qflow), src/mass_flow.f90:24-53
-  Subroutine compute_mean_mass_flow_U(Qflow)
+  Subroutine mean_mass_flow_U_array(U, Qflow)
+    Real(Int64), Contiguous, Intent(In) :: U(:,:,:)
+    Real(Int64), Intent(Out) :: Qflow
+    Call c3d_check(c3d_mean_mass_flow_u_array(ctx, U, Qflow), 'compute_mean_mass_flow_U')
+  End Subroutine mean_mass_flow_U_array
+
+  Subroutine mean_mass_flow_U_resident(Qflow)
     Real(Int64), Intent(Out) :: Qflow
     Call c3d_check(c3d_mean_mass_flow_u(ctx, Qflow), 'compute_mean_mass_flow_U')
-  End Subroutine compute_mean_mass_flow_U
+  End Subroutine mean_mass_flow_U_resident
 
-  ! compute_dPx_for_constant_mass_flow(U, dPdx), src/mass_flow.f90:144-175
-  Subroutine compute_dPx_for_constant_mass_flow(dPdx_out)
+  ! compute_dPx_for_constant_mass_flow(U, dPdx), src/mass_flow.f90:144-175:
+  ! dPdx = Qflow_x_0 - (H-masked bulk velocity of U)
+  Subroutine dPx_constant_mass_flow_array(U, dPdx_out)
+    Real(Int64), Contiguous, Intent(In) :: U(:,:,:)
+    Real(Int64), Intent(Out) :: dPdx_out
+    Real(Int64) :: q
+    Call c3d_check(c3d_mean_mass_flow_u_array(ctx, U, q), 'compute_dPx_for_constant_mass_flow')
+    dPdx_out = Qflow_x_0 - q
+  End Subroutine dPx_constant_mass_flow_array
+
+  Subroutine dPx_constant_mass_flow_resident(dPdx_out)
     Real(Int64), Intent(Out) :: dPdx_out
     Real(Int64) :: q
     Call c3d_check(c3d_mean_mass_flow_u(ctx, q), 'compute_dPx_for_constant_mass_flow')
     dPdx_out = Qflow_x_0 - q
-  End Subroutine compute_dPx_for_constant_mass_flow
+  End Subroutine dPx_constant_mass_flow_resident
 End Module mass_flow
 
 !------------------------------------------------------------------------------!
@@ -134,6 +186,13 @@ Contains
   Subroutine move_IB_geometry
     Call c3d_check(c3d_ib_move(ctx, t), 'setup_IB_geometry (move)')
   End Subroutine move_IB_geometry
+
+  ! remove_mean_per_body(f_), src/immersed_boundary_geometry.f90:400-442, in
+  ! place on a host vector of nb markers.
+  Subroutine remove_mean_per_body(f_)
+    Real(Int64), Contiguous, Intent(InOut) :: f_(:)
+    Call c3d_check(c3d_ib_remove_mean(ctx, f_), 'remove_mean_per_body')
+  End Subroutine remove_mean_per_body
 End Module immersed_boundary_geometry
 
 !------------------------------------------------------------------------------!

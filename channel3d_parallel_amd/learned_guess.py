@@ -4,7 +4,13 @@ and the iteration A/B comparison.
 Training pairs come from the IB solver itself: for every IB projection of a
 two-cylinder run with seeded rotation speeds, the features the network sees
 (per marker and 3x3 neighbourhood: the Schur residual of the previous force,
-b - S f_prev, and that force) and the converged force increment.
+b - S f_prev, and that force) and the converged force increment.  Each
+solve's pairs are normalised by that solve's own scales, as the library
+does at inference: the residual channels and the increment by the rms of
+the residual, the force channels by the rms of the force.  The increment
+solves S d = b - S f_prev, linear in the residual, and the start takes the
+step along the predicted direction that minimises the residual, so only
+the direction matters.
 The network is trained on the GPU by the library's own kernels
 (c3d_mlp_train).  Plain BiCGSTAB from a zero guess stays the reference:
 every variant is solved to the same absolute tolerance.
@@ -54,20 +60,40 @@ def stage(ch, st, dt, before_ib=None):
     ch.apply_boundary_conditions()
 
 
-def generate_pairs(nx, ny, nz, nth, nzb, runs, steps, dt, seed=11, max_pairs=None, rng_seed=5):
-    """Features X [n, 54] (unscaled) and force increments T [n, 4] from
-    `runs` runs of `steps` RK3 steps (reference warm start)."""
+def normalise(X, T):
+    """Per-solve normalisation of one solve's pairs (X [nb, 54] as
+    ib_last_pair returns them, T [nb, 4]): residual channels and T divided by
+    the rms of the residual (the centre marker's channels 0-3 over all
+    markers), force channels by the rms of dudn_x, dudn_y."""
+    X = X.astype(np.float64).reshape(-1, 9, 6)
+    rr = np.sqrt(np.mean(X[:, 4, 0:4] ** 2))
+    rf = np.sqrt(np.mean(X[:, 4, 4:6] ** 2))
+    ir = 1.0 / rr if rr > 0 else 0.0
+    jf = 1.0 / rf if rf > 0 else 0.0
+    X[:, :, 0:4] *= ir
+    X[:, :, 4:6] *= jf
+    return X.reshape(-1, 54), T * ir
+
+
+def generate_pairs(nx, ny, nz, nth, nzb, runs, steps, dt, seed=11, max_pairs=None, rng_seed=5,
+                   warm_steps=0):
+    """Normalised features X [n, 54] and force increments T [n, 4] from
+    `runs` runs: `warm_steps` RK3 steps past the start-up transient, then
+    `steps` steps whose every IB solve (reference warm start) gives nb pairs."""
     Xs, Ts = [], []
     for run in range(runs):
         s1, s2 = speeds(seed, run)
         ch, m = make_channel(nx, ny, nz, nth, nzb, s1, s2)
         with ch:
+            for _ in range(warm_steps):
+                ch.compute_time_step_RK3(dt)
             for _ in range(steps):
                 for st in (1, 2, 3):
                     stage(ch, st, dt)
                     X, dF = ch.ib_last_pair()
-                    Xs.append(X)
-                    Ts.append(dF.reshape(4, m.nb).T.astype(np.float64))
+                    Xn, Tn = normalise(X, dF.reshape(4, m.nb).T.astype(np.float64))
+                    Xs.append(Xn)
+                    Ts.append(Tn)
     X = np.concatenate(Xs)
     T = np.concatenate(Ts)
     if max_pairs and X.shape[0] > max_pairs:
@@ -98,6 +124,12 @@ def train(ch, X, T, epochs=40, batch=256, lr=1e-3, seed=1):
     Ts = (T / osc).astype(np.float32)
     hist = ch.mlp_train(Xs, Ts, epochs, batch=batch, lr=lr, seed=seed + 100)
     return hist, isc, osc
+
+
+def per_stage(rec, name):
+    """Iterations of `name` per RK3 stage (solves in stage order 1, 2, 3)."""
+    it = rec[name]
+    return {f"stage{st}": it[st - 1::3] for st in (1, 2, 3)}
 
 
 def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000, learned=True):

@@ -108,12 +108,34 @@ class Channel:
         return dict(U=U, V=V, W=W, P=P, P_interim=Pint)
 
     # reference procedures ----------------------------------------------------
-    def apply_boundary_conditions(self):
-        self._chk(self.L.c3d_apply_boundary_conditions(self.h))
+    def _uvw(self, U, V, W):
+        out = []
+        for a, s in ((U, self.shape_u), (V, self.shape_v), (W, self.shape_w)):
+            if a.shape != s or a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
+                raise ValueError(f"expected a Fortran-ordered float64 array of shape {s}")
+            out.append(a)
+        return out
 
-    def divergence(self):
+    def apply_boundary_conditions(self, U=None, V=None, W=None):
+        """apply_boundary_conditions(U_, V_, W_): on the resident velocity
+        without arguments, else in place on the given host arrays (the
+        resident velocity is then left alone)."""
+        if U is None and V is None and W is None:
+            self._chk(self.L.c3d_apply_boundary_conditions(self.h))
+            return None
+        U, V, W = self._uvw(U, V, W)
+        self._chk(self.L.c3d_apply_boundary_conditions_arrays(self.h, ptr(U), ptr(V), ptr(W)))
+        return U, V, W
+
+    def divergence(self, U=None, V=None, W=None):
+        """divergence(div_, U_, V_, W_) of the resident velocity, or of the
+        given host arrays."""
         d = self.zeros("rhs")
-        self._chk(self.L.c3d_divergence(self.h, ptr(d)))
+        if U is None and V is None and W is None:
+            self._chk(self.L.c3d_divergence(self.h, ptr(d)))
+        else:
+            U, V, W = self._uvw(U, V, W)
+            self._chk(self.L.c3d_divergence_arrays(self.h, ptr(U), ptr(V), ptr(W), ptr(d)))
         return d
 
     def gradient(self, p):
@@ -187,6 +209,26 @@ class Channel:
         self._chk(self.L.c3d_get_rhs(self.h, slot, ptr(fu), ptr(fv), ptr(fw)))
         return fu, fv, fw
 
+    def compute_rhs_arrays(self, U, V, W):
+        """compute_rhs_u/v/w(U_, V_, W_, rhs) of host arrays -> (Fu, Fv, Fw)
+        with the interior shapes of get_rhs."""
+        U, V, W = self._uvw(U, V, W)
+        fu = np.zeros((self.nx - 2, self.nyg - 2, self.nzg - 2), order="F")
+        fv = np.zeros((self.nxg - 2, self.ny - 2, self.nzg - 2), order="F")
+        fw = np.zeros((self.nxg - 2, self.nyg - 2, self.nz - 2), order="F")
+        self._chk(self.L.c3d_compute_rhs_arrays(self.h, ptr(U), ptr(V), ptr(W), ptr(fu), ptr(fv),
+                                                ptr(fw)))
+        return fu, fv, fw
+
+    def compute_rhs_u(self, U, V, W):
+        return self.compute_rhs_arrays(U, V, W)[0]
+
+    def compute_rhs_v(self, U, V, W):
+        return self.compute_rhs_arrays(U, V, W)[1]
+
+    def compute_rhs_w(self, U, V, W):
+        return self.compute_rhs_arrays(U, V, W)[2]
+
     def rk3_update(self, rk_step, dt):
         self._chk(self.L.c3d_rk3_update(self.h, rk_step, dt))
 
@@ -195,13 +237,24 @@ class Channel:
         self._chk(self.L.c3d_compute_dt(self.h, cfl, C.byref(v)))
         return v.value
 
-    def mean_mass_flow_u(self):
+    def mean_mass_flow_u(self, U=None):
+        """compute_mean_mass_flow_U of the resident U, or of a host array."""
         v = C.c_double()
-        self._chk(self.L.c3d_mean_mass_flow_u(self.h, C.byref(v)))
+        if U is None:
+            self._chk(self.L.c3d_mean_mass_flow_u(self.h, C.byref(v)))
+        else:
+            if U.shape != self.shape_u or not U.flags["F_CONTIGUOUS"] or U.dtype != np.float64:
+                raise ValueError(f"expected a Fortran-ordered float64 array of shape {self.shape_u}")
+            self._chk(self.L.c3d_mean_mass_flow_u_array(self.h, ptr(U), C.byref(v)))
         return v.value
 
     def add_to_u(self, d):
         self._chk(self.L.c3d_add_to_u(self.h, d))
+
+    def set_rhs_exact(self, on=True):
+        """Bitwise (reference divisions, no FMA) right-hand sides in
+        compute_time_step_RK3 instead of the fused round-off-level pass."""
+        self._chk(self.L.c3d_set_rhs_exact(self.h, int(bool(on))))
 
     def compute_time_step_RK3(self, dt, x_mass_cte=0, q0=0.0):
         """One compute_time_step_RK3 (src/time_integration.f90:86-193)."""
@@ -288,11 +341,21 @@ class Channel:
                                               ptr(Hc)))
         return dict(E1nHc=a, E1nH=b, Hu_int=Hu, Hv_int=Hv, Hw_int=Hw, Hc_ext=Hc)
 
-    def regT(self, with_dxn=False):
-        """E (regT) or E1n (regT_1n) of the resident velocity -> [3nb]."""
+    def regT(self, with_dxn=False, U=None, V=None, W=None):
+        """E (regT) or E1n (regT_1n) of the resident velocity, or of the
+        given host arrays -> [3nb]."""
         out = np.zeros(3 * self.nb)
-        self._chk(self.L.c3d_ib_interpolate(self.h, int(with_dxn), ptr(out)))
+        if U is None and V is None and W is None:
+            self._chk(self.L.c3d_ib_interpolate(self.h, int(with_dxn), ptr(out)))
+        else:
+            U, V, W = self._uvw(U, V, W)
+            self._chk(self.L.c3d_ib_interpolate_arrays(self.h, int(with_dxn), ptr(U), ptr(V),
+                                                       ptr(W), ptr(out)))
         return out
+
+    def compute_heaviside(self):
+        """compute_heaviside (src/heaviside.f90:20-54) for the current markers."""
+        self._chk(self.L.c3d_ib_compute_heaviside(self.h))
 
     def regTc_1n(self, p):
         p = np.asfortranarray(p, dtype=np.float64)

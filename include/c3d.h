@@ -92,6 +92,20 @@ int c3d_divergence(c3d_ctx *ctx, double *div);
  * staggered arrays (src/differential_operators.f90:16-52). */
 int c3d_gradient(c3d_ctx *ctx, const double *p, double *Gpx, double *Gpy, double *Gpz);
 
+/* Host-array forms of the procedures above, for callers that pass their own
+ * arrays (the reference calls apply_boundary_conditions and divergence on
+ * U_reg/V_reg/W_reg and the Heaviside fields, src/projection.f90:73,76,116,119,
+ * src/heaviside.f90:33,35,43,49).  The arrays are staged in device scratch
+ * owned by the context (allocated by c3d_create); the resident state is not
+ * touched and nothing is allocated per call. */
+/* apply_boundary_conditions(U_, V_, W_) on host arrays in place
+ * (src/boundary_conditions.f90:20-55). */
+int c3d_apply_boundary_conditions_arrays(c3d_ctx *ctx, double *U, double *V, double *W);
+/* divergence(div_, U_, V_, W_) of host arrays into a host rhs_p-shaped array
+ * (src/differential_operators.f90:54-74). */
+int c3d_divergence_arrays(c3d_ctx *ctx, const double *U, const double *V, const double *W,
+                          double *div);
+
 /* solve_poisson_equation(rhs) on a host rhs_p-shaped array, in place
  * (src/poisson.f90:33-101).  If rk_step == 1 the resident P receives
  * rhs/(dt*8/15) with neighbour-copied ghosts, as at src/poisson.f90:91-99. */
@@ -147,6 +161,14 @@ int c3d_compute_rhs(c3d_ctx *ctx, int slot);
  * shapes (nx-2,nyg-2,nzg-2), (nxg-2,ny-2,nzg-2), (nxg-2,nyg-2,nz-2). */
 int c3d_get_rhs(c3d_ctx *ctx, int slot, double *Fu, double *Fv, double *Fw);
 
+/* compute_rhs_u/v/w(U_, V_, W_, rhs) of host arrays (src/equations.f90:24-29,
+ * 160-165, 295-300) into host arrays of the interior shapes of c3d_get_rhs;
+ * NULL outputs are skipped.  Uses the pressure gradients of
+ * c3d_set_pressure_gradient and the body's H masks as the resident form does;
+ * the resident state is not touched. */
+int c3d_compute_rhs_arrays(c3d_ctx *ctx, const double *U, const double *V, const double *W,
+                           double *Fu, double *Fv, double *Fw);
+
 /* Stage update U(int) = Uo(int) + dt*sum_l rk_coef(rk_step,l)*F_l with the
  * Wray coefficients (src/time_integration.f90:105-107,134-136,163-168). */
 int c3d_rk3_update(c3d_ctx *ctx, int rk_step, double dt);
@@ -158,6 +180,10 @@ int c3d_compute_dt(c3d_ctx *ctx, double cfl, double *dt);
 /* compute_mean_mass_flow_U: H-masked trapezoid bulk velocity of the
  * resident U (src/mass_flow.f90:24-53). */
 int c3d_mean_mass_flow_u(c3d_ctx *ctx, double *qflow);
+/* compute_mean_mass_flow_U(U, Qflow) of a host array U(nx,nyg,nzg)
+ * (src/mass_flow.f90:24-53); compute_dPx_for_constant_mass_flow(U, dPdx)
+ * (:144-175) is Qflow_x_0 minus this value. */
+int c3d_mean_mass_flow_u_array(c3d_ctx *ctx, const double *U, double *qflow);
 
 /* U(2:nx-1,2:nyg-1,2:nzg-1) += d (src/time_integration.f90:190). */
 int c3d_add_to_u(c3d_ctx *ctx, double d);
@@ -167,8 +193,23 @@ int c3d_add_to_u(c3d_ctx *ctx, double d);
  * boundary conditions, non-IB projection, boundary conditions (with a body
  * set up: then the IB projection and boundary conditions, :118-123); then, if
  * x_mass_cte == 1, the constant-flow correction toward q0 (Qflow_x_0),
- * which also updates dPdx. */
+ * which also updates dPdx.  Returns C3D_ENAN when a velocity written by the
+ * step's projections is not finite (the NaN stop of compute_statistics,
+ * src/statistics.f90:142-144, checked at every step). */
 int c3d_time_step_rk3(c3d_ctx *ctx, double dt, int x_mass_cte, double q0);
+
+/* Right-hand sides of c3d_time_step_rk3.  on == 0 (the default, unless the
+ * environment sets C3D_RHS_EXACT=1): each stage evaluates compute_rhs_u/v/w
+ * and the stage update in one fused pass, with the spacing quotients as
+ * products with tabulated reciprocals and FMA contraction -- the reference's
+ * operations to round-off (relative 1e-12 after one step of the 32x64x32
+ * channel).  on != 0: the reference's divisions, no contraction, separate
+ * update pass: bitwise the oracle, as c3d_compute_rhs and c3d_rk3_update
+ * always are.  In the fused mode the stage slots hold F1 and F2 of the last
+ * step afterwards (F3 is not stored).  No reference counterpart: a choice
+ * of arithmetic for src/equations.f90:24-423 and
+ * src/time_integration.f90:101-168. */
+int c3d_set_rhs_exact(c3d_ctx *ctx, int on);
 
 /* Timing of resident projections with HIP events on the context stream:
  * a pass of `iters` back-to-back projections with an event between kernel
@@ -286,6 +327,15 @@ int c3d_ib_get_heaviside(c3d_ctx *ctx, double *E1nHc, double *E1nH, double *Hu_i
 /* regT (with_dxn 0) / regT_1n (1) of the resident U, V, W -> out[3nb]
  * (src/immersed_boundary_operators.f90:296-342). */
 int c3d_ib_interpolate(c3d_ctx *ctx, int with_dxn, double *out);
+/* regT (with_dxn 0) / regT_1n (1) of host arrays U_, V_, W_ -> out[3nb]
+ * (src/immersed_boundary_operators.f90:296-342, the Eu_ argument); the
+ * resident velocity is not touched. */
+int c3d_ib_interpolate_arrays(c3d_ctx *ctx, int with_dxn, const double *U, const double *V,
+                              const double *W, double *out);
+/* compute_heaviside (src/heaviside.f90:20-54) again for the current markers:
+ * Hu/Hv/Hw_interior, Hc_exterior, E1nHc_exterior and E1nH_exterior
+ * (c3d_ib_setup* and c3d_ib_move already run it). */
+int c3d_ib_compute_heaviside(c3d_ctx *ctx);
 /* regTc_1n of a host centre field p (rhs_p shape) -> out[nb] (:344-357). */
 int c3d_ib_interpolate_c(c3d_ctx *ctx, const double *p, double *out);
 /* regu / regv / regw (comp 0, 1, 2) of f[nb] into the host field F of that

@@ -135,6 +135,7 @@ def test_fortran_cylinders_matches_c_abi(tmp_path):
         ch.setup_IB(m)
         ch.set_pressure_gradient(dpdx, 0.0, 0.0)
         ch.set_IB_solver(tol=1e-10, maxit=1000, guess_mode=0)
+        ch.set_rhs_exact(True)  # the Fortran loop calls compute_rhs and the update separately
         for _ in range(nsteps):
             ch.compute_time_step_RK3(dt, x_mass_cte=0)
         ref = ch.get_fields()
@@ -190,6 +191,7 @@ def test_fortran_waves_matches_c_abi(tmp_path, body, p1, p2, p3, ramp):
         ch.setup_IB_wave(body, nxb, nzb, p1, p2, p3, ramp)
         ch.set_pressure_gradient(dpdx, 0.0, 0.0)
         ch.set_IB_solver(tol=1e-10, maxit=1000, guess_mode=0)
+        ch.set_rhs_exact(True)  # the Fortran loop calls compute_rhs and the update separately
         for _ in range(nsteps):
             ch.compute_time_step_RK3(dt, x_mass_cte=0)
         ref = ch.get_fields()
@@ -268,3 +270,85 @@ def test_fortran_restart_from_snapshot(tmp_path):
     dpdx_a, t_a = full[n:n + 2]
     dpdx_c, t_c = rest[n:n + 2]
     assert t_c == t_a and abs(dpdx_c - dpdx_a) <= 1e-9 * abs(dpdx_a)
+
+
+DROPIN_FD = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "dropin_fd")
+
+
+def test_fortran_host_array_procedures_leave_resident_state(tmp_path):
+    """The reference's host-array procedures (apply_boundary_conditions,
+    divergence, compute_rhs_u/v/w, compute_mean_mass_flow_U,
+    compute_dPx_for_constant_mass_flow, regT, regT_1n, regTc_1n, regu/v/w,
+    remove_mean_per_body, compute_heaviside), called by the Fortran host on its
+    own scratch arrays between two resident RK3 steps: the resident velocity is
+    bitwise unchanged by them, the two steps end bitwise where they end
+    without the calls, and each result matches the oracle on the same arrays."""
+    from channel3d_parallel_amd import ib_geometry
+    if not os.path.exists(DROPIN_FD):
+        pytest.fail("Fortran dropin_fd not built (make -C channel3d_parallel_amd/fortran)")
+    nx, ny, nz = 34, 33, 34
+    nu, dt, dpdx = 1.0 / 2800.0, 0.004, 3.0 / 2800.0
+    print(f"\n[case] fortran dropin_fd {nx}x{ny}x{nz}", flush=True)
+    g = orc.Grid(nx, ny, nz, grid_type=0)
+    m = ib_geometry.two_cylinders(g.scalars["Lzp"], nzb=16, nth=24)
+    nb = m.nb
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.0, grid_type=0)
+    Ua, Va, Wa = fields.white_faces(nx, ny, nz, seed=31)
+    rng = np.random.default_rng(32)
+    Pa = np.asfortranarray(rng.standard_normal(g.shape_rhs))
+    f = rng.standard_normal(3 * nb)
+    fmk, fin, fsc = tmp_path / "mk.bin", tmp_path / "in.bin", tmp_path / "sc.bin"
+    with open(fmk, "wb") as fh:
+        np.array([m.nb, m.nzb, m.nxb, m.nxb1], dtype=np.int32).tofile(fh)
+        np.concatenate([m.xb, m.yb, m.zb, m.normals, m.sb, m.ub]).tofile(fh)
+    np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
+    np.concatenate([Ua.ravel("F"), Va.ravel("F"), Wa.ravel("F"), Pa.ravel("F"), f]).tofile(fsc)
+    raws = []
+    for mode in (0, 1):
+        fout = tmp_path / f"out{mode}.bin"
+        subprocess.run([DROPIN_FD, str(nx), str(ny), str(nz), repr(nu), repr(dt), repr(dpdx),
+                        str(mode), str(fmk), str(fin), str(fsc), str(fout)],
+                       check=True, timeout=120, capture_output=True, text=True)
+        raws.append(np.fromfile(fout))
+    nstate = int(np.prod(g.shape_u) + np.prod(g.shape_v) + np.prod(g.shape_w)) + 4 * nb
+    assert np.array_equal(raws[0][:nstate], raws[1][:nstate]), "the calls changed the time steps"
+    assert raws[0][nstate] == 1.0 and raws[1][nstate] == 1.0, "resident velocity changed"
+    raw, off = raws[1], nstate + 1
+
+    def take(shape):
+        nonlocal off
+        n = int(np.prod(shape))
+        a = raw[off:off + n].reshape(shape, order="F")
+        off += n
+        return a
+
+    bc = [take(s) for s in (g.shape_u, g.shape_v, g.shape_w)]
+    dv = take(g.shape_rhs)
+    F = [take(s) for s in (g.shape_fu, g.shape_fv, g.shape_fw)]
+    qflow, dpx = take((2,))
+    Eu, E1nu, E1np = take((3 * nb,)), take((3 * nb,)), take((nb,))
+    R = [take(s) for s in (g.shape_u, g.shape_v, g.shape_w)]
+    fm = take((nb,))
+    assert off == raw.size
+    g.apply_bc(Ua, Va, Wa)
+    for a, b, n in zip(bc, (Ua, Va, Wa), "UVW"):
+        assert np.array_equal(a, b), "apply_boundary_conditions " + n
+    assert np.array_equal(dv, g.divergence(Ua, Va, Wa)), "divergence"
+    for a, b, n in zip(F, g.rhs(Ua, Va, Wa, nu, dpdx, 0.0, 0.0), "uvw"):
+        assert np.array_equal(a, b), "compute_rhs_" + n
+    ib = orc.IB(g, m)
+    ib.heaviside()
+    Hu = ib.heaviside_coeffs()["Hu_int"]
+    q = g.mean_mass_flow_u(Ua, Hu)
+    assert abs(qflow - q) <= 1e-13 * abs(q), (qflow, q)
+    assert abs(dpx - (1.0 - q)) <= 1e-13 * abs(q)
+    for a, b, n in ((Eu, ib.E(Ua, Va, Wa), "regT"), (E1nu, ib.E(Ua, Va, Wa, with_dxn=True), "regT_1n"),
+                    (E1np, ib.E1n_c(Pa), "regTc_1n")):
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        assert err <= 1e-13, (n, err)
+    for comp, n in enumerate(("regu", "regv", "regw")):
+        ref = ib.R(comp, f[comp * nb:(comp + 1) * nb])
+        assert np.max(np.abs(R[comp] - ref)) <= 1e-13 * np.max(np.abs(ref)), n
+        assert np.array_equal(R[comp] != 0, ref != 0), n + " support"
+    ref = ib.remove_mean(f[:nb])
+    assert np.max(np.abs(fm - ref)) <= 1e-13 * np.max(np.abs(f[:nb])), "remove_mean_per_body"

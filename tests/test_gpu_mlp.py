@@ -106,3 +106,25 @@ def test_learned_guess_in_ib_solve():
     assert hist[-1] < hist[0]
     assert max(rec["max_force_diff_learned_vs_zero"]) < 1e-6
     assert sum(rec["learned"]) <= sum(rec["zero"])
+
+
+def test_failed_buffer_growth_keeps_the_network_usable():
+    """A forward pass whose activation buffers cannot be allocated (300 M rows,
+    beyond HBM) returns C3D_ENOMEM and leaves the smaller buffer set that was
+    in use intact: the next passes at the old and a larger size still run and
+    give the same outputs."""
+    from channel3d_parallel_amd._lib import fptr
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((1024, 54)).astype(np.float32)
+    X2 = np.concatenate([X, rng.standard_normal((1024, 54)).astype(np.float32)])
+    print("\n[case] network buffers: 1024 rows, 300M rows (fails), 1024, 2048", flush=True)
+    with Channel(18, 17, 18) as ch:
+        ch.mlp_init(seed=3)
+        Y1 = ch.mlp_forward(X)
+        Yb = np.zeros((1, 4), dtype=np.float32)
+        rc = ch.L.c3d_mlp_forward(ch.h, fptr(X), 300_000_000, fptr(Yb))
+        assert rc == -3, rc
+        Y2 = ch.mlp_forward(X)
+        Y3 = ch.mlp_forward(X2)
+    assert np.array_equal(Y1, Y2)
+    assert np.allclose(Y3[:1024], Y1, rtol=1e-6, atol=1e-6)

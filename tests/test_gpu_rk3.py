@@ -89,11 +89,13 @@ def test_compute_dt_and_mass_flow():
     assert np.array_equal(U2, ref)
 
 
-@pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (100, 1e-10)])
-def test_channel_32x64x32_time_steps(nsteps, tol):
+@pytest.mark.parametrize("nsteps,tol,exact", [(1, 1e-12, False), (100, 1e-10, False),
+                                               (1, 1e-12, True), (100, 1e-10, True)])
+def test_channel_32x64x32_time_steps(nsteps, tol, exact):
     """32x64x32 cells (34x65x34 faces), tanh alpha 2, Re_b 2800,
     laminar parabola + 10% divergence-free perturbation, constant flow rate,
-    dt from compute_dt at CFL 0.5."""
+    dt from compute_dt at CFL 0.5; the fused right-hand sides (default) and
+    the bitwise ones (set_rhs_exact) against the oracle."""
     nx, ny, nz = 34, 65, 34
     g, U, V, W = state(nx, ny, nz, kind="ic", seed=1)
     q0 = g.mean_mass_flow_u(U)
@@ -101,6 +103,7 @@ def test_channel_32x64x32_time_steps(nsteps, tol):
     Pr = g.zeros("p")
     dpdx_r = DPDX
     with Channel(nx, ny, nz, nu=NU) as ch:
+        ch.set_rhs_exact(exact)
         ch.set_fields(U, V, W, P=np.zeros(g.shape_p, order="F"))
         ch.set_pressure_gradient(DPDX, 0.0, 0.0)
         for n in range(nsteps):
@@ -200,3 +203,31 @@ def test_monitor_vs_oracle(nx, ny, nz, gt):
     for k in ("meanU", "meanV", "meanW"):
         assert abs(m[k] - r[k]) <= 1e-13 * max(abs(r["meanU"]), 1e-300), k
     assert m_bad["nonfinite"] == 2.0 == g.monitor(U2, V, W2)["nonfinite"]
+
+
+@pytest.mark.parametrize("where", ["interior", "ib"])
+def test_time_step_stops_on_nan(where):
+    """A non-finite value in the state makes compute_time_step_RK3 return
+    C3D_ENAN (the NaN stop of compute_statistics, src/statistics.f90:142-144,
+    checked at every step); the steps before it return normally."""
+    from channel3d_parallel_amd import ib_geometry
+    from channel3d_parallel_amd.solver import C3DError
+    nx, ny, nz = 34, 33, 34
+    gt = 0 if where == "ib" else 1
+    g, U, V, W = state(nx, ny, nz, gt=gt, kind="ic", seed=4)
+    with Channel(nx, ny, nz, nu=NU, grid_type=gt) as ch:
+        ch.set_fields(U, V, W)
+        ch.set_pressure_gradient(DPDX, 0.0, 0.0)
+        if where == "ib":
+            m = ib_geometry.two_cylinders(g.scalars["Lzp"], nzb=16, nth=24)
+            ch.setup_IB(m)
+        ch.compute_time_step_RK3(1e-3)  # finite: no error
+        out = ch.get_fields()
+        Ub = out["U"]
+        Ub[nx // 2, ny // 2, nz // 2] = np.nan
+        ch.set_fields(Ub, out["V"], out["W"])
+        with pytest.raises(C3DError) as e:
+            ch.compute_time_step_RK3(1e-3)
+        assert e.value.code == -5, e.value
+        assert "non-finite" in str(e.value)
+        print(f"  {where}: {e.value}")

@@ -26,6 +26,6 @@ def test_invalid_sizes_rejected(nx, ny, nz, gt):
 
 
 @needs_lib
-def test_mixed_precision_not_available_yet_is_an_error():
+def test_unknown_precision_code_is_rejected():
     with pytest.raises(C3DError):
         Channel(34, 65, 34, precision=7)
