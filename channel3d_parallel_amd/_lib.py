@@ -48,6 +48,8 @@ SIGNATURES = [
     ("c3d_check_divergence", C.c_int, [C.c_void_p, _dp]),
     ("c3d_wall_shear", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     ("c3d_plane_statistics", C.c_int, [C.c_void_p, _dp]),
+    ("c3d_monitor_noalloc", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     ("c3d_monitor", C.c_int, [C.c_void_p, _dp]),
     ("c3d_set_pressure_gradient", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     ("c3d_get_pressure_gradient", C.c_int, [C.c_void_p, _dp, _dp, _dp]),

@@ -147,6 +147,10 @@ hipError_t launch_massflow(const DevGrid& g, const double* U, const double* H, d
 // interiors, maxima of the same, and the count of non-finite entries; part
 // holds monitor_part_size() doubles of scratch.
 int monitor_part_size();
+constexpr int kMonitorPartDoubles = 256 * 7;  // monitor_part_size()
+// out[0..6] = the results of c3d_monitor (maxima, sums / vol, non-finite
+// count) from the reduced sums r7 of launch_monitor, on the device.
+hipError_t launch_monitor_finish(const double* r7, double vol, double* out, hipStream_t s);
 hipError_t launch_monitor(const DevGrid& g, const double* U, const double* V, const double* W,
                           double* part, double* out7, hipStream_t s);
 // Plane sums of compute_statistics over the unique cells of every row

@@ -354,6 +354,27 @@ hipError_t launch_dt_max(const DevGrid& g, const double* U, const double* V, con
 int massflow_rows(const DevGrid& g) { return (g.nyg - 3) * (g.nzg - 2); }
 
 int monitor_part_size() { return kMonBlocks * kMon; }
+static_assert(kMonBlocks * kMon == kMonitorPartDoubles, "partials of the monitor kernels");
+
+namespace {
+// c3d_monitor's results from the reduced sums r7 (sums U, V, W; maxima U, V,
+// W; non-finite count): maxima, sums / vol, count.
+__global__ void k_monitor_finish(const double* __restrict__ r7, double vol, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  out[0] = r7[3];
+  out[1] = r7[4];
+  out[2] = r7[5];
+  out[3] = r7[0] / vol;
+  out[4] = r7[1] / vol;
+  out[5] = r7[2] / vol;
+  out[6] = r7[6];
+}
+}  // namespace
+
+hipError_t launch_monitor_finish(const double* r7, double vol, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_monitor_finish, dim3(1), dim3(64), 0, s, r7, vol, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_monitor(const DevGrid& g, const double* U, const double* V, const double* W,
                           double* part, double* out7, hipStream_t s) {

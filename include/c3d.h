@@ -137,6 +137,19 @@ int c3d_wall_shear(c3d_ctx *ctx, double *utau, double *wtau, double *Umean, doub
  * number of non-finite entries in those interiors (a NaN guard for a host
  * loop; the reference has none). */
 int c3d_monitor(c3d_ctx *ctx, double *out);
+/* The results of c3d_monitor for device arrays U(nx,nyg,nzg),
+ * V(nxg,ny,nzg), W(nxg,nyg,nz) that the caller owns, into the device array
+ * out[7], enqueued on `stream` (a hipStream_t; NULL: the null stream) and
+ * not waited for.  No context and no allocation: the kernels' scratch (the
+ * 256 x 7 partials and the reduced sums, 14 KB whatever the grid) is a
+ * static device array of the library, one per GPU, held until the process
+ * ends.  Face counts 3 .. 1000000 each; C3D_EINVAL otherwise or for NULL
+ * arrays.  Calls that overlap on one GPU share that scratch and race: serialise
+ * them (one stream, or events).  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc). */
+int c3d_monitor_noalloc(int nx, int ny, int nz, const double *U, const double *V, const double *W,
+                        double *out, void *stream);
 /* Plane-mean profiles of compute_statistics on the resident fields
  * (src/statistics.f90:41-111, one domain): means[q*nyg + j-1] for q = 0
  * Umean, 1 Vmean, 2 Wmean, 3 U2mean, 4 V2mean, 5 W2mean, 6 UVmean, from the
