@@ -56,6 +56,10 @@ struct IBState {
   double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr, *mpart = nullptr;
   double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
   double* fprev = nullptr;  // force before the last solve (learned-guess pairs)
+  // converged force of each RK3 stage of the last step (guess mode 5,
+  // allocated on first use): the start of the same stage of the next step
+  double* fstage[3] = {nullptr};
+  bool fstage_ok[3] = {false, false, false};
   // history-projection start (guess_mode 4), allocated on first use: ring of
   // the last kHist converged forces and their right-hand sides, and the
   // orthonormal rhs basis hQ with forces hX (S hX = hQ) built from it
@@ -145,12 +149,15 @@ struct c3d_ctx {
   const double *Hu = nullptr, *Hv = nullptr, *Hw = nullptr, *Hm = nullptr;
   IBState* ib = nullptr;
   struct MLPState* mlp = nullptr;
-  // mixed precision (C3D_PREC_MIXED): FP32 spectrum, FP64 divergence b and
-  // potential phi, FP32 residual; nref FP64 refinement sweeps per projection
+  // refinement (nref sweeps per projection; mixed precision by default, FP64
+  // contexts on request): FP64 divergence b, the first solve's potential
+  // phi64 and the accumulated correction phic (kept apart: their FP64 sum
+  // would round phi64's low bits away), the FP64 residual r64 of the first
+  // sweep (when nref >= 2), gauge-row scratch; mixed: FP32 spectrum S32
   cpx<float>* S32 = nullptr;
-  double *b64 = nullptr, *phi64 = nullptr, *gauge = nullptr;
-  float* r32 = nullptr;
-  int nref = 2;
+  double *b64 = nullptr, *phi64 = nullptr, *phic = nullptr, *r64 = nullptr, *gauge = nullptr;
+  int nref = 0;
+  bool mixed_inv64 = true;  // FP64 inverse half of every mixed pass (C3D_MIXED_INV64=0: FP32)
   // modes whose tridiagonal systems are solved with FP64 arithmetic in the
   // mixed path: |kxx + kzz| < tri_thr (the (0,0) mode and the lowest |k|);
   // the others use FP32 arithmetic.  tri_low == null: FP64 for every mode.
@@ -428,6 +435,19 @@ int ib_move_to(c3d_ctx* c, double t);
 
 extern "C" {
 
+namespace {
+// Device buffers of nref refinement sweeps, allocated on first need.
+int refine_buffers(c3d_ctx* c, int nref) {
+  int rc = 0;
+  if (nref >= 1 && !c->b64 &&
+      ((rc = dalloc(c, &c->b64, c->nr)) || (rc = dalloc(c, &c->phi64, c->nr)) ||
+       (rc = dalloc(c, &c->phic, c->nr)) || (rc = dalloc(c, &c->gauge, (size_t)c->nzp + 1))))
+    return rc;
+  if (nref >= 2 && !c->r64 && (rc = dalloc(c, &c->r64, c->nr))) return rc;
+  return 0;
+}
+}  // namespace
+
 int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
   if (!desc || !out) return C3D_EINVAL;
   *out = nullptr;
@@ -516,17 +536,21 @@ int c3d_create(const c3d_desc* desc, c3d_ctx** out) {
       (!c3d::tri_part_supported(c->g) && (rc = dalloc(c, &c->CP, c->nspec))) ||
       (rc = dalloc(c, &c->red, kRedRefine + (size_t)kMaxRef * kResSlots)))
     return cleanup(rc);
-  if (d.precision == C3D_PREC_MIXED &&
-      ((rc = dalloc(c, &c->S32, c->nspec)) || (rc = dalloc(c, &c->b64, c->nr)) ||
-       (rc = dalloc(c, &c->phi64, c->nr)) || (rc = dalloc(c, &c->r32, c->nr)) ||
-       (rc = dalloc(c, &c->gauge, (size_t)c->nzp + 1))))
-    return cleanup(rc);
+  if (d.precision == C3D_PREC_MIXED && (rc = dalloc(c, &c->S32, c->nspec))) return cleanup(rc);
   c->g.nonfinite = c->nanflag;
   {
     const char* e = getenv("C3D_RHS_EXACT");
     c->rhs_exact = e && atoi(e) != 0;
   }
-  if (d.precision == C3D_PREC_MIXED && (rc = mixed_tri_modes(c))) return cleanup(rc);
+  if (d.precision == C3D_PREC_MIXED) {
+    if ((rc = mixed_tri_modes(c))) return cleanup(rc);
+    const char* e = getenv("C3D_MIXED_INV64");
+    c->mixed_inv64 = !(e && atoi(e) == 0);
+    // sweeps that bring max|div| below 1e-10 at 384x193x384 (one sweep
+    // leaves ~1e-14 of the input's max|div|; two reach the FP64 floor)
+    c->nref = c->mixed_inv64 ? 1 : 2;
+    if ((rc = refine_buffers(c, c->nref))) return cleanup(rc);
+  }
   if (hipMemsetAsync(c->nanflag, 0, sizeof(unsigned), c->stream) != hipSuccess ||
       hipMemsetAsync(c->U, 0, c->nu * 8, c->stream) != hipSuccess ||
       hipMemsetAsync(c->V, 0, c->nv * 8, c->stream) != hipSuccess ||
@@ -672,71 +696,101 @@ int c3d_solve_poisson_equation(c3d_ctx* c, double* rhs, int rk_step, double dt) 
   return sync(c);
 }
 
-// Refinement residual of the mixed path: formed inside the forward x
-// transform (default), or as its own FP32 array read by the transform
-// (C3D_MIXED_FUSED=0); both give the same spectrum.
-static bool mixed_fused() {
-  static const bool on = [] {
-    const char* e = getenv("C3D_MIXED_FUSED");
-    return !(e && atoi(e) == 0);
-  }();
-  return on;
-}
-
 // compute_non_IB_projection (src/projection.f90:22-46) on the resident
-// velocity.  FP64: one spectral solve.  Mixed: the FP32-storage solve of the
-// divergence, then nref sweeps of FP64 residual r = b - D G phi and an FP32
-// correction solve, so that the potential converges to the FP64 solution of
-// the projection's own discrete operator.
+// velocity.  FP64 without refinement: one spectral solve, the reference's
+// operations.  With refinement (mixed precision, or FP64 on request): a first
+// solve for phi1, then nref sweeps that each transform the FP64 residual
+// r = b - M (phi1 + phic) (M = D G plus the spectral solver's gauge row) and
+// add the solution to phic; the velocity is corrected by G(phi1) + G(phic).
+// Mixed: the forward half of every pass (x and z transforms) on FP32
+// spectra, the inverse half (tridiagonal solution, inverse z and x
+// transforms) in FP64 (or FP32 with C3D_MIXED_INV64=0).
 static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev) {
   const bool rk1 = rk_step == 1;
   const bool mixed = c->desc.precision == C3D_PREC_MIXED;
+  const int nref = c->nref;
   const DevGrid& g = c->g;
   hipStream_t st = c->stream;
   if (ev) HIPCHK(c, hipEventRecord(ev[0], st));
-  if (mixed) {
-    HIPCHK(c, hipMemsetAsync(c->red + kRedRefine, 0, (size_t)kMaxRef * kResSlots * sizeof(unsigned long long), st));
-    // the FP64 divergence b is kept only for the refinement residuals
-    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, c->nref > 0 ? c->b64 : nullptr, nullptr,
-                                  c->S32, st));
-  } else {
+  if (!mixed && nref == 0) {
     HIPCHK(c, c3d::launch_fwd_x(g, true, c->U, c->V, c->W, nullptr, c->S, st));
-  }
-  if (ev) HIPCHK(c, hipEventRecord(ev[1], st));
-  HIPCHK(c, mixed ? c3d::launch_fft_z32(g, -1, c->S32, st) : c3d::launch_fft_z(g, -1, c->S, st));
-  if (ev) HIPCHK(c, hipEventRecord(ev[2], st));
-  HIPCHK(c, mixed ? c3d::launch_tri32(g, c->S32, c->tri_low, c->ntri_low, c->tri_thr, st)
-                  : c3d::launch_tri(g, c->S, c->CP, st));
-  if (ev) HIPCHK(c, hipEventRecord(ev[3], st));
-  HIPCHK(c, mixed ? c3d::launch_fft_z32(g, +1, c->S32, st) : c3d::launch_fft_z(g, +1, c->S, st));
-  if (ev) HIPCHK(c, hipEventRecord(ev[4], st));
-  if (!mixed)
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], st));
+    HIPCHK(c, c3d::launch_fft_z(g, -1, c->S, st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[2], st));
+    HIPCHK(c, c3d::launch_tri(g, c->S, c->CP, st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[3], st));
+    HIPCHK(c, c3d::launch_fft_z(g, +1, c->S, st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[4], st));
     HIPCHK(c, c3d::launch_inv_x(g, 1, c->S, c->Pint, c->P, dt, rk1, st));
-  else if (c->nref == 0)
-    HIPCHK(c, c3d::launch_inv_x32(g, 1, c->S32, nullptr, c->Pint, c->P, dt, rk1, st));
-  else
-    HIPCHK(c, c3d::launch_inv_x32(g, 0, c->S32, nullptr, c->phi64, c->P, dt, false, st));
-  if (ev) HIPCHK(c, hipEventRecord(ev[5], st));
-  for (int s = 1; mixed && s <= c->nref; s++) {
-    const bool last = s == c->nref;
-    unsigned long long* rslots = c->red + kRedRefine + (size_t)(s - 1) * kResSlots;
-    if (mixed_fused()) {  // residual formed inside the forward x transform
-      HIPCHK(c, c3d::launch_fwd_x_resid32(g, c->b64, c->phi64, c->gauge, c->S32, rslots, st));
-    } else {
-      HIPCHK(c, c3d::launch_residual(g, c->b64, c->phi64, c->gauge, c->r32, rslots, st));
-      HIPCHK(c, c3d::launch_fwd_x32(g, nullptr, nullptr, nullptr, nullptr, c->r32, c->S32, st));
-    }
-    HIPCHK(c, c3d::launch_fft_z32(g, -1, c->S32, st));
-    HIPCHK(c, c3d::launch_tri32(g, c->S32, c->tri_low, c->ntri_low, c->tri_thr, st));
-    HIPCHK(c, c3d::launch_fft_z32(g, +1, c->S32, st));
-    if (last)
-      HIPCHK(c, c3d::launch_inv_x32(g, 4, c->S32, c->phi64, c->Pint, c->P, dt, rk1, st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[5], st));
+    if (rk1) HIPCHK(c, c3d::launch_p_ghosts(g, c->P, st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[6], st));
+    HIPCHK(c, c3d::launch_grad_correct(g, c->U, c->V, c->W, c->Pint, dt, st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[7], st));
+    return 0;
+  }
+  // the FP32 inverse half of the mixed variant
+  const bool inv32 = mixed && !c->mixed_inv64;
+  auto fwd_z = [&]() -> hipError_t {
+    return mixed ? c3d::launch_fft_z32(g, -1, c->S32, st) : c3d::launch_fft_z(g, -1, c->S, st);
+  };
+  // inverse half of a pass: inv_x mode (0 phi, 1 phi/dt, 3 in1 + phi, 5 the
+  // final potential, see launch_inv_x64_ref) into out (and out2)
+  auto back = [&](int mode, const double* in1, const double* in2, double* out, double* out2,
+                  bool rk1w, bool events) -> int {
+    if (!mixed) HIPCHK(c, c3d::launch_tri(g, c->S, c->CP, st));
+    else if (inv32) HIPCHK(c, c3d::launch_tri32(g, c->S32, c->tri_low, c->ntri_low, c->tri_thr, st));
+    else HIPCHK(c, c3d::launch_tri32to64(g, c->S32, c->S, st));
+    if (events && ev) HIPCHK(c, hipEventRecord(ev[3], st));
+    HIPCHK(c, inv32 ? c3d::launch_fft_z32(g, +1, c->S32, st) : c3d::launch_fft_z(g, +1, c->S, st));
+    if (events && ev) HIPCHK(c, hipEventRecord(ev[4], st));
+    if (inv32)
+      HIPCHK(c, c3d::launch_inv_x32(g, mode, c->S32, in1, in2, out, out2, c->P, dt, rk1w, st));
     else
-      HIPCHK(c, c3d::launch_inv_x32(g, 3, c->S32, c->phi64, c->phi64, c->P, dt, false, st));
+      HIPCHK(c, c3d::launch_inv_x64_ref(g, mode, c->S, in1, in2, out, out2, c->P, dt, rk1w, st));
+    return 0;
+  };
+  int rc = 0;
+  if (nref > 0)
+    HIPCHK(c, hipMemsetAsync(c->red + kRedRefine, 0,
+                             (size_t)kMaxRef * kResSlots * sizeof(unsigned long long), st));
+  // first solve; the FP64 divergence b is kept only for the refinement
+  if (mixed)
+    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, nref > 0 ? c->b64 : nullptr, nullptr,
+                                  c->S32, st));
+  else
+    HIPCHK(c, c3d::launch_fwd_x_b64(g, c->U, c->V, c->W, c->b64, c->S, st));
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], st));
+  HIPCHK(c, fwd_z());
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], st));
+  if (nref == 0) rc = back(1, nullptr, nullptr, c->Pint, nullptr, rk1, true);
+  else rc = back(0, nullptr, nullptr, c->phi64, nullptr, false, true);
+  if (rc) return rc;
+  if (ev) HIPCHK(c, hipEventRecord(ev[5], st));
+  for (int s = 1; s <= nref; s++) {
+    // sweep 1: r = b - M phi1 (kept in r64 for later sweeps); sweep s > 1:
+    // r = r64 - M phic, M linear
+    unsigned long long* rslots = c->red + kRedRefine + (size_t)(s - 1) * kResSlots;
+    const double* bs = s == 1 ? c->b64 : c->r64;
+    const double* ps = s == 1 ? c->phi64 : c->phic;
+    double* rout = (s == 1 && nref >= 2) ? c->r64 : nullptr;
+    if (mixed)
+      HIPCHK(c, c3d::launch_fwd_x_resid32(g, bs, ps, c->gauge, c->S32, rslots, rout, st));
+    else
+      HIPCHK(c, c3d::launch_fwd_x_resid64(g, bs, ps, c->gauge, c->S, rslots, rout, st));
+    HIPCHK(c, fwd_z());
+    if (s < nref)
+      rc = back(s == 1 ? 0 : 3, c->phic, nullptr, c->phic, nullptr, false, false);
+    else
+      rc = back(5, c->phi64, s == 1 ? nullptr : c->phic, c->Pint, c->phic, rk1, false);
+    if (rc) return rc;
   }
   if (rk1) HIPCHK(c, c3d::launch_p_ghosts(g, c->P, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[6], st));
-  HIPCHK(c, c3d::launch_grad_correct(g, c->U, c->V, c->W, c->Pint, dt, st));
+  if (nref == 0)
+    HIPCHK(c, c3d::launch_grad_correct(g, c->U, c->V, c->W, c->Pint, dt, st));
+  else
+    HIPCHK(c, c3d::launch_grad_correct_split(g, c->U, c->V, c->W, c->phi64, c->phic, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[7], st));
   return 0;
 }
@@ -1214,17 +1268,14 @@ int c3d_set_rhs_exact(c3d_ctx* c, int on) {
 
 int c3d_set_refinement(c3d_ctx* c, int nref) {
   if (!c) return C3D_EINVAL;
-  if (c->desc.precision != C3D_PREC_MIXED)
-    return fail(c, C3D_EINVAL, "refinement sweeps apply to C3D_PREC_MIXED contexts");
   if (nref < 0 || nref > kMaxRef) return fail(c, C3D_EINVAL, "nref must be 0..%d", kMaxRef);
+  if (int rc = refine_buffers(c, nref)) return fail(c, rc, "refinement buffers: allocation failed");
   c->nref = nref;
   return C3D_OK;
 }
 
 int c3d_refinement_residuals(c3d_ctx* c, double* rmax, int n) {
   if (!c || !rmax || n < 0) return C3D_EINVAL;
-  if (c->desc.precision != C3D_PREC_MIXED)
-    return fail(c, C3D_EINVAL, "refinement sweeps apply to C3D_PREC_MIXED contexts");
   const int m = n < c->nref ? n : c->nref;
   std::vector<unsigned long long> bits((size_t)kMaxRef * kResSlots, 0ull);
   HIPCHK(c, hipMemcpyAsync(bits.data(), c->red + kRedRefine, bits.size() * sizeof(unsigned long long),
@@ -1542,6 +1593,23 @@ static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
                                         ib->mpart, c->stream));                                     // :57
   if (ib->guess == 1) HIPCHK(c, hipMemsetAsync(ib->fb, 0, 4 * nb * 8, c->stream));
   if (ib->guess == 2) ib->guess = 0;
+  // mode 5 starts from the converged force of the same RK3 stage of the
+  // previous step (the stage forces of consecutive steps differ by O(dt),
+  // those of consecutive stages by O(1))
+  const bool by_stage = ib->guess == 5 && rk_step >= 1 && rk_step <= 3;
+  if (by_stage) {
+    if (!ib->fstage[0]) {
+      int rc0 = 0;
+      for (int q = 0; q < 3 && !rc0; q++) rc0 = dalloc(c, &ib->fstage[q], 4 * nb);
+      if (rc0) {
+        ib->fstage[0] = nullptr;
+        return rc0;
+      }
+    }
+    if (ib->fstage_ok[rk_step - 1])
+      HIPCHK(c, hipMemcpyAsync(ib->fb, ib->fstage[rk_step - 1], 4 * nb * 8,
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
   HIPCHK(c, hipMemcpyAsync(ib->fprev, ib->fb, 4 * nb * 8, hipMemcpyDeviceToDevice, c->stream));
   ib->last_dt = dt;
   bool r_ready = false;
@@ -1560,6 +1628,11 @@ static int ib_projection_launch(c3d_ctx* c, double dt, int rk_step) {
   if (rc && rc != C3D_ENOCONV) return rc;
   const int conv = rc;
   if (!conv && ib->guess == 4 && (rc = ib_hist_push(c, dt))) return rc;
+  if (by_stage) {
+    HIPCHK(c, hipMemcpyAsync(ib->fstage[rk_step - 1], ib->fb, 4 * nb * 8, hipMemcpyDeviceToDevice,
+                             c->stream));
+    ib->fstage_ok[rk_step - 1] = true;
+  }
   if ((rc = ib_spread_force(c, ib->fb, dt))) return rc;                                  // :66-73
   if ((rc = ib_poisson_reg(c, dt, rk_step))) return rc;                                  // :76-80
   HIPCHK(c, c3d::ib::launch_ib_correct(c->g, ib->Uint, ib->Vint, ib->Wint, ib->Ureg, ib->Vreg,
@@ -2127,7 +2200,7 @@ int c3d_ib_bicgstab(c3d_ctx* c, double* x, const double* b, double tol, int maxi
 int c3d_ib_set_solver(c3d_ctx* c, double tol, int maxit, int guess_mode) {
   int rc = need_ib(c);
   if (rc) return rc;
-  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 4) return C3D_EINVAL;
+  if (!(tol >= 0.0) || maxit < 0 || guess_mode < 0 || guess_mode > 5) return C3D_EINVAL;
   c->ib->tol = tol;
   c->ib->maxit = maxit;
   c->ib->guess = guess_mode;

@@ -96,10 +96,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
       wprev[p] = (N % kBlock == 0 || n < N) ? W0[n + 1] : 0.0;
     }
   }
-  // Residual r = b - M phi (the operator of k_residual: D G on the unique
-  // cells, periodic in x and z, no gradient on the wall faces, plus the
-  // (0,0)-mode gauge row at j = 2), with U = phi and rhs_in = b; each lane
-  // walks its x positions down the tile's rows carrying phi(k-1), phi(k).
+  // Residual r = b - M phi (D G on the unique cells, periodic in x and z, no
+  // gradient on the wall faces, plus the (0,0)-mode gauge row at j = 2), with
+  // U = phi and rhs_in = b; each lane walks its x positions down the tile's
+  // rows carrying phi(k-1), phi(k).  bout (may be null) receives the FP64
+  // residual.
   const double* phi = U;
   const size_t pk = (size_t)(nxg - 2) * (nyg - 2);  // rhs_p plane stride
   const int rowj = nxg - 2;                          // rhs_p row stride
@@ -203,7 +204,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
               d[h] = (v[r + h][p][0] - v[r + h][p][1]) * rx +
                      (v[r + h][p][2] - v[r + h][p][3]) * ry + (v[r + h][p][4] - wm) * rz;
             } else if constexpr (RESID) {
-              // the expressions of k_residual, term by term
+              // r = b - D G phi on the unique cells (i = 2..nxp+1,
+              // j = 2..nyg-1, k = 2..nzp+1): periodic in x and z (the face
+              // left of cell 2 is the image of face nx-1, resp. nz-1), no
+              // gradient on the wall faces (j = 1, ny); phi carries its
+              // periodic duplicates (i = nxg-1, k = nzp+2)
               const int i = n + 2, k = k0 + t0 + r + h + 2;
               const int km = (k == 2) ? g.nz - 1 : k - 1;
               const double* q = v[r + h][p];
@@ -219,8 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
               amax = fmax(amax, fabs(d[h]));
               pm[p] = c0;
               p0[p] = q[1];
-              // the FFT input is the FP32 residual, as k_residual stores it
-              d[h] = (double)(float)d[h];
+              // the FP64 residual itself, when later sweeps start from it
+              if (bout) bout[ridx(i, j, k, nxg, nyg)] = d[h];
+              // an FP32 transform takes the residual rounded to FP32
+              if constexpr (sizeof(real) == 4) d[h] = (double)(float)d[h];
             } else {
               d[h] = v[r + h][p][0];
             }
@@ -275,7 +282,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
     __builtin_nontemporal_store(v.y, &d->y);
   }
   if constexpr (RESID) {
-    // max |r| of the workgroup into slot (workgroup % 64), as k_residual
+    // max |r| of the workgroup, one atomic per workgroup into one of 64
+    // slots (linear workgroup index % 64) so that the workgroups do not
+    // serialise on one address; the host takes the max of the slots
     __shared__ double sm[kBlock / 64];
     for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
     if ((tid & 63) == 0) sm[tid >> 6] = amax;
@@ -407,10 +416,12 @@ constexpr int kTriAll = 0;   // every mode
 constexpr int kTriHigh = 1;  // modes with |kxx + kzz| >= lam_thr
 constexpr int kTriList = 2;  // the modes list[0 .. nlist)
 
-template <typename real, typename acc, int M, int SEL>
-__global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __restrict__ S,
+// real: storage of the right-hand side read from S; rout: storage of the
+// solution written to Sout (in place when both are the same array).
+template <typename real, typename acc, int M, int SEL, typename rout = real>
+__global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
                                                      const int* __restrict__ list, int nlist,
-                                                     double lam_thr) {
+                                                     double lam_thr, cpx<rout>* Sout) {
   constexpr int P = 8;            // parts per mode
   constexpr int N = P * M;        // rows
   constexpr int MW = 16;          // modes per wave
@@ -451,6 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
   const acc lamz = (acc)g.kzz[kz];
   const bool zero_first = (kx == 0 && kz == 0 && p == 0);
   cpx<real>* s = S + mv + (size_t)(p * M) * stride;
+  cpx<rout>* so = Sout + mv + (size_t)(p * M) * stride;
 
   acc yr[M], yi[M], cp[M], sp[M];
 #pragma unroll
@@ -554,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
   for (int r = M - 1; r >= 0; r--) {
     const acc xr = (yr[r] - sp[r] * xar) - cp[r] * xnr;
     const acc xi = (yi[r] - sp[r] * xai) - cp[r] * xni;
-    if (valid) s[(size_t)r * stride] = cmk((real)xr, (real)xi);
+    if (valid) so[(size_t)r * stride] = cmk((rout)xr, (rout)xi);
     xnr = xr;
     xni = xi;
   }
@@ -567,14 +579,19 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* __res
 // MODE 0: out = phi (rhs_p), MODE 1: out = phi/dt (P_interim, with the zero
 // plane nzg that the divergence leaves), rk1: P = phi/(dt*8/15).
 // ---------------------------------------------------------------------------
-// Mixed-precision refinement modes: MODE 3: out = phi_in + phi (correction
-// added to the FP64 potential), MODE 4: out = (phi_in + phi)/dt (P_interim
-// from the corrected potential), rk1: P = (phi_in + phi)/(dt*8/15).
+// Refinement modes, the potential kept as the first solve's phi1 plus a
+// correction phic (their FP64 sum would round phi1's low bits away, and the
+// rounding of a large smooth potential, amplified by D G ~ 1/dy^2, bounds
+// max|div| far above the velocity's own rounding):
+//   MODE 3: out = phi_in + phi (correction accumulated),
+//   MODE 5: out2 = phic' = phi_in2 + phi (phi_in2 null: phi), out =
+//           P_interim = (phi_in + phic')/dt; rk1: P = (phi_in + phic')/(dt*8/15).
 template <typename real, int N, int TK, int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __restrict__ S,
                                                   const cpx<real>* __restrict__ tw,
                                                   double* out, double* __restrict__ P,
-                                                  double dt, int rk1, const double* phi_in) {
+                                                  double dt, int rk1, const double* phi_in,
+                                                  const double* phi_in2, double* out2) {
   constexpr int NP = TK / 2;
   constexpr int NT = kBlock / NP;
   constexpr int NH = N / 2 + 1;
@@ -675,19 +692,30 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
         if ((t & 1) == 0) zpair[p] = buf[(t >> 1) * LP + lp(n)];
         z = zpair[p];
         double phi = (double)((t & 1) ? z.y : z.x) / norm;
-        if constexpr (MODE == 3 || MODE == 4) phi = phi_in[ridx(2, j, k, nxg, nyg) + n] + phi;
-        const double o = (MODE == 1 || MODE == 4) ? phi / dt : phi;
+        const size_t at = ridx(2, j, k, nxg, nyg) + n;
+        double corr = 0.0;  // MODE 5: the new correction phic'
+        if constexpr (MODE == 3) phi = phi_in[at] + phi;
+        if constexpr (MODE == 5) {
+          corr = phi_in2 ? phi_in2[at] + phi : phi;
+          phi = phi_in[at] + corr;
+        }
+        const double o = (MODE == 1 || MODE == 5) ? phi / dt : phi;
         __builtin_nontemporal_store(o, &orow[n]);
+        if constexpr (MODE == 5) out2[at] = corr;
         if (rk1) prow[n] = phi / cP;
         if (n == 0) {  // x duplicate i = nxg-1 (offset nxp)
           orow[N] = o;
+          if constexpr (MODE == 5) out2[at + N] = corr;
           if (rk1) prow[N] = phi / cP;
         }
         if (dupz) {
+          const size_t at2 = ridx(2, j, kdup, nxg, nyg) + n;
           orow2[n] = o;
+          if constexpr (MODE == 5) out2[at2] = corr;
           if (rk1) prow2[n] = phi / cP;
           if (n == 0) {
             orow2[N] = o;
+            if constexpr (MODE == 5) out2[at2 + N] = corr;
             if (rk1) prow2[N] = phi / cP;
           }
         }
@@ -695,7 +723,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
     }
   }
   (void)idup;
-  if ((MODE == 1 || MODE == 4) && k0 == 0) {
+  if ((MODE == 1 || MODE == 5) && k0 == 0) {
     // rhs_p plane nzg is zero after the divergence; P_interim = 0/dt there.
     for (int i = 2 + tid; i <= nxg - 1; i += kBlock) out[ridx(i, j, g.nzg, nxg, nyg)] = 0.0;
   }
@@ -703,77 +731,6 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
     // plane nzg keeps its input, but the x-periodic copy (src/poisson.f90:110)
     // also covers it: rhs(nxg-1, j, nzg) = rhs(2, j, nzg).
     out[ridx(idup, j, g.nzg, nxg, nyg)] = out[ridx(2, j, g.nzg, nxg, nyg)];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// FP64 residual of the projection's discrete Poisson problem for the mixed
-// path: r = b - D(G(phi)) on the unique cells (i = 2..nxp+1, j = 2..nyg-1,
-// k = 2..nzp+1), stored in FP32.  b is the FP64 divergence of the interim
-// velocity and D, G are the staggered divergence and gradient with the
-// projection's index ranges: periodic in x and z (the face left of cell 2 is
-// the periodic image of face nx-1, resp. nz-1), no gradient on the wall
-// faces (j = 1, ny).  phi carries its periodic duplicates (i = nxg-1,
-// k = nzp+2).  |r| maxima per block go to rmax as double bits.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, const double* __restrict__ b,
-                                                     const double* __restrict__ phi,
-                                                     const double* __restrict__ gauge,
-                                                     float* __restrict__ r,
-                                                     unsigned long long* rmax) {
-  // one lane per (i, j) walking kGradKC z planes with the z neighbours of
-  // phi kept in registers
-  unsigned bx, by, bz;
-  xcd_tile(bx, by, bz);
-  const int i = bx * 64 + (threadIdx.x & 63) + 2;
-  const int j = by * 4 + (threadIdx.x >> 6) + 2;
-  const int kb = bz * kGradKC + 2;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg;
-  const int nzp = g.nzp;
-  double amax = 0.0;
-  if (i <= nx - 1 && j <= nyg - 1) {
-    const int im = (i == 2) ? nx - 1 : i - 1;  // periodic image of cell 1
-    const double rx = g.rdxf[i], ry = g.rdyf[j];
-    // the residual is stored in FP32: tabulated reciprocals of the spacings
-    // instead of divisions (within an ulp of FP64, far below FP32)
-    const double rhxm = g.rdxg[(i == 2) ? nx - 1 : i - 1], rhx = g.rdxg[i];
-    const double rhym = g.rdyg[j - 1], rhy = g.rdyg[j];
-    const double gj = (j == 2) ? *gauge : 0.0;  // (0,0)-mode row of the spectral operator
-    const int km0 = (kb == 2) ? nz - 1 : kb - 1;
-    double pm = phi[ridx(i, j, km0, nxg, nyg)];
-    double p0 = phi[ridx(i, j, kb, nxg, nyg)];
-#pragma unroll
-    for (int kk = 0; kk < kGradKC; kk++) {
-      const int k = kb + kk;
-      if (k > nzp + 1) break;
-      const int km = (k == 2) ? nz - 1 : k - 1;
-      const double pp = phi[ridx(i, j, k + 1, nxg, nyg)];
-      const double gxp = (phi[ridx(i + 1, j, k, nxg, nyg)] - p0) * rhx;
-      const double gxm = (p0 - phi[ridx(im, j, k, nxg, nyg)]) * rhxm;
-      const double gyp = (j <= ny - 1) ? (phi[ridx(i, j + 1, k, nxg, nyg)] - p0) * rhy : 0.0;
-      const double gym = (j >= 3) ? (p0 - phi[ridx(i, j - 1, k, nxg, nyg)]) * rhym : 0.0;
-      const double gzp = (pp - p0) * g.rdzg[k];
-      const double gzm = (p0 - pm) * g.rdzg[km];
-      const double lap = (gxp - gxm) * rx + (gyp - gym) * ry + (gzp - gzm) * g.rdzf[k];
-      const double res = (b[ridx(i, j, k, nxg, nyg)] - lap) - gj;
-      r[ridx(i, j, k, nxg, nyg)] = (float)res;
-      amax = fmax(amax, fabs(res));
-      pm = p0;
-      p0 = pp;
-    }
-  }
-  // block max, one atomic per block into one of 64 slots (linear workgroup
-  // index % 64) so that the ~1e4 workgroups do not serialise on one
-  // address; the host takes the max of the slots
-  __shared__ double sm[kBlock / 64];
-  for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = amax;
-  __syncthreads();
-  if (threadIdx.x == 0 && rmax) {
-    double m = sm[0];
-    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, sm[w]);
-    const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    atomicMax(rmax + (lin & 63), (unsigned long long)__double_as_longlong(m));
   }
 }
 
@@ -833,12 +790,16 @@ __global__ void k_pghost_z(DevGrid g, double* __restrict__ P) {
 // ---------------------------------------------------------------------------
 // Gradient correction: U = U - G(dt*Pint) on the index ranges of
 // src/differential_operators.f90:28-50 (faces outside them keep U).
+// SPLIT: U = U - (G(phi1) + G(phic)) for a potential kept as two parts (the
+// refinement's phi1 + phic); Pint is phi1 then, and each part's differences
+// are formed before they are added, so phic is not rounded into phi1.
 // ---------------------------------------------------------------------------
-template <int KC>
+template <int KC, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __restrict__ U,
                                                          double* __restrict__ V,
                                                          double* __restrict__ W,
                                                          const double* __restrict__ Pint,
+                                                         const double* __restrict__ phic,
                                                          double dt) {
   // one lane per (i, j), walking kGradKC consecutive z planes so that
   // p(k+1) of one plane is p(k) of the next; XCD-aware tile order so the
@@ -852,31 +813,51 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
   if (i > nx - 1 || j > nyg - 1) return;
   const double hx = g.dxg[i];
   const double hy = (j <= ny - 1) ? g.dyg[j] : 1.0;
-  double p = dt * Pint[ridx(i, j, kb, nxg, nyg)];
+  // potential at a cell: dt*Pint (the reference's operand), or the two parts
+  auto pot = [&](size_t at, double& c) -> double {
+    if constexpr (SPLIT) {
+      c = phic[at];
+      return Pint[at];
+    } else {
+      c = 0.0;
+      return dt * Pint[at];
+    }
+  };
+  // difference of the potential between two cells, each part on its own
+  auto dif = [](double a, double ac, double b, double bc) -> double {
+    if constexpr (SPLIT) return (a - b) + (ac - bc);
+    else return a - b;
+  };
+  double pc;
+  double p = pot(ridx(i, j, kb, nxg, nyg), pc);
   bool bad = false;
 #pragma unroll
   for (int kk = 0; kk < KC; kk++) {
     const int k = kb + kk;
     if (k > nzg - 1) break;
-    const double px = dt * Pint[ridx(i + 1, j, k, nxg, nyg)];
+    double pxc;
+    const double px = pot(ridx(i + 1, j, k, nxg, nyg), pxc);
     const size_t iu = fidx(i, j, k, nx, nyg);
-    const double u = U[iu] - (px - p) / hx;
+    const double u = U[iu] - dif(px, pxc, p, pc) / hx;
     U[iu] = u;
     bad |= !isfinite(u);
     if (j <= ny - 1) {
-      const double py = dt * Pint[ridx(i, j + 1, k, nxg, nyg)];
+      double pyc;
+      const double py = pot(ridx(i, j + 1, k, nxg, nyg), pyc);
       const size_t iv = fidx(i, j, k, nxg, ny);
-      const double v = V[iv] - (py - p) / hy;
+      const double v = V[iv] - dif(py, pyc, p, pc) / hy;
       V[iv] = v;
       bad |= !isfinite(v);
     }
     if (k <= nz - 1) {
-      const double pz = dt * Pint[ridx(i, j, k + 1, nxg, nyg)];
+      double pzc;
+      const double pz = pot(ridx(i, j, k + 1, nxg, nyg), pzc);
       const size_t iw = fidx(i, j, k, nxg, nyg);
-      const double w = W[iw] - (pz - p) / g.dzg[k];
+      const double w = W[iw] - dif(pz, pzc, p, pc) / g.dzg[k];
       W[iw] = w;
       bad |= !isfinite(w);
       p = pz;
+      pc = pzc;
     }
   }
   if (bad && g.nonfinite) atomicOr(g.nonfinite, 1u);
@@ -1125,17 +1106,42 @@ hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, co
   return hipGetLastError();
 }
 
-hipError_t launch_fwd_x_resid32(const DevGrid& g, const double* b, const double* phi, double* gauge,
-                                cpx<float>* S, unsigned long long* rmax, hipStream_t s) {
+namespace {
+template <typename real>
+hipError_t fwd_x_resid(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                       const cpx<real>* tw, cpx<real>* S, unsigned long long* rmax, double* rout,
+                       hipStream_t s) {
   const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   // gauge[0] = result, gauge[1 .. nzp] = row sums
   hipLaunchKernelGGL(k_gauge_rows, dim3(g.nzp), dim3(kBlock), 0, s, g, phi, gauge + 1);
   hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(kBlock), 0, s, g, (const double*)(gauge + 1), gauge);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
-    hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
-                       nullptr, nullptr, (const void*)b, g.twx32, S, nullptr,
-                       (const double*)gauge, rmax);
+    hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
+                       nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
+  }));
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_fwd_x_resid32(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                                cpx<float>* S, unsigned long long* rmax, double* rout, hipStream_t s) {
+  return fwd_x_resid<float>(g, b, phi, gauge, g.twx32, S, rmax, rout, s);
+}
+
+hipError_t launch_fwd_x_resid64(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                                cpx<double>* S, unsigned long long* rmax, double* rout,
+                                hipStream_t s) {
+  return fwd_x_resid<double>(g, b, phi, gauge, g.twx64, S, rmax, rout, s);
+}
+
+hipError_t launch_fwd_x_b64(const DevGrid& g, const double* U, const double* V, const double* W,
+                            double* b, cpx<double>* S, hipStream_t s) {
+  const int tk = tile_rows(g.nxp, g.nzp);
+  dim3 grid(g.nzp / tk, g.nym);
+  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
+    hipLaunchKernelGGL((k_fwd_x<double, NN, TK, kSrcUVWB>), grid, dim3(kBlock), 0, s, g, U, V, W,
+                       (const void*)nullptr, g.twx64, S, b, nullptr, nullptr);
   }));
   return hipGetLastError();
 }
@@ -1196,7 +1202,7 @@ hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t 
     return hipGetLastError();
   }
   C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<double, double, MM, kTriAll>), gridp,
-                                           dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0));
+                                           dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0, S));
   return hipGetLastError();
 }
 
@@ -1209,72 +1215,72 @@ hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlo
   const dim3 grid((unsigned)((nm + MB - 1) / MB));
   if (!low) {  // FP64 arithmetic on every mode
     C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll>), grid,
-                                             dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0));
+                                             dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0, S));
     return hipGetLastError();
   }
   if (nlow <= 0) return hipErrorInvalidValue;  // the (0,0) mode is always a low mode
   const dim3 gridl((unsigned)((nlow + MB - 1) / MB));
   C3D_DISPATCH_M(g.nym, {
     hipLaunchKernelGGL((k_tri_part<float, float, MM, kTriHigh>), grid, dim3(kBlock), 0, s, g, S,
-                       nullptr, 0, lam_thr);
+                       nullptr, 0, lam_thr, S);
     hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriList>), gridl, dim3(kBlock), 0, s, g, S,
-                       low, nlow, 0.0);
+                       low, nlow, 0.0, S);
   });
+  return hipGetLastError();
+}
+
+hipError_t launch_tri32to64(const DevGrid& g, cpx<float>* S32, cpx<double>* S64, hipStream_t s) {
+  const size_t nm = (size_t)g.nxh * g.nzp;
+  constexpr int MB = kBlock / 8;
+  const dim3 grid((unsigned)((nm + MB - 1) / MB));
+  C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll, double>), grid,
+                                           dim3(kBlock), 0, s, g, S32, nullptr, 0, 0.0, S64));
   return hipGetLastError();
 }
 
 hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
                         double dt, bool rk1, hipStream_t s) {
-  const int tk = tile_rows(g.nxp, g.nzp);
-  dim3 grid(g.nzp / tk, g.nym);
-  C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
-    if (mode == 0)
-      hipLaunchKernelGGL((k_inv_x<double, NN, TK, 0>), grid, dim3(kBlock), 0, s, g, S, g.twx64,
-                         out, P, dt, rk1 ? 1 : 0, (const double*)nullptr);
-    else
-      hipLaunchKernelGGL((k_inv_x<double, NN, TK, 1>), grid, dim3(kBlock), 0, s, g, S, g.twx64,
-                         out, P, dt, rk1 ? 1 : 0, (const double*)nullptr);
-  }));
-  return hipGetLastError();
+  if (mode != 0 && mode != 1) return hipErrorInvalidValue;
+  return launch_inv_x64_ref(g, mode, S, nullptr, nullptr, out, nullptr, P, dt, rk1, s);
 }
 
-hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
-                          double* out, double* P, double dt, bool rk1, hipStream_t s) {
+namespace {
+#define C3D_INV_CASE(MD)                                                                          \
+  case MD:                                                                                        \
+    hipLaunchKernelGGL((k_inv_x<real, NN, TK, MD>), grid, dim3(kBlock), 0, s, g, S, tw, out, P, dt, \
+                       rk1 ? 1 : 0, phi_in, phi_in2, out2);                                       \
+    break;
+template <typename real>
+hipError_t inv_x_modes(const DevGrid& g, int mode, const cpx<real>* S, const cpx<real>* tw,
+                       const double* phi_in, const double* phi_in2, double* out, double* out2,
+                       double* P, double dt, bool rk1, hipStream_t s) {
+  if ((mode == 3 && !phi_in) || (mode == 5 && (!phi_in || !out2))) return hipErrorInvalidValue;
   const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
     switch (mode) {
-      case 0:
-        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 0>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
-                           out, P, dt, rk1 ? 1 : 0, phi_in);
-        break;
-      case 1:
-        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 1>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
-                           out, P, dt, rk1 ? 1 : 0, phi_in);
-        break;
-      case 3:
-        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 3>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
-                           out, P, dt, rk1 ? 1 : 0, phi_in);
-        break;
-      case 4:
-        hipLaunchKernelGGL((k_inv_x<float, NN, TK, 4>), grid, dim3(kBlock), 0, s, g, S, g.twx32,
-                           out, P, dt, rk1 ? 1 : 0, phi_in);
-        break;
+      C3D_INV_CASE(0)
+      C3D_INV_CASE(1)
+      C3D_INV_CASE(3)
+      C3D_INV_CASE(5)
       default: return hipErrorInvalidValue;
     }
   }));
   return hipGetLastError();
 }
+#undef C3D_INV_CASE
+}  // namespace
 
-hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
-                           float* r, unsigned long long* rmax, hipStream_t s) {
-  // gauge[0] = result, gauge[1 .. nzp] = row sums
-  hipLaunchKernelGGL(k_gauge_rows, dim3(g.nzp), dim3(kBlock), 0, s, g, phi, gauge + 1);
-  hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(kBlock), 0, s, g, (const double*)(gauge + 1), gauge);
-  hipLaunchKernelGGL(k_residual,
-                     grid_pts(g, g.nxp, g.nyg - 2, (g.nzp + kGradKC - 1) / kGradKC),
-                     dim3(kBlock), 0, s, g, b, phi, (const double*)gauge, r, rmax);
-  return hipGetLastError();
+hipError_t launch_inv_x64_ref(const DevGrid& g, int mode, const cpx<double>* S, const double* phi_in,
+                              const double* phi_in2, double* out, double* out2, double* P, double dt,
+                              bool rk1, hipStream_t s) {
+  return inv_x_modes<double>(g, mode, S, g.twx64, phi_in, phi_in2, out, out2, P, dt, rk1, s);
+}
+
+hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
+                          const double* phi_in2, double* out, double* out2, double* P, double dt,
+                          bool rk1, hipStream_t s) {
+  return inv_x_modes<float>(g, mode, S, g.twx32, phi_in, phi_in2, out, out2, P, dt, rk1, s);
 }
 
 hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s) {
@@ -1296,18 +1302,40 @@ static int grad_kc() {
   return kc;
 }
 
-hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
-                               double dt, hipStream_t s) {
+namespace {
+template <bool SPLIT>
+hipError_t grad_correct_any(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
+                            const double* phic, double dt, hipStream_t s) {
   const int kc = grad_kc();
   const dim3 grid = grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kc - 1) / kc);
   switch (kc) {
-    case 1: hipLaunchKernelGGL(k_grad_correct<1>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
-    case 4: hipLaunchKernelGGL(k_grad_correct<4>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
-    case 8: hipLaunchKernelGGL(k_grad_correct<8>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
-    case 16: hipLaunchKernelGGL(k_grad_correct<16>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
-    default: hipLaunchKernelGGL(k_grad_correct<2>, grid, dim3(kBlock), 0, s, g, U, V, W, Pint, dt); break;
+#define C3D_GC_CASE(KC)                                                                             \
+  case KC:                                                                                          \
+    hipLaunchKernelGGL((k_grad_correct<KC, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint, phic, \
+                       dt);                                                                         \
+    break;
+    C3D_GC_CASE(1)
+    C3D_GC_CASE(4)
+    C3D_GC_CASE(8)
+    C3D_GC_CASE(16)
+#undef C3D_GC_CASE
+    default:
+      hipLaunchKernelGGL((k_grad_correct<2, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint, phic,
+                         dt);
+      break;
   }
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
+                               double dt, hipStream_t s) {
+  return grad_correct_any<false>(g, U, V, W, Pint, nullptr, dt, s);
+}
+
+hipError_t launch_grad_correct_split(const DevGrid& g, double* U, double* V, double* W,
+                                     const double* phi1, const double* phic, hipStream_t s) {
+  return grad_correct_any<true>(g, U, V, W, phi1, phic, 0.0, s);
 }
 
 hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s) {

@@ -63,18 +63,26 @@ hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t 
 // mode 1: out = P_interim = solution/dt (projection).  rk1: also write P.
 hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
                         double dt, bool rk1, hipStream_t s);
-// Mixed-precision projection pieces (FP32 spectral storage, FP64 tridiagonal
-// arithmetic, FP64 refinement).  launch_fwd_x32 with U != null: divergence of
-// U, V, W (FP64, also stored to b) -> FP32 spectrum; with U == null: the FP32
-// residual r -> FP32 spectrum.
+// Refinement pieces (mixed precision: FP32 forward spectra; FP64: the same
+// sweeps on FP64 spectra).  launch_fwd_x32 with U != null: divergence of
+// U, V, W (FP64, also stored to b when b != null) -> FP32 spectrum; with
+// U == null: the FP32 array r -> FP32 spectrum.
 hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
                           double* b, const float* r, cpx<float>* S, hipStream_t s);
-// The refinement residual r = b - M phi of launch_residual, formed on the fly
-// and rounded to FP32 as the transform's input (same values as
-// launch_residual followed by launch_fwd_x32 of r): the FP32 spectrum of r,
-// max |r| into rmax[0..64) as launch_residual, gauge as its scratch.
+// FP64 spectrum of div(U, V, W), the FP64 divergence also stored to b.
+hipError_t launch_fwd_x_b64(const DevGrid& g, const double* U, const double* V, const double* W,
+                            double* b, cpx<double>* S, hipStream_t s);
+// The refinement residual r = b - M phi formed on the fly as the forward
+// transform's input, M = D G on the unique cells plus the spectral solver's
+// (0,0)-mode gauge row (gauge: nzp + 1 doubles of scratch).  The FP32 variant
+// transforms r rounded to FP32.  max |r| as double bits into rmax[0..64):
+// workgroup w maxes into slot w % 64 (zeroed by the caller; the max over the
+// slots is max |r|).  rout (may be null) receives the FP64 residual.
 hipError_t launch_fwd_x_resid32(const DevGrid& g, const double* b, const double* phi, double* gauge,
-                                cpx<float>* S, unsigned long long* rmax, hipStream_t s);
+                                cpx<float>* S, unsigned long long* rmax, double* rout, hipStream_t s);
+hipError_t launch_fwd_x_resid64(const DevGrid& g, const double* b, const double* phi, double* gauge,
+                                cpx<double>* S, unsigned long long* rmax, double* rout,
+                                hipStream_t s);
 hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t s);
 // Tridiagonal solve of the FP32 spectrum (tri_part_supported sizes).  low ==
 // null: FP64 arithmetic on every mode.  Else FP32 arithmetic on the modes
@@ -82,21 +90,29 @@ hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t
 // low[] (the kx*nzp + kz indices of exactly the other modes).
 hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlow, double lam_thr,
                         hipStream_t s);
-// mode 0: out = phi, 1: out = phi/dt, 3: out = phi_in + phi,
-// 4: out = (phi_in + phi)/dt; rk1 writes P from the same potential.
+// The tridiagonal solve of the FP32 spectrum S32 with FP64 arithmetic on
+// every mode, the solution stored in FP64 to S64 (tri_part_supported sizes).
+hipError_t launch_tri32to64(const DevGrid& g, cpx<float>* S32, cpx<double>* S64, hipStream_t s);
+// Inverse x transform with the refinement modes, the potential kept as the
+// first solve's phi1 plus a correction phic:
+//   0: out = phi, 1: out = phi/dt, 3: out = phi_in + phi,
+//   5: out2 = phic' = phi_in2 + phi (phi_in2 may be null: phic' = phi),
+//      out = (phi_in + phic')/dt (P_interim);
+// rk1 writes P from the same potential.
+hipError_t launch_inv_x64_ref(const DevGrid& g, int mode, const cpx<double>* S, const double* phi_in,
+                              const double* phi_in2, double* out, double* out2, double* P, double dt,
+                              bool rk1, hipStream_t s);
 hipError_t launch_inv_x32(const DevGrid& g, int mode, const cpx<float>* S, const double* phi_in,
-                          double* out, double* P, double dt, bool rk1, hipStream_t s);
-// r = b - M phi (FP64, stored FP32) on the unique cells, M = D G plus the
-// spectral solver's (0,0)-mode gauge row; gauge is nzp + 1 doubles of scratch;
-// max |r| as double bits into rmax[0..64): workgroup w maxes into slot w % 64
-// (may be null; zeroed by the caller; the max over the slots is max |r|).
-hipError_t launch_residual(const DevGrid& g, const double* b, const double* phi, double* gauge,
-                           float* r, unsigned long long* rmax, hipStream_t s);
+                          const double* phi_in2, double* out, double* out2, double* P, double dt,
+                          bool rk1, hipStream_t s);
 // Ghost copies of P (src/poisson.f90:93-98), in the reference order.
 hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s);
 // U -= G(dt * Pint) on the gradient's index ranges.
 hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
                                double dt, hipStream_t s);
+// U -= G(phi1) + G(phic), each part differenced on its own.
+hipError_t launch_grad_correct_split(const DevGrid& g, double* U, double* V, double* W,
+                                     const double* phi1, const double* phic, hipStream_t s);
 // apply_boundary_conditions on device-resident staggered fields.
 hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s);
 // Divergence into an rhs_p-shaped array (plane nzg zero).

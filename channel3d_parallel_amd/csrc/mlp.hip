@@ -3,9 +3,7 @@
 //
 // GEMMs run on the matrix cores: v_mfma_f32_16x16x16_bf16 with BF16 operands
 // (round-to-nearest-even from the FP32 activations and master weights) and
-// FP32 accumulation.  One wavefront computes one 16x16 output tile, taking
-// its operands straight from global memory (the matrices are at most
-// 24,576 x 256, so the GEMMs are latency- rather than throughput-bound).
+// FP32 accumulation, on 64 x 64 output tiles staged through LDS.
 //
 // MFMA 16x16x16 operand layout (lane l, r = l % 16, q = l / 16):
 //   A: 4 values A(row r, k = 4q .. 4q+3)      B: 4 values B(k = 4q .. 4q+3, col r)
@@ -36,45 +34,86 @@ __device__ __forceinline__ short bf16_bits(float x) {
 //   op(B)(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n]
 // Epilogues: kEpiNone, kEpiBias (+bias[n]), kEpiBiasRelu, kEpiMask
 // (C *= (H(m,n) > 0), the ReLU derivative of the forward activation H).
-template <bool TA, bool TB, int EPI>
+//
+// A workgroup computes a T x T tile of C (T = 64, or 32 for products too
+// small to give every CU a 64 x 64 tile); its 4 wavefronts each own a
+// T/2 x T/2 quarter as (T/32) x (T/32) MFMA tiles.  K advances in slices of
+// 32: the workgroup converts the T x 32 slices of op(A) and op(B) to BF16
+// into LDS once (row-major [m][k] and [n][k], so both operand fragments are
+// 4 consecutive k of one row), and every wavefront then reads its fragments
+// from LDS.
+constexpr int kTK = 32;
+constexpr int kTKP = kTK + 4;  // LDS row pitch in BF16 elements (72 bytes)
+
+template <bool TA, bool TB, int EPI, int T>
 __global__ __launch_bounds__(kBlock) void k_gemm(int M, int N, int K, const float* __restrict__ A,
                                                  int lda, const float* __restrict__ B, int ldb,
                                                  float* __restrict__ C, int ldc,
                                                  const float* __restrict__ bias,
                                                  const float* __restrict__ H, int ldh) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int tilesN = (N + 15) / 16;
-  const int tile = blockIdx.x * (kBlock / 64) + wave;
-  const int tm = tile / tilesN, tn = tile % tilesN;
-  if (tm * 16 >= M) return;  // whole wavefront leaves together
+  constexpr int F = T / 32;  // MFMA tiles per wavefront along m and along n
+  __shared__ __align__(16) short As[T * kTKP];
+  __shared__ __align__(16) short Bs[T * kTKP];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int m0 = blockIdx.x * T, n0 = blockIdx.y * T;
+  const int wm = (wave >> 1) * (T / 2), wn = (wave & 1) * (T / 2);
   const int r = lane & 15, q = lane >> 4;
-  const int am = tm * 16 + r;  // row of A this lane feeds
-  const int bn = tn * 16 + r;  // column of B this lane feeds
-  float4_t acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    short4_t a, b;
+  float4_t acc[F][F];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int k = k0 + 4 * q + e;
-      float av = 0.f, bv = 0.f;
-      if (am < M && k < K) av = TA ? A[(size_t)k * lda + am] : A[(size_t)am * lda + k];
-      if (bn < N && k < K) bv = TB ? B[(size_t)bn * ldb + k] : B[(size_t)k * ldb + bn];
-      a[e] = bf16_bits(av);
-      b[e] = bf16_bits(bv);
+  for (int i = 0; i < F; i++)
+#pragma unroll
+    for (int j = 0; j < F; j++) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += kTK) {
+    // the slices, walked along the operand's contiguous index in memory
+#pragma unroll
+    for (int e = tid; e < T * kTK; e += kBlock) {
+      const int mm = TA ? e % T : e / kTK, kk = TA ? e / T : e % kTK;
+      const int m = m0 + mm, k = k0 + kk;
+      float v = 0.f;
+      if (m < M && k < K) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+      As[mm * kTKP + kk] = bf16_bits(v);
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int e = tid; e < T * kTK; e += kBlock) {
+      const int nn = TB ? e / kTK : e % T, kk = TB ? e % kTK : e / T;
+      const int n = n0 + nn, k = k0 + kk;
+      float v = 0.f;
+      if (n < N && k < K) v = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
+      Bs[nn * kTKP + kk] = bf16_bits(v);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kTK; ks += 16) {
+      short4_t a[F], b[F];
+#pragma unroll
+      for (int i = 0; i < F; i++)
+        a[i] = *reinterpret_cast<const short4_t*>(&As[(wm + 16 * i + r) * kTKP + ks + 4 * q]);
+#pragma unroll
+      for (int j = 0; j < F; j++)
+        b[j] = *reinterpret_cast<const short4_t*>(&Bs[(wn + 16 * j + r) * kTKP + ks + 4 * q]);
+#pragma unroll
+      for (int i = 0; i < F; i++)
+#pragma unroll
+        for (int j = 0; j < F; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();  // the slices are overwritten next
   }
 #pragma unroll
-  for (int e = 0; e < 4; e++) {
-    const int m = tm * 16 + 4 * q + e, n = tn * 16 + r;
-    if (m < M && n < N) {
-      float v = acc[e];
-      if (EPI == kEpiBias || EPI == kEpiBiasRelu) v += bias[n];
-      if (EPI == kEpiBiasRelu) v = fmaxf(v, 0.f);
-      if (EPI == kEpiMask) v = (H[(size_t)m * ldh + n] > 0.f) ? v : 0.f;
-      C[(size_t)m * ldc + n] = v;
-    }
-  }
+  for (int i = 0; i < F; i++)
+#pragma unroll
+    for (int j = 0; j < F; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int m = m0 + wm + 16 * i + 4 * q + e, n = n0 + wn + 16 * j + r;
+        if (m < M && n < N) {
+          float v = acc[i][j][e];
+          if (EPI == kEpiBias || EPI == kEpiBiasRelu) v += bias[n];
+          if (EPI == kEpiBiasRelu) v = fmaxf(v, 0.f);
+          if (EPI == kEpiMask) v = (H[(size_t)m * ldh + n] > 0.f) ? v : 0.f;
+          C[(size_t)m * ldc + n] = v;
+        }
+      }
 }
 
 // Mean-squared error over n x 4 outputs: dY = 2 (Y - T) / (4n); per-block
@@ -99,13 +138,26 @@ __global__ __launch_bounds__(kBlock) void k_mse(const float* __restrict__ Y, con
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// db[n] = sum over rows of dY(m, n), fixed order.
-__global__ void k_colsum(const float* __restrict__ dY, int M, int N, float* __restrict__ db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// db[n] = sum over rows of dY(m, n), fixed order: a workgroup owns
+// kColN columns; its kColR row slices each sum rows m = slice, slice + kColR,
+// ... (consecutive lanes read consecutive columns), then the slices are
+// added pairwise in LDS.
+constexpr int kColN = 16, kColR = kBlock / kColN;
+__global__ __launch_bounds__(kBlock) void k_colsum(const float* __restrict__ dY, int M, int N,
+                                                   float* __restrict__ db) {
+  __shared__ float sh[kColR][kColN];
+  const int c = threadIdx.x % kColN, r = threadIdx.x / kColN;
+  const int n = blockIdx.x * kColN + c;
   float s = 0.f;
-  for (int m = 0; m < M; m++) s += dY[(size_t)m * N + n];
-  db[n] = s;
+  if (n < N)
+    for (int m = r; m < M; m += kColR) s += dY[(size_t)m * N + n];
+  sh[r][c] = s;
+  __syncthreads();
+  for (int h = kColR / 2; h > 0; h >>= 1) {
+    if (r < h) sh[r][c] += sh[r + h][c];
+    __syncthreads();
+  }
+  if (r == 0 && n < N) db[n] = sh[0][c];
 }
 
 // Adam (FP32), bias-corrected step size lr_t passed in.
@@ -153,10 +205,18 @@ __global__ void k_apply(int nb, const float* __restrict__ Y, const float* __rest
 template <bool TA, bool TB, int EPI>
 hipError_t gemm(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
                 int ldc, const float* bias, const float* H, int ldh, hipStream_t s) {
-  const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
-  const int blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
-  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(blocks), dim3(kBlock), 0, s, M, N, K, A, lda, B,
-                     ldb, C, ldc, bias, H, ldh);
+  // 64 x 64 tiles when they give at least one workgroup per CU (256), else
+  // 32 x 32 (the training batches: 256 x 256 products)
+  const long long big = (long long)((M + 63) / 64) * ((N + 63) / 64);
+  if (big >= 256) {
+    const dim3 grid((M + 63) / 64, (N + 63) / 64);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 64>), grid, dim3(kBlock), 0, s, M, N, K, A, lda, B, ldb,
+                       C, ldc, bias, H, ldh);
+  } else {
+    const dim3 grid((M + 31) / 32, (N + 31) / 32);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 32>), grid, dim3(kBlock), 0, s, M, N, K, A, lda, B, ldb,
+                       C, ldc, bias, H, ldh);
+  }
   return hipGetLastError();
 }
 
@@ -210,8 +270,8 @@ hipError_t backward(const Layout& L, const float* P, const float* X, int n,
     hipError_t e = gemm<true, false, kEpiNone>(L.out[i], L.in[i], n, d, L.out[i], in, L.in[i],
                                                G + L.w[i], L.in[i], nullptr, nullptr, 0, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_colsum, dim3((L.out[i] + 63) / 64), dim3(64), 0, s, d, n, L.out[i],
-                       G + L.b[i]);
+    hipLaunchKernelGGL(k_colsum, dim3((L.out[i] + kColN - 1) / kColN), dim3(kBlock), 0, s, d, n,
+                       L.out[i], G + L.b[i]);
     if (i > 0) {
       e = gemm<false, false, kEpiMask>(n, L.in[i], L.out[i], d, L.out[i], P + L.w[i], L.in[i],
                                        dH[i - 1], L.in[i], nullptr, act[i - 1], L.in[i], s);

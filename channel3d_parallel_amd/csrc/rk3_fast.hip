@@ -9,6 +9,8 @@
 // X_s = X_0 + dt sum_l a_sl F_l into a second set of fields (the stage's
 // input is still read by the neighbouring workgroups' halos, so the update
 // cannot be in place).  F_s is stored for the later stages that use it.
+#include <cstdlib>
+
 #include "kernels.hpp"
 #include "rk3_point.hpp"
 
@@ -24,8 +26,11 @@ constexpr int kBf = 256;
 constexpr double kC11 = 8.0 / 15.0, kC21 = 1.0 / 4.0, kC22 = 5.0 / 12.0, kC31 = 1.0 / 4.0,
                  kC33 = 3.0 / 4.0;
 
-template <int KC, int STAGE>
-__global__ __launch_bounds__(kBf) void k_rhs_stage(DevGrid g, RkStage a) {
+// PIPE: the loads of plane k+2 are issued before plane k is evaluated
+// (software pipelining through registers); otherwise each plane is loaded
+// into LDS right before the barrier that precedes its use.
+template <int KC, int STAGE, bool PIPE>
+__device__ __forceinline__ void rhs_stage_body(const DevGrid& g, const RkStage& a) {
   __shared__ double tile[3 * 3 * kTP];
   unsigned bx, by, bz;
   xcd_tile(bx, by, bz);
@@ -33,15 +38,22 @@ __global__ __launch_bounds__(kBf) void k_rhs_stage(DevGrid g, RkStage a) {
   const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   TileAcc A{tile, i0, j0, kb};
-  auto stage = [&](int k) { stage_tile_plane(g, a.U, a.V, a.W, tile, A, k, kBf); };
-  stage(kb - 1);
-  stage(kb);
+  // planes kb-1 and kb in LDS, kb+1 in registers; with PIPE, at plane k the
+  // loads of plane k+2 are in flight while k is evaluated
+  PlaneRegs nxt;
+  load_plane(g, a.U, a.V, a.W, A, kb - 1, nxt);
+  store_plane(tile, A, kb - 1, nxt);
+  load_plane(g, a.U, a.V, a.W, A, kb, nxt);
+  store_plane(tile, A, kb, nxt);
+  if (PIPE) load_plane(g, a.U, a.V, a.W, A, kb + 1, nxt);
   const double dt = a.dt;
   for (int kk = 0; kk < KC; kk++) {
     const int k = kb + kk;
     if (k > nzg - 1) break;  // uniform across the workgroup
-    stage(k + 1);
+    if (!PIPE) load_plane(g, a.U, a.V, a.W, A, k + 1, nxt);
+    store_plane(tile, A, k + 1, nxt);  // its slot held plane k-2, done with last iteration
     __syncthreads();
+    if (PIPE && kk + 1 < KC && k + 1 <= nzg - 1) load_plane(g, a.U, a.V, a.W, A, k + 2, nxt);
     if (i <= nxg - 1 && j <= nyg - 1) {
       if (i <= nx - 1) {  // U(2:nx-1, 2:nyg-1, 2:nzg-1)
         const size_t f = (size_t)(i - 2) + (size_t)(nx - 2) * ((size_t)(j - 2) + (size_t)(nyg - 2) * (size_t)(k - 2));
@@ -81,7 +93,43 @@ __global__ __launch_bounds__(kBf) void k_rhs_stage(DevGrid g, RkStage a) {
   }
 }
 
+template <int KC, int STAGE, bool PIPE>
+__global__ __launch_bounds__(kBf) void k_rhs_stage(DevGrid g, RkStage a) {
+  rhs_stage_body<KC, STAGE, PIPE>(g, a);
+}
+
+// the pipelined form held to 128 VGPRs (4 waves per SIMD instead of 3, a
+// few spilled registers)
+template <int KC, int STAGE>
+__global__ __launch_bounds__(kBf) __attribute__((amdgpu_waves_per_eu(4))) void k_rhs_stage_w4(
+    DevGrid g, RkStage a) {
+  rhs_stage_body<KC, STAGE, true>(g, a);
+}
+
 inline dim3 gpts_f(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
+
+// Plane loading of the fused kernel: 0 load-then-barrier (default: 5.43 ms
+// per RK3 step at 384x193x384), 1 pipelined (5.61 ms: 3 instead of 4 waves
+// per SIMD), 2 pipelined at 4 waves per SIMD (5.72 ms, spills).
+// C3D_RHS_PIPE overrides the default (tuning knob).
+int rhs_pipe() {
+  static const int v = [] {
+    const char* e = getenv("C3D_RHS_PIPE");
+    const int x = e ? atoi(e) : 0;
+    return (x >= 0 && x <= 2) ? x : 0;
+  }();
+  return v;
+}
+
+template <int STAGE>
+void launch_stage_kernel(const DevGrid& g, const RkStage& a, dim3 grid, hipStream_t s) {
+  constexpr int KC = 8;
+  switch (rhs_pipe()) {
+    case 1: hipLaunchKernelGGL((k_rhs_stage<KC, STAGE, true>), grid, dim3(kBf), 0, s, g, a); break;
+    case 2: hipLaunchKernelGGL((k_rhs_stage_w4<KC, STAGE>), grid, dim3(kBf), 0, s, g, a); break;
+    default: hipLaunchKernelGGL((k_rhs_stage<KC, STAGE, false>), grid, dim3(kBf), 0, s, g, a); break;
+  }
+}
 
 }  // namespace
 
@@ -89,9 +137,9 @@ hipError_t launch_rhs_stage(const DevGrid& g, int stage, const RkStage& a, hipSt
   constexpr int KC = 8;
   const dim3 grid = gpts_f(g.nxg - 2, g.nyg - 2, (g.nzg - 2 + KC - 1) / KC);
   switch (stage) {
-    case 1: hipLaunchKernelGGL((k_rhs_stage<KC, 1>), grid, dim3(kBf), 0, s, g, a); break;
-    case 2: hipLaunchKernelGGL((k_rhs_stage<KC, 2>), grid, dim3(kBf), 0, s, g, a); break;
-    case 3: hipLaunchKernelGGL((k_rhs_stage<KC, 3>), grid, dim3(kBf), 0, s, g, a); break;
+    case 1: launch_stage_kernel<1>(g, a, grid, s); break;
+    case 2: launch_stage_kernel<2>(g, a, grid, s); break;
+    case 3: launch_stage_kernel<3>(g, a, grid, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

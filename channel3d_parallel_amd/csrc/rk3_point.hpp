@@ -35,6 +35,46 @@ struct TileAcc {
   __device__ __forceinline__ double W(int i, int j, int k) const { return at(2, i, j, k); }
 };
 
+// Plane k of U, V, W over the tile of A with its one-cell x/y halo, held in
+// registers: element e = threadIdx.x + q * 256 (q = 0, 1) of the kTP tile
+// cells; cells outside the arrays are zero.  load_plane issues the global
+// loads, store_plane writes them to the plane's ring slot, so a kernel can
+// overlap the loads of a later plane with the work on the current one.
+struct PlaneRegs {
+  double u[2], v[2], w[2];
+};
+static_assert(kTP <= 2 * 256, "two tile cells per thread of a 256-thread workgroup");
+
+__device__ __forceinline__ void load_plane(const DevGrid& g, const double* __restrict__ U,
+                                           const double* __restrict__ V,
+                                           const double* __restrict__ W, const TileAcc& A, int k,
+                                           PlaneRegs& r) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = threadIdx.x + q * 256;
+    const int ii = A.i0 - 1 + e % kTX, jj = A.j0 - 1 + e / kTX;
+    const bool ok = e < kTP && ii >= 1 && jj >= 1 && k >= 1;
+    r.u[q] = (ok && ii <= nx && jj <= nyg && k <= nzg) ? U[fidx(ii, jj, k, nx, nyg)] : 0.0;
+    r.v[q] = (ok && ii <= nxg && jj <= ny && k <= nzg) ? V[fidx(ii, jj, k, nxg, ny)] : 0.0;
+    r.w[q] = (ok && ii <= nxg && jj <= nyg && k <= nz) ? W[fidx(ii, jj, k, nxg, nyg)] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void store_plane(double* tile, const TileAcc& A, int k,
+                                            const PlaneRegs& r) {
+  const int sl = A.slot(k);
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = threadIdx.x + q * 256;
+    if (e < kTP) {
+      tile[(0 * 3 + sl) * kTP + e] = r.u[q];
+      tile[(1 * 3 + sl) * kTP + e] = r.v[q];
+      tile[(2 * 3 + sl) * kTP + e] = r.w[q];
+    }
+  }
+}
+
 // Stages plane k of U, V, W (with the one-cell x/y halo around the tile of
 // A) into its ring slot of the LDS tile; cells outside the arrays are zero.
 __device__ __forceinline__ void stage_tile_plane(const DevGrid& g, const double* __restrict__ U,

@@ -76,14 +76,16 @@ def normalise(X, T):
 
 
 def generate_pairs(nx, ny, nz, nth, nzb, runs, steps, dt, seed=11, max_pairs=None, rng_seed=5,
-                   warm_steps=0):
+                   warm_steps=0, base_mode=0):
     """Normalised features X [n, 54] and force increments T [n, 4] from
     `runs` runs: `warm_steps` RK3 steps past the start-up transient, then
-    `steps` steps whose every IB solve (reference warm start) gives nb pairs."""
+    `steps` steps whose every IB solve gives nb pairs.  The solves start from
+    guess mode `base_mode` (0: the previous force, the base the learned start
+    builds on)."""
     Xs, Ts = [], []
     for run in range(runs):
         s1, s2 = speeds(seed, run)
-        ch, m = make_channel(nx, ny, nz, nth, nzb, s1, s2)
+        ch, m = make_channel(nx, ny, nz, nth, nzb, s1, s2, dt_guess_mode=base_mode)
         with ch:
             for _ in range(warm_steps):
                 ch.compute_time_step_RK3(dt)
@@ -132,36 +134,72 @@ def per_stage(rec, name):
     return {f"stage{st}": it[st - 1::3] for st in (1, 2, 3)}
 
 
-def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000, learned=True):
+def ab_compare(ch, dt, steps, tol=1e-10, maxit=2000, learned=None, stage_ch=None):
     """At every IB projection of `steps` RK3 steps, solve from the zero guess,
-    from the previous force (the reference's warm start), from the learned
-    guess (when `learned`) and from the least-squares history start
-    (guess_mode 4, fed by its own solves of the same states), all from the
-    same state; continue with the zero-guess solution.  Returns per-solve
-    iteration counts and force differences."""
-    names = [("previous", 0)] + ([("learned", 3)] if learned else []) + [("history", 4), ("zero", 1)]
-    rec = {name: [] for name, _ in names}
-    for name, _ in names[:-1]:
-        rec[f"max_force_diff_{name}_vs_zero"] = []
-
-    def variants(c, st):
-        snap = c.get_fields()
-        fb0 = c.get_IB_force()
-        out = {}
-        for name, mode in names:
-            c.set_fields(snap["U"], snap["V"], snap["W"], P=snap["P"])
-            c.set_IB_force(fb0)
-            c.set_IB_solver(tol=tol, maxit=maxit, guess_mode=mode)
-            it, _ = c.compute_IB_projection(dt, st)
-            rec[name].append(it)
-            out[name] = c.get_IB_force()
-        scale = max(np.max(np.abs(out["zero"])), 1e-300)
-        for name, _ in names[:-1]:
-            rec[f"max_force_diff_{name}_vs_zero"].append(
-                float(np.max(np.abs(out[name] - out["zero"])) / scale))
-        return True  # the zero-guess solve ran last: the state continues from it
-
+    from the previous force (the reference's warm start) and from the
+    least-squares history start (guess_mode 4, fed by its own solves of the
+    same states) on `ch`, all from the same state, and continue with the
+    zero-guess solution.  The same-stage start (guess_mode 5 on `stage_ch`,
+    which keeps each stage's force of the previous step) and the learned
+    guess (guess_mode 3 on `learned`, a channel with the trained network)
+    run on channels of their own, which repeat the stage's projection from
+    the same interim velocity, so that the stage forces each keeps are its
+    own.  Returns per-solve iteration counts and force differences against
+    the zero-guess force."""
+    names = [("previous", 0), ("history", 4), ("zero", 1)]
+    sides = [(n, c, m) for n, c, m in (("stage", stage_ch, 5), ("learned", learned, 3)) if c is not None]
+    rec = {n: [] for n, _ in names}
+    rec.update({n: [] for n, _, _ in sides})
+    for n in list(rec):
+        if n != "zero":
+            rec[f"max_force_diff_{n}_vs_zero"] = []
     for _ in range(steps):
         for st in (1, 2, 3):
-            stage(ch, st, dt, before_ib=variants)
+            if st == 1:
+                ch.save_state()
+            ch.compute_rhs(st)
+            ch.rk3_update(st, dt)
+            ch.apply_boundary_conditions()
+            pre = ch.get_fields()  # the interim velocity entering the projections
+            ch.compute_non_IB_projection(dt, st)
+            ch.apply_boundary_conditions()
+            snap = ch.get_fields()
+            fb0 = ch.get_IB_force()
+            out = {}
+            for n, c, m in sides:
+                c.set_fields(pre["U"], pre["V"], pre["W"], P=pre["P"])
+                c.compute_non_IB_projection(dt, st)
+                c.apply_boundary_conditions()
+                c.set_IB_force(fb0)
+                c.set_IB_solver(tol=tol, maxit=maxit, guess_mode=m)
+                it, _ = c.compute_IB_projection(dt, st)
+                rec[n].append(it)
+                out[n] = c.get_IB_force()
+            for n, m in names:  # the zero-guess solve last: the state continues from it
+                ch.set_fields(snap["U"], snap["V"], snap["W"], P=snap["P"])
+                ch.set_IB_force(fb0)
+                ch.set_IB_solver(tol=tol, maxit=maxit, guess_mode=m)
+                it, _ = ch.compute_IB_projection(dt, st)
+                rec[n].append(it)
+                out[n] = ch.get_IB_force()
+            scale = max(np.max(np.abs(out["zero"])), 1e-300)
+            for n in out:
+                if n != "zero":
+                    rec[f"max_force_diff_{n}_vs_zero"].append(
+                        float(np.max(np.abs(out[n] - out["zero"])) / scale))
+            ch.apply_boundary_conditions()
     return rec
+
+
+def side_channels(ch, nx, ny, nz, nth, nzb, s1, s2, isc=None, osc=None):
+    """Channels for ab_compare's per-stage starts, on the body of `ch`: one
+    for guess_mode 5 and, when the scales are given, one with a copy of
+    `ch`'s trained network for guess_mode 3."""
+    stage_ch, _ = make_channel(nx, ny, nz, nth, nzb, s1, s2)
+    learned = None
+    if isc is not None:
+        learned, _ = make_channel(nx, ny, nz, nth, nzb, s1, s2)
+        learned.mlp_init(1)
+        learned.mlp_set_scales(isc, osc)
+        learned.mlp_set_params(ch.mlp_params())
+    return stage_ch, learned

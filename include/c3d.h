@@ -38,7 +38,7 @@ extern "C" {
 #define C3D_ENAN (-5)     /* non-finite value detected */
 
 #define C3D_PREC_FP64 0  /* plain FP64 path */
-#define C3D_PREC_MIXED 1 /* FP32 spectral solve + FP64 iterative refinement;
+#define C3D_PREC_MIXED 1 /* FP32 forward transforms + FP64 refinement;
                             wall-normal length ny-1 in {64, 128, 192} */
 
 typedef struct c3d_ctx c3d_ctx;
@@ -233,14 +233,23 @@ int c3d_set_rhs_exact(c3d_ctx *ctx, int on);
 int c3d_time_projection(c3d_ctx *ctx, int iters, double dt, double *total_ms, double *kernel_ms,
                         int nk);
 int c3d_stage_count(void);
-/* Mixed precision (C3D_PREC_MIXED contexts): number of FP64 refinement
- * sweeps per projection (0..8, default 2).  The first solve and every
- * correction run the x/z transforms on FP32 spectra with FP64 tridiagonal
- * arithmetic; each sweep forms the FP64 residual b - D G phi of the
- * projection's own staggered operator.  No reference counterpart: the
- * reference solves in FP64 only (src/poisson.f90:33-101). */
+/* Refinement sweeps per projection (0..8): C3D_PREC_MIXED contexts default
+ * to 1, C3D_PREC_FP64 contexts to 0 (the reference's single solve).  A first
+ * solve gives the potential phi1; each sweep transforms the FP64 residual
+ * b - M(phi1 + phic) of the projection's own staggered operator (M = D G plus
+ * the spectral solver's (0,0)-mode gauge row) and adds its solution to the
+ * correction phic; the velocity is corrected by G(phi1) + G(phic), the two
+ * parts differenced separately, and P_interim = (phi1 + phic)/dt.  Mixed
+ * passes run the forward x/z transforms on FP32 spectra and the tridiagonal
+ * solution, inverse z and x transforms in FP64 (FP32 throughout if the
+ * environment sets C3D_MIXED_INV64=0 before c3d_create; the default sweep
+ * count is then 2).  On an FP64 context the sweeps are FP64 defect
+ * corrections.  The first call asking for sweeps allocates their device
+ * buffers (3-4 rhs_p-sized FP64 arrays); C3D_ENOMEM if that fails.  No
+ * reference counterpart: the reference solves once in FP64
+ * (src/poisson.f90:33-101). */
 int c3d_set_refinement(c3d_ctx *ctx, int nref);
-/* max |b - D G phi| entering each sweep of the last mixed projection,
+/* max |residual| entering each sweep of the last projection,
  * rmax[0..min(n, nref)); returns the count written. */
 int c3d_refinement_residuals(c3d_ctx *ctx, double *rmax, int n);
 /* Streaming device-to-device copy of nbytes (multiple of 16), `iters`
@@ -370,12 +379,14 @@ int c3d_ib_bicgstab(c3d_ctx *ctx, double *x, const double *b, double tol, int ma
  * cg_max_iter, and the initial guess: guess_mode 0 = the previous force
  * fb (the reference's warm start), 1 = zero, 2 = the force last set by
  * c3d_ib_set_force for the next solve only, then 0, 3 = the previous force
- * plus the learned increment (c3d_mlp_*), 4 = the least-squares combination
- * of the last 8 converged forces of the same dt, fitted through their
- * right-hand sides (for a static body the Schur operator depends on dt only;
- * a moving body clears the stored forces at every move; the previous force
- * while none is stored).  Mode 4 is an addition with no counterpart in
- * the reference; BiCGSTAB starts from the true residual in every mode. */
+ * plus the learned increment (c3d_mlp_*), 4 = the least-squares
+ * combination of the last 8 converged forces of the same dt, fitted through
+ * their right-hand sides (for a static body the Schur operator depends on dt
+ * only; a moving body clears the stored forces at every move; the previous
+ * force while none is stored), 5 = the converged force of the same RK3
+ * stage (rk_step) of the previous step (the previous force until one is
+ * stored).  Modes 3-5 are additions with no counterpart in the reference;
+ * BiCGSTAB starts from the true residual in every mode. */
 int c3d_ib_set_solver(c3d_ctx *ctx, double tol, int maxit, int guess_mode);
 /* on != 0 (the default): each BiCGSTAB iteration of the IB solves is
  * replayed from a HIP graph captured once per (solution vector, dt, rk_step

@@ -319,7 +319,7 @@ def test_history_start():
 
     with ch:
         learned_guess.stage(ch, 1, dt, before_ib=twice)
-        rec = learned_guess.ab_compare(ch, dt, steps=3, learned=False)
+        rec = learned_guess.ab_compare(ch, dt, steps=3)
     print(f"  repeat: {repeat}")
     print(f"  iterations zero {rec['zero']} previous {rec['previous']} history {rec['history']}")
     print(f"  max |force diff| / max |force| vs zero: history "

@@ -1,10 +1,15 @@
-"""Mixed-precision projection (FP32 spectra, FP64 tridiagonal arithmetic and
-FP64 residual refinement) against the FP64 oracle.
+"""Refined projections against the FP64 oracle: the mixed-precision path
+(FP32 forward transforms, FP64 tridiagonal solve and inverse transforms,
+FP64 residual refinement) and FP64 defect correction.
 
-Bars: without refinement the FP32 spectral storage limits the potential to
-relative 1e-5; each refinement sweep must cut the FP64 residual by >= 1e3;
-with two sweeps the fields agree with the FP64 oracle to relative 1e-11 and
-max|div| is no worse than twice the plain FP64 GPU path's on the same input.
+The refined potential is kept as the first solve's phi1 plus a correction
+phic, each differenced on its own in the velocity correction.  Bars: without
+refinement the FP32 spectra limit the potential to relative 1e-5; each sweep
+cuts the FP64 residual by >= 1e3 until it meets the FP64 floor; one sweep
+(the mixed default) brings max|div| to 1e-10 or below and the fields to the
+FP64 oracle within relative 1e-11; two sweeps reach the floor (no worse than
+the plain FP64 path's max|div|).  The all-FP32 variant (C3D_MIXED_INV64=0)
+needs one sweep more.
 """
 import numpy as np
 import pytest
@@ -51,22 +56,25 @@ def test_mixed_projection(nx, ny, nz, gt, rk):
             r = res[nref]
             print(f"  nref={nref}: max|div|={r['div']:.2e} (fp64 {div64:.2e}) "
                   f"relU={r['eU']:.2e} relPint={r['eP']:.2e} relP={r['eP1']:.2e} "
-                  f"residuals={np.array2string(r['rr'], precision=2)}")
+                  f"residuals={np.array2string(r['rr'], precision=2, floatmode='unique')}")
     assert res[0]["eP"] < 1e-5 and res[0]["eU"] < 1e-5
-    for nref in (2, 3):
+    for nref in (1, 2, 3):
         rr = res[nref]["rr"]
-        assert np.all(rr[1:] <= 1e-3 * rr[:-1]), rr
+        assert np.all(rr[1:] <= np.maximum(1e-3 * rr[:-1], 4.0 * div64)), rr
         assert res[nref]["eU"] < 1e-11 and res[nref]["eP"] < 1e-11 and res[nref]["eP1"] < 1e-11
-        assert res[nref]["div"] <= 2.0 * div64
+    assert res[1]["div"] <= max(1e-10, div64)
+    for nref in (2, 3):
+        assert res[nref]["div"] <= div64
 
 
 def test_mixed_projection_384():
-    """The c3 workload itself: 384x193x384 cells (386x193x386 faces), tanh
-    alpha 2, the seed-3 k^-5/3 random-phase field at unit RMS.  Two sweeps
-    reach the FP64 oracle to relative 1e-11 (U, V, W, P_interim), every sweep
-    cuts the FP64 residual by >= 1e3, max|div| is within twice the plain FP64
-    GPU path's, and the wall stress is the FP64 path's; 32 back-to-back
-    solves from the same input give the same bits."""
+    """The 384x192x384-cell workload (386x193x386 faces), tanh alpha 2, the
+    seed-3 k^-5/3 random-phase field at unit RMS.  One sweep (the default)
+    reaches the FP64 oracle to relative 1e-11 (U, V, W, P_interim) and
+    max|div| <= 1e-10, which the plain FP64 path (7.5e-10) does not; every
+    sweep cuts the FP64 residual by >= 1e3 until the FP64 floor; the wall
+    stress is the FP64 path's; 32 back-to-back solves from the same input
+    give the same bits."""
     nx, ny, nz, rk, dt = 386, 193, 386, 2, 1e-3
     print(f"\n[case] mixed {nx}x{ny}x{nz} k^-5/3 seed 3", flush=True)
     g = orc.Grid(nx, ny, nz)
@@ -96,9 +104,9 @@ def test_mixed_projection_384():
             r = res[nref]
             print(f"  nref={nref}: max|div|={r['div']:.2e} (fp64 {div64:.2e}) relU={r['eU']:.2e} "
                   f"relPint={r['eP']:.2e} utau={r['tau']:.15e} (fp64 {tau64:.15e}) "
-                  f"residuals={np.array2string(r['rr'], precision=2)}", flush=True)
+                  f"residuals={np.array2string(r['rr'], precision=2, floatmode='unique')}", flush=True)
         first = out
-        # 32 back-to-back solves of the same input (the c3 batch) repeat the bits
+        # 32 back-to-back solves of the same input (the 32-solve batch) repeat the bits
         for _ in range(32):
             cm.set_fields(U, V, W)
             cm.compute_non_IB_projection(dt, rk)
@@ -106,12 +114,65 @@ def test_mixed_projection_384():
         again = cm.get_fields()
     assert np.array_equal(again["U"], first["U"]) and np.array_equal(again["P_interim"],
                                                                       first["P_interim"])
-    # without refinement the FP32 spectra leave 1.5e-7 in P_interim and
-    # 4.5e-5 in the velocity at this size (the correction's gradient
-    # amplifies the potential's FP32 round-off by 1/dy_min)
+    # without refinement the FP32 forward transforms leave the potential's
+    # error in P_interim and the velocity (amplified by the correction's
+    # gradient, 1/dy_min)
     assert res[0]["eP"] < 1e-5 and res[0]["eU"] < 1e-4
     rr = res[2]["rr"]
-    assert rr.size == 2 and np.all(rr[1:] <= 1e-3 * rr[:-1]), rr
-    assert res[2]["eU"] < 1e-11 and res[2]["eP"] < 1e-11
-    assert res[2]["div"] <= 2.0 * div64
-    assert abs(res[2]["tau"] - tau64) <= 1e-12 * abs(tau64)
+    assert rr.size == 2 and np.all(rr[1:] <= np.maximum(1e-3 * rr[:-1], 4.0 * div64)), rr
+    for nref in (1, 2):
+        assert res[nref]["eU"] < 1e-11 and res[nref]["eP"] < 1e-11
+        assert res[nref]["div"] <= min(1e-10, div64)
+        assert abs(res[nref]["tau"] - tau64) <= 1e-12 * abs(tau64)
+
+
+def test_fp64_defect_correction_192():
+    """FP64 defect correction on the 192x128x192-cell workload (194x129x194
+    faces, i.i.d. N(0,1) faces, no-slip walls): one sweep takes max|div| from
+    the single solve's 3.5e-11 to the FP64 floor.  On the cells the solve owns
+    that floor is the rounding of the residual b - M phi, a few ulps of
+    max|b| (2643 here: 4 eps max|b| = 2.3e-12); the reference's divergence
+    loop (src/differential_operators.f90:64-72) also evaluates the periodic
+    duplicate cells i = nx, k = nz, whose spacing x(nx) - x(nx-1) differs from
+    x(2) - x(1) by rounding (relative 5.8e-15 here), so their divergence
+    differs from their twins' by that fraction of the x- and z-derivative
+    terms, ~2e-12 whatever the velocity.  The fields stay within relative
+    1e-12 of the FP64 oracle."""
+    nx, ny, nz, rk, dt = 194, 129, 194, 2, 0.01
+    print(f"\n[case] fp64 defect correction {nx}x{ny}x{nz} white faces seed 2", flush=True)
+    g = orc.Grid(nx, ny, nz)
+    U, V, W = fields.white_faces(nx, ny, nz, seed=2)
+    g.apply_bc(U, V, W)
+    bmax = g.max_abs_div(U, V, W)
+    Ur, Vr, Wr = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    ref = g.projection(Ur, Vr, Wr, rk_step=rk, dt=dt)
+    g.apply_bc(Ur, Vr, Wr)
+    x, z = g.get(0), g.get(2)
+    seam = max(abs((x[nx - 1] - x[nx - 2]) / (x[1] - x[0]) - 1),
+               abs((z[nz - 1] - z[nz - 2]) / (z[1] - z[0]) - 1))
+    nxp, nzp = nx - 2, nz - 2
+    res = {}
+    with Channel(nx, ny, nz) as c:
+        for nref in (0, 1, 2):
+            c.set_refinement(nref)
+            c.set_fields(U, V, W)
+            c.compute_non_IB_projection(dt, rk)
+            c.apply_boundary_conditions()
+            out = c.get_fields()
+            d = np.abs(c.divergence())
+            res[nref] = dict(div=c.check_divergence(), inner=float(d[:nxp, :, :nzp].max()),
+                             rr=c.refinement_residuals(),
+                             eU=max(rel(out["U"], Ur), rel(out["V"], Vr), rel(out["W"], Wr)),
+                             eP=rel(out["P_interim"], ref["P_interim"]))
+            r = res[nref]
+            print(f"  nref={nref}: max|div|={r['div']:.3e} owned cells {r['inner']:.3e} "
+                  f"relU={r['eU']:.2e} relPint={r['eP']:.2e} "
+                  f"residuals={np.array2string(r['rr'], precision=2, floatmode='unique')}", flush=True)
+    print(f"  periodic seam spacing mismatch {seam:.2e}, input max|div| {bmax:.4e}")
+    assert seam > 0.0  # the duplicate cells' spacing is not their twins'
+    for nref in (0, 1, 2):
+        assert res[nref]["eU"] < 1e-12 and res[nref]["eP"] < 1e-12
+    assert res[1]["div"] <= 0.1 * res[0]["div"]
+    for nref in (1, 2):
+        assert res[nref]["inner"] <= 4.0 * np.finfo(float).eps * bmax
+        assert res[nref]["div"] <= 4e-12

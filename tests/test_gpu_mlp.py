@@ -35,10 +35,13 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_gemm_layout(ta, tb):
-    print(f"\n[case] bf16 mfma gemm ta={ta} tb={tb}", flush=True)
+@pytest.mark.parametrize("M,N,K", [(48, 40, 54), (1100, 1030, 70)])
+def test_gemm_layout(ta, tb, M, N, K):
+    """Both tile sizes: 32 x 32 for the small product, 64 x 64 when the
+    64-tile grid has at least 256 workgroups (18 x 17 here); ragged edges in
+    M, N and K."""
+    print(f"\n[case] bf16 mfma gemm ta={ta} tb={tb} {M}x{N}x{K}", flush=True)
     rng = np.random.default_rng(4)
-    M, N, K = 48, 40, 54
     A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
     B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
     with Channel(18, 17, 18) as ch:
@@ -88,24 +91,40 @@ def test_training_lowers_loss():
     assert np.mean((Y - T) ** 2) < 0.1 * np.mean(T ** 2)
 
 
-def test_learned_guess_in_ib_solve():
-    """Small two-cylinder case: train on pairs from two seeded runs, then
-    compare zero, previous-force and learned starts on a held-out run."""
-    print("\n[case] learned guess A/B, 34x33x34, 24x16 markers per cylinder", flush=True)
-    nx, ny, nz, nth, nzb, dt = 34, 33, 34, 24, 16, 0.004
-    X, T = learned_guess.generate_pairs(nx, ny, nz, nth, nzb, runs=2, steps=3, dt=dt, seed=11)
+def test_learned_guess_two_cylinders_128():
+    """The learned-start workload itself: two counter-rotating cylinders in
+    the 128x128x128-cell channel (130x129x130 faces, uniform y), 96 x 128
+    markers per cylinder (nb = 24,576).  20k training pairs from two runs
+    with rotation speeds drawn from U[0.3, 0.7] U_b, the network trained on
+    the GPU by the library's kernels, then, on a held-out run, every start
+    solved from the same states to the same tolerance.  The learned start's
+    forces equal the zero-guess forces to the solver's accuracy; iterations
+    per RK3 stage of every start are printed (the counts are measurements,
+    not asserted: no start is faster than zero by a fixed factor)."""
+    nx, ny, nz, nth, nzb, dt = 130, 129, 130, 96, 128, 0.01
+    print(f"\n[case] learned guess, {nx}x{ny}x{nz} faces, {nth}x{nzb} markers per cylinder", flush=True)
+    X, T = learned_guess.generate_pairs(nx, ny, nz, nth, nzb, runs=2, steps=3, dt=dt, seed=11,
+                                        max_pairs=20000, warm_steps=2)
+    assert X.shape == (20000, 54) and T.shape == (20000, 4)
     s1, s2 = learned_guess.speeds(99, 0)
     ch, m = learned_guess.make_channel(nx, ny, nz, nth, nzb, s1, s2)
+    assert m.nb == 2 * 96 * 128
     with ch:
-        hist, _, _ = learned_guess.train(ch, X, T, epochs=30, batch=256, lr=1e-3)
-        rec = learned_guess.ab_compare(ch, dt, steps=2)
-    print(f"  pairs {X.shape[0]}; loss {hist[0]:.3e} -> {hist[-1]:.3e}")
-    print(f"  iterations zero {rec['zero']} previous {rec['previous']} learned {rec['learned']}")
-    print(f"  max |force diff| / max |force|: learned {max(rec['max_force_diff_learned_vs_zero']):.2e} "
-          f"previous {max(rec['max_force_diff_previous_vs_zero']):.2e}")
-    assert hist[-1] < hist[0]
-    assert max(rec["max_force_diff_learned_vs_zero"]) < 1e-6
-    assert sum(rec["learned"]) <= sum(rec["zero"])
+        for _ in range(2):
+            ch.compute_time_step_RK3(dt)
+        hist, isc, osc = learned_guess.train(ch, X, T, epochs=60, batch=256, lr=1e-3)
+        sch, lch = learned_guess.side_channels(ch, nx, ny, nz, nth, nzb, s1, s2, isc, osc)
+        with sch, lch:
+            rec = learned_guess.ab_compare(ch, dt, steps=2, learned=lch, stage_ch=sch)
+    print(f"  loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    for n in ("zero", "previous", "stage", "learned", "history"):
+        print(f"  {n:9s} per stage {learned_guess.per_stage(rec, n)} total {sum(rec[n])}")
+    diffs = {n: max(rec[f"max_force_diff_{n}_vs_zero"]) for n in ("previous", "stage", "learned", "history")}
+    print(f"  max |force - zero-guess force| / max |force|: {diffs}")
+    assert hist[-1] < 0.5 * hist[0]
+    # every start converges to the zero-guess force (absolute residual 1e-10)
+    for n, d in diffs.items():
+        assert d < 1e-8, (n, d)
 
 
 def test_failed_buffer_growth_keeps_the_network_usable():
