@@ -782,7 +782,7 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
     if (s < nref)
       rc = back(s == 1 ? 0 : 3, c->phic, nullptr, c->phic, nullptr, false, false);
     else
-      rc = back(5, c->phi64, s == 1 ? nullptr : c->phic, c->Pint, c->phic, rk1, false);
+      rc = back(5, c->phi64, s == 1 ? nullptr : c->phic, nullptr, c->phic, rk1, false);
     if (rc) return rc;
   }
   if (rk1) HIPCHK(c, c3d::launch_p_ghosts(g, c->P, st));
@@ -790,7 +790,8 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
   if (nref == 0)
     HIPCHK(c, c3d::launch_grad_correct(g, c->U, c->V, c->W, c->Pint, dt, st));
   else
-    HIPCHK(c, c3d::launch_grad_correct_split(g, c->U, c->V, c->W, c->phi64, c->phic, st));
+    HIPCHK(c, c3d::launch_grad_correct_split(g, c->U, c->V, c->W, c->phi64, c->phic, c->Pint, dt,
+                                             st));
   if (ev) HIPCHK(c, hipEventRecord(ev[7], st));
   return 0;
 }

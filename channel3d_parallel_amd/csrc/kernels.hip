@@ -18,7 +18,6 @@ namespace c3d {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGradKC = 8;  // z planes per lane in the residual kernel
 constexpr int kGradKCDefault = 2;  // z planes per lane in the gradient correction
 
 // Rows of the z tile handled by one x-transform workgroup (row pairs are
@@ -50,7 +49,8 @@ constexpr int kSrcUVW = 1;    // divergence of U, V, W computed on the fly
 constexpr int kSrcUVWB = 2;   // same, and the FP64 divergence also stored to bout
 constexpr int kSrcRhs32 = 3;  // rhs_p-shaped FP32 array (refinement residual)
 constexpr int kSrcUVWc = 4;   // as kSrcUVW with cached U and W loads (fields that fit in cache)
-constexpr int kSrcResid = 5;  // FP64 residual b - M phi of the mixed path's refinement, on the fly
+constexpr int kSrcResid = 5;  // FP64 residual b - M phi of the refinement, on the fly
+constexpr int kSrcResid4 = 6; // the same, 4 rows per load batch instead of 2
 
 template <typename real, int N, int TK, int SRC>
 __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __restrict__ U,
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
                                                   const double* __restrict__ gauge,
                                                   unsigned long long* __restrict__ rmax) {
   constexpr bool FROM_UVW = (SRC == kSrcUVW || SRC == kSrcUVWB || SRC == kSrcUVWc);
-  constexpr bool RESID = (SRC == kSrcResid);
+  constexpr bool RESID = (SRC == kSrcResid || SRC == kSrcResid4);
   constexpr bool NTL = (SRC != kSrcUVWc);  // non-temporal U and W loads
   constexpr int NV = FROM_UVW ? 5 : (RESID ? 6 : 1);  // values loaded per point
   constexpr int NP = TK / 2;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   //    batch of RB rows are all issued before any is used.
   constexpr int NPL = (N + kBlock - 1) / kBlock;  // x positions per lane
   // rows per load batch (the residual loads 6 values per point: 2 rows)
-  constexpr int RB = TK < 4 ? TK : (RESID ? 2 : 4);
+  constexpr int RB = TK < 4 ? TK : (SRC == kSrcResid ? 2 : 4);
   const double ry = g.rdyf[j];
   double wprev[NPL];  // W(k-1) of the next row: the W row of the row before
   if constexpr (FROM_UVW) {
@@ -424,9 +424,12 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
                                                      double lam_thr, cpx<rout>* Sout) {
   constexpr int P = 8;            // parts per mode
   constexpr int N = P * M;        // rows
-  constexpr int MW = 16;          // modes per wave
+  // modes per wave: a row access of a wave is MW consecutive modes, 256
+  // contiguous bytes of the input spectrum (16 FP64 or 32 FP32 complex)
+  constexpr int MW = 256 / (int)sizeof(cpx<real>);
   constexpr int PW = 64 / MW;     // parts per wave
-  static_assert(PW * 2 == P, "two waves cover the parts of a mode group");
+  constexpr int WG = P / PW;      // waves that cover the parts of a mode group
+  static_assert(WG * PW == P && kBlock / 64 % WG == 0, "waves must tile the parts");
   constexpr int MB = kBlock / P;  // modes per workgroup
   constexpr int CPT = M + 1;      // padded pitch of the row tables
   constexpr int RP = 8 * P + 1;   // padded pitch of a mode's interface rows
@@ -441,8 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
   }
   const int lane = tid & 63;
   const int wv = tid >> 6;
-  const int p = (wv & 1) * PW + lane / MW;
-  const int mb = (wv >> 1) * MW + lane % MW;
+  const int p = (wv % WG) * PW + lane / MW;
+  const int mb = (wv / WG) * MW + lane % MW;
   const int nzp = g.nzp;
   const size_t stride = (size_t)g.nxh * nzp;
   bool valid;
@@ -584,8 +587,9 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
 // rounding of a large smooth potential, amplified by D G ~ 1/dy^2, bounds
 // max|div| far above the velocity's own rounding):
 //   MODE 3: out = phi_in + phi (correction accumulated),
-//   MODE 5: out2 = phic' = phi_in2 + phi (phi_in2 null: phi), out =
-//           P_interim = (phi_in + phic')/dt; rk1: P = (phi_in + phic')/(dt*8/15).
+//   MODE 5: out2 = phic' = phi_in2 + phi (phi_in2 null: phi); rk1: P =
+//           (phi_in + phic')/(dt*8/15).  P_interim = (phi1 + phic')/dt is
+//           written by the split gradient correction, which reads both.
 template <typename real, int N, int TK, int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __restrict__ S,
                                                   const cpx<real>* __restrict__ tw,
@@ -697,24 +701,25 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
         if constexpr (MODE == 3) phi = phi_in[at] + phi;
         if constexpr (MODE == 5) {
           corr = phi_in2 ? phi_in2[at] + phi : phi;
-          phi = phi_in[at] + corr;
+          if (rk1) phi = phi_in[at] + corr;  // the potential of P
         }
-        const double o = (MODE == 1 || MODE == 5) ? phi / dt : phi;
-        __builtin_nontemporal_store(o, &orow[n]);
+        constexpr bool OUT = MODE != 5;  // MODE 5 writes phic' (and P), not out
+        const double o = (MODE == 1) ? phi / dt : phi;
+        if constexpr (OUT) __builtin_nontemporal_store(o, &orow[n]);
         if constexpr (MODE == 5) out2[at] = corr;
         if (rk1) prow[n] = phi / cP;
         if (n == 0) {  // x duplicate i = nxg-1 (offset nxp)
-          orow[N] = o;
+          if constexpr (OUT) orow[N] = o;
           if constexpr (MODE == 5) out2[at + N] = corr;
           if (rk1) prow[N] = phi / cP;
         }
         if (dupz) {
           const size_t at2 = ridx(2, j, kdup, nxg, nyg) + n;
-          orow2[n] = o;
+          if constexpr (OUT) orow2[n] = o;
           if constexpr (MODE == 5) out2[at2] = corr;
           if (rk1) prow2[n] = phi / cP;
           if (n == 0) {
-            orow2[N] = o;
+            if constexpr (OUT) orow2[N] = o;
             if constexpr (MODE == 5) out2[at2 + N] = corr;
             if (rk1) prow2[N] = phi / cP;
           }
@@ -723,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_inv_x(DevGrid g, const cpx<real>* __
     }
   }
   (void)idup;
-  if ((MODE == 1 || MODE == 5) && k0 == 0) {
+  if (MODE == 1 && k0 == 0) {
     // rhs_p plane nzg is zero after the divergence; P_interim = 0/dt there.
     for (int i = 2 + tid; i <= nxg - 1; i += kBlock) out[ridx(i, j, g.nzg, nxg, nyg)] = 0.0;
   }
@@ -792,7 +797,9 @@ __global__ void k_pghost_z(DevGrid g, double* __restrict__ P) {
 // src/differential_operators.f90:28-50 (faces outside them keep U).
 // SPLIT: U = U - (G(phi1) + G(phic)) for a potential kept as two parts (the
 // refinement's phi1 + phic); Pint is phi1 then, and each part's differences
-// are formed before they are added, so phic is not rounded into phi1.
+// are formed before they are added, so phic is not rounded into phi1.  The
+// split kernel also writes P_interim = (phi1 + phic)/dt to pout on the whole
+// rhs_p shape (the duplicates i = nxg-1 from i = 2, the zero plane nzg).
 // ---------------------------------------------------------------------------
 template <int KC, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __restrict__ U,
@@ -800,8 +807,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
                                                          double* __restrict__ W,
                                                          const double* __restrict__ Pint,
                                                          const double* __restrict__ phic,
-                                                         double dt) {
-  // one lane per (i, j), walking kGradKC consecutive z planes so that
+                                                         double* __restrict__ pout, double dt) {
+  // one lane per (i, j), walking KC consecutive z planes so that
   // p(k+1) of one plane is p(k) of the next; XCD-aware tile order so the
   // p(j+1) halo row is usually in the same L2
   unsigned bx, by, bz;
@@ -835,6 +842,15 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
   for (int kk = 0; kk < KC; kk++) {
     const int k = kb + kk;
     if (k > nzg - 1) break;
+    if constexpr (SPLIT) {
+      const double o = (p + pc) / dt;
+      pout[ridx(i, j, k, nxg, nyg)] = o;
+      if (i == 2) pout[ridx(nxg - 1, j, k, nxg, nyg)] = o;
+      if (k == nzg - 1) {
+        pout[ridx(i, j, nzg, nxg, nyg)] = 0.0;
+        if (i == 2) pout[ridx(nxg - 1, j, nzg, nxg, nyg)] = 0.0;
+      }
+    }
     double pxc;
     const double px = pot(ridx(i + 1, j, k, nxg, nyg), pxc);
     const size_t iu = fidx(i, j, k, nx, nyg);
@@ -1116,9 +1132,17 @@ hipError_t fwd_x_resid(const DevGrid& g, const double* b, const double* phi, dou
   // gauge[0] = result, gauge[1 .. nzp] = row sums
   hipLaunchKernelGGL(k_gauge_rows, dim3(g.nzp), dim3(kBlock), 0, s, g, phi, gauge + 1);
   hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(kBlock), 0, s, g, (const double*)(gauge + 1), gauge);
+  static const bool rb4 = [] {
+    const char* e = getenv("C3D_RESID_RB");
+    return e && atoi(e) == 4;
+  }();
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
-    hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
-                       nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
+    if (rb4)
+      hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid4>), grid, dim3(kBlock), 0, s, g, phi,
+                         nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
+    else
+      hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
+                         nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
   }));
   return hipGetLastError();
 }
@@ -1254,7 +1278,8 @@ template <typename real>
 hipError_t inv_x_modes(const DevGrid& g, int mode, const cpx<real>* S, const cpx<real>* tw,
                        const double* phi_in, const double* phi_in2, double* out, double* out2,
                        double* P, double dt, bool rk1, hipStream_t s) {
-  if ((mode == 3 && !phi_in) || (mode == 5 && (!phi_in || !out2))) return hipErrorInvalidValue;
+  if ((mode == 3 && !phi_in) || (mode == 5 && (!out2 || (rk1 && !phi_in))))
+    return hipErrorInvalidValue;
   const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
@@ -1305,14 +1330,14 @@ static int grad_kc() {
 namespace {
 template <bool SPLIT>
 hipError_t grad_correct_any(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
-                            const double* phic, double dt, hipStream_t s) {
+                            const double* phic, double* pout, double dt, hipStream_t s) {
   const int kc = grad_kc();
   const dim3 grid = grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kc - 1) / kc);
   switch (kc) {
-#define C3D_GC_CASE(KC)                                                                             \
-  case KC:                                                                                          \
-    hipLaunchKernelGGL((k_grad_correct<KC, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint, phic, \
-                       dt);                                                                         \
+#define C3D_GC_CASE(KC)                                                                         \
+  case KC:                                                                                      \
+    hipLaunchKernelGGL((k_grad_correct<KC, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint,     \
+                       phic, pout, dt);                                                         \
     break;
     C3D_GC_CASE(1)
     C3D_GC_CASE(4)
@@ -1320,22 +1345,24 @@ hipError_t grad_correct_any(const DevGrid& g, double* U, double* V, double* W, c
     C3D_GC_CASE(16)
 #undef C3D_GC_CASE
     default:
-      hipLaunchKernelGGL((k_grad_correct<2, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint, phic,
-                         dt);
+      hipLaunchKernelGGL((k_grad_correct<2, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint,
+                         phic, pout, dt);
       break;
   }
   return hipGetLastError();
 }
+
 }  // namespace
 
 hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
                                double dt, hipStream_t s) {
-  return grad_correct_any<false>(g, U, V, W, Pint, nullptr, dt, s);
+  return grad_correct_any<false>(g, U, V, W, Pint, nullptr, nullptr, dt, s);
 }
 
 hipError_t launch_grad_correct_split(const DevGrid& g, double* U, double* V, double* W,
-                                     const double* phi1, const double* phic, hipStream_t s) {
-  return grad_correct_any<true>(g, U, V, W, phi1, phic, 0.0, s);
+                                     const double* phi1, const double* phic, double* Pint, double dt,
+                                     hipStream_t s) {
+  return grad_correct_any<true>(g, U, V, W, phi1, phic, Pint, dt, s);
 }
 
 hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s) {

@@ -96,8 +96,9 @@ hipError_t launch_tri32to64(const DevGrid& g, cpx<float>* S32, cpx<double>* S64,
 // Inverse x transform with the refinement modes, the potential kept as the
 // first solve's phi1 plus a correction phic:
 //   0: out = phi, 1: out = phi/dt, 3: out = phi_in + phi,
-//   5: out2 = phic' = phi_in2 + phi (phi_in2 may be null: phic' = phi),
-//      out = (phi_in + phic')/dt (P_interim);
+//   5: out2 = phic' = phi_in2 + phi (phi_in2 may be null: phic' = phi), out
+//      unused (launch_grad_correct_split writes P_interim); rk1: P from
+//      phi_in + phic';
 // rk1 writes P from the same potential.
 hipError_t launch_inv_x64_ref(const DevGrid& g, int mode, const cpx<double>* S, const double* phi_in,
                               const double* phi_in2, double* out, double* out2, double* P, double dt,
@@ -110,9 +111,11 @@ hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s);
 // U -= G(dt * Pint) on the gradient's index ranges.
 hipError_t launch_grad_correct(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
                                double dt, hipStream_t s);
-// U -= G(phi1) + G(phic), each part differenced on its own.
+// U -= G(phi1) + G(phic), each part differenced on its own; Pint =
+// (phi1 + phic)/dt on the whole rhs_p shape.
 hipError_t launch_grad_correct_split(const DevGrid& g, double* U, double* V, double* W,
-                                     const double* phi1, const double* phic, hipStream_t s);
+                                     const double* phi1, const double* phic, double* Pint, double dt,
+                                     hipStream_t s);
 // apply_boundary_conditions on device-resident staggered fields.
 hipError_t launch_apply_bc(const DevGrid& g, double* U, double* V, double* W, hipStream_t s);
 // Divergence into an rhs_p-shaped array (plane nzg zero).
