@@ -50,7 +50,6 @@ constexpr int kSrcUVWB = 2;   // same, and the FP64 divergence also stored to bo
 constexpr int kSrcRhs32 = 3;  // rhs_p-shaped FP32 array (refinement residual)
 constexpr int kSrcUVWc = 4;   // as kSrcUVW with cached U and W loads (fields that fit in cache)
 constexpr int kSrcResid = 5;  // FP64 residual b - M phi of the refinement, on the fly
-constexpr int kSrcResid4 = 6; // the same, 4 rows per load batch instead of 2
 
 template <typename real, int N, int TK, int SRC>
 __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __restrict__ U,
@@ -63,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
                                                   const double* __restrict__ gauge,
                                                   unsigned long long* __restrict__ rmax) {
   constexpr bool FROM_UVW = (SRC == kSrcUVW || SRC == kSrcUVWB || SRC == kSrcUVWc);
-  constexpr bool RESID = (SRC == kSrcResid || SRC == kSrcResid4);
+  constexpr bool RESID = (SRC == kSrcResid);
   constexpr bool NTL = (SRC != kSrcUVWc);  // non-temporal U and W loads
   constexpr int NV = FROM_UVW ? 5 : (RESID ? 6 : 1);  // values loaded per point
   constexpr int NP = TK / 2;
@@ -85,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   //    batch of RB rows are all issued before any is used.
   constexpr int NPL = (N + kBlock - 1) / kBlock;  // x positions per lane
   // rows per load batch (the residual loads 6 values per point: 2 rows)
-  constexpr int RB = TK < 4 ? TK : (SRC == kSrcResid ? 2 : 4);
+  constexpr int RB = TK < 4 ? TK : (RESID ? 2 : 4);
   const double ry = g.rdyf[j];
   double wprev[NPL];  // W(k-1) of the next row: the W row of the row before
   if constexpr (FROM_UVW) {
@@ -1132,17 +1131,9 @@ hipError_t fwd_x_resid(const DevGrid& g, const double* b, const double* phi, dou
   // gauge[0] = result, gauge[1 .. nzp] = row sums
   hipLaunchKernelGGL(k_gauge_rows, dim3(g.nzp), dim3(kBlock), 0, s, g, phi, gauge + 1);
   hipLaunchKernelGGL(k_gauge_final, dim3(1), dim3(kBlock), 0, s, g, (const double*)(gauge + 1), gauge);
-  static const bool rb4 = [] {
-    const char* e = getenv("C3D_RESID_RB");
-    return e && atoi(e) == 4;
-  }();
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
-    if (rb4)
-      hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid4>), grid, dim3(kBlock), 0, s, g, phi,
-                         nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
-    else
-      hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
-                         nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
+    hipLaunchKernelGGL((k_fwd_x<real, NN, TK, kSrcResid>), grid, dim3(kBlock), 0, s, g, phi,
+                       nullptr, nullptr, (const void*)b, tw, S, rout, (const double*)gauge, rmax);
   }));
   return hipGetLastError();
 }

@@ -10,6 +10,8 @@
 //   D: 4 values D(row 4q .. 4q+3, col r)
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #include "mlp.hpp"
 
 namespace c3d {
@@ -39,13 +41,77 @@ __device__ __forceinline__ short bf16_bits(float x) {
 // small to give every CU a 64 x 64 tile); its 4 wavefronts each own a
 // T/2 x T/2 quarter as (T/32) x (T/32) MFMA tiles.  K advances in slices of
 // 32: the workgroup converts the T x 32 slices of op(A) and op(B) to BF16
-// into LDS once (row-major [m][k] and [n][k], so both operand fragments are
-// 4 consecutive k of one row), and every wavefront then reads its fragments
-// from LDS.
+// into LDS (row-major [m][k] and [n][k], so both operand fragments are 4
+// consecutive k of one row), and every wavefront then reads its fragments
+// from LDS.  The next slice's loads are issued into registers before the
+// current slice's MFMAs; with VA / VB the operand is read as float4 along
+// its contiguous index (that extent and the leading dimension multiples of
+// 4, the base 16-byte aligned).
 constexpr int kTK = 32;
 constexpr int kTKP = kTK + 4;  // LDS row pitch in BF16 elements (72 bytes)
 
-template <bool TA, bool TB, int EPI, int T>
+// Slice of an operand X with rows r (m or n) and reductions k: KC means
+// X[r*ld + k] (k contiguous), else X[k*ld + r].  E values per thread.
+template <bool KC, bool V, int T>
+struct Slice {
+  static constexpr int E = T * kTK / kBlock;
+  float v[E];
+  __device__ __forceinline__ void load(const float* __restrict__ X, int ld, int R, int K, int r0,
+                                       int k0) {
+    const int tid = threadIdx.x;
+    if constexpr (V) {
+#pragma unroll
+      for (int g = 0; g < E / 4; g++) {
+        const int e = tid + g * kBlock;  // float4 group
+        int rr, kk;
+        if constexpr (KC) { rr = e / (kTK / 4); kk = 4 * (e % (kTK / 4)); }
+        else { kk = e / (T / 4); rr = 4 * (e % (T / 4)); }
+        const int r = r0 + rr, k = k0 + kk;
+        float4_t x = float4_t{0.f, 0.f, 0.f, 0.f};
+        if (r < R && k < K)
+          x = *reinterpret_cast<const float4_t*>(KC ? &X[(size_t)r * ld + k] : &X[(size_t)k * ld + r]);
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[4 * g + c] = x[c];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const int e = tid + i * kBlock;
+        const int rr = KC ? e / kTK : e % T, kk = KC ? e % kTK : e / T;
+        const int r = r0 + rr, k = k0 + kk;
+        v[i] = (r < R && k < K) ? (KC ? X[(size_t)r * ld + k] : X[(size_t)k * ld + r]) : 0.f;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(short* Xs) const {
+    const int tid = threadIdx.x;
+    if constexpr (V) {
+#pragma unroll
+      for (int g = 0; g < E / 4; g++) {
+        const int e = tid + g * kBlock;
+        if constexpr (KC) {
+          const int rr = e / (kTK / 4), kk = 4 * (e % (kTK / 4));
+          *reinterpret_cast<short4_t*>(&Xs[rr * kTKP + kk]) =
+              short4_t{bf16_bits(v[4 * g]), bf16_bits(v[4 * g + 1]), bf16_bits(v[4 * g + 2]),
+                       bf16_bits(v[4 * g + 3])};
+        } else {
+          const int kk = e / (T / 4), rr = 4 * (e % (T / 4));
+#pragma unroll
+          for (int c = 0; c < 4; c++) Xs[(rr + c) * kTKP + kk] = bf16_bits(v[4 * g + c]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const int e = tid + i * kBlock;
+        const int rr = KC ? e / kTK : e % T, kk = KC ? e % kTK : e / T;
+        Xs[rr * kTKP + kk] = bf16_bits(v[i]);
+      }
+    }
+  }
+};
+
+template <bool TA, bool TB, int EPI, int T, bool VA, bool VB>
 __global__ __launch_bounds__(kBlock) void k_gemm(int M, int N, int K, const float* __restrict__ A,
                                                  int lda, const float* __restrict__ B, int ldb,
                                                  float* __restrict__ C, int ldc,
@@ -63,25 +129,18 @@ __global__ __launch_bounds__(kBlock) void k_gemm(int M, int N, int K, const floa
   for (int i = 0; i < F; i++)
 #pragma unroll
     for (int j = 0; j < F; j++) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  Slice<!TA, VA, T> sa;
+  Slice<TB, VB, T> sb;
+  sa.load(A, lda, M, K, m0, 0);
+  sb.load(B, ldb, N, K, n0, 0);
   for (int k0 = 0; k0 < K; k0 += kTK) {
-    // the slices, walked along the operand's contiguous index in memory
-#pragma unroll
-    for (int e = tid; e < T * kTK; e += kBlock) {
-      const int mm = TA ? e % T : e / kTK, kk = TA ? e / T : e % kTK;
-      const int m = m0 + mm, k = k0 + kk;
-      float v = 0.f;
-      if (m < M && k < K) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
-      As[mm * kTKP + kk] = bf16_bits(v);
-    }
-#pragma unroll
-    for (int e = tid; e < T * kTK; e += kBlock) {
-      const int nn = TB ? e / kTK : e % T, kk = TB ? e % kTK : e / T;
-      const int n = n0 + nn, k = k0 + kk;
-      float v = 0.f;
-      if (n < N && k < K) v = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
-      Bs[nn * kTKP + kk] = bf16_bits(v);
-    }
+    sa.store(As);
+    sb.store(Bs);
     __syncthreads();
+    if (k0 + kTK < K) {  // next slice in flight during this slice's MFMAs
+      sa.load(A, lda, M, K, m0, k0 + kTK);
+      sb.load(B, ldb, N, K, n0, k0 + kTK);
+    }
 #pragma unroll
     for (int ks = 0; ks < kTK; ks += 16) {
       short4_t a[F], b[F];
@@ -202,22 +261,38 @@ __global__ void k_apply(int nb, const float* __restrict__ Y, const float* __rest
   fb[t] = fprev ? fprev[t] + d : d;
 }
 
+template <bool TA, bool TB, int EPI, int T>
+hipError_t gemm_tile(int M, int N, int K, const float* A, int lda, const float* B, int ldb,
+                     float* C, int ldc, const float* bias, const float* H, int ldh, hipStream_t s) {
+  // float4 operand loads: the contiguous extent and the leading dimension
+  // multiples of 4, the base 16-byte aligned
+  auto al = [](const float* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool va = al(A) && lda % 4 == 0 && (TA ? M : K) % 4 == 0;
+  const bool vb = al(B) && ldb % 4 == 0 && (TB ? K : N) % 4 == 0;
+  const dim3 grid((M + T - 1) / T, (N + T - 1) / T);
+  if (va && vb)
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, T, true, true>), grid, dim3(kBlock), 0, s, M, N, K, A,
+                       lda, B, ldb, C, ldc, bias, H, ldh);
+  else if (vb)
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, T, false, true>), grid, dim3(kBlock), 0, s, M, N, K, A,
+                       lda, B, ldb, C, ldc, bias, H, ldh);
+  else if (va)
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, T, true, false>), grid, dim3(kBlock), 0, s, M, N, K, A,
+                       lda, B, ldb, C, ldc, bias, H, ldh);
+  else
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, T, false, false>), grid, dim3(kBlock), 0, s, M, N, K, A,
+                       lda, B, ldb, C, ldc, bias, H, ldh);
+  return hipGetLastError();
+}
+
 template <bool TA, bool TB, int EPI>
 hipError_t gemm(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
                 int ldc, const float* bias, const float* H, int ldh, hipStream_t s) {
   // 64 x 64 tiles when they give at least one workgroup per CU (256), else
   // 32 x 32 (the training batches: 256 x 256 products)
   const long long big = (long long)((M + 63) / 64) * ((N + 63) / 64);
-  if (big >= 256) {
-    const dim3 grid((M + 63) / 64, (N + 63) / 64);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 64>), grid, dim3(kBlock), 0, s, M, N, K, A, lda, B, ldb,
-                       C, ldc, bias, H, ldh);
-  } else {
-    const dim3 grid((M + 31) / 32, (N + 31) / 32);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 32>), grid, dim3(kBlock), 0, s, M, N, K, A, lda, B, ldb,
-                       C, ldc, bias, H, ldh);
-  }
-  return hipGetLastError();
+  if (big >= 256) return gemm_tile<TA, TB, EPI, 64>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, s);
+  return gemm_tile<TA, TB, EPI, 32>(M, N, K, A, lda, B, ldb, C, ldc, bias, H, ldh, s);
 }
 
 }  // namespace

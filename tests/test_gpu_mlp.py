@@ -35,11 +35,13 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(48, 40, 54), (1100, 1030, 70)])
+@pytest.mark.parametrize("M,N,K", [(48, 40, 54), (1100, 1030, 70), (256, 256, 256), (1024, 1024, 128)])
 def test_gemm_layout(ta, tb, M, N, K):
-    """Both tile sizes: 32 x 32 for the small product, 64 x 64 when the
-    64-tile grid has at least 256 workgroups (18 x 17 here); ragged edges in
-    M, N and K."""
+    """Both tile sizes: 32 x 32 for the small products, 64 x 64 when the
+    64-tile grid has at least 256 workgroups (18 x 17 and 16 x 16 here);
+    ragged edges in M, N and K; scalar operand loads where an extent is not a
+    multiple of 4, float4 loads where it is (all of the last two shapes, some
+    operands of the first two)."""
     print(f"\n[case] bf16 mfma gemm ta={ta} tb={tb} {M}x{N}x{K}", flush=True)
     rng = np.random.default_rng(4)
     A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
