@@ -176,3 +176,38 @@ def test_fp64_defect_correction_192():
     for nref in (1, 2):
         assert res[nref]["inner"] <= 4.0 * np.finfo(float).eps * bmax
         assert res[nref]["div"] <= 4e-12
+
+
+def test_mixed_fp32_inverse_variant(monkeypatch):
+    """C3D_MIXED_INV64=0 (read at context creation): the inverse half in FP32
+    too.  Its first solve is less accurate, so its default is two sweeps,
+    which reach the FP64 oracle to relative 1e-11 and the plain FP64 path's
+    max|div|."""
+    nx, ny, nz, rk, dt = 66, 65, 66, 2, 0.01
+    print(f"\n[case] mixed FP32 inverse half {nx}x{ny}x{nz}", flush=True)
+    g = orc.Grid(nx, ny, nz)
+    U, V, W = fields.white_faces(nx, ny, nz, seed=2)
+    g.apply_bc(U, V, W)
+    Ur, Vr, Wr = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    ref = g.projection(Ur, Vr, Wr, rk_step=rk, dt=dt)
+    g.apply_bc(Ur, Vr, Wr)
+    with Channel(nx, ny, nz) as c64:
+        c64.set_fields(U, V, W)
+        c64.compute_non_IB_projection(dt, rk)
+        c64.apply_boundary_conditions()
+        div64 = c64.check_divergence()
+    monkeypatch.setenv("C3D_MIXED_INV64", "0")
+    with Channel(nx, ny, nz, precision=PREC_MIXED) as cm:
+        cm.set_fields(U, V, W)
+        cm.compute_non_IB_projection(dt, rk)
+        rr = cm.refinement_residuals()
+        cm.apply_boundary_conditions()
+        out = cm.get_fields()
+        div = cm.check_divergence()
+    eU = max(rel(out["U"], Ur), rel(out["V"], Vr), rel(out["W"], Wr))
+    eP = rel(out["P_interim"], ref["P_interim"])
+    print(f"  default sweeps {rr.size}: max|div|={div:.2e} (fp64 {div64:.2e}) relU={eU:.2e} "
+          f"relPint={eP:.2e} residuals={np.array2string(rr, precision=2, floatmode='unique')}")
+    assert rr.size == 2 and rr[1] <= 1e-3 * rr[0]
+    assert eU < 1e-11 and eP < 1e-11
+    assert div <= div64
