@@ -51,6 +51,7 @@ struct IBState {
   double *Hu_int = nullptr, *Hv_int = nullptr, *Hw_int = nullptr;
   double *E1nHc = nullptr, *E1nH = nullptr;
   double *Uint = nullptr, *Vint = nullptr, *Wint = nullptr;  // U_interim ...
+  double* aux = nullptr;  // 3nb
   double* vec[8] = {nullptr};  // x r rhat p v h s t  (4nb each)
   double *rhs_ib = nullptr, *fb = nullptr, *sc = nullptr, *part = nullptr, *mpart = nullptr;
   double *fin = nullptr, *fout = nullptr;  // 4nb staging for host-array calls
@@ -1359,12 +1360,12 @@ static int ib_poisson_reg(c3d_ctx* c, double dt, int rk_step) {
   return 0;
 }
 
-// U_reg = R f (src/projection.f90:66-73): the three spreads of the aux
-// vector (formed per entry from f), boundary conditions.
+// U_reg = R f (src/projection.f90:66-73): aux, three spreads, boundary conditions.
 static int ib_spread_force(c3d_ctx* c, const double* f, double dt) {
   IBState* ib = c->ib;
-  HIPCHK(c, c3d::ib::launch_spread3_aux(ib->m, ib->sp, f, dt, c->desc.nu, ib->Ureg, ib->Vreg,
-                                        ib->Wreg, c->stream));
+  const int nb = ib->m.nb;
+  HIPCHK(c, c3d::ib::launch_aux(ib->m, f, dt, c->desc.nu, ib->aux, c->stream));
+  HIPCHK(c, c3d::ib::launch_spread3(ib->m, ib->sp, ib->aux, ib->Ureg, ib->Vreg, ib->Wreg, c->stream));
   HIPCHK(c, c3d::launch_apply_bc(c->g, ib->Ureg, ib->Vreg, ib->Wreg, c->stream));
   return 0;
 }
@@ -1690,7 +1691,7 @@ int ib_create(c3d_ctx* c, int nb, int nzb, int nxb, int nxb1) {
       (rc = dalloc(c, &ib->Hv_int, c->nv)) || (rc = dalloc(c, &ib->Hw_int, c->nw)) ||
       (rc = dalloc(c, &ib->Uint, c->nu)) || (rc = dalloc(c, &ib->Vint, c->nv)) ||
       (rc = dalloc(c, &ib->Wint, c->nw)) || (rc = dalloc(c, &ib->E1nHc, (size_t)nb)) ||
-      (rc = dalloc(c, &ib->E1nH, n3)) ||
+      (rc = dalloc(c, &ib->E1nH, n3)) || (rc = dalloc(c, &ib->aux, n3)) ||
       (rc = dalloc(c, &ib->rhs_ib, n4)) || (rc = dalloc(c, &ib->fb, n4)) ||
       (rc = dalloc(c, &ib->fin, n4)) || (rc = dalloc(c, &ib->fout, n4)) ||
       (rc = dalloc(c, &ib->fprev, n4)) ||

@@ -430,6 +430,15 @@ __global__ __launch_bounds__(kBlock) void k_rhs_u(DevGrid g, Markers m, const do
     for (int c = 0; c < 3; c++) out[c * m.nb + l] = -r[c] + m.ub[c * m.nb + l];
 }
 
+__global__ void k_aux(Markers m, const double* __restrict__ f, double dt, double nu,
+                      double* __restrict__ aux) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = m.nb;
+  if (t >= 3 * nb) return;
+  const int l = t % nb;
+  aux[t] = dt * nu * f[t] - dt * f[3 * nb + l] * m.normals[t];
+}
+
 // Static coefficients of the spreading entries: w*factor*sb[l] per entry.
 __global__ void k_spread_coeffs(int nb, double A, double factor, const double* __restrict__ phi,
                                 const double* __restrict__ sb, Spread sp) {
@@ -465,42 +474,6 @@ __global__ __launch_bounds__(kBlock) void k_spread(Spread sp, const double* __re
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= sp.nnodes) return;
   spread_node(sp, t, f, F);
-}
-
-// spread_node with the Schur complement's aux vector formed per entry from
-// the force f = (dudn (3nb), pj (nb)): aux_c[l] = dt*nu*dudn_c[l] -
-// dt*pj[l]*n_c[l] (src/projection.f90:109-114).
-__device__ __forceinline__ void spread_node_aux(const Spread& sp, int t, int c, int nb,
-                                                const double* __restrict__ f,
-                                                const double* __restrict__ normals, double dt,
-                                                double nu, double* __restrict__ F) {
-  double acc = 0.0;
-  const int n = sp.cnt[t];
-  const double* fc = f + (size_t)c * nb;
-  const double* nc = normals + (size_t)c * nb;
-  const double* pj = f + 3 * (size_t)nb;
-#pragma unroll 8
-  for (int i = 0; i < n; i++) {
-    const int e = sp.slot[i] + t;
-    const int l = sp.ml[e];
-    const double aux = dt * nu * fc[l] - dt * pj[l] * nc[l];
-    acc = acc + sp.wfs[e] * aux;
-  }
-  F[sp.node[t]] = acc;
-}
-
-__global__ __launch_bounds__(kBlock) void k_spread3_aux(Spread s0, Spread s1, Spread s2,
-                                                        const double* __restrict__ f,
-                                                        const double* __restrict__ normals,
-                                                        double dt, double nu, int nb,
-                                                        double* __restrict__ F0,
-                                                        double* __restrict__ F1,
-                                                        double* __restrict__ F2) {
-  const int c = blockIdx.y;
-  const Spread sp = c == 0 ? s0 : (c == 1 ? s1 : s2);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= sp.nnodes) return;
-  spread_node_aux(sp, t, c, nb, f, normals, dt, nu, c == 0 ? F0 : (c == 1 ? F1 : F2));
 }
 
 // The three components in one launch, block row c = component c, reading
@@ -959,6 +932,12 @@ hipError_t launch_rhs_u(const DevGrid& g, const Markers& m, const double* U, con
   return hipGetLastError();
 }
 
+hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, double* aux,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_aux, blocks_for((size_t)3 * m.nb), dim3(kBlock), 0, s, m, f, dt, nu, aux);
+  return hipGetLastError();
+}
+
 hipError_t launch_spread3(const Markers& m, const Spread sp[3], const double* f, double* F0,
                           double* F1, double* F2, hipStream_t s) {
   int mx = 0;
@@ -967,18 +946,6 @@ hipError_t launch_spread3(const Markers& m, const Spread sp[3], const double* f,
   dim3 grid = blocks_for((size_t)mx);
   grid.y = 3;
   hipLaunchKernelGGL(k_spread3, grid, dim3(kBlock), 0, s, sp[0], sp[1], sp[2], f, m.nb, F0, F1, F2);
-  return hipGetLastError();
-}
-
-hipError_t launch_spread3_aux(const Markers& m, const Spread sp[3], const double* f, double dt,
-                              double nu, double* F0, double* F1, double* F2, hipStream_t s) {
-  int mx = 0;
-  for (int c = 0; c < 3; c++) mx = sp[c].nnodes > mx ? sp[c].nnodes : mx;
-  if (mx == 0) return hipSuccess;
-  dim3 grid = blocks_for((size_t)mx);
-  grid.y = 3;
-  hipLaunchKernelGGL(k_spread3_aux, grid, dim3(kBlock), 0, s, sp[0], sp[1], sp[2], f, m.normals, dt,
-                     nu, m.nb, F0, F1, F2);
   return hipGetLastError();
 }
 

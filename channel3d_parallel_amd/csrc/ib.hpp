@@ -98,6 +98,9 @@ hipError_t launch_schur_rows(const DevGrid& g, const Markers& m, const double* p
 // Velocity-rhs rows: out[c*nb+l] = -E_c(U)[l] + ub[c*nb+l].
 hipError_t launch_rhs_u(const DevGrid& g, const Markers& m, const double* U, const double* V,
                         const double* W, double* out, hipStream_t s);
+// aux[c*nb+l] = dt*nu*f[c*nb+l] - dt*f[3nb+l]*n[c*nb+l]   (c = 0..2)
+hipError_t launch_aux(const Markers& m, const double* f, double dt, double nu, double* aux,
+                      hipStream_t s);
 // R: Reg_c[node] = sum over entries of w*factor*sb*f_c, written at support
 // nodes only (other nodes keep the zeros they were created with).
 hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const double* f,
@@ -105,11 +108,6 @@ hipError_t launch_spread(const Markers& m, const Spread& sp, int comp, const dou
 // The three components at once: F_c from f + c*nb with map sp[c].
 hipError_t launch_spread3(const Markers& m, const Spread sp[3], const double* f, double* F0,
                           double* F1, double* F2, hipStream_t s);
-// launch_spread3 of the Schur complement's aux vector
-// aux[c*nb+l] = dt*nu*f[c*nb+l] - dt*f[3nb+l]*n[c*nb+l] (c = 0..2), each
-// entry forming its aux value from the force f (4nb) itself.
-hipError_t launch_spread3_aux(const Markers& m, const Spread sp[3], const double* f, double dt,
-                              double nu, double* F0, double* F1, double* F2, hipStream_t s);
 // Fills sp.ml and sp.wfs of component comp from the stencils (once per body).
 hipError_t launch_spread_coeffs(const Markers& m, const Spread& sp, int comp, hipStream_t s);
 
