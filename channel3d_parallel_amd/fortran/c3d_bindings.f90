@@ -247,6 +247,22 @@ Module c3d_bindings
        Integer(c_int), Value :: maxit, guess_mode
      End Function c3d_ib_set_solver
 
+     ! Refinement sweeps per projection (mixed precision; FP64 defect
+     ! correction on an FP64 context) and the residual entering each sweep.
+     Integer(c_int) Function c3d_set_refinement(ctx, nref) Bind(C, name="c3d_set_refinement")
+       Import :: c_int, c_ptr
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: nref
+     End Function c3d_set_refinement
+
+     Integer(c_int) Function c3d_refinement_residuals(ctx, rmax, n) &
+          Bind(C, name="c3d_refinement_residuals")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: rmax(*)
+       Integer(c_int), Value :: n
+     End Function c3d_refinement_residuals
+
      Integer(c_int) Function c3d_ib_set_force(ctx, fb) Bind(C, name="c3d_ib_set_force")
        Import :: c_int, c_ptr, c_double
        Type(c_ptr), Value :: ctx

@@ -52,16 +52,20 @@ Contains
 
   ! Counterpart of `initialize` for the projection state
   ! (src/initialization.f90:24-460): grid, Dyy, wavenumbers, resident fields.
+  ! prec (optional): C3D_PREC_FP64 (default, the reference's arithmetic)
+  ! or C3D_PREC_MIXED.
   Subroutine c3d_initialize(nx_global, ny_global, nz_global, grid_type, Lxp, Lzp, Ly_channel, &
-                            alpha_stretch, nu_in, min_buffer_width)
+                            alpha_stretch, nu_in, min_buffer_width, prec)
     Integer(Int32), Intent(In) :: nx_global, ny_global, nz_global, grid_type
     Real(Int64), Intent(In)    :: Lxp, Lzp, Ly_channel, alpha_stretch, nu_in
     Real(Int64), Intent(In), Optional :: min_buffer_width  ! grid_type 2 only
+    Integer(Int32), Intent(In), Optional :: prec
     Type(c3d_desc) :: d
     Integer(c_int) :: rc
     d%nx = nx_global; d%ny = ny_global; d%nz = nz_global; d%grid_type = grid_type
     d%Lxp = Lxp; d%Lzp = Lzp; d%Ly = Ly_channel; d%alpha = alpha_stretch; d%nu = nu_in
     d%precision = C3D_PREC_FP64
+    If (Present(prec)) d%precision = prec
     d%min_buffer_width = 0d0
     If (Present(min_buffer_width)) d%min_buffer_width = min_buffer_width
     rc = c3d_create(d, ctx)

@@ -4,8 +4,10 @@
 ! src/time_integration.f90:117-123 (BC, projection, BC).                     !
 !                                                                            !
 ! usage: projection_driver nx ny nz grid_type alpha dt rk_step in out        !
+!                          [precision nref]                                  !
 !   in : raw FP64 U, V, W in the reference's Fortran layout                  !
 !   out: raw FP64 U, V, W, P, P_interim, then max|div|                       !
+!   precision: 0 FP64 (default), 1 mixed; nref >= 0: refinement sweeps       !
 !------------------------------------------------------------------------------!
 Program projection_driver
   Use iso_fortran_env, Only : Int32, Int64, error_unit
@@ -15,14 +17,21 @@ Program projection_driver
   Use monitor
   Implicit None
 
-  Integer(Int32) :: nxa, nya, nza, gtype, rk, iu
+  Integer(Int32) :: nxa, nya, nza, gtype, rk, iu, prec, nref
   Real(Int64)    :: alpha, dt_in, max_div
   Character(256) :: arg, fin, fout
   Real(Int64), Allocatable :: U(:,:,:), V(:,:,:), W(:,:,:), P(:,:,:), Pint(:,:,:)
 
-  If (command_argument_count() /= 9) Then
-     Write(error_unit,*) 'usage: projection_driver nx ny nz grid_type alpha dt rk_step in out'
+  If (command_argument_count() /= 9 .And. command_argument_count() /= 11) Then
+     Write(error_unit,*) 'usage: projection_driver nx ny nz grid_type alpha dt rk_step in out ', &
+          '[precision nref]'
      Error Stop 2
+  End If
+  prec = C3D_PREC_FP64
+  nref = -1
+  If (command_argument_count() == 11) Then
+     Call get_command_argument(10, arg); Read(arg,*) prec
+     Call get_command_argument(11, arg); Read(arg,*) nref
   End If
   Call get_command_argument(1, arg); Read(arg,*) nxa
   Call get_command_argument(2, arg); Read(arg,*) nya
@@ -34,7 +43,9 @@ Program projection_driver
   Call get_command_argument(8, fin)
   Call get_command_argument(9, fout)
 
-  Call c3d_initialize(nxa, nya, nza, gtype, 8d0*Atan(1d0), 4d0*Atan(1d0), 2d0, alpha, 1d0)
+  Call c3d_initialize(nxa, nya, nza, gtype, 8d0*Atan(1d0), 4d0*Atan(1d0), 2d0, alpha, 1d0, &
+                      prec=prec)
+  If (nref >= 0) Call c3d_check(c3d_set_refinement(ctx, nref), 'c3d_set_refinement')
   dt = dt_in
   rk_step = rk
 

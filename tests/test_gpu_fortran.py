@@ -62,17 +62,21 @@ def test_fortran_rk3_time_loop(tmp_path):
     assert abs(utau_gpu - utau) <= 1e-10 * utau
 
 
-@pytest.mark.parametrize("nx,ny,nz,gt,rk", [(34, 65, 34, 1, 1), (66, 33, 34, 0, 2)])
-def test_fortran_projection_driver(tmp_path, nx, ny, nz, gt, rk):
+@pytest.mark.parametrize("nx,ny,nz,gt,rk,prec,nref", [(34, 65, 34, 1, 1, 0, -1), (66, 33, 34, 0, 2, 0, -1),
+                                                     (34, 65, 34, 1, 2, 0, 1), (34, 65, 34, 1, 1, 1, -1)])
+def test_fortran_projection_driver(tmp_path, nx, ny, nz, gt, rk, prec, nref):
+    """The Fortran host's projection: FP64 (the reference's arithmetic), FP64
+    with one defect-correction sweep, and mixed precision with its default
+    sweep (c3d_initialize's prec argument, c3d_set_refinement)."""
     if not os.path.exists(DRIVER):
         pytest.fail("Fortran driver not built (make -C channel3d_parallel_amd/fortran)")
-    print(f"\n[case] fortran driver {nx}x{ny}x{nz} gt={gt} rk={rk}", flush=True)
+    print(f"\n[case] fortran driver {nx}x{ny}x{nz} gt={gt} rk={rk} prec={prec} nref={nref}", flush=True)
     U, V, W = fields.white_faces(nx, ny, nz, seed=21)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     np.concatenate([U.ravel("F"), V.ravel("F"), W.ravel("F")]).tofile(fin)
     dt = 3e-3
     subprocess.run([DRIVER, str(nx), str(ny), str(nz), str(gt), "2.0", repr(dt), str(rk),
-                    str(fin), str(fout)], check=True, timeout=120)
+                    str(fin), str(fout), str(prec), str(nref)], check=True, timeout=120)
     raw = np.fromfile(fout)
     g = orc.Grid(nx, ny, nz, grid_type=gt)
     shapes = [g.shape_u, g.shape_v, g.shape_w, g.shape_p, g.shape_rhs]
@@ -86,14 +90,16 @@ def test_fortran_projection_driver(tmp_path, nx, ny, nz, gt, rk):
     P = np.zeros(g.shape_p, order="F")
     ref = g.projection(U, V, W, rk_step=rk, dt=dt, P=P)
     g.apply_bc(U, V, W)  # the driver reads back before the trailing BC; compare interiors
+    tol = 1e-11 if prec == 1 else 1e-12  # the mixed path's bar
     for name, a, b in (("U", outs[0], U), ("V", outs[1], V), ("W", outs[2], W),
                        ("P_interim", outs[4], ref["P_interim"])):
         sl = (slice(1, -2), slice(1, -1), slice(1, -2))
         r = np.linalg.norm(a[sl] - b[sl]) / np.linalg.norm(b[sl])
-        assert r <= 1e-12, (name, r)
+        assert r <= tol, (name, r)
     if rk == 1:
-        assert np.linalg.norm(outs[3] - P) <= 1e-12 * np.linalg.norm(P)
-    assert max_div <= max(1e-12, 4 * g.max_abs_div(U, V, W))
+        assert np.linalg.norm(outs[3] - P) <= tol * np.linalg.norm(P)
+    # one mixed sweep leaves ~2e-7 of the first solve's divergence (1e-10 bar)
+    assert max_div <= (1e-10 if prec == 1 else max(1e-12, 4 * g.max_abs_div(U, V, W)))
 
 
 CYL_FD = os.path.join(ROOT, "channel3d_parallel_amd", "fortran", "bin", "cylinders_fd")
