@@ -756,8 +756,7 @@ static int projection_launch(c3d_ctx* c, double dt, int rk_step, hipEvent_t* ev)
                              (size_t)kMaxRef * kResSlots * sizeof(unsigned long long), st));
   // first solve; the FP64 divergence b is kept only for the refinement
   if (mixed)
-    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, nref > 0 ? c->b64 : nullptr, nullptr,
-                                  c->S32, st));
+    HIPCHK(c, c3d::launch_fwd_x32(g, c->U, c->V, c->W, nref > 0 ? c->b64 : nullptr, c->S32, st));
   else
     HIPCHK(c, c3d::launch_fwd_x_b64(g, c->U, c->V, c->W, c->b64, c->S, st));
   if (ev) HIPCHK(c, hipEventRecord(ev[1], st));

@@ -49,6 +49,26 @@ __device__ __forceinline__ void dft_small(cpx<real>* v) {
     v[2] = a0 - a2;
     v[1] = a1 + a3;
     v[3] = a1 - a3;
+  } else if constexpr (R == 6) {
+    // even and odd samples through two 3-point DFTs, then
+    // X_k = A_k + w6^k B_k, X_{k+3} = A_k - w6^k B_k (k = 0, 1, 2) with
+    // w6 = exp(SIGN 2 pi i / 6): w6^1 = 1/2 + SIGN i sqrt(3)/2,
+    // w6^2 = -1/2 + SIGN i sqrt(3)/2
+    const real s3 = (real)0.86602540378443864676372317075294;  // sqrt(3)/2
+    cpx<real> a[3] = {v[0], v[2], v[4]};
+    cpx<real> b[3] = {v[1], v[3], v[5]};
+    dft_small<real, 3, SIGN>(a);
+    dft_small<real, 3, SIGN>(b);
+    const cpx<real> b1 = cmk((real)0.5 * b[1].x - (real)SIGN * s3 * b[1].y,
+                             (real)0.5 * b[1].y + (real)SIGN * s3 * b[1].x);
+    const cpx<real> b2 = cmk((real)-0.5 * b[2].x - (real)SIGN * s3 * b[2].y,
+                             (real)-0.5 * b[2].y + (real)SIGN * s3 * b[2].x);
+    v[0] = a[0] + b[0];
+    v[3] = a[0] - b[0];
+    v[1] = a[1] + b1;
+    v[4] = a[1] - b1;
+    v[2] = a[2] + b2;
+    v[5] = a[2] - b2;
   } else if constexpr (R == 8) {
     const real c = (real)0.70710678118654752440084436210485;  // sqrt(2)/2
     cpx<real> e[4] = {v[0], v[2], v[4], v[6]};
@@ -140,7 +160,7 @@ __device__ __forceinline__ void fft_lds(cpx<real>* buf, const cpx<real>* __restr
   else if constexpr (N == 128) fft_passes<real, N, NT, SIGN, 1, 8, 4, 4>::run(buf, tw, t);
   else if constexpr (N == 192) fft_passes<real, N, NT, SIGN, 1, 8, 8, 3>::run(buf, tw, t);
   else if constexpr (N == 256) fft_passes<real, N, NT, SIGN, 1, 8, 8, 4>::run(buf, tw, t);
-  else if constexpr (N == 384) fft_passes<real, N, NT, SIGN, 1, 8, 8, 2, 3>::run(buf, tw, t);
+  else if constexpr (N == 384) fft_passes<real, N, NT, SIGN, 1, 8, 8, 6>::run(buf, tw, t);
   else if constexpr (N == 512) fft_passes<real, N, NT, SIGN, 1, 8, 8, 8>::run(buf, tw, t);
   else static_assert(N == 16, "unsupported FFT length");
 }

@@ -18,7 +18,11 @@ namespace c3d {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGradKCDefault = 2;  // z planes per lane in the gradient correction
+// z planes per lane in the gradient correction: the single potential, and
+// the two-part potential of the refinement (which loads twice the potential
+// words per face, so reusing p(k+1) over more planes pays)
+constexpr int kGradKCDefault = 2;
+constexpr int kGradKCSplitDefault = 4;
 
 // Rows of the z tile handled by one x-transform workgroup (row pairs are
 // packed into one complex transform).  Default: 16 rows (256-byte kz runs in
@@ -47,7 +51,6 @@ int tile_rows(int nxp, int nzp) {
 constexpr int kSrcRhs = 0;    // rhs_p-shaped FP64 array
 constexpr int kSrcUVW = 1;    // divergence of U, V, W computed on the fly
 constexpr int kSrcUVWB = 2;   // same, and the FP64 divergence also stored to bout
-constexpr int kSrcRhs32 = 3;  // rhs_p-shaped FP32 array (refinement residual)
 constexpr int kSrcUVWc = 4;   // as kSrcUVW with cached U and W loads (fields that fit in cache)
 constexpr int kSrcResid = 5;  // FP64 residual b - M phi of the refinement, on the fly
 
@@ -106,10 +109,16 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   double pm[NPL], p0[NPL];
   double amax = 0.0;
   double gj = 0.0, rhy = 0.0, rhym = 0.0;
+  // the y neighbours' row offsets: on the first and last rows (no gradient
+  // on the wall face) the row itself, read with a zero coefficient, so the
+  // loads need no branch
+  int jpo = 0, jmo = 0;
   if constexpr (RESID) {
     gj = (j == 2) ? *gauge : 0.0;
-    rhy = g.rdyg[j];
-    rhym = g.rdyg[j - 1];
+    rhy = (j <= ny - 1) ? g.rdyg[j] : 0.0;
+    rhym = (j >= 3) ? g.rdyg[j - 1] : 0.0;
+    jpo = (j <= ny - 1) ? rowj : 0;
+    jmo = (j >= 3) ? rowj : 0;
     const int kf = k0 + 2, km0 = (kf == 2) ? g.nz - 1 : kf - 1;
 #pragma unroll
     for (int p = 0; p < NPL; p++) {
@@ -164,17 +173,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
             v[r][p][0] = Br[n];
             v[r][p][1] = Pr[n + pk];                          // phi(i, j, k+1)
             v[r][p][2] = Pr[n + 1];                           // phi(i+1, j, k)
-            v[r][p][3] = (n == 0) ? Pr[N - 1] : Pr[n - 1];    // phi(i-1), periodic at i = 2
-            v[r][p][4] = (j <= ny - 1) ? Pr[n + rowj] : 0.0;  // phi(i, j+1, k)
-            v[r][p][5] = (j >= 3) ? Pr[n - rowj] : 0.0;       // phi(i, j-1, k)
+            v[r][p][3] = Pr[(n == 0) ? N - 1 : n - 1];  // phi(i-1), periodic at i = 2
+            v[r][p][4] = Pr[n + jpo];                   // phi(i, j+1, k)
+            v[r][p][5] = Pr[n - jmo];                   // phi(i, j-1, k)
           }
-        }
-      } else if constexpr (SRC == kSrcRhs32) {
-        const float* Rr = static_cast<const float*>(rhs_in) + ridx(2, j, k, nxg, nyg);
-#pragma unroll
-        for (int p = 0; p < NPL; p++) {
-          const int n = tid + p * kBlock;
-          if (N % kBlock == 0 || n < N) v[r][p][0] = (double)Rr[n];
         }
       } else {
         const double* Rr = static_cast<const double*>(rhs_in) + ridx(2, j, k, nxg, nyg);
@@ -214,8 +216,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
               const double c0 = p0[p];
               const double gxp = (q[2] - c0) * g.rdxg[i];
               const double gxm = (c0 - q[3]) * g.rdxg[(i == 2) ? nx - 1 : i - 1];
-              const double gyp = (j <= ny - 1) ? (q[4] - c0) * rhy : 0.0;
-              const double gym = (j >= 3) ? (c0 - q[5]) * rhym : 0.0;
+              const double gyp = (q[4] - c0) * rhy;
+              const double gym = (c0 - q[5]) * rhym;
               const double gzp = (q[1] - c0) * g.rdzg[k];
               const double gzm = (c0 - pm[p]) * g.rdzg[km];
               const double lap = (gxp - gxm) * rx + (gyp - gym) * ry + (gzp - gzm) * g.rdzf[k];
@@ -417,15 +419,15 @@ constexpr int kTriList = 2;  // the modes list[0 .. nlist)
 
 // real: storage of the right-hand side read from S; rout: storage of the
 // solution written to Sout (in place when both are the same array).
-template <typename real, typename acc, int M, int SEL, typename rout = real>
+// P: parts per mode; MWX (when set) overrides the modes per wave.
+template <typename real, typename acc, int M, int SEL, typename rout = real, int P = 8, int MWX = 0>
 __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
                                                      const int* __restrict__ list, int nlist,
                                                      double lam_thr, cpx<rout>* Sout) {
-  constexpr int P = 8;            // parts per mode
   constexpr int N = P * M;        // rows
   // modes per wave: a row access of a wave is MW consecutive modes, 256
   // contiguous bytes of the input spectrum (16 FP64 or 32 FP32 complex)
-  constexpr int MW = 256 / (int)sizeof(cpx<real>);
+  constexpr int MW = MWX ? MWX : 256 / (int)sizeof(cpx<real>);
   constexpr int PW = 64 / MW;     // parts per wave
   constexpr int WG = P / PW;      // waves that cover the parts of a mode group
   static_assert(WG * PW == P && kBlock / 64 % WG == 0, "waves must tile the parts");
@@ -1104,19 +1106,17 @@ hipError_t launch_fwd_x(const DevGrid& g, bool from_uvw, const double* U, const 
 }
 
 hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
-                          double* b, const float* r, cpx<float>* S, hipStream_t s) {
+                          double* b, cpx<float>* S, hipStream_t s) {
+  if (!U || !V || !W) return hipErrorInvalidValue;
   const int tk = tile_rows(g.nxp, g.nzp);
   dim3 grid(g.nzp / tk, g.nym);
   C3D_DISPATCH_N(g.nxp, C3D_DISPATCH_TK(tk, {
-    if (U && b)
+    if (b)
       hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcUVWB>), grid, dim3(kBlock), 0, s, g, U, V, W,
                          (const void*)nullptr, g.twx32, S, b, nullptr, nullptr);
-    else if (U)
+    else
       hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcUVW>), grid, dim3(kBlock), 0, s, g, U, V, W,
                          (const void*)nullptr, g.twx32, S, nullptr, nullptr, nullptr);
-    else
-      hipLaunchKernelGGL((k_fwd_x<float, NN, TK, kSrcRhs32>), grid, dim3(kBlock), 0, s, g, U, V, W,
-                         (const void*)r, g.twx32, S, nullptr, nullptr, nullptr);
   }));
   return hipGetLastError();
 }
@@ -1208,6 +1208,16 @@ hipError_t launch_fft_z32(const DevGrid& g, int sign, cpx<float>* S, hipStream_t
     default: return hipErrorInvalidValue;            \
   }
 
+// 16 parts of nym/16 rows (152 VGPRs at nym = 192: 3 waves per SIMD against
+// 2 with 8 parts of 24 rows).
+#define C3D_DISPATCH_M16(nym, BODY)                  \
+  switch (nym) {                                     \
+    case 64: { constexpr int MM = 4; BODY; } break;    \
+    case 128: { constexpr int MM = 8; BODY; } break;   \
+    case 192: { constexpr int MM = 12; BODY; } break;  \
+    default: return hipErrorInvalidValue;            \
+  }
+
 hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t s) {
   const size_t nm = (size_t)g.nxh * g.nzp;
   const dim3 grid((unsigned)((nm + kBlock - 1) / kBlock));
@@ -1245,11 +1255,14 @@ hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlo
 }
 
 hipError_t launch_tri32to64(const DevGrid& g, cpx<float>* S32, cpx<double>* S64, hipStream_t s) {
+  // FP32 right-hand side, FP64 solution: 16 parts of nym/16 rows, 16 modes
+  // per wave (94 against 99 us with 8 parts at 384x193x384; the FP64 solve
+  // gains nothing from it and keeps 8 parts)
   const size_t nm = (size_t)g.nxh * g.nzp;
-  constexpr int MB = kBlock / 8;
+  constexpr int MB = kBlock / 16;
   const dim3 grid((unsigned)((nm + MB - 1) / MB));
-  C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll, double>), grid,
-                                           dim3(kBlock), 0, s, g, S32, nullptr, 0, 0.0, S64));
+  C3D_DISPATCH_M16(g.nym, hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll, double, 16, 16>),
+                                             grid, dim3(kBlock), 0, s, g, S32, nullptr, 0, 0.0, S64));
   return hipGetLastError();
 }
 
@@ -1306,23 +1319,24 @@ hipError_t launch_p_ghosts(const DevGrid& g, double* P, hipStream_t s) {
   return hipGetLastError();
 }
 
-// z planes per lane of the gradient correction (2 measured fastest at
-// 384x193x384: more workgroups in flight); C3D_GRAD_KC (2, 4, 8, 16)
-// overrides it (tuning knob).
-static int grad_kc() {
-  static const int kc = [] {
+// z planes per lane of the gradient correction (at 384x193x384, 2 measured
+// fastest for the single potential: 0.303 against 0.305 ms for 1, more
+// workgroups in flight; 4 for the two-part potential: 0.457 against
+// 0.468 ms for 2); C3D_GRAD_KC (1, 2, 4, 8, 16) overrides both (tuning knob).
+static int grad_kc(bool split) {
+  static const int forced = [] {
     const char* e = getenv("C3D_GRAD_KC");
     const int v = e ? atoi(e) : 0;
-    return (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ? v : kGradKCDefault;
+    return (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ? v : 0;
   }();
-  return kc;
+  return forced ? forced : (split ? kGradKCSplitDefault : kGradKCDefault);
 }
 
 namespace {
 template <bool SPLIT>
 hipError_t grad_correct_any(const DevGrid& g, double* U, double* V, double* W, const double* Pint,
                             const double* phic, double* pout, double dt, hipStream_t s) {
-  const int kc = grad_kc();
+  const int kc = grad_kc(SPLIT);
   const dim3 grid = grid_pts(g, g.nx - 2, g.nyg - 2, (g.nzg - 2 + kc - 1) / kc);
   switch (kc) {
 #define C3D_GC_CASE(KC)                                                                         \
@@ -1331,14 +1345,13 @@ hipError_t grad_correct_any(const DevGrid& g, double* U, double* V, double* W, c
                        phic, pout, dt);                                                         \
     break;
     C3D_GC_CASE(1)
+    C3D_GC_CASE(2)
     C3D_GC_CASE(4)
     C3D_GC_CASE(8)
     C3D_GC_CASE(16)
 #undef C3D_GC_CASE
     default:
-      hipLaunchKernelGGL((k_grad_correct<2, SPLIT>), grid, dim3(kBlock), 0, s, g, U, V, W, Pint,
-                         phic, pout, dt);
-      break;
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -64,11 +64,10 @@ hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t 
 hipError_t launch_inv_x(const DevGrid& g, int mode, const cpx<double>* S, double* out, double* P,
                         double dt, bool rk1, hipStream_t s);
 // Refinement pieces (mixed precision: FP32 forward spectra; FP64: the same
-// sweeps on FP64 spectra).  launch_fwd_x32 with U != null: divergence of
-// U, V, W (FP64, also stored to b when b != null) -> FP32 spectrum; with
-// U == null: the FP32 array r -> FP32 spectrum.
+// sweeps on FP64 spectra).  launch_fwd_x32: divergence of U, V, W (FP64,
+// also stored to b when b != null) -> FP32 spectrum.
 hipError_t launch_fwd_x32(const DevGrid& g, const double* U, const double* V, const double* W,
-                          double* b, const float* r, cpx<float>* S, hipStream_t s);
+                          double* b, cpx<float>* S, hipStream_t s);
 // FP64 spectrum of div(U, V, W), the FP64 divergence also stored to b.
 hipError_t launch_fwd_x_b64(const DevGrid& g, const double* U, const double* V, const double* W,
                             double* b, cpx<double>* S, hipStream_t s);
