@@ -71,7 +71,7 @@ def test_mixed_projection_384():
     """The 384x192x384-cell workload (386x193x386 faces), tanh alpha 2, the
     seed-3 k^-5/3 random-phase field at unit RMS.  One sweep (the default)
     reaches the FP64 oracle to relative 1e-11 (U, V, W, P_interim) and
-    max|div| <= 1e-10, which the plain FP64 path (7.5e-10) does not; every
+    max|div| <= 1e-10, which the plain FP64 path (7.2e-10) does not; every
     sweep cuts the FP64 residual by >= 1e3 until the FP64 floor; the wall
     stress is the FP64 path's; 32 back-to-back solves from the same input
     give the same bits."""
