@@ -7,14 +7,18 @@ Bars (stated per check):
   E, E1n, R, mean removal     relative 1e-13 (summation order)
   Heaviside outputs           absolute 1e-12 (one Poisson solve)
   Schur products              relative 1e-12 of the largest entry
+  BiCGSTAB iterates           relative 1e-13 for the first 7 iterations
+                              (the iteration itself, step by step)
   BiCGSTAB / IB projection    both converge to the absolute tolerance; the
                               GPU solution satisfies the oracle's Schur
                               system to that tolerance; solutions agree to
                               1e-7 relative and fields to 1e-8.  Iteration
-                              counts agree only to ~40%: on this system a
+                              counts agree to 10% on the projection cases;
+                              on the random right-hand side of
+                              test_bicgstab_solution only to ~40%: there a
                               1e-14 perturbation of the oracle's own initial
                               guess moves its count from 125 to 103 while
-                              the solution moves by 1e-9 relative, so the
+                              the solution moves by 1e-9 relative, so that
                               count is not a parity quantity.
 """
 import numpy as np
@@ -106,8 +110,40 @@ def test_bicgstab_solution():
     assert eg < tol ** 2 and eo < tol ** 2
     # the GPU solution solves the oracle's system
     assert np.linalg.norm(ib.schur(xg, dt, NU) - b) < 5 * tol
+    # The counts are not expected to agree closely on this random right-hand
+    # side: the iterates are identical to rounding for 7 iterations (see
+    # test_bicgstab_iterates_match_oracle), after which the unpreconditioned
+    # iteration amplifies the two Schur products' rounding differences
+    # (~1e-10 at iteration 8, ~4e-6 at 9), so the paths to the tolerance differ
+    # (93 against 125 iterations measured).
     assert abs(itg - ito) <= 0.4 * ito
     assert rel(xg, xo) < 1e-7
+
+
+def test_bicgstab_iterates_match_oracle():
+    """BiCGSTAB step by step (src/projection.f90:146-186): stopped after
+    k = 1..7 iterations (maxit = k, tolerance out of reach, C3D_ENOCONV), the
+    GPU iterate and residual equal the oracle's to rounding (measured 4e-16 to
+    1e-15 relative), which pins the update order and every scalar (rho,
+    alpha, omega, beta) of the reference's iteration."""
+    import ctypes as C
+    from channel3d_parallel_amd._lib import ptr
+    g, m, ib, ch = case(34, 33, 34, 24, 16)
+    ib.heaviside()
+    rng = np.random.default_rng(9)
+    dt = 0.005
+    b = ib.schur(rng.standard_normal(4 * m.nb), dt, NU)
+    with ch:
+        for k in range(1, 8):
+            print(f"  [case] bicgstab maxit={k}", flush=True)
+            x = np.zeros(4 * m.nb)
+            it, err = C.c_int(), C.c_double()
+            rc = ch.L.c3d_ib_bicgstab(ch.h, ptr(x), ptr(b), 1e-30, k, dt, 2, C.byref(it), C.byref(err))
+            xo, ito, eo = ib.bicgstab(b, tol=1e-30, maxit=k, dt=dt, nu=NU)
+            print(f"    rel x {rel(x, xo):.2e}; ||r||^2 gpu {err.value:.6e} oracle {eo:.6e}")
+            assert rc == -4 and it.value == k and ito == k
+            assert rel(x, xo) < 1e-13
+            assert abs(err.value - eo) <= 1e-12 * eo
 
 
 def _laminar(g, nx, ny, nz):
@@ -139,7 +175,7 @@ def test_ib_projection_parity():
         slip, div = ch.check_slip(), ch.check_divergence()
     print(f"  iterations gpu={itg} oracle={ito}; slip={slip:.2e}; div={div:.2e}")
     assert eg < tol ** 2 and eo < tol ** 2
-    assert abs(itg - ito) <= 0.4 * ito
+    assert abs(itg - ito) <= 0.1 * ito  # measured 116 against 114
     assert slip < 1e-9 and div < 1e-12
     assert abs(slip - ib.max_slip(r["U"], r["V"], r["W"], fbg)) < 1e-12
     assert rel(fbg, fbo) < 1e-6
@@ -279,7 +315,8 @@ def test_twocyl130_projection_golden():
         fb = ch.get_IB_force()
         slip, div = ch.check_slip(), ch.check_divergence()
     print(f"  iterations gpu={it} oracle={ref['iterations']}; slip={slip:.2e}; div={div:.2e}")
-    assert err < ref["tol"] ** 2 and abs(it - ref["iterations"]) <= 0.4 * ref["iterations"]
+    # measured 25 against the oracle's 25
+    assert err < ref["tol"] ** 2 and abs(it - ref["iterations"]) <= 0.1 * ref["iterations"]
     assert slip < 1e-9 and div < 1e-12
     for name, a in (("U", r["U"]), ("V", r["V"]), ("W", r["W"])):
         w = fields.uniform(77, a.size).reshape(a.shape, order="F")
