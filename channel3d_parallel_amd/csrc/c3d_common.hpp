@@ -56,6 +56,18 @@ __device__ __forceinline__ void xcd_tile(unsigned& bx, unsigned& by, unsigned& b
   bz = lid / (gridDim.x * gridDim.y);
 }
 
+// Whole-wavefront lane shift of an FP64 value through DPP (wave_shl:1, two
+// 32-bit moves, no LDS): lane L receives the value of lane L+1.  Lane 63, and
+// any lane whose source lane is inactive, receives 0, so the caller supplies
+// those values itself.
+__device__ __forceinline__ double wave_shl1(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x130, 0xf, 0xf, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Fortran column-major offset of 1-based (i,j,k) in an (n1,n2,*) array.
 __host__ __device__ __forceinline__ size_t fidx(int i, int j, int k, int n1, int n2) {
   return (size_t)(i - 1) + (size_t)n1 * ((size_t)(j - 1) + (size_t)n2 * (size_t)(k - 1));

@@ -819,6 +819,11 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
   const int kb = bz * KC + 2;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   if (i > nx - 1 || j > nyg - 1) return;
+  // the potential at i+1 is the next lane's (a wavefront holds 64
+  // consecutive i of one row): moved across lanes, not loaded again.  The
+  // wave's last face ie (lane 63, or i = nx-1 in a row's last wave) takes
+  // it from a load at one address for the whole wave (one cache line).
+  const int ie = min((int)bx * 64 + 65, nx - 1);
   const double hx = g.dxg[i];
   const double hy = (j <= ny - 1) ? g.dyg[j] : 1.0;
   // potential at a cell: dt*Pint (the reference's operand), or the two parts
@@ -852,8 +857,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
         if (i == 2) pout[ridx(nxg - 1, j, nzg, nxg, nyg)] = 0.0;
       }
     }
-    double pxc;
-    const double px = pot(ridx(i + 1, j, k, nxg, nyg), pxc);
+    const double pxn = wave_shl1(p);
+    const double pxcn = SPLIT ? wave_shl1(pc) : 0.0;
+    double pxce;
+    const double pxe = pot(ridx(ie + 1, j, k, nxg, nyg), pxce);
+    const double px = (i == ie) ? pxe : pxn;
+    const double pxc = (i == ie) ? pxce : pxcn;
     const size_t iu = fidx(i, j, k, nx, nyg);
     const double u = U[iu] - dif(px, pxc, p, pc) / hx;
     U[iu] = u;
