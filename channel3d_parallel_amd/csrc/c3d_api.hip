@@ -2525,6 +2525,7 @@ int c3d_mlp_train(c3d_ctx* c, const float* X, const float* T, int n, int epochs,
   float *DX = nullptr, *DT = nullptr;
   if (hipMalloc(&DX, (size_t)n * N::kIn * 4) != hipSuccess ||
       hipMalloc(&DT, (size_t)n * N::kOut * 4) != hipSuccess) {
+    (void)hipGetLastError();  // the launchers report hipGetLastError()
     if (DX) (void)hipFree(DX);
     return fail(c, C3D_ENOMEM, "training set buffers");
   }
@@ -2593,6 +2594,7 @@ int c3d_mlp_gemm_test(c3d_ctx* c, int M, int N, int K, int ta, int tb, const flo
   const size_t na = (size_t)M * K, nbv = (size_t)K * N, nc = (size_t)M * N;
   if (hipMalloc(&a, na * 4) != hipSuccess || hipMalloc(&b, nbv * 4) != hipSuccess ||
       hipMalloc(&d, nc * 4) != hipSuccess) {
+    (void)hipGetLastError();  // the launchers report hipGetLastError()
     rc = fail(c, C3D_ENOMEM, "gemm test buffers");
   } else {
     const int lda = ta ? M : K, ldb = tb ? K : N;
