@@ -4,7 +4,9 @@ full time step) against the oracle.
 Tolerances: right-hand sides and stage updates are bitwise (same operations
 in the same order, FMA contraction off); the bulk-flow reduction sums in a
 different order (relative 1e-14); full time steps include the projection
-(relative L2 1e-12 after one step, 1e-10 after 100 steps of the 32x64x32-cell channel).
+(relative L2 1e-12 after one step, 1e-10 after 100 steps of the 32x64x32-cell
+channel; at 192x128x192 one step to 1e-12 once the constant-flow correction's
+uniform shift, the oracle's serial-summation error, is taken out).
 """
 import numpy as np
 import pytest
@@ -126,6 +128,75 @@ def test_channel_32x64x32_time_steps(nsteps, tol, exact):
     # reduction (~1e-14) is amplified by 1/dt, so the bound is absolute
     assert abs(dpdx - dpdx_r) <= 1e-11
     assert div <= 1e-10
+    assert abs(q - q0) <= 1e-12
+
+
+def _bulk_flow_exact(g, U):
+    """mean_mass_flow_U (src/mass_flow.f90:24-53) with both sums exactly
+    rounded (math.fsum), the terms formed as the oracle forms them."""
+    import math
+    nx, nyg, nzg = g.nx, g.nyg, g.nzg
+    yg = g.get(4)
+    dy = yg[2:nyg - 1] - yg[1:nyg - 2]  # yg(j) - yg(j-1), j = 3..nyg-1
+    Ui = U[1:nx - 1, :, 1:nzg - 1]
+    terms = (Ui[:, 2:nyg - 1, :] + Ui[:, 1:nyg - 2, :]) * 0.5 * dy[None, :, None]
+    return math.fsum(terms.ravel()) / math.fsum(np.broadcast_to(dy[None, :, None], terms.shape).ravel())
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_channel_192x128x192_time_step(exact):
+    """One RK3 step at the 192x128x192-cell size (194x129x194 faces, tanh
+    alpha 2): laminar parabola plus a divergence-free perturbation, constant
+    flow rate, dt from compute_dt at CFL 0.5, both right-hand-side modes
+    against the oracle.
+
+    The constant-flow correction adds d = Q0 - Q(U) to every interior U face
+    (src/mass_flow.f90).  The oracle sums Q in the reference's serial order,
+    whose rounding grows with the cell count: against an exactly rounded sum
+    it is off by 3e-14 at 32x64x32, 5e-12 here and 5e-11 at 384x192x384,
+    while the library's row sums and fixed-order tree match the exact sum.
+    So U is compared after removing the uniform shift the two corrections
+    differ by, and that shift must equal the oracle's own summation error on
+    the field it corrected (to 1e-13); V, W and P to 1e-12 as at 32x64x32."""
+    nx, ny, nz = 194, 129, 194
+    g, U, V, W = state(nx, ny, nz, kind="ic", seed=1)
+    q0 = g.mean_mass_flow_u(U)
+    Ur, Vr, Wr = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    Pr = g.zeros("p")
+    with Channel(nx, ny, nz, nu=NU) as ch:
+        ch.set_rhs_exact(exact)
+        ch.set_fields(U, V, W, P=np.zeros(g.shape_p, order="F"))
+        ch.set_pressure_gradient(DPDX, 0.0, 0.0)
+        dt = ch.compute_dt(0.5)
+        dt_r = g.compute_dt(Ur, Vr, Wr, 0.5, NU)
+        assert abs(dt - dt_r) <= 1e-15 * dt_r
+        ch.compute_time_step_RK3(dt, x_mass_cte=1, q0=q0)
+        dpdx_r, _ = g.time_step_rk3(Ur, Vr, Wr, Pr, dt_r, NU, DPDX, q0)
+        out = ch.get_fields()
+        dpdx = ch.get_pressure_gradient()[0]
+        div = ch.check_divergence()
+        q = ch.mean_mass_flow_u()
+    for name, a, b in (("V", out["V"], Vr), ("W", out["W"], Wr), ("P", out["P"], Pr)):
+        r = rel(a, b)
+        print(f"  one step {name}: rel L2 = {r:.3e}")
+        assert r <= 1e-12, name
+    # the oracle's correction d_r (dPdx = (DPDX + d/dt)/2) and the field it corrected
+    d_r = (2.0 * dpdx_r - DPDX) * dt_r
+    Ustar = Ur.copy(order="F")
+    Ustar[1:nx - 1, 1:ny, 1:nz] -= d_r
+    expected = g.mean_mass_flow_u(Ustar) - _bulk_flow_exact(g, Ustar)
+    diff = (out["U"] - Ur)[1:nx - 1, 1:ny, 1:nz]
+    shift = float(diff.mean())
+    r = np.linalg.norm(diff - shift) / np.linalg.norm(Ur[1:nx - 1, 1:ny, 1:nz])
+    print(f"  one step U: rel L2 = {rel(out['U'], Ur):.3e}; uniform shift {shift:.3e} "
+          f"(oracle summation error {expected:.3e}); without it rel L2 = {r:.3e}")
+    assert abs(shift - expected) <= 1e-13
+    assert r <= 1e-12
+    div_r = g.max_abs_div(Ur, Vr, Wr)
+    print(f"  dPdx gpu={dpdx:.15e} oracle={dpdx_r:.15e}; max|div| gpu={div:.2e} oracle={div_r:.2e}")
+    # the dPdx update carries the same difference, divided by 2 dt
+    assert abs((dpdx - dpdx_r) - shift / (2.0 * dt)) <= 1e-11
+    assert div <= max(1e-12, 4 * div_r)
     assert abs(q - q0) <= 1e-12
 
 
