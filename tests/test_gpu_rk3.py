@@ -5,8 +5,9 @@ Tolerances: right-hand sides and stage updates are bitwise (same operations
 in the same order, FMA contraction off); the bulk-flow reduction sums in a
 different order (relative 1e-14); full time steps include the projection
 (relative L2 1e-12 after one step, 1e-10 after 100 steps of the 32x64x32-cell
-channel; at 192x128x192 one step to 1e-12 once the constant-flow correction's
-uniform shift, the oracle's serial-summation error, is taken out).
+channel; at 192x128x192 and 384x192x384 one step to 1e-12 once the
+constant-flow correction's uniform shift, the oracle's serial-summation error,
+is taken out).
 """
 import numpy as np
 import pytest
@@ -143,12 +144,14 @@ def _bulk_flow_exact(g, U):
     return math.fsum(terms.ravel()) / math.fsum(np.broadcast_to(dy[None, :, None], terms.shape).ravel())
 
 
-@pytest.mark.parametrize("exact", [False, True])
-def test_channel_192x128x192_time_step(exact):
-    """One RK3 step at the 192x128x192-cell size (194x129x194 faces, tanh
-    alpha 2): laminar parabola plus a divergence-free perturbation, constant
-    flow rate, dt from compute_dt at CFL 0.5, both right-hand-side modes
-    against the oracle.
+@pytest.mark.parametrize("nx,ny,nz,exact", [(194, 129, 194, False), (194, 129, 194, True),
+                                             (386, 193, 386, False)])
+def test_channel_time_step_large(nx, ny, nz, exact):
+    """One RK3 step at 192x128x192 cells (194x129x194 faces; both
+    right-hand-side modes) and at 384x192x384 (386x193x386 faces, the
+    benchmarked size; the default fused mode), tanh alpha 2: laminar parabola
+    plus a divergence-free perturbation, constant flow rate, dt from
+    compute_dt at CFL 0.5, against the oracle.
 
     The constant-flow correction adds d = Q0 - Q(U) to every interior U face
     (src/mass_flow.f90).  The oracle sums Q in the reference's serial order,
@@ -158,7 +161,6 @@ def test_channel_192x128x192_time_step(exact):
     So U is compared after removing the uniform shift the two corrections
     differ by, and that shift must equal the oracle's own summation error on
     the field it corrected (to 1e-13); V, W and P to 1e-12 as at 32x64x32."""
-    nx, ny, nz = 194, 129, 194
     g, U, V, W = state(nx, ny, nz, kind="ic", seed=1)
     q0 = g.mean_mass_flow_u(U)
     Ur, Vr, Wr = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
