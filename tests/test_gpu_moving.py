@@ -265,3 +265,58 @@ def test_device_map_dense_markers():
         ch.setup_IB(m)
         b = operators(ch, nb, f)
     assert_same(a, b)
+
+
+def test_time_step_moving_walls_benchmark_size():
+    """The benchmarked moving-wall workload: standing wave (amplitude 0.03,
+    omega 2, one wavelength in x), 128 x 64 markers per wall (nb = 16,384),
+    130x129x66 faces on the buffered grid (alpha 2.6, min_buffer_width 0.1),
+    constant pressure gradient 3 nu.  One RK3 step with the device
+    re-set-up at every stage against the oracle composed stage by stage, at
+    the bars of test_time_steps_with_moving_walls."""
+    nx, ny, nz, nxb, nzb, mbw, alpha = 130, 129, 66, 128, 64, 0.1, 2.6
+    p1, p2, p3, ramp = 0.03, 2.0, 1.0, 0.0
+    print(f"\n[case] standing wave {nx}x{ny}x{nz}, {nxb}x{nzb} markers per wall: one RK3 step", flush=True)
+    g = orc.Grid(nx, ny, nz, alpha=alpha, grid_type=2, min_buffer_width=mbw)
+    U, V, W = fields.channel_ic(nx, ny, nz, pert=0.0, alpha=alpha, grid_type=2, min_buffer_width=mbw)
+    g.apply_bc(U, V, W)
+    dPdx, dt, tol, t0 = 3.0 * NU, 0.005, 1e-12, 0.0
+    rk_t = (8.0 / 15.0, 2.0 / 3.0, 1.0)
+    Ur, Vr, Wr = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    Us = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F")]
+    fb = np.zeros(4 * 2 * nxb * nzb)
+    F, its = [], []
+    for st in (1, 2, 3):
+        F.append(g.rhs(Ur, Vr, Wr, NU, dPdx))
+        for q, X in enumerate((Ur, Vr, Wr)):
+            g.rk_update(q, st, dt, F[0][q], F[1][q] if st > 1 else None,
+                        F[2][q] if st > 2 else None, Us[q], X)
+        m = orc.WaveMarkers(g, "standing_wave", nxb, nzb, p1, p2, p3, t0 + rk_t[st - 1] * dt, ramp)
+        assert m.stop == 0
+        ib = orc.IB(g, m)
+        ib.heaviside()
+        g.apply_bc(Ur, Vr, Wr)
+        out = g.projection(Ur, Vr, Wr, rk_step=st, dt=dt, P=g.zeros("p"))
+        g.apply_bc(Ur, Vr, Wr)
+        fb[:] = 0.0
+        it, _ = ib.projection(Ur, Vr, Wr, out["U_interim"], out["V_interim"], out["W_interim"],
+                              out["P_interim"], fb, dt, NU, tol=tol, maxit=2000, rk_step=st,
+                              P=g.zeros("p"))
+        its.append(it)
+        g.apply_bc(Ur, Vr, Wr)
+    with Channel(nx, ny, nz, alpha=alpha, grid_type=2, nu=NU, min_buffer_width=mbw) as ch:
+        ch.set_fields(U, V, W, P=np.zeros(g.shape_p, order="F"))
+        ch.set_pressure_gradient(dPdx, 0.0, 0.0)
+        ch.time = t0
+        ch.setup_IB_wave("standing_wave", nxb, nzb, p1, p2, p3, ramp)
+        ch.set_IB_solver(tol=tol, maxit=2000, guess_mode=0)
+        ch.compute_time_step_RK3(dt, x_mass_cte=0)
+        r = ch.get_fields()
+        last, accum, err = ch.ib_stats()
+        div, slip = ch.check_divergence(), ch.check_slip()
+    scale = np.linalg.norm(Ur)
+    d = [np.linalg.norm(a - b) / scale for a, b in ((r["U"], Ur), (r["V"], Vr), (r["W"], Wr))]
+    print(f"  oracle iterations {its}; gpu total {accum}; max|div| {div:.2e} slip {slip:.2e}; "
+          f"|difference| / |U|: U {d[0]:.2e} V {d[1]:.2e} W {d[2]:.2e}")
+    assert err < tol ** 2 and div < 1e-12 and slip < 1e-11
+    assert max(d) < 1e-9
