@@ -312,18 +312,19 @@ __global__ __launch_bounds__(kBlock) void k_fft_z(cpx<real>* __restrict__ S,
   const int tid = threadIdx.x;
   const size_t col0 = (size_t)blockIdx.x * NC;
   {
+    // every lane loads (element 0 in place of one past the end) and keeps
+    // zero where it has no element: no branch per load, so that all ITEMS
+    // loads are in flight together (behind branches, the FP32 ones each
+    // waited for their data)
     constexpr int ITEMS = (NC * N + kBlock - 1) / kBlock;
     real vx[ITEMS], vy[ITEMS];
 #pragma unroll
     for (int q = 0; q < ITEMS; q++) {
       const int e = tid + q * kBlock;
-      vx[q] = (real)0;
-      vy[q] = (real)0;
-      if (e < NC * N && col0 + e / N < (size_t)ncol) {
-        const cpx<real> t = S[col0 * N + e];
-        vx[q] = t.x;
-        vy[q] = t.y;
-      }
+      const bool ok = e < NC * N && col0 + e / N < (size_t)ncol;
+      const cpx<real> t = S[ok ? col0 * N + e : 0];
+      vx[q] = ok ? t.x : (real)0;
+      vy[q] = ok ? t.y : (real)0;
     }
 #pragma unroll
     for (int q = 0; q < ITEMS; q++) {
@@ -438,10 +439,15 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
   __shared__ acc red[MB * RP];
   const int tid = threadIdx.x;
   for (int e = tid; e < N; e += kBlock) {
+    // three loads in flight at once: clamped in-bounds indices (dyy_l and
+    // dyy_u hold N-1 entries), the end rows' zero couplings selected after
     const int q = e / M, r = e % M;
-    td[q * CPT + r] = (acc)g.dyy_d[e];
-    tl[q * CPT + r] = (e == 0) ? (acc)0 : (acc)g.dyy_l[e - 1];
-    tu[q * CPT + r] = (e == N - 1) ? (acc)0 : (acc)g.dyy_u[e];
+    const double d = g.dyy_d[e];
+    const double l = g.dyy_l[e > 0 ? e - 1 : 0];
+    const double u = g.dyy_u[e < N - 1 ? e : 0];
+    td[q * CPT + r] = (acc)d;
+    tl[q * CPT + r] = (e == 0) ? (acc)0 : (acc)l;
+    tu[q * CPT + r] = (e == N - 1) ? (acc)0 : (acc)u;
   }
   const int lane = tid & 63;
   const int wv = tid >> 6;
@@ -468,15 +474,19 @@ __global__ __launch_bounds__(kBlock) void k_tri_part(DevGrid g, cpx<real>* S,
   cpx<real>* s = S + mv + (size_t)(p * M) * stride;
   cpx<rout>* so = Sout + mv + (size_t)(p * M) * stride;
 
+  // All M rows are loaded unconditionally and converted afterwards: mv is a
+  // valid mode for every lane, so the addresses stay in bounds, and without
+  // a branch per row (whose FP32 -> FP64 conversion waited for its load) the
+  // M loads of a lane are in flight together.
   acc yr[M], yi[M], cp[M], sp[M];
+  {
+    cpx<real> t[M];
 #pragma unroll
-  for (int r = 0; r < M; r++) {
-    yr[r] = (acc)0;
-    yi[r] = (acc)0;
-    if (valid) {
-      const cpx<real> t = s[(size_t)r * stride];
-      yr[r] = (acc)t.x;
-      yi[r] = (acc)t.y;
+    for (int r = 0; r < M; r++) t[r] = s[(size_t)r * stride];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      yr[r] = valid ? (acc)t[r].x : (acc)0;
+      yi[r] = valid ? (acc)t[r].y : (acc)0;
     }
   }
   __syncthreads();  // row tables
@@ -1242,25 +1252,41 @@ hipError_t launch_tri(const DevGrid& g, cpx<double>* S, double* CP, hipStream_t 
 
 bool tri_part_supported(const DevGrid& g) { return g.nym == 64 || g.nym == 128 || g.nym == 192; }
 
-hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlow, double lam_thr,
-                        hipStream_t s) {
+namespace {
+// The FP32 solves of launch_tri32 in P parts of MM rows (16 modes per wave
+// when P = 16).
+template <int P, int MM>
+hipError_t tri32_parts(const DevGrid& g, cpx<float>* S, const int* low, int nlow, double lam_thr,
+                       hipStream_t s) {
+  constexpr int MWX = P == 16 ? 16 : 0;
+  constexpr int MB = kBlock / P;
   const size_t nm = (size_t)g.nxh * g.nzp;
-  constexpr int MB = kBlock / 8;
   const dim3 grid((unsigned)((nm + MB - 1) / MB));
   if (!low) {  // FP64 arithmetic on every mode
-    C3D_DISPATCH_M(g.nym, hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll>), grid,
-                                             dim3(kBlock), 0, s, g, S, nullptr, 0, 0.0, S));
+    hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriAll, float, P, MWX>), grid, dim3(kBlock),
+                       0, s, g, S, nullptr, 0, 0.0, S);
     return hipGetLastError();
   }
   if (nlow <= 0) return hipErrorInvalidValue;  // the (0,0) mode is always a low mode
   const dim3 gridl((unsigned)((nlow + MB - 1) / MB));
-  C3D_DISPATCH_M(g.nym, {
-    hipLaunchKernelGGL((k_tri_part<float, float, MM, kTriHigh>), grid, dim3(kBlock), 0, s, g, S,
-                       nullptr, 0, lam_thr, S);
-    hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriList>), gridl, dim3(kBlock), 0, s, g, S,
-                       low, nlow, 0.0, S);
-  });
+  hipLaunchKernelGGL((k_tri_part<float, float, MM, kTriHigh, float, P, MWX>), grid, dim3(kBlock), 0,
+                     s, g, S, nullptr, 0, lam_thr, S);
+  hipLaunchKernelGGL((k_tri_part<float, double, MM, kTriList, float, P, MWX>), gridl, dim3(kBlock),
+                     0, s, g, S, low, nlow, 0.0, S);
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_tri32(const DevGrid& g, cpx<float>* S, const int* low, int nlow, double lam_thr,
+                        hipStream_t s) {
+  // 192 rows as 16 parts of 12: 8 parts of 24 with all of a lane's FP32 rows
+  // in flight need more than 256 registers (one wave per SIMD)
+  switch (g.nym) {
+    case 64: return tri32_parts<8, 8>(g, S, low, nlow, lam_thr, s);
+    case 128: return tri32_parts<8, 16>(g, S, low, nlow, lam_thr, s);
+    case 192: return tri32_parts<16, 12>(g, S, low, nlow, lam_thr, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_tri32to64(const DevGrid& g, cpx<float>* S32, cpx<double>* S64, hipStream_t s) {
