@@ -107,6 +107,19 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
   const size_t pk = (size_t)(nxg - 2) * (nyg - 2);  // rhs_p plane stride
   const int rowj = nxg - 2;                          // rhs_p row stride
   double pm[NPL], p0[NPL];
+  // per-lane spacing reciprocals, loaded once for the whole tile (lanes past
+  // the row end read row position N-1's and do not use them)
+  double rxa[NPL], cxp[NPL], cxm[NPL];
+#pragma unroll
+  for (int p = 0; p < NPL; p++) {
+    const int i = min(tid + p * kBlock, N - 1) + 2;
+    rxa[p] = g.rdxf[i];
+    cxp[p] = cxm[p] = 0.0;
+    if constexpr (RESID) {
+      cxp[p] = g.rdxg[i];
+      cxm[p] = g.rdxg[(i == 2) ? nx - 1 : i - 1];
+    }
+  }
   double amax = 0.0;
   double gj = 0.0, rhy = 0.0, rhym = 0.0;
   // the y neighbours' row offsets: on the first and last rows (no gradient
@@ -191,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
     for (int p = 0; p < NPL; p++) {
       const int n = tid + p * kBlock;
       if (N % kBlock == 0 || n < N) {
-        const double rx = g.rdxf[n + 2];
+        const double rx = rxa[p];
 #pragma unroll
         for (int r = 0; r < RB; r += 2) {  // rows 2q, 2q+1 -> one complex
           double d[2];
@@ -214,8 +227,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_x(DevGrid g, const double* __res
               const int km = (k == 2) ? g.nz - 1 : k - 1;
               const double* q = v[r + h][p];
               const double c0 = p0[p];
-              const double gxp = (q[2] - c0) * g.rdxg[i];
-              const double gxm = (c0 - q[3]) * g.rdxg[(i == 2) ? nx - 1 : i - 1];
+              const double gxp = (q[2] - c0) * cxp[p];
+              const double gxm = (c0 - q[3]) * cxm[p];
               const double gyp = (q[4] - c0) * rhy;
               const double gym = (c0 - q[5]) * rhym;
               const double gzp = (q[1] - c0) * g.rdzg[k];
