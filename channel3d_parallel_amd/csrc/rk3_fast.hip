@@ -32,12 +32,24 @@ constexpr double kC11 = 8.0 / 15.0, kC21 = 1.0 / 4.0, kC22 = 5.0 / 12.0, kC31 = 
 template <int KC, int STAGE, bool PIPE>
 __device__ __forceinline__ void rhs_stage_body(const DevGrid& g, const RkStage& a) {
   __shared__ double tile[3 * 3 * kTP];
+  __shared__ double xt[kXT * kTX];
   unsigned bx, by, bz;
   xcd_tile(bx, by, bz);
   const int i0 = bx * 64 + 2, j0 = by * 4 + 2, kb = bz * KC + 2;
-  const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
+  // j is the same on all lanes of a wavefront (64 x positions of one row):
+  // wave-uniform, so its spacing factors are scalar loads
+  const int i = i0 + (threadIdx.x & 63);
+  const int j = __builtin_amdgcn_readfirstlane(j0 + (int)(threadIdx.x >> 6));
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
-  TileAcc A{tile, i0, j0, kb};
+  // the tile's x spacing factors, i0-1 .. i0+64, clamped to the tables'
+  // entries 0 .. nx+1 (lanes past the domain read but do not use them);
+  // made visible by the first plane's barrier below
+  for (int e = threadIdx.x; e < kXT * kTX; e += kBf) {
+    const int t = e / kTX, ii = min(i0 - 1 + e % kTX, nx + 1);
+    const double* src = t == kXrdxg ? g.rdxg : t == kXrdxf ? g.rdxf : t == kXnudxf ? g.nudxf : g.nudxg;
+    xt[e] = src[ii];
+  }
+  TileAccX A{{tile, i0, j0, kb}, xt};
   // planes kb-1 and kb in LDS, kb+1 in registers; with PIPE, at plane k the
   // loads of plane k+2 are in flight while k is evaluated
   PlaneRegs nxt;

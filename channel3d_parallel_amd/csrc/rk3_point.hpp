@@ -35,6 +35,30 @@ struct TileAcc {
   __device__ __forceinline__ double W(int i, int j, int k) const { return at(2, i, j, k); }
 };
 
+// The x-direction spacing factors of the point functions (1/dxg, 1/dxf,
+// nu/dxf, nu/dxg), read from the grid's arrays or, for a TileAccX, from the
+// workgroup's LDS copy of the tile's x range i0-1 .. i0+64 (the same values:
+// an LDS read in place of a global load per point).
+enum { kXrdxg = 0, kXrdxf, kXnudxf, kXnudxg, kXT };
+template <int T>
+__device__ __forceinline__ const double* xtab_array(const DevGrid& g) {
+  if constexpr (T == kXrdxg) return g.rdxg;
+  else if constexpr (T == kXrdxf) return g.rdxf;
+  else if constexpr (T == kXnudxf) return g.nudxf;
+  else return g.nudxg;
+}
+struct TileAccX : TileAcc {
+  const double* xt;  // [kXT][kTX]
+};
+template <int T>
+__device__ __forceinline__ double xtab(const DevGrid& g, const TileAcc&, int i) {
+  return xtab_array<T>(g)[i];
+}
+template <int T>
+__device__ __forceinline__ double xtab(const DevGrid&, const TileAccX& A, int i) {
+  return A.xt[T * kTX + (i - A.i0 + 1)];
+}
+
 // Plane k of U, V, W over the tile of A with its one-cell x/y halo, held in
 // registers: element e = threadIdx.x + q * 256 (q = 0, 1) of the kTP tile
 // cells; cells outside the arrays are zero.  load_plane issues the global
@@ -103,7 +127,7 @@ __device__ __forceinline__ double rhs_u_at(const DevGrid& g, const Acc& A,
                                          double dPdx, int i, int j, int k) {
   const double uc1 = 0.5 * (A.U(i - 1, j, k) + A.U(i, j, k));
   const double uc2 = 0.5 * (A.U(i, j, k) + A.U(i + 1, j, k));
-  const double a = quot<FAST>(-(uc2 * uc2 - uc1 * uc1), g.dxg[i], g.rdxg[i]);
+  const double a = quot<FAST>(-(uc2 * uc2 - uc1 * uc1), g.dxg[i], xtab<kXrdxg>(g, A, i));
   const double uv1 = (0.5 * (A.U(i, j, k) + A.U(i, j + 1, k))) * (0.5 * (A.V(i, j, k) + A.V(i + 1, j, k)));
   const double uv0 =
       (0.5 * (A.U(i, j - 1, k) + A.U(i, j, k))) * (0.5 * (A.V(i, j - 1, k) + A.V(i + 1, j - 1, k)));
@@ -116,8 +140,9 @@ __device__ __forceinline__ double rhs_u_at(const DevGrid& g, const Acc& A,
   const double t1m = quot<FAST>(A.U(i, j, k) - A.U(i, j - 1, k), g.dyg[j - 1], g.rdyg[j - 1]);
   // 1.0/dx_3 and nu*1.0/dx_2 are index-only factors, tabulated (same
   // correctly rounded quotients as evaluating them here)
-  const double visc = g.rdxg[i] * (g.nudxf[i + 1] * (A.U(i + 1, j, k) - A.U(i, j, k)) -
-                                   g.nudxf[i] * (A.U(i, j, k) - A.U(i - 1, j, k))) +
+  const double visc = xtab<kXrdxg>(g, A, i) *
+                          (xtab<kXnudxf>(g, A, i + 1) * (A.U(i + 1, j, k) - A.U(i, j, k)) -
+                           xtab<kXnudxf>(g, A, i) * (A.U(i, j, k) - A.U(i - 1, j, k))) +
                       g.rdyf[j] * (nu * t1 - nu * t1m) +
                       g.rdzf[k] * (g.nudzg[k] * (A.U(i, j, k + 1) - A.U(i, j, k)) -
                                    g.nudzg[k - 1] * (A.U(i, j, k) - A.U(i, j, k - 1)));
@@ -141,13 +166,14 @@ __device__ __forceinline__ double rhs_v_at(const DevGrid& g, const Acc& A,
   const double uv1 = (0.5 * (A.U(i, j, k) + A.U(i, j + 1, k))) * (0.5 * (A.V(i, j, k) + A.V(i + 1, j, k)));
   const double uv0 =
       (0.5 * (A.U(i - 1, j, k) + A.U(i - 1, j + 1, k))) * (0.5 * (A.V(i - 1, j, k) + A.V(i, j, k)));
-  const double b = quot<FAST>(-(uv1 - uv0), g.dxf[i], g.rdxf[i]);
+  const double b = quot<FAST>(-(uv1 - uv0), g.dxf[i], xtab<kXrdxf>(g, A, i));
   const double vw1 = (0.5 * (A.V(i, j, k) + A.V(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i, j + 1, k)));
   const double vw0 =
       (0.5 * (A.V(i, j, k - 1) + A.V(i, j, k))) * (0.5 * (A.W(i, j, k - 1) + A.W(i, j + 1, k - 1)));
   const double c = quot<FAST>(-(vw1 - vw0), g.dzf[k], g.rdzf[k]);
-  const double visc = g.rdxf[i] * (g.nudxg[i] * (A.V(i + 1, j, k) - A.V(i, j, k)) -
-                                   g.nudxg[i - 1] * (A.V(i, j, k) - A.V(i - 1, j, k))) +
+  const double visc = xtab<kXrdxf>(g, A, i) *
+                          (xtab<kXnudxg>(g, A, i) * (A.V(i + 1, j, k) - A.V(i, j, k)) -
+                           xtab<kXnudxg>(g, A, i - 1) * (A.V(i, j, k) - A.V(i - 1, j, k))) +
                       g.rdyg[j] * (g.nudyf[j + 1] * (A.V(i, j + 1, k) - A.V(i, j, k)) -
                                    g.nudyf[j] * (A.V(i, j, k) - A.V(i, j - 1, k))) +
                       g.rdzf[k] * (g.nudzg[k] * (A.V(i, j, k + 1) - A.V(i, j, k)) -
@@ -172,15 +198,16 @@ __device__ __forceinline__ double rhs_w_at(const DevGrid& g, const Acc& A,
   const double uw1 = (0.5 * (A.U(i, j, k) + A.U(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i + 1, j, k)));
   const double uw0 =
       (0.5 * (A.U(i - 1, j, k) + A.U(i - 1, j, k + 1))) * (0.5 * (A.W(i - 1, j, k) + A.W(i, j, k)));
-  const double b = quot<FAST>(-(uw1 - uw0), g.dxf[i], g.rdxf[i]);
+  const double b = quot<FAST>(-(uw1 - uw0), g.dxf[i], xtab<kXrdxf>(g, A, i));
   const double vw1 = (0.5 * (A.V(i, j, k) + A.V(i, j, k + 1))) * (0.5 * (A.W(i, j, k) + A.W(i, j + 1, k)));
   const double vw0 =
       (0.5 * (A.V(i, j - 1, k) + A.V(i, j - 1, k + 1))) * (0.5 * (A.W(i, j - 1, k) + A.W(i, j, k)));
   const double c = quot<FAST>(-(vw1 - vw0), g.dyf[j], g.rdyf[j]);
   const double t1 = quot<FAST>(A.W(i, j + 1, k) - A.W(i, j, k), g.dyg[j], g.rdyg[j]);
   const double t1m = quot<FAST>(A.W(i, j, k) - A.W(i, j - 1, k), g.dyg[j - 1], g.rdyg[j - 1]);
-  const double visc = g.rdxf[i] * (g.nudxg[i] * (A.W(i + 1, j, k) - A.W(i, j, k)) -
-                                   g.nudxg[i - 1] * (A.W(i, j, k) - A.W(i - 1, j, k))) +
+  const double visc = xtab<kXrdxf>(g, A, i) *
+                          (xtab<kXnudxg>(g, A, i) * (A.W(i + 1, j, k) - A.W(i, j, k)) -
+                           xtab<kXnudxg>(g, A, i - 1) * (A.W(i, j, k) - A.W(i - 1, j, k))) +
                       g.rdyf[j] * (nu * t1 - nu * t1m) +
                       g.rdzg[k] * (g.nudzf[k + 1] * (A.W(i, j, k + 1) - A.W(i, j, k)) -
                                    g.nudzf[k] * (A.W(i, j, k) - A.W(i, j, k - 1)));
