@@ -41,12 +41,22 @@ __global__ __launch_bounds__(kB) void k_rhs(DevGrid g, const double* __restrict_
                                             double dPdy, double dPdz, double* __restrict__ Fu,
                                             double* __restrict__ Fv, double* __restrict__ Fw) {
   __shared__ double tile[3 * 3 * kTP];
+  __shared__ double xt[kXT * kTX];
   unsigned bx, by, bz;
   xcd_tile(bx, by, bz);  // neighbouring tiles on one XCD (shared halo rows)
   const int i0 = bx * 64 + 2, j0 = by * 4 + 2, kb = bz * KC + 2;
-  const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
+  // j is the same on all lanes of a wavefront (one row of 64 x positions)
+  const int i = i0 + (threadIdx.x & 63);
+  const int j = __builtin_amdgcn_readfirstlane(j0 + (int)(threadIdx.x >> 6));
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
-  TileAcc A{tile, i0, j0, kb};
+  // LDS copy of the tile's x spacing factors (the same table values, so the
+  // result stays bitwise); the first plane's barrier makes it visible
+  for (int e = threadIdx.x; e < kXT * kTX; e += kB) {
+    const int t = e / kTX, ii = min(i0 - 1 + e % kTX, nx + 1);
+    const double* src = t == kXrdxg ? g.rdxg : t == kXrdxf ? g.rdxf : t == kXnudxf ? g.nudxf : g.nudxg;
+    xt[e] = src[ii];
+  }
+  TileAccX A{{tile, i0, j0, kb}, xt};
   auto stage = [&](int k) { stage_tile_plane(g, U, V, W, tile, A, k, kB); };
   stage(kb - 1);
   stage(kb);
