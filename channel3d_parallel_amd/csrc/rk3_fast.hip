@@ -120,9 +120,10 @@ __global__ __launch_bounds__(kBf) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 inline dim3 gpts_f(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
 
-// Plane loading of the fused kernel: 0 load-then-barrier (default: 5.43 ms
-// per RK3 step at 384x193x384), 1 pipelined (5.61 ms: 3 instead of 4 waves
-// per SIMD), 2 pipelined at 4 waves per SIMD (5.72 ms, spills).
+// Plane loading of the fused kernel: 0 load-then-barrier (the default), 1
+// pipelined, 2 pipelined at 4 waves per SIMD.  Measured at 384x193x384
+// before the x factors moved to LDS: 5.43, 5.61 (3 instead of 4 waves per
+// SIMD) and 5.72 ms (spills) per RK3 step.
 // C3D_RHS_PIPE overrides the default (tuning knob).
 int rhs_pipe() {
   static const int v = [] {
