@@ -838,7 +838,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_correct(DevGrid g, double* __re
   unsigned bx, by, bz;
   xcd_tile(bx, by, bz);
   const int i = bx * 64 + (threadIdx.x & 63) + 2;
-  const int j = by * 4 + (threadIdx.x >> 6) + 2;
+  // one row per wavefront: j is wave-uniform (scalar row arithmetic, 1/dy
+  // a scalar load)
+  const int j = __builtin_amdgcn_readfirstlane((int)(by * 4 + (threadIdx.x >> 6) + 2));
   const int kb = bz * KC + 2;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxg = g.nxg, nyg = g.nyg, nzg = g.nzg;
   if (i > nx - 1 || j > nyg - 1) return;
